@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark: plain GPU tile-fit scoring on MI355X (one GPU).
+
+Workload (default): 1920x1080 synthetic RGB float32 image (seed 0, 600 primitives),
+adaptive quadtree tiles of 8-64 px split while the tile's luminance variance > 1e-3,
+1024 candidates per tile per step drawn on the device, stack depth 16, float64 R-D
+objective (lambda of quality 75).  A step is one greedy round over every tile: score all
+candidates against the cached canvases (one launch per tile-size class), take the first
+minimum, accept and composite (one launch).  Inputs are resident in HBM before timing.
+
+Prints one JSON line (see the driver contract): value = candidate*pixel evaluations per
+second over the K timed steps; roofline = the dominant scoring kernel's algorithmic FP32
+rate (FLOP per evaluation x evaluations per launch / mean launch time measured with HIP
+events on the launch stream) against MI355X's 157.3 TFLOP/s FP32 vector peak;
+cpu_baseline = the CPU oracle (C, float32, OpenMP) on a bounded sample of the same step.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "candidate·pixel evals/s; time-to-PSNR, refined+batched vs plain GPU search"
+PEAK_FP32 = 157.3e12
+# Algorithmic FLOP per candidate-pixel evaluation (add/sub/mul/div = 1, FMA-shaped = 2,
+# compares 0, per-candidate constants hoisted): coverage + composite (10) + SSE (9).
+FLOP_PER_EVAL = {"triangle": 32, "ellipse": 33, "rect": 28}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flop_per_eval_mix() -> float:
+    # family = ((h & 0xFFFF) * 3) >> 16: counts of 16-bit values mapping to 0, 1, 2
+    counts = np.bincount((np.arange(65536, dtype=np.int64) * 3) >> 16, minlength=3) / 65536.0
+    return float(counts[0] * FLOP_PER_EVAL["ellipse"] + counts[1] * FLOP_PER_EVAL["rect"]
+                 + counts[2] * FLOP_PER_EVAL["triangle"])
+
+
+def build_workload(args):
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.synth import CONFIG_C2, image_checksum, synth_image
+
+    c = dict(CONFIG_C2)
+    img = synth_image(c["height"], c["width"], c["n_prims"], seed=c["seed"])
+    qt = decompose_quadtree_variance(img, threshold=c["var_threshold"], min_tile=c["min_tile"],
+                                     max_tile=c["max_tile"])
+    tiles = TileSet.from_images([img], [qt])
+    return c, img, tiles, image_checksum(img)
+
+
+def traffic_from_profiles(kernel: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(tiles, c, lam, seed, budget_s, gpu_rec0):
+    """Time the C oracle on a bounded tile sample of step 0 and check its winners."""
+    from oracle import coracle
+
+    coracle.build()
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    order = np.random.default_rng(1).permutation(tiles.n_tiles)
+
+    def run(n):
+        ids = np.sort(order[:n]).astype(np.int32)
+        ts = tiles.ts[ids]
+        offs = np.concatenate(([0], np.cumsum(ts.astype(np.int64) ** 2)[:-1]))
+        tgt = np.concatenate([tiles.tile_pixels(int(t)).ravel() for t in ids])
+        fit = coracle.PlainFit(tgt, offs, ts, c["n_cand"], c["depth"], seed, lam, tile_ids=ids)
+        t0 = time.perf_counter()
+        res = fit.run(1, n_threads=cores)
+        return ids, ts, res, time.perf_counter() - t0
+
+    n = max(4, tiles.n_tiles // 500)
+    while True:
+        ids, ts, res, dt = run(n)
+        if dt > 0.3 * budget_s or n >= tiles.n_tiles:
+            break
+        n = min(tiles.n_tiles, int(n * min(20.0, max(2.0, 0.6 * budget_s / max(dt, 1e-3)))))
+    evals = int((ts.astype(np.int64) ** 2).sum()) * c["n_cand"]
+    agree = bool(np.array_equal(res["win_idx"][0], gpu_rec0["win_idx"][ids])
+                 and np.array_equal(res["win_acc"][0], gpu_rec0["win_acc"][ids])
+                 and res["win_obj"][0].tobytes() == np.ascontiguousarray(gpu_rec0["win_obj"][ids]).tobytes())
+    return {
+        "value": evals / dt, "unit": "candidate·pixel evals/s", "cores": cores, "kind": "port",
+        "sample": f"step 0 of the same fit on {len(ids)} of {tiles.n_tiles} tiles "
+                  f"({evals / 1e9:.2f} G evals in {dt:.1f} s; C float32 oracle, OpenMP {cores} threads)",
+        "winners_match_gpu": agree,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the tile-fit path "
+              "shards by tiles with no collective, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from weft_amd import _lib
+    from weft_amd.fit import PlainFitter
+    from weft_amd.rd import tile_lambda
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    c, img, tiles, checksum = build_workload(args)
+    seed = 0
+    lam = tile_lambda(75)
+    total_steps = args.warmup + args.steps
+    log(f"[bench] {tiles.n_tiles} tiles, sizes {dict(zip(*np.unique(tiles.ts, return_counts=True)))}, "
+        f"{tiles.n_pixels} px, {c['n_cand']} candidates, steps {args.warmup}+{args.steps}")
+    fitter = PlainFitter(tiles, n_cand=c["n_cand"], depth=c["depth"], seed=seed, lam=lam,
+                         max_steps=max(total_steps, 1))
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        fitter.step()
+    torch.cuda.synchronize()
+    log("[bench] warmup done")
+
+    # Per-class HIP events on the launch stream for the scoring kernels.
+    stream = torch.cuda.current_stream()
+    ev = {cl["ts"]: [] for cl in fitter.classes}
+    t0 = time.perf_counter()
+    torch.cuda.synchronize()
+    start_ev = torch.cuda.Event(enable_timing=True)
+    end_ev = torch.cuda.Event(enable_timing=True)
+    start_ev.record(stream)
+    import ctypes
+    for _ in range(args.steps):
+        s = fitter.step_idx
+        cands = fitter._cands(s)
+        for (w, start), cl in zip(fitter._works, fitter.classes):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            _lib.check(fitter.L.weft_fit_score(ctypes.byref(fitter._state), ctypes.byref(w), ctypes.byref(cands),
+                                               None, None, fitter.d_part_obj.data_ptr() + 8 * start,
+                                               fitter.d_part_idx.data_ptr() + 4 * start, _lib.stream_ptr()),
+                       "weft_fit_score")
+            b.record(stream)
+            ev[cl["ts"]].append((a, b))
+        fitter.select()
+        fitter.step_idx += 1
+    end_ev.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start_ev.elapsed_time(end_ev)
+    log(f"[bench] timed {args.steps} steps: wall {wall * 1e3:.2f} ms, events {gpu_ms:.2f} ms")
+
+    # Evaluations actually performed: a tile is scored in step s iff its stack was not full.
+    rec = fitter.records(0, total_steps)
+    acc = rec["win_acc"].astype(np.int64)
+    nsel_before = np.cumsum(acc, axis=0) - acc
+    px = tiles.ts.astype(np.int64) ** 2
+    active = nsel_before < c["depth"]
+    evals_steps = (active * px[None, :]).sum(axis=1) * c["n_cand"]
+    evals_timed = int(evals_steps[args.warmup:].sum())
+    value = evals_timed / wall
+
+    fpe = flop_per_eval_mix()
+    per_class = {}
+    for cl in fitter.classes:
+        ts = cl["ts"]
+        times = [a.elapsed_time(b) * 1e-3 for a, b in ev[ts]]
+        ids = tiles.ts == ts
+        ev_launch = float(np.mean([(active[s][ids] * px[ids]).sum() * c["n_cand"]
+                                   for s in range(args.warmup, total_steps)]))
+        per_class[ts] = dict(mean_s=float(np.mean(times)), total_s=float(np.sum(times)),
+                             evals_per_launch=ev_launch,
+                             tflops=ev_launch * fpe / float(np.mean(times)) / 1e12)
+    dom = max(per_class, key=lambda k: per_class[k]["total_s"])
+    d = per_class[dom]
+    achieved = d["evals_per_launch"] * fpe / d["mean_s"]
+    kname = f"weft_fit_score_ts{dom}"
+    roofline = {"bound": "valu", "kernel": kname, "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic_from_profiles(kname),
+                "flop_per_eval": fpe, "evals_per_launch": d["evals_per_launch"],
+                "mean_launch_ms": d["mean_s"] * 1e3,
+                "per_class": {str(k): {"mean_launch_ms": v["mean_s"] * 1e3, "tflops": v["tflops"]}
+                              for k, v in per_class.items()}}
+    log(f"[bench] {value / 1e12:.3f} T evals/s; dominant {kname}: {achieved / 1e12:.1f} TFLOP/s")
+
+    cpu = None
+    if not args.no_cpu:
+        rec0 = fitter.records(0, 1)
+        rec0 = {k: v[0] for k, v in rec0.items()}
+        cpu = cpu_baseline(tiles, c, lam, seed, args.cpu_seconds, rec0)
+        log(f"[bench] cpu baseline {cpu['value'] / 1e9:.3f} G evals/s on {cpu['cores']} cores; "
+            f"winners match: {cpu['winners_match_gpu']}")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "candidate·pixel evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain GPU tile-fit scoring, 1920x1080 synthetic image (seed 0, 600 "
+                               "primitives), quadtree 8-64 px (luma variance > 1e-3), 1024 drawn "
+                               "candidates/tile/step, depth 16",
+                   "tiles": int(tiles.n_tiles), "pixels": int(tiles.n_pixels),
+                   "tile_sizes": {str(int(k)): int(v) for k, v in zip(*np.unique(tiles.ts, return_counts=True))},
+                   "n_cand": c["n_cand"], "depth": c["depth"], "image_sha256": checksum,
+                   "evals_per_step": int(evals_steps[args.warmup]), "parallelism": "1 GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

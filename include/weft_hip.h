@@ -1,0 +1,189 @@
+/*
+ * weft_hip.h -- C ABI of libweft_hip.so, the MI355X (gfx950) implementation of the
+ * Weft encoder tile-fit scoring path and the primitive-walk renderer it shares.
+ *
+ * Every entry point returns 0 on success and a negative WEFT_E* code (or a negated
+ * hipError_t) on failure; weft_last_error() describes the last failure.  No C++
+ * exception crosses this boundary.  Pointers named d_* are DEVICE pointers (for
+ * example torch tensors' data_ptr()); `stream` is a hipStream_t (NULL = default
+ * stream).  All launches are asynchronous on `stream`.
+ *
+ * Reference interfaces replaced (jarmak-personal/weft, paths relative to its root):
+ *   weft_init / weft_shutdown     gpu_context.get_cuda_state / reset_cuda_state
+ *                                 (src/weft/gpu_context.py:16-62) and the NVRTC build
+ *                                 + cuModuleGetFunction of gpu_render._get_render_state
+ *                                 (src/weft/gpu_render.py:696-734)
+ *   weft_score_evals              kernel batch_tile_objective
+ *                                 (src/weft/gpu_render.py:113-152) as launched by
+ *                                 gpu_batch_score_prepacked (gpu_render.py:455-566) and
+ *                                 gpu_batch_objectives (gpu_render.py:737-845); MSE per
+ *                                 eval, but with the CPU scorer's semantics
+ *                                 (render.py:127-234 + encoder.py:271-285)
+ *   weft_fit_*                    the greedy round of encoder._fit_adaptive_state
+ *                                 (src/weft/encoder.py:2745-2811) / _greedy_add
+ *                                 (encoder.py:752-817) with the current stack kept as a
+ *                                 cached canvas on the device
+ *   weft_render_tiles             kernel decode_tile_pixels (gpu_render.py:194-237) as
+ *                                 launched by gpu_render_tiles_to_image
+ *                                 (gpu_render.py:594-690), with render.py's alpha clamp
+ */
+#ifndef WEFT_HIP_H
+#define WEFT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Packed primitive, 60 bytes: the reference's PrimData / _PRIM_DTYPE
+ * (src/weft/gpu_render.py:26-32, 241-247).  Kinds 0-4 as in src/weft/constants.py:38-42;
+ * kind 6 = rotated ellipse, 7 = rotated rectangle (geom = cx, cy, rx, ry, cos t, sin t);
+ * kind 8 = triangle drawn from (cx, cy, rx, ry, t), geom = its three vertices. */
+typedef struct WeftPrim {
+    int32_t kind;
+    float geom[7];
+    float color0[3];
+    float color1[3];
+    float alpha;
+} WeftPrim;
+
+/* One evaluation: the reference's EvalDesc (src/weft/gpu_render.py:106-111). */
+typedef struct WeftEval {
+    int32_t prim_offset;
+    int32_t prim_count;
+    int32_t tile_offset; /* in floats into the packed tile buffer */
+    int32_t tile_size;
+} WeftEval;
+
+/* Decode descriptor: the reference's TileDecodeDesc (src/weft/gpu_render.py:179-192). */
+typedef struct WeftTileDecodeDesc {
+    int32_t prim_offset;
+    int32_t prim_count;
+    int32_t target_x0;
+    int32_t target_y0;
+    int32_t target_w;
+    int32_t target_h;
+    int32_t source_size;
+    int32_t _pad;
+    float res0_r;
+    float res0_g;
+    float res0_b;
+    float _pad2;
+} WeftTileDecodeDesc;
+
+/* Quantized candidate code (no reference counterpart; the configurations' random
+ * candidates).  Field meanings are documented at weft_draw_codes. */
+typedef struct WeftCode {
+    uint16_t family, cx, cy, rx, ry, theta, r, g, b, alpha;
+} WeftCode;
+
+/* Device-resident state of a plain tile fit: one entry per tile, SoA. */
+typedef struct WeftFitState {
+    const float* d_target;   /* packed tiles, tile k at float offset 3*d_tile_pix[k] */
+    float* d_canvas;         /* unclipped composite of the tile's accepted stack */
+    const int64_t* d_tile_pix;
+    const int32_t* d_tile_ts;
+    int32_t* d_tile_nsel;    /* primitives accepted so far */
+    int32_t* d_tile_bits;    /* _estimate_bits of the accepted stack */
+    double* d_tile_obj;      /* current objective (MSE + lambda * bits) */
+    int32_t n_tiles;
+    int32_t depth;           /* maximum stack depth */
+    double lam;
+    int32_t obj_mode;        /* 0: float64 objective (_greedy_add), 1: float32 (_fit_adaptive_state) */
+    int32_t _pad;
+} WeftFitState;
+
+/* Work list for one tile-size class: work item i scores candidates
+ * [d_work_cand0[i], d_work_cand0[i] + block) of tile d_work_tile[i]. */
+typedef struct WeftWork {
+    const int32_t* d_work_tile;
+    const int32_t* d_work_cand0;
+    int32_t n_work;
+    int32_t tile_size;       /* 8, 16, 32 or 64: every tile in the list has this size */
+    int32_t block;           /* threads per workgroup (= candidates per work item), 64..1024 */
+    int32_t _pad;
+} WeftWork;
+
+/* Candidate source: explicit records or drawn on the device from (seed, step). */
+typedef struct WeftCands {
+    const WeftPrim* d_prims;     /* NULL: draw codes on the device */
+    const int64_t* d_tile_cand;  /* per tile, offset of its first candidate in d_prims */
+    uint64_t seed;
+    uint32_t step;
+    int32_t n_cand;              /* candidates per tile */
+    int32_t has_general;         /* nonzero if any explicit candidate has kind 1, 2 or 3 */
+    int32_t _pad;
+} WeftCands;
+
+enum {
+    WEFT_OK = 0,
+    WEFT_E_NOT_INITIALISED = -1000,
+    WEFT_E_INVALID = -1001,
+    WEFT_E_UNSUPPORTED_TILE = -1002,
+    WEFT_E_MODULE = -1003,
+    WEFT_E_TOO_LARGE = -1004,
+};
+
+/* Load the gfx950 code object embedded in the library on `device` and build the
+ * candidate tables.  Idempotent per device. */
+int weft_init(int device);
+/* Unload the module and drop every cached handle (reset_cuda_state). */
+int weft_shutdown(void);
+const char* weft_last_error(void);
+/* Library and kernel ABI version (for the Python loader's sanity check). */
+int weft_abi_version(void);
+
+/* Reference-semantics scoring of explicit primitive stacks (one lane per eval):
+ * out_mse[i] = float32(SSE_i / (3 * ts^2)), SSE in NumPy's pairwise float32 order
+ * over clip(render_tile(stack_i)) - target.  d_out_sse may be NULL.
+ * Tile sizes 8, 16, 32, 64. */
+int weft_score_evals(const WeftEval* d_evals, int n_evals, const WeftPrim* d_prims,
+                     const float* d_tiles, float* d_out_mse, float* d_out_sse, void* stream);
+
+/* Plain fit, part 1: objective of every tile's current canvas (call once after the
+ * canvas is set).  d_tile_obj[t] = MSE(clip(canvas_t), target_t) + lam * bits_t. */
+int weft_fit_init(const WeftFitState* st, void* stream);
+
+/* Plain fit, part 2: score candidates of the tiles in `work` against their cached
+ * canvases.  Per candidate k of tile t (index t * n_cand + k) writes SSE / MSE when
+ * the tables are given, and per work item the first minimum of the objective into
+ * d_part_obj / d_part_idx.  Tiles whose stack is full are skipped. */
+int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands* cands,
+                   float* d_out_sse, float* d_out_mse, double* d_part_obj, int32_t* d_part_idx,
+                   void* stream);
+
+/* Plain fit, part 3: per tile reduce its work items' partial minima (work items
+ * d_tile_work0[t] .. + d_tile_nwork[t] in the concatenated partial arrays), accept the
+ * winner when its objective beats the tile's current one, composite it onto the canvas
+ * and update the state.  Records per tile: winner index, objective, accepted flag and,
+ * when candidates are drawn on the device, the winner's code (d_win_code may be NULL). */
+int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_t* d_tile_work0,
+                    const int32_t* d_tile_nwork, const double* d_part_obj,
+                    const int32_t* d_part_idx, int32_t* d_win_idx, double* d_win_obj,
+                    int32_t* d_win_acc, WeftCode* d_win_code, void* stream);
+
+/* Draw candidate codes / records for the given tiles exactly as weft_fit_score does on
+ * the device (for checking and serialization).  d_codes / d_prims: n_tiles * n_cand.
+ *   family : 0 ellipse (kind 6), 1 rectangle (kind 7), 2 triangle (kind 8)
+ *   cx, cy : code / 1024 px, tile-local, in [0, ts)
+ *   rx, ry : code / 1024 px, log-uniform in [4, 64)
+ *   theta  : angle = code * pi / 4096, code < 4096
+ *   r,g,b  : code / 255 ; alpha : code / 65535 in [0.3, 1] */
+int weft_draw_codes(uint64_t seed, uint32_t step, const int32_t* d_tile_ids,
+                    const int32_t* d_tile_ts, int n_tiles, int n_cand, WeftCode* d_codes,
+                    WeftPrim* d_prims, void* stream);
+
+/* Host-side copies of the candidate tables (log-radius codes, cos, sin: 4096 each). */
+int weft_get_tables(uint16_t* logr, float* cosv, float* sinv);
+
+/* Decode: render every tile's stack into an H x W x 3 float32 image (RES0 bias, clip
+ * to [0, 1]); pixels no tile covers are left untouched. */
+int weft_render_tiles(const WeftTileDecodeDesc* d_descs, int n_tiles, const WeftPrim* d_prims,
+                      float* d_out, int width, int height, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WEFT_HIP_H */

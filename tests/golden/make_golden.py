@@ -1,0 +1,266 @@
+"""Write golden input/output vectors from the reference implementation.
+
+Run once in the build container (the only place the reference tree exists):
+
+    python tests/golden/make_golden.py
+
+It imports the reference's own CPU functions from /root/reference/src and records,
+for seeded inputs, what they return.  Only data is written (``*.npz`` with plain
+arrays, no pickles); the fixtures are what the CPU oracle and the HIP kernels are
+pinned against.  Functions captured (reference file:line):
+
+- ``render.render_tile``            src/weft/render.py:127-234  (kinds 0-4, all-const
+  fast path, residual bias, final clip)
+- ``encoder._tile_objective``       src/weft/encoder.py:271-285
+- ``gpu_score.batch_mse``           src/weft/gpu_score.py:45-57
+- ``encoder._greedy_add`` (GPU scorer forced to ``None``)  src/weft/encoder.py:752-817
+- ``encoder._tile_lambda``          src/weft/encoder.py:167-174
+- ``encoder._estimate_bits``        src/weft/encoder.py:259-268
+- ``primitives.encode_tile``        src/weft/primitives.py:291-298
+- ``quadtree.decompose_quadtree``   src/weft/quadtree.py:42-91
+
+Every primitive parameter is a float32-representable value, so the float32 records the
+HIP path receives carry exactly what the reference's Python floats carry.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+REF_SRC = "/root/reference/src"
+OUT_DIR = os.path.dirname(os.path.abspath(__file__))
+
+
+def _f(x) -> float:
+    return float(np.float32(x))
+
+
+def _rand_prim(rng, ts, kind):
+    from weft.primitives import Primitive
+
+    lo, hi = -0.25 * ts, 1.25 * ts
+
+    def xy():
+        return _f(rng.uniform(lo, hi))
+
+    color0 = tuple(_f(v) for v in rng.random(3))
+    color1 = None
+    r = rng.random()
+    if r < 0.1:
+        alpha = _f(rng.uniform(-0.3, 0.0))
+    elif r < 0.2:
+        alpha = _f(rng.uniform(1.0, 1.4))
+    else:
+        alpha = _f(rng.uniform(0.05, 1.0))
+    if kind == 0:
+        geom = ()
+    elif kind == 1:
+        geom = (xy(), xy(), xy(), xy())
+        if rng.random() < 0.8:
+            color1 = tuple(_f(v) for v in rng.random(3))
+    elif kind == 2:
+        geom = (xy(), xy(), xy(), xy(), _f(rng.uniform(0.0, 4.0)))
+    elif kind == 3:
+        geom = (xy(), xy(), xy(), xy(), xy(), xy(), _f(rng.uniform(0.0, 4.0)))
+    else:
+        if rng.random() < 0.08:
+            # degenerate (collinear) triangle: |den| < 1e-6 must give an empty mask
+            x0, y0 = xy(), xy()
+            geom = (x0, y0, x0, y0, xy(), xy())
+        else:
+            geom = tuple(xy() for _ in range(6))
+    return Primitive(kind=kind, geom=geom, color0=color0, color1=color1, alpha=alpha)
+
+
+def _pack(prims):
+    n = len(prims)
+    kind = np.zeros(n, np.int32)
+    geom = np.zeros((n, 7), np.float32)
+    c0 = np.zeros((n, 3), np.float32)
+    c1 = np.zeros((n, 3), np.float32)
+    has_c1 = np.zeros(n, np.int32)
+    alpha = np.zeros(n, np.float32)
+    for i, p in enumerate(prims):
+        kind[i] = p.kind
+        geom[i, : len(p.geom)] = p.geom
+        c0[i] = p.color0
+        if p.color1 is not None:
+            c1[i] = p.color1
+            has_c1[i] = 1
+        alpha[i] = p.alpha
+    return kind, geom, c0, c1, has_c1, alpha
+
+
+def _stack_arrays(cases):
+    """Flatten a list of primitive stacks into offset/count + concatenated records."""
+    offs, cnts, allp = [], [], []
+    for prims in cases:
+        offs.append(len(allp))
+        cnts.append(len(prims))
+        allp.extend(prims)
+    k, g, c0, c1, h1, a = _pack(allp)
+    return dict(prim_off=np.array(offs, np.int32), prim_cnt=np.array(cnts, np.int32),
+                kind=k, geom=g, color0=c0, color1=c1, has_color1=h1, alpha=a)
+
+
+def render_cases(rng):
+    from weft.render import render_tile
+
+    stacks, sizes, resid, outs = [], [], [], []
+    plan = []
+    for ts in (8, 16, 32, 64):
+        n_cases = 6 if ts == 64 else 14
+        for i in range(n_cases):
+            plan.append((ts, i))
+    for ts, i in plan:
+        depth = int(rng.integers(1, 9))
+        mode = i % 7
+        if mode == 0:
+            kinds = [0] * depth               # all-const fast path
+        elif mode == 1:
+            kinds = [4] * depth               # hard-edge triangles only
+        elif mode == 2:
+            kinds = list(rng.choice([0, 4], depth))
+        elif mode == 3:
+            kinds = list(rng.choice([0, 1, 4], depth))
+        else:
+            kinds = list(rng.choice([0, 1, 2, 3, 4], depth))
+        prims = [_rand_prim(rng, ts, int(k)) for k in kinds]
+        rr = None
+        if i % 5 == 4:
+            rr = tuple(int(v) for v in rng.integers(-40, 41, 3))
+        out = render_tile(prims, tile_size=ts, residual_rgb=rr)
+        stacks.append(prims)
+        sizes.append(ts)
+        resid.append(rr if rr is not None else (0, 0, 0))
+        outs.append(np.asarray(out, np.float32).ravel())
+    d = _stack_arrays(stacks)
+    d.update(tile_size=np.array(sizes, np.int32), residual=np.array(resid, np.int32),
+             has_residual=np.array([1 if (i % 5 == 4) else 0 for _, i in plan], np.int32),
+             out_off=np.cumsum([0] + [s * s * 3 for s in sizes[:-1]]).astype(np.int64),
+             out=np.concatenate(outs))
+    return d
+
+
+def objective_cases(rng):
+    import weft.encoder as enc
+    from weft.gpu_score import batch_mse
+
+    tiles, stacks, objs, dists, lams, bm = [], [], [], [], [], []
+    for ts in (8, 16, 32, 64):
+        for i in range(5):
+            src = rng.random((ts, ts, 3), dtype=np.float32)
+            depth = int(rng.integers(0, 6))
+            prims = [_rand_prim(rng, ts, int(k)) for k in rng.choice([0, 1, 4], depth)]
+            q = int(rng.integers(0, 101))
+            lam = enc._tile_lambda(q)
+            obj, pred = enc._tile_objective(src, prims, (0, 0, 0), lam)
+            dist = float(np.mean((src - pred) ** 2))
+            tiles.append(src.ravel())
+            stacks.append(prims)
+            objs.append(obj)
+            dists.append(dist)
+            lams.append(lam)
+            bm.append(float(batch_mse(src, [pred])[0]))
+    d = _stack_arrays(stacks)
+    sizes = [int(round((len(t) // 3) ** 0.5)) for t in tiles]
+    d.update(tile_size=np.array(sizes, np.int32),
+             src_off=np.cumsum([0] + [len(t) for t in tiles[:-1]]).astype(np.int64),
+             src=np.concatenate(tiles), obj=np.array(objs, np.float64),
+             dist=np.array(dists, np.float64), lam=np.array(lams, np.float64),
+             batch_mse=np.array(bm, np.float32))
+    return d
+
+
+def greedy_cases(rng):
+    import weft.encoder as enc
+    import weft.gpu_render as gr
+
+    gr.gpu_batch_objectives = lambda *_a, **_k: None  # the reference's own CPU-fallback seam
+    out_rows = []
+    tiles, pools, sel_idx, sel_cnt, best_objs, lams, maxp = [], [], [], [], [], [], []
+    for ts in (8, 16, 32):
+        for i in range(4):
+            src = rng.random((ts, ts, 3), dtype=np.float32)
+            n_pool = int(rng.integers(6, 14))
+            pool = [_rand_prim(rng, ts, int(k)) for k in rng.choice([0, 1, 4], n_pool)]
+            for p in pool:  # keep alphas in range so candidates are realistic
+                p.alpha = _f(min(1.0, max(0.2, p.alpha)))
+            lam = enc._tile_lambda(int(rng.integers(40, 100)))
+            mp = int(rng.integers(2, 6))
+            selected, best_obj, _ = enc._greedy_add(src, [], pool, mp, (0, 0, 0), lam, 4)
+            idx = [next(j for j, p in enumerate(pool) if p is s) for s in selected]
+            tiles.append(src.ravel())
+            pools.append(pool)
+            sel_idx.append(idx + [-1] * (8 - len(idx)))
+            sel_cnt.append(len(idx))
+            best_objs.append(best_obj)
+            lams.append(lam)
+            maxp.append(mp)
+    d = _stack_arrays(pools)
+    sizes = [int(round((len(t) // 3) ** 0.5)) for t in tiles]
+    d.update(tile_size=np.array(sizes, np.int32),
+             src_off=np.cumsum([0] + [len(t) for t in tiles[:-1]]).astype(np.int64),
+             src=np.concatenate(tiles), sel_idx=np.array(sel_idx, np.int32),
+             sel_cnt=np.array(sel_cnt, np.int32), best_obj=np.array(best_objs, np.float64),
+             lam=np.array(lams, np.float64), max_prims=np.array(maxp, np.int32))
+    return d
+
+
+def misc_cases(rng):
+    import weft.encoder as enc
+    from weft.primitives import TileRecord, encode_tile
+    from weft.quadtree import decompose_quadtree
+
+    lam = np.array([enc._tile_lambda(q) for q in range(-5, 106)], np.float64)
+    # serialization: stacks of quantizable primitives (kinds 0-4), values inside and
+    # outside the [0, 15] geometry range and the [0, 1] colour/alpha ranges.
+    stacks, blobs = [], []
+    for i in range(24):
+        depth = int(rng.integers(0, 6))
+        prims = [_rand_prim(rng, 16, int(k)) for k in rng.choice([0, 1, 2, 3, 4], depth)]
+        for p in prims:
+            if p.kind == 1 and p.color1 is None:
+                p.color1 = (0.25, 0.5, 0.75)
+            if rng.random() < 0.3:
+                p.color0 = (_f(rng.uniform(-0.2, 1.2)), p.color0[1], p.color0[2])
+        stacks.append(prims)
+        blobs.append(encode_tile(TileRecord(primitives=prims)))
+    bits = np.array([[enc._estimate_bits(s, False), enc._estimate_bits(s, True)] for s in stacks],
+                    np.int64)
+    d = _stack_arrays(stacks)
+    d = {"ser_" + k: v for k, v in d.items()}
+    d.update(lam_q=np.arange(-5, 106, dtype=np.int32), lam=lam,
+             ser_blob=np.frombuffer(b"".join(blobs), np.uint8).copy(),
+             ser_blob_off=np.cumsum([0] + [len(b) for b in blobs]).astype(np.int64),
+             est_bits=bits)
+    # quadtree on a small image with structure, reference criterion and sizes
+    img = np.zeros((72, 100, 3), np.float32)
+    yy, xx = np.mgrid[0:72, 0:100]
+    img[..., 0] = (xx / 100.0).astype(np.float32)
+    img[..., 1] = ((xx - 50) ** 2 + (yy - 30) ** 2 < 400).astype(np.float32)
+    img[..., 2] = (rng.random((72, 100)) * 0.3).astype(np.float32) * (xx > 60)
+    qts = []
+    for thr in (0.03, 0.08, 0.12, 0.5):
+        tl = decompose_quadtree(img, split_threshold=thr, min_tile=8, max_tile=32)
+        qts.append(np.array([[t.x, t.y, t.size, thr * 1000] for t in tl], np.int32))
+    d.update(qt_image=img, qt_tiles=np.concatenate(qts),
+             qt_counts=np.array([len(q) for q in qts], np.int32),
+             qt_thr=np.array([0.03, 0.08, 0.12, 0.5], np.float64))
+    return d
+
+
+def main() -> None:
+    sys.path.insert(0, REF_SRC)
+    rng = np.random.default_rng(20261203)
+    np.savez_compressed(os.path.join(OUT_DIR, "ref_render_tile.npz"), **render_cases(rng))
+    np.savez_compressed(os.path.join(OUT_DIR, "ref_tile_objective.npz"), **objective_cases(rng))
+    np.savez_compressed(os.path.join(OUT_DIR, "ref_greedy_add.npz"), **greedy_cases(rng))
+    np.savez_compressed(os.path.join(OUT_DIR, "ref_misc.npz"), **misc_cases(rng))
+    print("wrote fixtures to", OUT_DIR)
+
+
+if __name__ == "__main__":
+    main()

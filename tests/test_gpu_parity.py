@@ -1,0 +1,286 @@
+"""GPU parity: the HIP path against the CPU oracle and the reference's own fixtures.
+
+Integer/index/byte results must be bit-identical; float32 SSE, MSE and pixels are
+compared bit for bit as well (the kernels reproduce NumPy's operation order).  The only
+tolerance is for the Gaussian kinds 2/3, whose exp is NumPy's SIMD exp on the CPU and
+the device expf here: 1e-5 relative on MSE, 1e-6 absolute on pixels.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import bits_equal, golden_prims, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def test_draw_codes_match_oracle(gpu, oracle_lib):
+    from weft_amd.fit import draw_codes
+
+    ids = np.array([0, 5, 17, 123456], np.int32)
+    ts = np.array([8, 16, 32, 64], np.int32)
+    _say("draw_codes: 4 tiles x 300 candidates, seed 99 step 7")
+    codes, prims = draw_codes(ids, ts, 300, seed=99, step=7)
+    for j in range(4):
+        oc = oracle_lib.draw_codes(99, 7, int(ids[j]), 300, int(ts[j]))
+        assert bits_equal(codes[j * 300:(j + 1) * 300], oc.view(codes.dtype))
+        op = oracle_lib.codes_to_prims(oc)
+        assert bits_equal(prims[j * 300:(j + 1) * 300].view(np.uint8), op.view(np.uint8))
+
+
+def _has_gauss(P):
+    return bool(np.isin(P["kind"], (2, 3)).any())
+
+
+def test_score_evals_matches_reference_render(gpu, oracle_lib):
+    """weft_score_evals over the reference's render_tile fixtures (kinds 0-4, 8-64 px)."""
+    from weft_amd.gpu_render import gpu_batch_score_prepacked
+
+    d = load_golden("ref_render_tile.npz")
+    rng = np.random.default_rng(5)
+    n = len(d["tile_size"])
+    tiles, offs, arrays, counts, sizes, refs, gauss = [], {}, [], [], [], [], []
+    run = 0
+    for i in range(n):
+        if d["has_residual"][i]:
+            continue
+        ts = int(d["tile_size"][i])
+        src = rng.random((ts, ts, 3), dtype=np.float32)
+        pred = d["out"][d["out_off"][i]:d["out_off"][i] + ts * ts * 3].reshape(ts, ts, 3)
+        refs.append(np.mean((src - pred) ** 2))  # the reference's dist, encoder.py:282
+        P = golden_prims(d, i)
+        gauss.append(_has_gauss(P))
+        offs[len(sizes)] = run
+        tiles.append(src.ravel())
+        run += src.size
+        arrays.append(P)
+        counts.append(np.array([len(P)], np.int32))
+        sizes.append(ts)
+    _say(f"score_evals: {len(sizes)} stacks from the reference render fixtures")
+    out = gpu_batch_score_prepacked(np.concatenate(tiles), offs, list(range(len(sizes))), sizes,
+                                    arrays, counts)
+    assert out is not None
+    for j, (o, r, g) in enumerate(zip(out, refs, gauss)):
+        if g:
+            assert abs(float(o[0]) - float(r)) <= 1e-5 * abs(float(r)) + 1e-12, j
+        else:
+            assert o[0].tobytes() == np.float32(r).tobytes(), (j, float(o[0]), float(r))
+
+
+def test_drop_in_objective_matches_reference(gpu):
+    """gpu_batch_objectives == float(np.mean((src - render_tile(stack))**2)) of the reference."""
+    from weft_amd.gpu_render import gpu_batch_objectives
+    from weft_amd.primitives import Primitive
+
+    d = load_golden("ref_tile_objective.npz")
+    for i in range(len(d["tile_size"])):
+        ts = int(d["tile_size"][i])
+        src = d["src"][d["src_off"][i]:d["src_off"][i] + ts * ts * 3].reshape(ts, ts, 3)
+        P = golden_prims(d, i)
+        prims = []
+        o = int(d["prim_off"][i])
+        for k in range(len(P)):
+            kind = int(P["kind"][k])
+            ng = {0: 0, 1: 4, 4: 6}[kind]
+            c1 = tuple(float(v) for v in d["color1"][o + k]) if d["has_color1"][o + k] else None
+            prims.append(Primitive(kind, tuple(float(v) for v in P["geom"][k][:ng]),
+                                   tuple(float(v) for v in P["color0"][k]), c1, float(P["alpha"][k])))
+        mse = gpu_batch_objectives(src, [prims])
+        assert mse is not None
+        assert float(mse[0]) == float(d["dist"][i]), i
+
+
+def test_drop_in_greedy_matches_reference(gpu):
+    """The reference's _greedy_add driven by the GPU scorer picks the same stacks."""
+    from weft_amd.gpu_render import gpu_batch_objectives
+    from weft_amd.primitives import Primitive
+    from weft_amd.rd import estimate_bits
+
+    g = load_golden("ref_greedy_add.npz")
+    for i in range(len(g["tile_size"])):
+        ts = int(g["tile_size"][i])
+        src = g["src"][g["src_off"][i]:g["src_off"][i] + ts * ts * 3].reshape(ts, ts, 3)
+        P = golden_prims(g, i)
+        o = int(g["prim_off"][i])
+        pool = []
+        for k in range(len(P)):
+            kind = int(P["kind"][k])
+            ng = {0: 0, 1: 4, 4: 6}[kind]
+            c1 = tuple(float(v) for v in g["color1"][o + k]) if g["has_color1"][o + k] else None
+            pool.append(Primitive(kind, tuple(float(v) for v in P["geom"][k][:ng]),
+                                  tuple(float(v) for v in P["color0"][k]), c1, float(P["alpha"][k])))
+        lam = float(g["lam"][i])
+        maxp = int(g["max_prims"][i])
+        order = list(range(len(pool)))
+        selected: list[int] = []
+        best = float(gpu_batch_objectives(src, [[]])[0]) + lam * estimate_bits([])
+        while order and len(selected) < maxp:
+            sets = [[pool[j] for j in selected] + [pool[c]] for c in order]
+            mses = gpu_batch_objectives(src, sets)
+            objs = [float(mses[k]) + lam * estimate_bits([pool[j].kind for j in selected] + [pool[c].kind])
+                    for k, c in enumerate(order)]
+            mi = int(np.argmin(np.asarray(objs, np.float64)))
+            if objs[mi] >= best:
+                break
+            selected.append(order.pop(mi))
+            best = objs[mi]
+        ref = g["sel_idx"][i][: g["sel_cnt"][i]].tolist()
+        assert selected == ref, (i, selected, ref)
+        assert best == float(g["best_obj"][i])
+
+
+def _fit_pair(oracle_lib, tiles, n_cand, depth, steps, seed, lam, blocks=None, keep_sse=True):
+    from oracle.coracle import PlainFit
+    from weft_amd.fit import PlainFitter
+
+    f = PlainFitter(tiles, n_cand=n_cand, depth=depth, seed=seed, lam=lam, max_steps=steps,
+                    blocks=blocks, keep_sse=keep_sse)
+    o = PlainFit(tiles.target, tiles.pix_off, tiles.ts, n_cand, depth, seed, lam)
+    sses = []
+    for s in range(steps):
+        f.step()
+        if keep_sse:
+            sses.append(f.sse_table().copy())
+    ro = o.run(steps, want_sse=keep_sse)
+    return f, o, sses, ro
+
+
+def test_fit_c1_bit_exact(gpu, oracle_lib):
+    """Config 1: 256x256 image, 64 tiles of 32 px, 256 candidates, depth 8, 8 steps."""
+    from weft_amd.primitives import encode_code_tiles
+    from weft_amd.quadtree import TileSet, uniform_tiles
+    from weft_amd.rd import tile_lambda
+    from weft_amd.synth import CONFIG_C1, synth_image
+
+    c = CONFIG_C1
+    img = synth_image(c["height"], c["width"], c["n_prims"], seed=c["seed"])
+    tiles = TileSet.from_images([img], [uniform_tiles(c["height"], c["width"], c["tile"])], pad_multiple=32)
+    _say("fit C1: 64 tiles x 256 candidates x 8 steps, GPU vs oracle")
+    f, o, sses, ro = _fit_pair(oracle_lib, tiles, c["n_cand"], c["depth"], 8, 0, tile_lambda(75))
+    rec = f.records()
+    for s in range(8):
+        assert bits_equal(sses[s], ro["sse"][s]), f"SSE table differs at step {s}"
+    assert np.array_equal(rec["win_idx"], ro["win_idx"])
+    assert np.array_equal(rec["win_acc"], ro["win_acc"])
+    assert bits_equal(rec["win_obj"], ro["win_obj"])
+    acc = ro["win_acc"].astype(bool)
+    assert bits_equal(rec["win_code"][acc], ro["win_code"][acc].view(rec["win_code"].dtype))
+    assert bits_equal(f.canvas(), o.canvas)
+    st = f.state()
+    assert np.array_equal(st["nsel"], o.n_sel) and np.array_equal(st["bits"], o.sel_bits)
+    assert bits_equal(st["obj"], o.cur_obj)
+    stacks_g = [[rec["win_code"][s, t] for s in range(8) if rec["win_acc"][s, t]] for t in range(f.n_tiles)]
+    stacks_o = [[ro["win_code"][s, t] for s in range(8) if ro["win_acc"][s, t]] for t in range(f.n_tiles)]
+    assert encode_code_tiles(stacks_g) == encode_code_tiles(stacks_o)
+    assert acc.sum() > 64  # the fit actually accepts primitives
+
+
+def test_fit_mixed_tile_sizes_bit_exact(gpu, oracle_lib):
+    """Quadtree tiles of 8-64 px (64-px tiles use 100 KiB of LDS), odd candidate count."""
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.synth import synth_image
+
+    img = synth_image(200, 136, 30, seed=3)
+    qt = decompose_quadtree_variance(img, threshold=1e-3, min_tile=8, max_tile=64)
+    tiles = TileSet.from_images([img], [qt])
+    sizes = set(tiles.ts.tolist())
+    _say(f"fit mixed: {tiles.n_tiles} tiles, sizes {sorted(sizes)}, 200 candidates, 3 steps")
+    assert {8, 64} <= sizes or len(sizes) >= 3
+    f, o, sses, ro = _fit_pair(oracle_lib, tiles, 200, 4, 3, 11, 1e-5,
+                               blocks={8: 128, 16: 64, 32: 192, 64: 256})
+    rec = f.records()
+    for s in range(3):
+        assert bits_equal(sses[s], ro["sse"][s]), f"SSE table differs at step {s}"
+    assert np.array_equal(rec["win_idx"], ro["win_idx"])
+    assert bits_equal(f.canvas(), o.canvas)
+
+
+def test_fit_explicit_candidates_all_kinds(gpu, oracle_lib):
+    """Explicit candidate records of every kind over non-trivial canvases."""
+    from weft_amd.fit import PlainFitter
+    from weft_amd.quadtree import TileSet, uniform_tiles
+    from oracle.coracle import PRIM_DTYPE
+
+    rng = np.random.default_rng(21)
+    for ts in (8, 16, 32, 64):
+        img = rng.random((ts * 2, ts * 2, 3), dtype=np.float32)
+        tiles = TileSet.from_images([img], [uniform_tiles(ts * 2, ts * 2, ts)], pad_multiple=ts)
+        T, P = tiles.n_tiles, 150
+        cands = np.zeros(T * P, PRIM_DTYPE)
+        kinds = rng.choice([0, 1, 2, 3, 4, 6, 7, 8, 9], T * P)
+        cands["kind"] = kinds
+        cands["geom"][:, :6] = rng.uniform(-0.25 * ts, 1.25 * ts, (T * P, 6)).astype(np.float32)
+        er = np.isin(kinds, (6, 7))
+        cands["geom"][er, 2:4] = rng.uniform(1.0, ts, (er.sum(), 2)).astype(np.float32)
+        ang = rng.uniform(0, np.pi, er.sum())
+        cands["geom"][er, 4] = np.cos(ang).astype(np.float32)
+        cands["geom"][er, 5] = np.sin(ang).astype(np.float32)
+        g2 = kinds == 2
+        cands["geom"][g2, 4] = rng.uniform(0, 4, g2.sum()).astype(np.float32)
+        g3 = kinds == 3
+        cands["geom"][g3, 6] = rng.uniform(0, 4, g3.sum()).astype(np.float32)
+        deg = (kinds == 4) & (rng.random(T * P) < 0.1)
+        cands["geom"][deg, 2:4] = cands["geom"][deg, 0:2]
+        cands["color0"] = rng.random((T * P, 3)).astype(np.float32)
+        cands["color1"] = rng.random((T * P, 3)).astype(np.float32)
+        cands["alpha"] = rng.uniform(-0.2, 1.2, T * P).astype(np.float32)
+        canvas = rng.uniform(-0.05, 1.05, tiles.target.shape).astype(np.float32)
+        _say(f"explicit candidates: ts={ts}, {T} tiles x {P}, kinds 0-4,6-9")
+        f = PlainFitter(tiles, n_cand=P, depth=4, seed=0, lam=0.0, cand_prims=cands, keep_sse=True,
+                        blocks={ts: 64 if ts < 64 else 128})
+        f.set_canvas(canvas)
+        f.score(step=0)
+        sse = f.sse_table()
+        for t in range(T):
+            s0 = int(tiles.pix_off[t]) * 3
+            src = tiles.target[s0:s0 + ts * ts * 3].reshape(ts, ts, 3)
+            cv = canvas[s0:s0 + ts * ts * 3].reshape(ts, ts, 3)
+            osse, _ = oracle_lib.score_candidates(src, cv, cands[t * P:(t + 1) * P])
+            gauss = np.isin(kinds[t * P:(t + 1) * P], (2, 3))
+            assert bits_equal(sse[t][~gauss], osse[~gauss]), (ts, t)
+            np.testing.assert_allclose(sse[t][gauss], osse[gauss], rtol=1e-5, atol=1e-9)
+
+
+def test_render_tiles_matches_oracle(gpu, oracle_lib):
+    """Decode walk: scale-independent grid, RES0 bias, clip (render_tile semantics)."""
+    from weft_amd.gpu_render import gpu_render_tiles_to_image
+    from weft_amd.records import TILE_DECODE_DTYPE
+    from oracle.coracle import PRIM_DTYPE
+
+    rng = np.random.default_rng(8)
+    d = load_golden("ref_render_tile.npz")
+    H, W = 160, 200
+    descs, prims, expect = [], [], []
+    y = x = 0
+    for i in range(len(d["tile_size"])):
+        P = golden_prims(d, i)
+        if _has_gauss(P):
+            continue
+        src = int(d["tile_size"][i])
+        out = int(rng.choice([src // 2, src, src + 8, 2 * src])) if src <= 32 else src
+        if x + out > W:
+            x, y = 0, y + 64
+        if y + out > H:
+            break
+        res = rng.integers(-40, 41, 3)
+        desc = np.zeros(1, TILE_DECODE_DTYPE)[0]
+        desc["prim_offset"], desc["prim_count"] = sum(len(p) for p in prims), len(P)
+        desc["target_x0"], desc["target_y0"], desc["target_w"], desc["target_h"] = x, y, out, out
+        desc["source_size"] = src
+        bias = res.astype(np.float32) / np.float32(255.0)
+        desc["res0_r"], desc["res0_g"], desc["res0_b"] = bias
+        descs.append(desc)
+        prims.append(P)
+        expect.append((x, y, out, oracle_lib.render_tile(P, out, source_size=src, residual=res)))
+        x += out
+    _say(f"render_tiles: {len(descs)} tiles into {H}x{W}")
+    img = gpu_render_tiles_to_image(np.array(descs, TILE_DECODE_DTYPE),
+                                    np.concatenate(prims).astype(PRIM_DTYPE), H, W)
+    assert img is not None
+    for (x, y, out, ref) in expect:
+        assert bits_equal(img[y:y + out, x:x + out], ref), (x, y, out)

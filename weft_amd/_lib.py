@@ -1,0 +1,162 @@
+"""ctypes binding of libweft_hip.so (include/weft_hip.h).
+
+The library is built in-tree by ``python -m weft_amd.build``.  PyTorch is imported
+before the library is loaded so that the process holds exactly one HIP runtime
+(torch's); tensors are used only as device buffers and their ``data_ptr()`` values go
+straight into the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
+ABI_VERSION = 1
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_int32 = ctypes.c_int32
+c_uint32 = ctypes.c_uint32
+c_uint64 = ctypes.c_uint64
+c_double = ctypes.c_double
+
+
+class WeftHipError(RuntimeError):
+    """A HIP launch or argument error reported by libweft_hip.so."""
+
+
+class WeftHipUnavailable(RuntimeError):
+    """The native library is missing or no GPU is visible."""
+
+
+class FitState(ctypes.Structure):
+    _fields_ = [
+        ("d_target", c_void_p), ("d_canvas", c_void_p), ("d_tile_pix", c_void_p),
+        ("d_tile_ts", c_void_p), ("d_tile_nsel", c_void_p), ("d_tile_bits", c_void_p),
+        ("d_tile_obj", c_void_p), ("n_tiles", c_int32), ("depth", c_int32), ("lam", c_double),
+        ("obj_mode", c_int32), ("_pad", c_int32),
+    ]
+
+
+class Work(ctypes.Structure):
+    _fields_ = [
+        ("d_work_tile", c_void_p), ("d_work_cand0", c_void_p), ("n_work", c_int32),
+        ("tile_size", c_int32), ("block", c_int32), ("_pad", c_int32),
+    ]
+
+
+class Cands(ctypes.Structure):
+    _fields_ = [
+        ("d_prims", c_void_p), ("d_tile_cand", c_void_p), ("seed", c_uint64), ("step", c_uint32),
+        ("n_cand", c_int32), ("has_general", c_int32), ("_pad", c_int32),
+    ]
+
+
+assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 32 and ctypes.sizeof(Cands) == 40
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+_device: int | None = None
+
+
+def _bind(L: ctypes.CDLL) -> None:
+    L.weft_abi_version.restype = c_int
+    L.weft_last_error.restype = ctypes.c_char_p
+    L.weft_init.argtypes = [c_int]
+    L.weft_shutdown.argtypes = []
+    L.weft_score_evals.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.weft_fit_init.argtypes = [ctypes.POINTER(FitState), c_void_p]
+    L.weft_fit_score.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Work), ctypes.POINTER(Cands),
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.weft_fit_select.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.weft_draw_codes.argtypes = [c_uint64, c_uint32, c_void_p, c_void_p, c_int, c_int, c_void_p,
+                                  c_void_p, c_void_p]
+    L.weft_get_tables.argtypes = [c_void_p, c_void_p, c_void_p]
+    L.weft_render_tiles.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
+    for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_score",
+                 "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles"):
+        getattr(L, name).restype = c_int
+
+
+def library() -> ctypes.CDLL:
+    """Load (once) and return the native library; raises if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        import torch  # noqa: F401  -- one HIP runtime per process: torch's, loaded first
+
+        if not os.path.exists(LIB_PATH):
+            raise WeftHipUnavailable(
+                f"{LIB_PATH} is not built; run `python -m weft_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        _bind(L)
+        if L.weft_abi_version() != ABI_VERSION:
+            raise WeftHipUnavailable("libweft_hip.so ABI version mismatch; rebuild it")
+        _lib = L
+        return L
+
+
+def gpu_available() -> bool:
+    import torch
+
+    return bool(torch.cuda.is_available())
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = library().weft_last_error().decode(errors="replace")
+        raise WeftHipError(f"{what} failed ({rc}): {msg}")
+
+
+def ensure_init(device: int = 0) -> ctypes.CDLL:
+    """Load the library and initialise it on `device` (raises when no GPU is visible)."""
+    global _device
+    L = library()
+    if _device == device:
+        return L
+    if not gpu_available():
+        raise WeftHipUnavailable("no HIP device is visible")
+    import torch
+
+    torch.cuda.init()
+    with torch.cuda.device(device):
+        check(L.weft_init(device), "weft_init")
+    _device = device
+    return L
+
+
+def shutdown() -> None:
+    global _device
+    if _lib is not None:
+        check(_lib.weft_shutdown(), "weft_shutdown")
+    _device = None
+
+
+def stream_ptr(stream=None) -> int:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def tables() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """The candidate tables the kernels use (log-radius codes, cos, sin)."""
+    logr = np.zeros(4096, np.uint16)
+    cosv = np.zeros(4096, np.float32)
+    sinv = np.zeros(4096, np.float32)
+    check(library().weft_get_tables(logr.ctypes.data, cosv.ctypes.data, sinv.ctypes.data),
+          "weft_get_tables")
+    return logr, cosv, sinv
+
+
+EXPORTED_SYMBOLS = (
+    "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_score_evals",
+    "weft_fit_init", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
+    "weft_render_tiles",
+)

@@ -1,0 +1,69 @@
+// Kernel-argument blocks shared by the gfx950 code object (weft_kernels.hip) and the
+// host launcher (weft_host.cpp).  Each kernel takes one of these by value.
+#pragma once
+
+#include "weft_hip.h"
+
+#define WEFT_ABI_VERSION 1
+#define WEFT_MAX_BLOCK 1024
+#define WEFT_N_CLASSES 6
+#define WEFT_TABLE_N 4096
+
+struct WeftScoreEvalsArgs {
+    const WeftEval* evals;
+    const WeftPrim* prims;
+    const float* tiles;
+    float* out_mse;
+    float* out_sse;
+    int32_t n_evals;
+    int32_t _pad;
+};
+
+struct WeftFitInitArgs {
+    WeftFitState st;
+};
+
+struct WeftFitScoreArgs {
+    WeftFitState st;
+    WeftWork wk;
+    WeftCands cd;
+    float* out_sse;
+    float* out_mse;
+    double* part_obj;
+    int32_t* part_idx;
+};
+
+struct WeftFitSelectArgs {
+    WeftFitState st;
+    WeftCands cd;
+    const int32_t* tile_work0;
+    const int32_t* tile_nwork;
+    const double* part_obj;
+    const int32_t* part_idx;
+    int32_t* win_idx;
+    double* win_obj;
+    int32_t* win_acc;
+    WeftCode* win_code;
+};
+
+struct WeftDrawArgs {
+    const int32_t* tile_ids;
+    const int32_t* tile_ts;
+    WeftCode* codes;
+    WeftPrim* prims;
+    uint64_t seed;
+    uint32_t step;
+    int32_t n_tiles;
+    int32_t n_cand;
+    int32_t _pad;
+};
+
+struct WeftRenderArgs {
+    const WeftTileDecodeDesc* descs;
+    const WeftPrim* prims;
+    float* out;
+    int32_t n_tiles;
+    int32_t width;
+    int32_t height;
+    int32_t _pad;
+};

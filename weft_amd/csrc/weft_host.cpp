@@ -1,0 +1,266 @@
+// weft_host.cpp -- libweft_hip.so: the C ABI declared in include/weft_hip.h.
+//
+// Loads the gfx950 code object that the build embeds in this library
+// (weft_kernels.hsaco, linked in by weft_embed.S) with hipModuleLoadData and launches
+// its kernels with hipModuleLaunchKernel.  The reference compiled CUDA source with
+// NVRTC at run time and kept module handles in a Python dict
+// (src/weft/gpu_render.py:696-734, src/weft/gpu_context.py:16-62); here the code object
+// is built offline and the handles live in this file.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "weft_abi.h"
+#include "weft_hip.h"
+
+extern "C" const unsigned char weft_hsaco_start[];
+extern "C" const unsigned char weft_hsaco_end[];
+
+namespace {
+
+struct Module {
+    int device = -1;
+    hipModule_t mod = nullptr;
+    hipFunction_t f_evals = nullptr;
+    hipFunction_t f_init = nullptr;
+    hipFunction_t f_score[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipFunction_t f_score_gen[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipFunction_t f_select = nullptr;
+    hipFunction_t f_draw = nullptr;
+    hipFunction_t f_render = nullptr;
+};
+
+Module g_mod;
+std::mutex g_mu;
+uint16_t g_logr[WEFT_TABLE_N];
+float g_cos[WEFT_TABLE_N];
+float g_sin[WEFT_TABLE_N];
+bool g_tables = false;
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return -(int)e;
+}
+
+#define WEFT_HIP(call, what)                          \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+void build_tables() {
+    if (g_tables) return;
+    for (int k = 0; k < WEFT_TABLE_N; ++k) {
+        // log-uniform radius over [4, 64) px, stored as code = round(r * 1024)
+        double r = 4.0 * std::pow(16.0, ((double)k + 0.5) / (double)WEFT_TABLE_N);
+        g_logr[k] = (uint16_t)std::llround(r * 1024.0);
+        double th = (double)k * 3.14159265358979323846 / (double)WEFT_TABLE_N;
+        g_cos[k] = (float)std::cos(th);
+        g_sin[k] = (float)std::sin(th);
+    }
+    g_tables = true;
+}
+
+int upload_global(const char* name, const void* src, size_t bytes) {
+    hipDeviceptr_t dptr = nullptr;
+    size_t sz = 0;
+    WEFT_HIP(hipModuleGetGlobal(&dptr, &sz, g_mod.mod, name), name);
+    if (sz != bytes) return set_err(WEFT_E_MODULE, std::string("size mismatch for ") + name);
+    WEFT_HIP(hipMemcpyHtoD(dptr, const_cast<void*>(src), bytes), name);
+    return WEFT_OK;
+}
+
+int require_init() {
+    if (!g_mod.mod) return set_err(WEFT_E_NOT_INITIALISED, "weft_init has not been called");
+    return WEFT_OK;
+}
+
+int launch(hipFunction_t f, unsigned grid, unsigned block, unsigned shmem, void* stream, void* args) {
+    if (grid == 0) return WEFT_OK;
+    void* params[] = {args};
+    WEFT_HIP(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, shmem, (hipStream_t)stream, params,
+                                   nullptr),
+             "hipModuleLaunchKernel");
+    return WEFT_OK;
+}
+
+int ts_slot(int ts) {
+    switch (ts) {
+    case 8: return 0;
+    case 16: return 1;
+    case 32: return 2;
+    case 64: return 3;
+    default: return -1;
+    }
+}
+
+unsigned score_lds_bytes(int ts) {
+    // 3 float2 per pixel + sort permutation + per-wave class counts/offsets + reduction
+    return (unsigned)(24 * ts * ts + 4 * WEFT_MAX_BLOCK + 4 * 16 * 8 * 2 + 8 * 16 + 4 * 16);
+}
+
+}  // namespace
+
+extern "C" {
+
+int weft_abi_version(void) { return WEFT_ABI_VERSION; }
+
+const char* weft_last_error(void) { return g_err.c_str(); }
+
+int weft_init(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_mod.mod && g_mod.device == device) return WEFT_OK;
+    if (g_mod.mod) return set_err(WEFT_E_INVALID, "weft_init: already initialised on another device");
+    int n = 0;
+    WEFT_HIP(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) return set_err(WEFT_E_INVALID, "weft_init: no such device");
+    WEFT_HIP(hipSetDevice(device), "hipSetDevice");
+    Module m;
+    m.device = device;
+    WEFT_HIP(hipModuleLoadData(&m.mod, weft_hsaco_start), "hipModuleLoadData");
+    g_mod = m;
+    struct {
+        hipFunction_t* f;
+        const char* name;
+    } fns[] = {
+        {&g_mod.f_evals, "weft_score_evals_kernel"},
+        {&g_mod.f_init, "weft_fit_init_kernel"},
+        {&g_mod.f_score[0], "weft_fit_score_ts8"},
+        {&g_mod.f_score[1], "weft_fit_score_ts16"},
+        {&g_mod.f_score[2], "weft_fit_score_ts32"},
+        {&g_mod.f_score[3], "weft_fit_score_ts64"},
+        {&g_mod.f_score_gen[0], "weft_fit_score_gen_ts8"},
+        {&g_mod.f_score_gen[1], "weft_fit_score_gen_ts16"},
+        {&g_mod.f_score_gen[2], "weft_fit_score_gen_ts32"},
+        {&g_mod.f_score_gen[3], "weft_fit_score_gen_ts64"},
+        {&g_mod.f_select, "weft_fit_select_kernel"},
+        {&g_mod.f_draw, "weft_draw_kernel"},
+        {&g_mod.f_render, "weft_render_tiles_kernel"},
+    };
+    for (auto& fn : fns) {
+        hipError_t e = hipModuleGetFunction(fn.f, g_mod.mod, fn.name);
+        if (e != hipSuccess) {
+            (void)hipModuleUnload(g_mod.mod);
+            g_mod = Module();
+            return hip_fail(e, fn.name);
+        }
+    }
+    build_tables();
+    int rc = upload_global("weft_tab_logr", g_logr, sizeof(g_logr));
+    if (rc == WEFT_OK) rc = upload_global("weft_tab_cos", g_cos, sizeof(g_cos));
+    if (rc == WEFT_OK) rc = upload_global("weft_tab_sin", g_sin, sizeof(g_sin));
+    if (rc != WEFT_OK) {
+        (void)hipModuleUnload(g_mod.mod);
+        g_mod = Module();
+    }
+    return rc;
+}
+
+int weft_shutdown(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_mod.mod) {
+        hipError_t e = hipModuleUnload(g_mod.mod);
+        g_mod = Module();
+        if (e != hipSuccess) return hip_fail(e, "hipModuleUnload");
+    }
+    return WEFT_OK;
+}
+
+int weft_get_tables(uint16_t* logr, float* cosv, float* sinv) {
+    build_tables();
+    if (logr) std::memcpy(logr, g_logr, sizeof(g_logr));
+    if (cosv) std::memcpy(cosv, g_cos, sizeof(g_cos));
+    if (sinv) std::memcpy(sinv, g_sin, sizeof(g_sin));
+    return WEFT_OK;
+}
+
+int weft_score_evals(const WeftEval* d_evals, int n_evals, const WeftPrim* d_prims,
+                     const float* d_tiles, float* d_out_mse, float* d_out_sse, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n_evals < 0) return set_err(WEFT_E_INVALID, "weft_score_evals: n_evals < 0");
+    if (n_evals == 0) return WEFT_OK;
+    if (!d_evals || !d_prims || !d_tiles || !d_out_mse)
+        return set_err(WEFT_E_INVALID, "weft_score_evals: null pointer");
+    WeftScoreEvalsArgs a{d_evals, d_prims, d_tiles, d_out_mse, d_out_sse, n_evals, 0};
+    return launch(g_mod.f_evals, (unsigned)((n_evals + 255) / 256), 256, 0, stream, &a);
+}
+
+int weft_fit_init(const WeftFitState* st, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || st->n_tiles < 0) return set_err(WEFT_E_INVALID, "weft_fit_init: bad state");
+    WeftFitInitArgs a{*st};
+    return launch(g_mod.f_init, (unsigned)((st->n_tiles + 255) / 256), 256, 0, stream, &a);
+}
+
+int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands* cands,
+                   float* d_out_sse, float* d_out_mse, double* d_part_obj, int32_t* d_part_idx,
+                   void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !work || !cands || !d_part_obj || !d_part_idx)
+        return set_err(WEFT_E_INVALID, "weft_fit_score: null argument");
+    const int slot = ts_slot(work->tile_size);
+    if (slot < 0) return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_fit_score: tile size must be 8, 16, 32 or 64");
+    if (work->block < 64 || work->block > WEFT_MAX_BLOCK || (work->block % 64) != 0)
+        return set_err(WEFT_E_INVALID, "weft_fit_score: block must be a multiple of 64 in [64, 1024]");
+    if (cands->n_cand < 1) return set_err(WEFT_E_INVALID, "weft_fit_score: n_cand < 1");
+    if (work->n_work <= 0) return WEFT_OK;
+    WeftFitScoreArgs a{*st, *work, *cands, d_out_sse, d_out_mse, d_part_obj, d_part_idx};
+    hipFunction_t f = cands->has_general ? g_mod.f_score_gen[slot] : g_mod.f_score[slot];
+    return launch(f, (unsigned)work->n_work, (unsigned)work->block,
+                  score_lds_bytes(work->tile_size), stream, &a);
+}
+
+int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_t* d_tile_work0,
+                    const int32_t* d_tile_nwork, const double* d_part_obj,
+                    const int32_t* d_part_idx, int32_t* d_win_idx, double* d_win_obj,
+                    int32_t* d_win_acc, WeftCode* d_win_code, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !cands || !d_tile_work0 || !d_tile_nwork || !d_part_obj || !d_part_idx ||
+        !d_win_idx || !d_win_obj || !d_win_acc)
+        return set_err(WEFT_E_INVALID, "weft_fit_select: null argument");
+    WeftFitSelectArgs a{*st, *cands, d_tile_work0, d_tile_nwork, d_part_obj, d_part_idx,
+                        d_win_idx, d_win_obj, d_win_acc, d_win_code};
+    return launch(g_mod.f_select, (unsigned)st->n_tiles, 256, 0, stream, &a);
+}
+
+int weft_draw_codes(uint64_t seed, uint32_t step, const int32_t* d_tile_ids,
+                    const int32_t* d_tile_ts, int n_tiles, int n_cand, WeftCode* d_codes,
+                    WeftPrim* d_prims, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n_tiles < 0 || n_cand < 0 || !d_tile_ids || !d_tile_ts)
+        return set_err(WEFT_E_INVALID, "weft_draw_codes: bad arguments");
+    const long long total = (long long)n_tiles * n_cand;
+    if (total == 0) return WEFT_OK;
+    if (total > (1ll << 31) * 255ll) return set_err(WEFT_E_TOO_LARGE, "weft_draw_codes: too many");
+    WeftDrawArgs a{d_tile_ids, d_tile_ts, d_codes, d_prims, seed, step, n_tiles, n_cand, 0};
+    return launch(g_mod.f_draw, (unsigned)((total + 255) / 256), 256, 0, stream, &a);
+}
+
+int weft_render_tiles(const WeftTileDecodeDesc* d_descs, int n_tiles, const WeftPrim* d_prims,
+                      float* d_out, int width, int height, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n_tiles < 0 || width < 0 || height < 0) return set_err(WEFT_E_INVALID, "weft_render_tiles: bad sizes");
+    if (n_tiles == 0) return WEFT_OK;
+    if (!d_descs || !d_prims || !d_out) return set_err(WEFT_E_INVALID, "weft_render_tiles: null pointer");
+    WeftRenderArgs a{d_descs, d_prims, d_out, n_tiles, width, height, 0};
+    return launch(g_mod.f_render, (unsigned)n_tiles, 256, 0, stream, &a);
+}
+
+}  // extern "C"

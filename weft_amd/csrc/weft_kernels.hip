@@ -1,0 +1,887 @@
+// weft_kernels.hip -- gfx950 kernels for the Weft tile-fit scoring path and the
+// primitive-walk renderer.  Built offline into a code object (hipcc --genco) that
+// libweft_hip.so embeds and loads with hipModuleLoadData.
+//
+// Exactness contract (reference = jarmak-personal/weft, src/weft/render.py:127-234,
+// src/weft/encoder.py:271-285): every float32 operation is the one NumPy performs for
+// render_tile / np.mean, in the same order, with no fused multiply-add (the file is
+// compiled with -ffp-contract=off) and IEEE division.  Per-candidate SSE is summed in
+// NumPy's pairwise order: 8 interleaved accumulators over leaf blocks of 96 elements
+// (128 for 64-px tiles), a balanced binary tree over the leaves, and 8192-element
+// chunks added left to right.  Kinds 2/3 use the device expf, which is not NumPy's
+// SIMD exp: those two kinds agree to ~1e-7, every other kind is bit-exact.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "weft_abi.h"
+
+// Candidate tables, filled by weft_init through hipModuleGetGlobal.
+__device__ uint16_t weft_tab_logr[WEFT_TABLE_N];
+__device__ float weft_tab_cos[WEFT_TABLE_N];
+__device__ float weft_tab_sin[WEFT_TABLE_N];
+
+namespace {
+
+enum : int { CL_CONST = 0, CL_ELL = 1, CL_RECT = 2, CL_TRI = 3, CL_GEN = 4, CL_NONE = 5 };
+
+__device__ __forceinline__ float clip01(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f); }
+
+__device__ __forceinline__ int prim_bytes(int kind) {
+    // encoder.py:242-256 (unknown kinds cost 20 bytes) + 17-byte records for 6/7/8
+    switch (kind) {
+    case 0: return 7;
+    case 1: return 18;
+    case 2: return 17;
+    case 3: return 21;
+    case 4: return 19;
+    case 6: case 7: case 8: return 17;
+    default: return 20;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Generic per-pixel composite of one primitive (all kinds), render.py:184-229.
+// rgb holds the running unclipped canvas; (px, py) the sample position.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float tri_alpha(const float* g, float px, float py) {
+    double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
+    double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
+    if (fabs(den) < 1e-6) return 0.0f;
+    float A = (float)(y1 - y2), B = (float)(x2 - x1);
+    float C = (float)(y2 - y0), E = (float)(x0 - x2);
+    float D = (float)den;
+    float dx = px - (float)x2;
+    float dy = py - (float)y2;
+    float na = A * dx + B * dy;
+    float a = na / D;
+    float nb = C * dx + E * dy;
+    float b = nb / D;
+    float c = (1.0f - a) - b;
+    return (a >= 0.0f && b >= 0.0f && c >= 0.0f) ? 1.0f : 0.0f;
+}
+
+__device__ __forceinline__ float ell_rect_alpha(int kind, const float* g, float px, float py) {
+    float rx = g[2], ry = g[3];
+    if (!(rx > 0.0f) || !(ry > 0.0f)) return 0.0f;
+    float ca = g[4] / rx, sa = g[5] / rx, cb = g[4] / ry, sb = g[5] / ry;
+    float dx = px - g[0];
+    float dy = py - g[1];
+    float p = ca * dx + sa * dy;
+    float q = cb * dy - sb * dx;
+    if (kind == 6) return (p * p + q * q) <= 1.0f ? 1.0f : 0.0f;
+    return (fabsf(p) <= 1.0f && fabsf(q) <= 1.0f) ? 1.0f : 0.0f;
+}
+
+__device__ __forceinline__ float seg_dist_f32(float px, float py, float x0, float y0, float x1,
+                                              float y1) {
+    float vx = x1 - x0, vy = y1 - y0;
+    float den = (vx * vx + vy * vy) + 1e-8f;
+    float t = ((px - x0) * vx + (py - y0) * vy) / den;
+    t = clip01(t);
+    float cx = x0 + t * vx;
+    float cy = y0 + t * vy;
+    float dx = px - cx, dy = py - cy;
+    return sqrtf(dx * dx + dy * dy);
+}
+
+__device__ __forceinline__ float gauss_alpha(float dist, float thickness, float a) {
+    double sigma = (double)thickness > 0.5 ? (double)thickness : 0.5;
+    double inv_2s2 = 1.0 / (2.0 * sigma * sigma);
+    float arg = (-(dist * dist)) * (float)inv_2s2;
+    return expf(arg) * a;
+}
+
+__device__ void composite_px(const WeftPrim& P, float px, float py, float& r, float& g, float& b) {
+    const float a = clip01(P.alpha);
+    float s0 = P.color0[0], s1 = P.color0[1], s2 = P.color0[2];
+    float al;
+    switch (P.kind) {
+    case 0:
+        al = a;
+        break;
+    case 1: {
+        double x0 = P.geom[0], y0 = P.geom[1], x1 = P.geom[2], y1 = P.geom[3];
+        double vx = x1 - x0, vy = y1 - y0;
+        double den = vx * vx + vy * vy + 1e-8;
+        float VX = (float)vx, VY = (float)vy, DN = (float)den;
+        float t = clip01(((px - (float)x0) * VX + (py - (float)y0) * VY) / DN);
+        float omt = 1.0f - t;
+        s0 = P.color0[0] * omt + P.color1[0] * t;
+        s1 = P.color0[1] * omt + P.color1[1] * t;
+        s2 = P.color0[2] * omt + P.color1[2] * t;
+        al = a;
+        break;
+    }
+    case 2: {
+        double x0 = P.geom[0], y0 = P.geom[1], x1 = P.geom[2], y1 = P.geom[3];
+        double vx = x1 - x0, vy = y1 - y0;
+        double den = vx * vx + vy * vy + 1e-8;
+        float VX = (float)vx, VY = (float)vy, DN = (float)den;
+        float X0 = (float)x0, Y0 = (float)y0;
+        float t = clip01(((px - X0) * VX + (py - Y0) * VY) / DN);
+        float cx = X0 + t * VX, cy = Y0 + t * VY;
+        float dx = px - cx, dy = py - cy;
+        al = gauss_alpha(sqrtf(dx * dx + dy * dy), P.geom[4], a);
+        break;
+    }
+    case 3: {
+        // _eval_curve_distance: 32 float32 points (np.linspace in float64, cast), 31 segments
+        float best = 1e9f;
+        float prevx = 0.0f, prevy = 0.0f;
+        for (int i = 0; i < 32; ++i) {
+            float t = (i == 31) ? 1.0f : (float)((double)i * (1.0 / 31.0));
+            float u = 1.0f - t;
+            float uu = u * u, tu2 = (2.0f * u) * t, tt = t * t;
+            float x = (uu * P.geom[0] + tu2 * P.geom[2]) + tt * P.geom[4];
+            float y = (uu * P.geom[1] + tu2 * P.geom[3]) + tt * P.geom[5];
+            if (i > 0) best = fminf(best, seg_dist_f32(px, py, prevx, prevy, x, y));
+            prevx = x;
+            prevy = y;
+        }
+        al = gauss_alpha(best, P.geom[6], a);
+        break;
+    }
+    case 4:
+    case 8:
+        al = tri_alpha(P.geom, px, py) * a;
+        break;
+    case 6:
+    case 7:
+        al = ell_rect_alpha(P.kind, P.geom, px, py) * a;
+        break;
+    default:
+        return;  // render.py:225-226 skips unknown kinds
+    }
+    const float oma = 1.0f - al;
+    r = s0 * al + r * oma;
+    g = s1 * al + g * oma;
+    b = s2 * al + b * oma;
+}
+
+// Out-of-line copy for the per-pixel loops of the generic paths: keeps the unrolled
+// pixel bodies small and the fast classes' register allocation unaffected.
+__device__ __noinline__ void composite_px_call(const WeftPrim& P, float px, float py, float& r,
+                                               float& g, float& b) {
+    composite_px(P, px, py, r, g, b);
+}
+
+// ---------------------------------------------------------------------------
+// NumPy pairwise float32 summation, streamed in element order.
+// ---------------------------------------------------------------------------
+// Leaf size of NumPy's pairwise recursion for n = 3*TS*TS elements (halving at
+// multiples of 8 until <= 128): 96 for 8/16/32-px tiles, 128 for 64-px tiles.
+template <int TS> struct PwCfg { static constexpr int BLOCK = (TS == 64) ? 128 : 96; };
+
+struct PwState {
+    float r[8];
+    float s[8];
+
+    // Push a finished leaf (block j, 0-based) into the binary counter.  j is
+    // wave-uniform, so the branches are scalar.
+    __device__ __forceinline__ void push(float v, int j) {
+        if (j & 1) {
+            v = s[0] + v;
+            if (j & 2) {
+                v = s[1] + v;
+                if (j & 4) {
+                    v = s[2] + v;
+                    if (j & 8) {
+                        v = s[3] + v;
+                        if (j & 16) {
+                            v = s[4] + v;
+                            if (j & 32) {
+                                v = s[5] + v;
+                                if (j & 64) { v = s[6] + v; s[7] = v; }
+                                else s[6] = v;
+                            } else s[5] = v;
+                        } else s[4] = v;
+                    } else s[3] = v;
+                } else s[2] = v;
+            } else s[1] = v;
+        } else {
+            s[0] = v;
+        }
+    }
+
+    // Element `off` (0..23, compile-time) of the 24-element group starting at element
+    // `base` (runtime, wave-uniform, a multiple of 24).  Accumulators start each leaf at
+    // +0, and 0 + e == e exactly, so this equals NumPy's "first eight initialise".
+    template <int BLOCK>
+    __device__ __forceinline__ void feed(int off, float e, int base) {
+        r[off & 7] = r[off & 7] + e;
+        if ((off & 7) == 7) {
+            const int end = base + off + 1;
+            if (end % BLOCK == 0) {
+                const float v = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+                push(v, end / BLOCK - 1);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) r[j] = 0.0f;
+            }
+        }
+    }
+
+    template <int TS>
+    __device__ __forceinline__ float result() const {
+        if constexpr (TS == 8) return s[1];
+        else if constexpr (TS == 16) return s[3];
+        else if constexpr (TS == 32) return s[5];
+        else return s[6] + s[5];  // 8192-element chunk, then the 4096-element rest
+    }
+};
+
+// Pairwise SSE of one tile, pixels in row-major order, 8 pixels (24 elements) per
+// unrolled group.  Px provides row(y) and pixel(y, x, e0, e1, e2) (the three squared
+// errors of pixel (y, x)).
+template <int TS, class Px>
+__device__ __forceinline__ float pairwise_tile(Px& px) {
+    constexpr int BLOCK = PwCfg<TS>::BLOCK;
+    PwState st;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) st.r[j] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) st.s[j] = 0.0f;
+    int base = 0;
+#pragma unroll 1
+    for (int y = 0; y < TS; ++y) {
+        px.row(y);
+#pragma unroll 1
+        for (int gx = 0; gx < TS; gx += 8) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                float e0, e1, e2;
+                px.pixel(y, gx + i, e0, e1, e2);
+                st.template feed<BLOCK>(3 * i + 0, e0, base);
+                st.template feed<BLOCK>(3 * i + 1, e1, base);
+                st.template feed<BLOCK>(3 * i + 2, e2, base);
+            }
+            base += 24;
+        }
+    }
+    return st.template result<TS>();
+}
+
+// ---------------------------------------------------------------------------
+// Pixel sources.
+// ---------------------------------------------------------------------------
+
+// Reference-semantics stack walk over a zero canvas; target read from global memory.
+struct PxStack {
+    const float* tgt;
+    const WeftPrim* prims;
+    int n;
+    int ts;
+    float py;
+    __device__ __forceinline__ void row(int y) { py = (float)y + 0.5f; }
+    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+        float r = 0.0f, g = 0.0f, b = 0.0f;
+        const float px = (float)x + 0.5f;
+        for (int i = 0; i < n; ++i) composite_px_call(prims[i], px, py, r, g, b);
+        const float* t = tgt + 3 * (y * ts + x);
+        float d0 = t[0] - clip01(r), d1 = t[1] - clip01(g), d2 = t[2] - clip01(b);
+        e0 = d0 * d0;
+        e1 = d1 * d1;
+        e2 = d2 * d2;
+    }
+};
+
+// Current canvas against the target, both in global memory (tile objective).
+struct PxCanvasGlobal {
+    const float* tgt;
+    const float* cv;
+    int ts;
+    __device__ __forceinline__ void row(int) {}
+    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+        const int o = 3 * (y * ts + x);
+        float d0 = tgt[o] - clip01(cv[o]);
+        float d1 = tgt[o + 1] - clip01(cv[o + 1]);
+        float d2 = tgt[o + 2] - clip01(cv[o + 2]);
+        e0 = d0 * d0;
+        e1 = d1 * d1;
+        e2 = d2 * d2;
+    }
+};
+
+// LDS tile: per pixel 3 float2 = (c_r, c_g), (c_b, t_r), (t_g, t_b).
+template <int TS>
+struct PxLdsBase {
+    const float2* L;
+    float ks0, ks1, ks2, om;
+    __device__ __forceinline__ void finish(int y, int x, bool in, float& e0, float& e1, float& e2) {
+        const int p = y * TS + x;
+        const float2 a = L[3 * p], b = L[3 * p + 1], c = L[3 * p + 2];
+        const float w0 = ks0 + a.x * om;
+        const float w1 = ks1 + a.y * om;
+        const float w2 = ks2 + b.x * om;
+        const float v0 = clip01(in ? w0 : a.x);
+        const float v1 = clip01(in ? w1 : a.y);
+        const float v2 = clip01(in ? w2 : b.x);
+        const float d0 = b.y - v0, d1 = c.x - v1, d2 = c.y - v2;
+        e0 = d0 * d0;
+        e1 = d1 * d1;
+        e2 = d2 * d2;
+    }
+};
+
+template <int TS>
+struct PxConst : PxLdsBase<TS> {
+    __device__ __forceinline__ void row(int) {}
+    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+        this->finish(y, x, true, e0, e1, e2);
+    }
+};
+
+// Ellipse / rectangle: p = ca*dx + sa*dy, q = cb*dy - sb*dx (dy terms per row).
+template <int TS, bool ELL>
+struct PxEllRect : PxLdsBase<TS> {
+    float cx, cy, ca, sa, cb, sb;
+    float p2, q1;
+    __device__ __forceinline__ void row(int y) {
+        const float dy = ((float)y + 0.5f) - cy;
+        p2 = sa * dy;
+        q1 = cb * dy;
+    }
+    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+        const float dx = ((float)x + 0.5f) - cx;
+        const float p = ca * dx + p2;
+        const float q = q1 - sb * dx;
+        bool in;
+        if constexpr (ELL) in = (p * p + q * q) <= 1.0f;
+        else in = (fabsf(p) <= 1.0f) && (fabsf(q) <= 1.0f);
+        this->finish(y, x, in, e0, e1, e2);
+    }
+};
+
+// Triangle (render.py:114-124): a = (A dx + B dy) / D, b = (C dx + E dy) / D.
+template <int TS>
+struct PxTri : PxLdsBase<TS> {
+    float X2, Y2, A, B, C, E, D;
+    float bdy, edy;
+    __device__ __forceinline__ void row(int y) {
+        const float dy = ((float)y + 0.5f) - Y2;
+        bdy = B * dy;
+        edy = E * dy;
+    }
+    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+        const float dx = ((float)x + 0.5f) - X2;
+        const float a = (A * dx + bdy) / D;
+        const float b = (C * dx + edy) / D;
+        const float c = (1.0f - a) - b;
+        const bool in = (a >= 0.0f) && (b >= 0.0f) && (c >= 0.0f);
+        this->finish(y, x, in, e0, e1, e2);
+    }
+};
+
+// Any kind over the LDS canvas through the generic composite.
+template <int TS>
+struct PxGeneral {
+    const float2* L;
+    WeftPrim P;
+    float py;
+    __device__ __forceinline__ void row(int y) { py = (float)y + 0.5f; }
+    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+        const int p = y * TS + x;
+        const float2 a = L[3 * p], b = L[3 * p + 1], c = L[3 * p + 2];
+        float r = a.x, g = a.y, bb = b.x;
+        composite_px(P, (float)x + 0.5f, py, r, g, bb);
+        const float d0 = b.y - clip01(r), d1 = c.x - clip01(g), d2 = c.y - clip01(bb);
+        e0 = d0 * d0;
+        e1 = d1 * d1;
+        e2 = d2 * d2;
+    }
+};
+
+template <int TS>
+__device__ __noinline__ float score_general(const float2* L, WeftPrim P) {
+    PxGeneral<TS> px;
+    px.L = L;
+    px.P = P;
+    return pairwise_tile<TS>(px);
+}
+
+// ---------------------------------------------------------------------------
+// Candidate codes, drawn from (seed, step, tile, index).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t smix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ uint64_t draw_base(uint64_t seed, uint32_t step, uint32_t tile) {
+    return smix(smix(smix(seed) ^ (uint64_t)step) ^ (uint64_t)tile);
+}
+
+__device__ __forceinline__ int draw_family(uint64_t base, uint32_t idx) {
+    const uint64_t h1 = smix(base ^ (2ull * idx));
+    return (int)(((h1 & 0xFFFFull) * 3ull) >> 16);
+}
+
+__device__ __forceinline__ WeftCode draw_code(uint64_t base, uint32_t idx, int ts) {
+    const uint64_t h1 = smix(base ^ (2ull * idx));
+    const uint64_t h2 = smix(base ^ (2ull * idx + 1ull));
+    const uint64_t span = (uint64_t)ts * 1024ull;
+    WeftCode c;
+    c.family = (uint16_t)(((h1 & 0xFFFFull) * 3ull) >> 16);
+    c.cx = (uint16_t)((((h1 >> 16) & 0xFFFFull) * span) >> 16);
+    c.cy = (uint16_t)((((h1 >> 32) & 0xFFFFull) * span) >> 16);
+    c.rx = weft_tab_logr[(h1 >> 48) & 0xFFFull];
+    c.ry = weft_tab_logr[h2 & 0xFFFull];
+    c.theta = (uint16_t)((h2 >> 12) & 0xFFFull);
+    c.r = (uint16_t)((h2 >> 24) & 0xFFull);
+    c.g = (uint16_t)((h2 >> 32) & 0xFFull);
+    c.b = (uint16_t)((h2 >> 40) & 0xFFull);
+    c.alpha = (uint16_t)(19661u + ((((h2 >> 48) & 0xFFFFull) * (65536ull - 19661ull)) >> 16));
+    return c;
+}
+
+__device__ __forceinline__ WeftPrim code_to_prim(const WeftCode& c) {
+    WeftPrim P;
+    const float cx = (float)c.cx * (1.0f / 1024.0f);
+    const float cy = (float)c.cy * (1.0f / 1024.0f);
+    const float rx = (float)c.rx * (1.0f / 1024.0f);
+    const float ry = (float)c.ry * (1.0f / 1024.0f);
+    const float ct = weft_tab_cos[c.theta & 0xFFF], st = weft_tab_sin[c.theta & 0xFFF];
+    P.color0[0] = (float)((double)c.r / 255.0);
+    P.color0[1] = (float)((double)c.g / 255.0);
+    P.color0[2] = (float)((double)c.b / 255.0);
+    P.color1[0] = P.color1[1] = P.color1[2] = 0.0f;
+    P.alpha = (float)((double)c.alpha / 65535.0);
+    P.geom[6] = 0.0f;
+    if (c.family == 2) {
+        const float ux[3] = {0.0f, 0.8660254037844386f, -0.8660254037844386f};
+        const float uy[3] = {-1.0f, 0.5f, 0.5f};
+        P.kind = 8;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float lx = rx * ux[k];
+            const float ly = ry * uy[k];
+            P.geom[2 * k] = cx + (ct * lx - st * ly);
+            P.geom[2 * k + 1] = cy + (st * lx + ct * ly);
+        }
+    } else {
+        P.kind = c.family == 0 ? 6 : 7;
+        P.geom[0] = cx;
+        P.geom[1] = cy;
+        P.geom[2] = rx;
+        P.geom[3] = ry;
+        P.geom[4] = ct;
+        P.geom[5] = st;
+    }
+    return P;
+}
+
+__device__ __forceinline__ WeftPrim load_prim(const WeftPrim* p) {
+    WeftPrim P;
+    P.kind = p->kind;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) P.geom[i] = p->geom[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        P.color0[i] = p->color0[i];
+        P.color1[i] = p->color1[i];
+    }
+    P.alpha = p->alpha;
+    return P;
+}
+
+__device__ __forceinline__ int class_of_kind(int kind) {
+    switch (kind) {
+    case 0: return CL_CONST;
+    case 4: case 8: return CL_TRI;
+    case 6: return CL_ELL;
+    case 7: return CL_RECT;
+    case 1: case 2: case 3: return CL_GEN;
+    default: return CL_CONST;  // skipped kinds leave the canvas unchanged
+    }
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const unsigned lane = __lane_id();
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+__device__ __forceinline__ void argmin_pair(double& o, int& i, double o2, int i2) {
+    if (o2 < o || (o2 == o && i2 < i)) {
+        o = o2;
+        i = i2;
+    }
+}
+
+__device__ __forceinline__ double objective(float mse, int bits, double lam, int mode) {
+    if (mode == 0) return (double)mse + lam * (double)bits;
+    const float r32 = (float)bits;
+    const float o = mse + (float)lam * r32;
+    return (double)o;
+}
+
+// ---------------------------------------------------------------------------
+// Plain fit: score one work item (one tile, `blockDim.x` candidates).
+// ---------------------------------------------------------------------------
+template <int TS, bool GEN>
+__device__ void fit_score_body(const WeftFitScoreArgs& A) {
+    extern __shared__ float4 weft_smem[];
+    constexpr int NPX = TS * TS;
+    float2* L = reinterpret_cast<float2*>(weft_smem);
+    int* perm = reinterpret_cast<int*>(L + 3 * NPX);
+    int* wcnt = perm + WEFT_MAX_BLOCK;           // [16 waves][8]
+    int* woff = wcnt + 16 * 8;                   // [16 waves][8]
+    double* red_o = reinterpret_cast<double*>(woff + 16 * 8);
+    int* red_i = reinterpret_cast<int*>(red_o + 16);
+
+    const int w = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int wave = tid >> 6;
+    const int nwaves = nthr >> 6;
+    const int tile = A.wk.d_work_tile[w];
+    const int c0 = A.wk.d_work_cand0[w];
+    const WeftFitState& st = A.st;
+
+    if (st.d_tile_nsel[tile] >= st.depth) {
+        if (tid == 0) {
+            A.part_obj[w] = __builtin_huge_val();
+            A.part_idx[w] = -1;
+        }
+        return;
+    }
+
+    // Stage canvas and target into LDS (coalesced, 3 floats per pixel each).
+    const int64_t base = st.d_tile_pix[tile] * 3;
+    const float* T = st.d_target + base;
+    const float* Cv = st.d_canvas + base;
+    float* Lf = reinterpret_cast<float*>(L);
+    for (int i = tid; i < 3 * NPX; i += nthr) {
+        const int p = i / 3, ch = i - 3 * (i / 3);
+        Lf[6 * p + ch] = Cv[i];
+        Lf[6 * p + 3 + ch] = T[i];
+    }
+
+    // Candidate class, then a counting sort by class so waves are class-uniform.
+    const int k = c0 + tid;
+    const bool valid = k < A.cd.n_cand;
+    const bool drawn = A.cd.d_prims == nullptr;
+    const uint64_t dbase = draw_base(A.cd.seed, A.cd.step, (uint32_t)tile);
+    const int64_t cand_base = drawn ? 0
+                              : (A.cd.d_tile_cand ? A.cd.d_tile_cand[tile]
+                                                  : (int64_t)tile * A.cd.n_cand);
+    int cls = CL_NONE;
+    if (valid) {
+        if (drawn) {
+            const int fam = draw_family(dbase, (uint32_t)k);
+            cls = fam == 0 ? CL_ELL : (fam == 1 ? CL_RECT : CL_TRI);
+        } else {
+            const WeftPrim* pp = A.cd.d_prims + cand_base + k;
+            cls = class_of_kind(pp->kind);
+            if ((cls == CL_ELL || cls == CL_RECT) && !(pp->geom[2] > 0.0f && pp->geom[3] > 0.0f))
+                cls = CL_CONST;  // non-positive radius: covers nothing
+            if (cls == CL_TRI) {
+                // degenerate triangles never cover a pixel: they behave as the no-op constant
+                const float* g = pp->geom;
+                double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
+                double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
+                if (fabs(den) < 1e-6) cls = CL_CONST;
+            }
+        }
+    }
+    uint64_t my_ballot = 0;
+#pragma unroll
+    for (int c = 0; c < WEFT_N_CLASSES; ++c) {
+        const uint64_t bal = __ballot(cls == c);
+        if (cls == c) my_ballot = bal;
+        if ((tid & 63) == 0) wcnt[wave * 8 + c] = __popcll(bal);
+    }
+    __syncthreads();
+    if (tid < WEFT_N_CLASSES) {
+        int off = 0;
+        for (int c = 0; c < tid; ++c)
+            for (int ww = 0; ww < nwaves; ++ww) off += wcnt[ww * 8 + c];
+        for (int ww = 0; ww < nwaves; ++ww) {
+            woff[ww * 8 + tid] = off;
+            off += wcnt[ww * 8 + tid];
+        }
+    }
+    __syncthreads();
+    perm[woff[wave * 8 + cls] + __popcll(my_ballot & lanemask_lt())] = valid ? k : -1;
+    __syncthreads();
+
+    // This lane's sorted candidate.
+    const int ks = perm[tid];
+    WeftPrim P;
+    int mycls = CL_NONE;
+    if (ks >= 0) {
+        if (drawn) P = code_to_prim(draw_code(dbase, (uint32_t)ks, TS));
+        else P = load_prim(A.cd.d_prims + cand_base + ks);
+        mycls = class_of_kind(P.kind);
+    }
+    const float a = clip01(P.alpha);
+    float ks0 = P.color0[0] * a, ks1 = P.color0[1] * a, ks2 = P.color0[2] * a;
+    float om = 1.0f - a;
+    if (mycls == CL_TRI) {
+        const float* g = P.geom;
+        double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
+        double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
+        if (fabs(den) < 1e-6) mycls = CL_CONST;
+    }
+    if ((mycls == CL_ELL || mycls == CL_RECT) && !(P.geom[2] > 0.0f && P.geom[3] > 0.0f))
+        mycls = CL_CONST;
+    if (mycls == CL_CONST && P.kind != 0) {
+        // skipped kind or degenerate triangle: the canvas passes through unchanged
+        ks0 = ks1 = ks2 = 0.0f;
+        om = 1.0f;
+    }
+
+    float sse = 0.0f;
+    if (mycls == CL_ELL) {
+        PxEllRect<TS, true> px;
+        px.L = L; px.ks0 = ks0; px.ks1 = ks1; px.ks2 = ks2; px.om = om;
+        px.cx = P.geom[0]; px.cy = P.geom[1];
+        px.ca = P.geom[4] / P.geom[2]; px.sa = P.geom[5] / P.geom[2];
+        px.cb = P.geom[4] / P.geom[3]; px.sb = P.geom[5] / P.geom[3];
+        sse = pairwise_tile<TS>(px);
+    } else if (mycls == CL_RECT) {
+        PxEllRect<TS, false> px;
+        px.L = L; px.ks0 = ks0; px.ks1 = ks1; px.ks2 = ks2; px.om = om;
+        px.cx = P.geom[0]; px.cy = P.geom[1];
+        px.ca = P.geom[4] / P.geom[2]; px.sa = P.geom[5] / P.geom[2];
+        px.cb = P.geom[4] / P.geom[3]; px.sb = P.geom[5] / P.geom[3];
+        sse = pairwise_tile<TS>(px);
+    } else if (mycls == CL_TRI) {
+        PxTri<TS> px;
+        px.L = L; px.ks0 = ks0; px.ks1 = ks1; px.ks2 = ks2; px.om = om;
+        const float* g = P.geom;
+        double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
+        double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
+        px.A = (float)(y1 - y2); px.B = (float)(x2 - x1);
+        px.C = (float)(y2 - y0); px.E = (float)(x0 - x2);
+        px.D = (float)den; px.X2 = (float)x2; px.Y2 = (float)y2;
+        sse = pairwise_tile<TS>(px);
+    } else if (mycls == CL_CONST) {
+        PxConst<TS> px;
+        px.L = L; px.ks0 = ks0; px.ks1 = ks1; px.ks2 = ks2; px.om = om;
+        sse = pairwise_tile<TS>(px);
+    } else if (mycls == CL_GEN) {
+        // kinds 1-3 need the generic composite; the GEN=false variants are only
+        // launched when the host has checked that no such candidate is present
+        if constexpr (GEN) sse = score_general<TS>(L, P);
+        else sse = __builtin_nanf("");
+    }
+
+    // Tables, objective and the work item's first minimum.
+    double obj = __builtin_huge_val();
+    int idx = 0x7fffffff;
+    if (ks >= 0) {
+        const float mse = sse / (float)(3 * NPX);
+        const int64_t ti = (int64_t)tile * A.cd.n_cand + ks;
+        if (A.out_sse) A.out_sse[ti] = sse;
+        if (A.out_mse) A.out_mse[ti] = mse;
+        obj = objective(mse, st.d_tile_bits[tile] + prim_bytes(P.kind) * 8, st.lam, st.obj_mode);
+        idx = ks;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o2 = __shfl_xor(obj, off, 64);
+        const int i2 = __shfl_xor(idx, off, 64);
+        argmin_pair(obj, idx, o2, i2);
+    }
+    if ((tid & 63) == 0) {
+        red_o[wave] = obj;
+        red_i[wave] = idx;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double bo = red_o[0];
+        int bi = red_i[0];
+        for (int ww = 1; ww < nwaves; ++ww) argmin_pair(bo, bi, red_o[ww], red_i[ww]);
+        A.part_obj[w] = bo;
+        A.part_idx[w] = bi == 0x7fffffff ? -1 : bi;
+    }
+}
+
+template <int TS>
+__device__ float tile_sse_global(const float* tgt, const float* cv) {
+    PxCanvasGlobal px{tgt, cv, TS};
+    return pairwise_tile<TS>(px);
+}
+
+template <int TS>
+__device__ float stack_sse(const float* tgt, const WeftPrim* prims, int n) {
+    PxStack px{tgt, prims, n, TS, 0.0f};
+    return pairwise_tile<TS>(px);
+}
+
+}  // namespace
+
+// ===========================================================================
+// Kernels
+// ===========================================================================
+
+// One lane per evaluation (reference batch_tile_objective with CPU semantics).
+extern "C" __global__ __launch_bounds__(256) void weft_score_evals_kernel(WeftScoreEvalsArgs A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n_evals) return;
+    const WeftEval ev = A.evals[i];
+    const float* tgt = A.tiles + ev.tile_offset;
+    const WeftPrim* pr = A.prims + ev.prim_offset;
+    float sse;
+    switch (ev.tile_size) {
+    case 8: sse = stack_sse<8>(tgt, pr, ev.prim_count); break;
+    case 16: sse = stack_sse<16>(tgt, pr, ev.prim_count); break;
+    case 32: sse = stack_sse<32>(tgt, pr, ev.prim_count); break;
+    case 64: sse = stack_sse<64>(tgt, pr, ev.prim_count); break;
+    default: sse = __builtin_nanf(""); break;
+    }
+    const float n = (float)(3 * ev.tile_size * ev.tile_size);
+    A.out_mse[i] = sse / n;
+    if (A.out_sse) A.out_sse[i] = sse;
+}
+
+// One lane per tile: objective of the current canvas.
+extern "C" __global__ __launch_bounds__(256) void weft_fit_init_kernel(WeftFitInitArgs A) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const WeftFitState& st = A.st;
+    if (t >= st.n_tiles) return;
+    const int64_t base = st.d_tile_pix[t] * 3;
+    const float* tg = st.d_target + base;
+    const float* cv = st.d_canvas + base;
+    const int ts = st.d_tile_ts[t];
+    float sse;
+    switch (ts) {
+    case 8: sse = tile_sse_global<8>(tg, cv); break;
+    case 16: sse = tile_sse_global<16>(tg, cv); break;
+    case 32: sse = tile_sse_global<32>(tg, cv); break;
+    case 64: sse = tile_sse_global<64>(tg, cv); break;
+    default: sse = __builtin_nanf(""); break;
+    }
+    const float mse = sse / (float)(3 * ts * ts);
+    st.d_tile_obj[t] = objective(mse, st.d_tile_bits[t], st.lam, st.obj_mode);
+}
+
+// Fast variants: candidate kinds 0, 4, 6, 7, 8 (hard-edge coverage over the cached canvas).
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_ts8(WeftFitScoreArgs A) { fit_score_body<8, false>(A); }
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_ts16(WeftFitScoreArgs A) { fit_score_body<16, false>(A); }
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_ts32(WeftFitScoreArgs A) { fit_score_body<32, false>(A); }
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_ts64(WeftFitScoreArgs A) { fit_score_body<64, false>(A); }
+// Variants that also accept kinds 1-3 (linear gradient, Gaussian line, Gaussian curve).
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_ts8(WeftFitScoreArgs A) { fit_score_body<8, true>(A); }
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_ts16(WeftFitScoreArgs A) { fit_score_body<16, true>(A); }
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_ts32(WeftFitScoreArgs A) { fit_score_body<32, true>(A); }
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_ts64(WeftFitScoreArgs A) { fit_score_body<64, true>(A); }
+
+// One workgroup per tile: pick the winner, accept, composite, update the state.
+extern "C" __global__ __launch_bounds__(256) void weft_fit_select_kernel(WeftFitSelectArgs A) {
+    __shared__ double s_obj;
+    __shared__ int s_idx;
+    __shared__ int s_acc;
+    const int t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const WeftFitState& st = A.st;
+    if (st.d_tile_nsel[t] >= st.depth) {
+        if (tid == 0) {
+            A.win_idx[t] = -1;
+            A.win_obj[t] = __builtin_huge_val();
+            A.win_acc[t] = 0;
+        }
+        return;
+    }
+    if (tid == 0) {
+        double bo = __builtin_huge_val();
+        int bi = 0x7fffffff;
+        const int w0 = A.tile_work0[t], nw = A.tile_nwork[t];
+        for (int w = w0; w < w0 + nw; ++w) {
+            const int i2 = A.part_idx[w];
+            if (i2 >= 0) argmin_pair(bo, bi, A.part_obj[w], i2);
+        }
+        const bool found = bi != 0x7fffffff;
+        s_idx = found ? bi : -1;
+        s_obj = bo;
+        s_acc = found && (bo < st.d_tile_obj[t]);
+    }
+    __syncthreads();
+    const int bi = s_idx;
+    const int acc = s_acc;
+    const int ts = st.d_tile_ts[t];
+    WeftPrim P;
+    P.kind = -1;
+    if (acc) {
+        if (A.cd.d_prims == nullptr) {
+            const WeftCode code = draw_code(draw_base(A.cd.seed, A.cd.step, (uint32_t)t), (uint32_t)bi, ts);
+            P = code_to_prim(code);
+            if (tid == 0 && A.win_code) A.win_code[t] = code;
+        } else {
+            const int64_t cb = A.cd.d_tile_cand ? A.cd.d_tile_cand[t] : (int64_t)t * A.cd.n_cand;
+            P = load_prim(A.cd.d_prims + cb + bi);
+        }
+        float* cv = st.d_canvas + st.d_tile_pix[t] * 3;
+        for (int p = tid; p < ts * ts; p += blockDim.x) {
+            const int y = p / ts, x = p - y * ts;
+            float r = cv[3 * p], g = cv[3 * p + 1], b = cv[3 * p + 2];
+            composite_px(P, (float)x + 0.5f, (float)y + 0.5f, r, g, b);
+            cv[3 * p] = r;
+            cv[3 * p + 1] = g;
+            cv[3 * p + 2] = b;
+        }
+    }
+    if (tid == 0) {
+        A.win_idx[t] = bi;
+        A.win_obj[t] = s_obj;
+        A.win_acc[t] = acc;
+        if (acc) {
+            st.d_tile_nsel[t] += 1;
+            st.d_tile_bits[t] += prim_bytes(P.kind) * 8;
+            st.d_tile_obj[t] = s_obj;
+        }
+    }
+}
+
+// Candidate codes / records as the fit kernels draw them.
+extern "C" __global__ __launch_bounds__(256) void weft_draw_kernel(WeftDrawArgs A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)A.n_tiles * A.n_cand) return;
+    const int j = (int)(i / A.n_cand);
+    const int k = (int)(i - (int64_t)j * A.n_cand);
+    const WeftCode c = draw_code(draw_base(A.seed, A.step, (uint32_t)A.tile_ids[j]), (uint32_t)k, A.tile_ts[j]);
+    if (A.codes) A.codes[i] = c;
+    if (A.prims) {
+        const WeftPrim P = code_to_prim(c);
+        WeftPrim* o = A.prims + i;
+        o->kind = P.kind;
+        for (int q = 0; q < 7; ++q) o->geom[q] = P.geom[q];
+        for (int q = 0; q < 3; ++q) {
+            o->color0[q] = P.color0[q];
+            o->color1[q] = P.color1[q];
+        }
+        o->alpha = P.alpha;
+    }
+}
+
+// Decode: one workgroup per tile, threads over the tile's output pixels.
+extern "C" __global__ __launch_bounds__(256) void weft_render_tiles_kernel(WeftRenderArgs A) {
+    const int t = blockIdx.x;
+    if (t >= A.n_tiles) return;
+    const WeftTileDecodeDesc d = A.descs[t];
+    const WeftPrim* pr = A.prims + d.prim_offset;
+    const float stepx = d.source_size == d.target_w ? 1.0f
+                        : (float)((double)d.source_size / (double)d.target_w);
+    const float stepy = d.source_size == d.target_h ? 1.0f
+                        : (float)((double)d.source_size / (double)d.target_h);
+    const int npx = d.target_w * d.target_h;
+    for (int p = threadIdx.x; p < npx; p += blockDim.x) {
+        const int ly = p / d.target_w, lx = p - ly * d.target_w;
+        const float fx = ((float)lx + 0.5f) * stepx;
+        const float fy = ((float)ly + 0.5f) * stepy;
+        float r = 0.0f, g = 0.0f, b = 0.0f;
+        for (int i = 0; i < d.prim_count; ++i) composite_px(pr[i], fx, fy, r, g, b);
+        r = clip01(r + d.res0_r);
+        g = clip01(g + d.res0_g);
+        b = clip01(b + d.res0_b);
+        const int gx = d.target_x0 + lx, gy = d.target_y0 + ly;
+        if (gx >= 0 && gx < A.width && gy >= 0 && gy < A.height) {
+            const int64_t o = ((int64_t)gy * A.width + gx) * 3;
+            A.out[o] = r;
+            A.out[o + 1] = g;
+            A.out[o + 2] = b;
+        }
+    }
+}

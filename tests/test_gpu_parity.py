@@ -200,14 +200,15 @@ def test_fit_mixed_tile_sizes_bit_exact(gpu, oracle_lib):
     assert bits_equal(f.canvas(), o.canvas)
 
 
-def test_fit_explicit_candidates_all_kinds(gpu, oracle_lib):
+@pytest.mark.parametrize("ts", [8, 16, 32, 64])
+def test_fit_explicit_candidates_all_kinds(gpu, oracle_lib, ts):
     """Explicit candidate records of every kind over non-trivial canvases."""
     from weft_amd.fit import PlainFitter
     from weft_amd.quadtree import TileSet, uniform_tiles
     from oracle.coracle import PRIM_DTYPE
 
-    rng = np.random.default_rng(21)
-    for ts in (8, 16, 32, 64):
+    rng = np.random.default_rng(21 + ts)
+    if True:
         img = rng.random((ts * 2, ts * 2, 3), dtype=np.float32)
         tiles = TileSet.from_images([img], [uniform_tiles(ts * 2, ts * 2, ts)], pad_multiple=ts)
         T, P = tiles.n_tiles, 150

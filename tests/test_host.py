@@ -1,0 +1,195 @@
+"""Host-side logic and the C ABI library, without a GPU."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, bits_equal, golden_prims, load_golden
+
+
+def _header_functions() -> list[str]:
+    text = open(os.path.join(ROOT, "include", "weft_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(weft_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from weft_amd import _lib
+
+    L = _lib.library()
+    names = _header_functions()
+    assert len(names) >= 11
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.EXPORTED_SYMBOLS)
+    assert L.weft_abi_version() == _lib.ABI_VERSION
+
+
+def test_code_object_targets_gfx950():
+    from weft_amd import build
+
+    blob = open(build.HSACO, "rb").read()
+    # an ELF code object or a clang offload bundle holding one (hipcc --genco)
+    assert blob[:4] == b"\x7fELF" or blob.startswith(b"__CLANG_OFFLOAD_BUNDLE__")
+    assert b"gfx950" in blob
+    for k in (b"weft_fit_score_ts32", b"weft_score_evals_kernel", b"weft_render_tiles_kernel"):
+        assert k in blob
+
+
+def test_entry_points_fail_cleanly_without_init():
+    from weft_amd import _lib
+
+    L = _lib.library()
+    if _lib.gpu_available():
+        pytest.skip("a GPU is visible; the no-device path is exercised on CPU hosts")
+    assert L.weft_score_evals(None, 1, None, None, None, None, None) == -1000
+    assert b"weft_init" in L.weft_last_error()
+
+
+def test_tables_match_oracle(oracle_lib):
+    from weft_amd import _lib
+
+    a = _lib.tables()
+    b = oracle_lib.tables()
+    for x, y in zip(a, b):
+        assert bits_equal(x, y)
+
+
+def test_codes_to_records_match_oracle(oracle_lib):
+    from weft_amd.records import codes_to_records
+
+    for ts in (8, 16, 32, 64):
+        codes = oracle_lib.draw_codes(123, 4, 77, 500, ts)
+        ours = codes_to_records(codes)
+        theirs = oracle_lib.codes_to_prims(codes)
+        assert bits_equal(ours.view(np.uint8), theirs.view(np.uint8))
+
+
+def test_serialization_matches_reference_bytes():
+    """encode_tile of kinds 0-4 reproduces the reference's PRIM bytes (fixture)."""
+    from weft_amd.primitives import Primitive, TileRecord, encode_tile
+
+    m = load_golden("ref_misc.npz")
+    n = len(m["ser_prim_off"])
+    for i in range(n):
+        P = golden_prims(m, i, pref="ser_")
+        o = int(m["ser_prim_off"][i])
+        prims = []
+        for k in range(len(P)):
+            kind = int(P["kind"][k])
+            ng = {0: 0, 1: 4, 2: 5, 3: 7, 4: 6}[kind]
+            c1 = tuple(float(v) for v in m["ser_color1"][o + k]) if m["ser_has_color1"][o + k] else None
+            prims.append(Primitive(kind, tuple(float(v) for v in P["geom"][k][:ng]),
+                                   tuple(float(v) for v in P["color0"][k]), c1, float(P["alpha"][k])))
+        blob = bytes(m["ser_blob"][m["ser_blob_off"][i]:m["ser_blob_off"][i + 1]])
+        assert encode_tile(TileRecord(prims)) == blob, i
+
+
+def test_estimate_bits_and_lambda_match_reference():
+    from weft_amd.rd import estimate_bits, tile_lambda
+
+    m = load_golden("ref_misc.npz")
+    for q, lam in zip(m["lam_q"], m["lam"]):
+        assert tile_lambda(int(q)) == float(lam)
+    for i in range(len(m["ser_prim_off"])):
+        P = golden_prims(m, i, pref="ser_")
+        assert estimate_bits(P["kind"], False) == int(m["est_bits"][i, 0])
+        assert estimate_bits(P["kind"], True) == int(m["est_bits"][i, 1])
+
+
+def test_decode_encode_roundtrip_idempotent():
+    from weft_amd.primitives import (Primitive, TileRecord, decode_tiles, encode_code_tiles, encode_tile,
+                                     encode_tiles)
+
+    rng = np.random.default_rng(2)
+    tiles = []
+    for _ in range(20):
+        prims = []
+        for k in rng.choice([0, 1, 2, 3, 4, 6, 7, 8], int(rng.integers(0, 6))):
+            k = int(k)
+            g = {0: (), 1: tuple(rng.uniform(0, 15, 4)), 2: tuple(rng.uniform(0, 15, 4)) + (1.5,),
+                 3: tuple(rng.uniform(0, 15, 6)) + (0.7,), 4: tuple(rng.uniform(0, 15, 6)),
+                 6: tuple(rng.uniform(0, 30, 4)) + (1.0,), 7: tuple(rng.uniform(0, 30, 4)) + (2.0,),
+                 8: tuple(rng.uniform(0, 30, 4)) + (0.2,)}[k]
+            prims.append(Primitive(k, g, tuple(rng.random(3)), tuple(rng.random(3)) if k == 1 else None,
+                                   float(rng.random())))
+        tiles.append(TileRecord(prims))
+    blob, toc = encode_tiles(tiles)
+    again, toc2 = encode_tiles(decode_tiles(blob, toc))
+    assert again == blob and toc2 == toc
+    # parametric records: code bytes == bytes of the decoded primitive
+    from oracle import coracle
+    from weft_amd.primitives import code_to_primitive
+
+    codes = coracle.draw_codes(1, 2, 3, 50, 32)
+    b1, _ = encode_code_tiles([list(codes)])
+    b2 = encode_tile(TileRecord([code_to_primitive(c) for c in codes]))
+    assert b1 == b2
+    assert struct.unpack_from("<B", b1, 0)[0] == 50 and len(b1) == 1 + 50 * 17
+
+
+def test_quadtree_matches_reference():
+    from weft_amd.quadtree import decompose_quadtree
+
+    m = load_golden("ref_misc.npz")
+    img = m["qt_image"]
+    start = 0
+    for thr, cnt in zip(m["qt_thr"], m["qt_counts"]):
+        ref = m["qt_tiles"][start:start + cnt, :3]
+        start += cnt
+        got = np.array([[t.x, t.y, t.size] for t in decompose_quadtree(img, split_threshold=float(thr))])
+        assert np.array_equal(got, ref)
+
+
+def test_synthetic_images_are_deterministic():
+    from weft_amd.synth import image_checksum, synth_image
+
+    a = synth_image(96, 128, 20, seed=0)
+    b = synth_image(96, 128, 20, seed=0)
+    assert image_checksum(a) == image_checksum(b)
+    assert a.dtype == np.float32 and a.min() >= 0 and a.max() <= 1
+    assert image_checksum(synth_image(96, 128, 20, seed=1)) != image_checksum(a)
+
+
+def test_tileset_packing_and_assembly():
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.synth import synth_image
+
+    img = synth_image(100, 150, 15, seed=2)
+    qt = decompose_quadtree_variance(img, threshold=1e-3)
+    ts = TileSet.from_images([img], [qt])
+    assert ts.n_pixels == 128 * 192
+    back = ts.assemble(ts.target, 0, 100, 150)
+    assert bits_equal(back, img)
+
+
+def test_drop_in_returns_none_without_gpu():
+    """The reference contract: no GPU -> None, so the caller falls back to its CPU path."""
+    from weft_amd import _lib
+    from weft_amd.gpu_render import gpu_batch_objectives, gpu_batch_score_prepacked
+    from weft_amd.primitives import Primitive
+
+    if _lib.gpu_available():
+        pytest.skip("GPU visible")
+    tile = np.zeros((16, 16, 3), np.float32)
+    assert gpu_batch_objectives(tile, [[Primitive(0, (), (0.5, 0.5, 0.5))]]) is None
+    assert gpu_batch_objectives(tile, []).shape == (0,)
+    assert gpu_batch_score_prepacked(tile.ravel(), {0: 0}, [0], [16], [], [np.zeros(0, np.int32)]) == [
+        pytest.approx(np.zeros(0))]
+
+
+def test_fitter_rejects_unsupported_tile_sizes():
+    from weft_amd import _lib
+    from weft_amd.quadtree import TileSet
+
+    if _lib.gpu_available():
+        pytest.skip("GPU visible")
+    ts = TileSet([0], [0], [12], [0], np.zeros(12 * 12 * 3, np.float32), [0])
+    from weft_amd.fit import PlainFitter
+
+    with pytest.raises(Exception):
+        PlainFitter(ts, n_cand=64, depth=2)

@@ -106,8 +106,10 @@ int ts_slot(int ts) {
 }
 
 unsigned score_lds_bytes(int ts) {
+    // one band of rows (whole tile up to 32 px, 32-row halves of 64-px tiles):
     // 3 float2 per pixel + sort permutation + per-wave class counts/offsets + reduction
-    return (unsigned)(24 * ts * ts + 4 * WEFT_MAX_BLOCK + 4 * 16 * 8 * 2 + 8 * 16 + 4 * 16);
+    const int rows = ts > 32 ? 32 : ts;
+    return (unsigned)(24 * rows * ts + 4 * WEFT_MAX_BLOCK + 4 * 16 * 8 * 2 + 8 * 16 + 4 * 16);
 }
 
 }  // namespace

@@ -230,20 +230,16 @@ struct PwState {
     }
 };
 
-// Pairwise SSE of one tile, pixels in row-major order, 8 pixels (24 elements) per
-// unrolled group.  Px provides row(y) and pixel(y, x, e0, e1, e2) (the three squared
-// errors of pixel (y, x)).
+// Pairwise SSE over rows [y0, y1) of a tile, pixels in row-major order, 8 pixels
+// (24 elements) per unrolled group; the state carries over between calls so a tile can
+// be processed in row bands.  Px provides row(y) and pixel(y, x, e0, e1, e2) (the three
+// squared errors of pixel (y, x)).
 template <int TS, class Px>
-__device__ __forceinline__ float pairwise_tile(Px& px) {
+__device__ __forceinline__ void pairwise_rows(Px& px, PwState& st, int y0, int y1) {
     constexpr int BLOCK = PwCfg<TS>::BLOCK;
-    PwState st;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) st.r[j] = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) st.s[j] = 0.0f;
-    int base = 0;
+    int base = 3 * TS * y0;
 #pragma unroll 1
-    for (int y = 0; y < TS; ++y) {
+    for (int y = y0; y < y1; ++y) {
         px.row(y);
 #pragma unroll 1
         for (int gx = 0; gx < TS; gx += 8) {
@@ -258,6 +254,20 @@ __device__ __forceinline__ float pairwise_tile(Px& px) {
             base += 24;
         }
     }
+}
+
+__device__ __forceinline__ void pw_init(PwState& st) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) st.r[j] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) st.s[j] = 0.0f;
+}
+
+template <int TS, class Px>
+__device__ __forceinline__ float pairwise_tile(Px& px) {
+    PwState st;
+    pw_init(st);
+    pairwise_rows<TS>(px, st, 0, TS);
     return st.template result<TS>();
 }
 
@@ -302,13 +312,14 @@ struct PxCanvasGlobal {
     }
 };
 
-// LDS tile: per pixel 3 float2 = (c_r, c_g), (c_b, t_r), (t_g, t_b).
+// LDS band of rows [y0, y0 + band): per pixel 3 float2 = (c_r, c_g), (c_b, t_r), (t_g, t_b).
 template <int TS>
 struct PxLdsBase {
     const float2* L;
+    int y0;
     float ks0, ks1, ks2, om;
     __device__ __forceinline__ void finish(int y, int x, bool in, float& e0, float& e1, float& e2) {
-        const int p = y * TS + x;
+        const int p = (y - y0) * TS + x;
         const float2 a = L[3 * p], b = L[3 * p + 1], c = L[3 * p + 2];
         const float w0 = ks0 + a.x * om;
         const float w1 = ks1 + a.y * om;
@@ -376,11 +387,12 @@ struct PxTri : PxLdsBase<TS> {
 template <int TS>
 struct PxGeneral {
     const float2* L;
+    int y0;
     WeftPrim P;
     float py;
     __device__ __forceinline__ void row(int y) { py = (float)y + 0.5f; }
     __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
-        const int p = y * TS + x;
+        const int p = (y - y0) * TS + x;
         const float2 a = L[3 * p], b = L[3 * p + 1], c = L[3 * p + 2];
         float r = a.x, g = a.y, bb = b.x;
         composite_px(P, (float)x + 0.5f, py, r, g, bb);
@@ -392,11 +404,12 @@ struct PxGeneral {
 };
 
 template <int TS>
-__device__ __noinline__ float score_general(const float2* L, WeftPrim P) {
+__device__ __noinline__ void score_general(const float2* L, WeftPrim P, PwState& st, int y0, int y1) {
     PxGeneral<TS> px;
     px.L = L;
+    px.y0 = y0;
     px.P = P;
-    return pairwise_tile<TS>(px);
+    pairwise_rows<TS>(px, st, y0, y1);
 }
 
 // ---------------------------------------------------------------------------
@@ -519,12 +532,57 @@ __device__ __forceinline__ double objective(float mse, int bits, double lam, int
 // ---------------------------------------------------------------------------
 // Plain fit: score one work item (one tile, `blockDim.x` candidates).
 // ---------------------------------------------------------------------------
+// Rows of a tile held in LDS at a time: whole tiles up to 32 px (<= 24 KiB), 32-row
+// halves of 64-px tiles (48 KiB), so several workgroups fit on a CU.
+template <int TS> struct Band { static constexpr int ROWS = TS > 32 ? 32 : TS; };
+
+// Per-candidate constants of the hard-edge classes.
+struct CandConsts {
+    float ks0, ks1, ks2, om;               // s * a per channel, 1 - a
+    float g0, g1, k11, k12, k21, k22, D;   // class-specific coverage constants
+};
+
+template <int TS, bool GEN>
+__device__ __forceinline__ void score_band(int mycls, const CandConsts& c, const WeftPrim& P,
+                                           const float2* L, PwState& pw, int y0, int y1) {
+    if (mycls == CL_ELL) {
+        PxEllRect<TS, true> px;
+        px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
+        px.cx = c.g0; px.cy = c.g1; px.ca = c.k11; px.sa = c.k12; px.cb = c.k21; px.sb = c.k22;
+        pairwise_rows<TS>(px, pw, y0, y1);
+    } else if (mycls == CL_RECT) {
+        PxEllRect<TS, false> px;
+        px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
+        px.cx = c.g0; px.cy = c.g1; px.ca = c.k11; px.sa = c.k12; px.cb = c.k21; px.sb = c.k22;
+        pairwise_rows<TS>(px, pw, y0, y1);
+    } else if (mycls == CL_TRI) {
+        PxTri<TS> px;
+        px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
+        px.X2 = c.g0; px.Y2 = c.g1; px.A = c.k11; px.B = c.k12; px.C = c.k21; px.E = c.k22; px.D = c.D;
+        pairwise_rows<TS>(px, pw, y0, y1);
+    } else if (mycls == CL_CONST) {
+        PxConst<TS> px;
+        px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
+        pairwise_rows<TS>(px, pw, y0, y1);
+    } else if (mycls == CL_GEN) {
+        // kinds 1-3 need the generic composite; the GEN=false variants are only
+        // launched when the host has checked that no such candidate is present
+        if constexpr (GEN) score_general<TS>(L, P, pw, y0, y1);
+        else pw.s[0] = pw.s[1] = pw.s[3] = pw.s[5] = pw.s[6] = __builtin_nanf("");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain fit: score one work item (one tile, `blockDim.x` candidates).
+// ---------------------------------------------------------------------------
 template <int TS, bool GEN>
 __device__ void fit_score_body(const WeftFitScoreArgs& A) {
     extern __shared__ float4 weft_smem[];
+    constexpr int ROWS = Band<TS>::ROWS;
     constexpr int NPX = TS * TS;
+    constexpr int BPX = ROWS * TS;  // pixels per LDS band
     float2* L = reinterpret_cast<float2*>(weft_smem);
-    int* perm = reinterpret_cast<int*>(L + 3 * NPX);
+    int* perm = reinterpret_cast<int*>(L + 3 * BPX);
     int* wcnt = perm + WEFT_MAX_BLOCK;           // [16 waves][8]
     int* woff = wcnt + 16 * 8;                   // [16 waves][8]
     double* red_o = reinterpret_cast<double*>(woff + 16 * 8);
@@ -545,17 +603,6 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
             A.part_idx[w] = -1;
         }
         return;
-    }
-
-    // Stage canvas and target into LDS (coalesced, 3 floats per pixel each).
-    const int64_t base = st.d_tile_pix[tile] * 3;
-    const float* T = st.d_target + base;
-    const float* Cv = st.d_canvas + base;
-    float* Lf = reinterpret_cast<float*>(L);
-    for (int i = tid; i < 3 * NPX; i += nthr) {
-        const int p = i / 3, ch = i - 3 * (i / 3);
-        Lf[6 * p + ch] = Cv[i];
-        Lf[6 * p + 3 + ch] = T[i];
     }
 
     // Candidate class, then a counting sort by class so waves are class-uniform.
@@ -606,67 +653,72 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
     perm[woff[wave * 8 + cls] + __popcll(my_ballot & lanemask_lt())] = valid ? k : -1;
     __syncthreads();
 
-    // This lane's sorted candidate.
+    // This lane's sorted candidate and its constants.
     const int ks = perm[tid];
     WeftPrim P;
+    P.kind = -1;
+    P.alpha = 0.0f;
+    P.color0[0] = P.color0[1] = P.color0[2] = 0.0f;
     int mycls = CL_NONE;
     if (ks >= 0) {
         if (drawn) P = code_to_prim(draw_code(dbase, (uint32_t)ks, TS));
         else P = load_prim(A.cd.d_prims + cand_base + ks);
         mycls = class_of_kind(P.kind);
     }
+    CandConsts cc;
     const float a = clip01(P.alpha);
-    float ks0 = P.color0[0] * a, ks1 = P.color0[1] * a, ks2 = P.color0[2] * a;
-    float om = 1.0f - a;
+    cc.ks0 = P.color0[0] * a;
+    cc.ks1 = P.color0[1] * a;
+    cc.ks2 = P.color0[2] * a;
+    cc.om = 1.0f - a;
+    cc.g0 = cc.g1 = cc.k11 = cc.k12 = cc.k21 = cc.k22 = cc.D = 0.0f;
     if (mycls == CL_TRI) {
         const float* g = P.geom;
         double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
         double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
-        if (fabs(den) < 1e-6) mycls = CL_CONST;
+        if (fabs(den) < 1e-6) {
+            mycls = CL_CONST;
+        } else {
+            cc.k11 = (float)(y1 - y2); cc.k12 = (float)(x2 - x1);
+            cc.k21 = (float)(y2 - y0); cc.k22 = (float)(x0 - x2);
+            cc.D = (float)den; cc.g0 = (float)x2; cc.g1 = (float)y2;
+        }
     }
-    if ((mycls == CL_ELL || mycls == CL_RECT) && !(P.geom[2] > 0.0f && P.geom[3] > 0.0f))
-        mycls = CL_CONST;
+    if (mycls == CL_ELL || mycls == CL_RECT) {
+        if (!(P.geom[2] > 0.0f && P.geom[3] > 0.0f)) {
+            mycls = CL_CONST;
+        } else {
+            cc.g0 = P.geom[0]; cc.g1 = P.geom[1];
+            cc.k11 = P.geom[4] / P.geom[2]; cc.k12 = P.geom[5] / P.geom[2];
+            cc.k21 = P.geom[4] / P.geom[3]; cc.k22 = P.geom[5] / P.geom[3];
+        }
+    }
     if (mycls == CL_CONST && P.kind != 0) {
-        // skipped kind or degenerate triangle: the canvas passes through unchanged
-        ks0 = ks1 = ks2 = 0.0f;
-        om = 1.0f;
+        // skipped kind, degenerate triangle or empty ellipse: the canvas passes through
+        cc.ks0 = cc.ks1 = cc.ks2 = 0.0f;
+        cc.om = 1.0f;
     }
 
-    float sse = 0.0f;
-    if (mycls == CL_ELL) {
-        PxEllRect<TS, true> px;
-        px.L = L; px.ks0 = ks0; px.ks1 = ks1; px.ks2 = ks2; px.om = om;
-        px.cx = P.geom[0]; px.cy = P.geom[1];
-        px.ca = P.geom[4] / P.geom[2]; px.sa = P.geom[5] / P.geom[2];
-        px.cb = P.geom[4] / P.geom[3]; px.sb = P.geom[5] / P.geom[3];
-        sse = pairwise_tile<TS>(px);
-    } else if (mycls == CL_RECT) {
-        PxEllRect<TS, false> px;
-        px.L = L; px.ks0 = ks0; px.ks1 = ks1; px.ks2 = ks2; px.om = om;
-        px.cx = P.geom[0]; px.cy = P.geom[1];
-        px.ca = P.geom[4] / P.geom[2]; px.sa = P.geom[5] / P.geom[2];
-        px.cb = P.geom[4] / P.geom[3]; px.sb = P.geom[5] / P.geom[3];
-        sse = pairwise_tile<TS>(px);
-    } else if (mycls == CL_TRI) {
-        PxTri<TS> px;
-        px.L = L; px.ks0 = ks0; px.ks1 = ks1; px.ks2 = ks2; px.om = om;
-        const float* g = P.geom;
-        double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
-        double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
-        px.A = (float)(y1 - y2); px.B = (float)(x2 - x1);
-        px.C = (float)(y2 - y0); px.E = (float)(x0 - x2);
-        px.D = (float)den; px.X2 = (float)x2; px.Y2 = (float)y2;
-        sse = pairwise_tile<TS>(px);
-    } else if (mycls == CL_CONST) {
-        PxConst<TS> px;
-        px.L = L; px.ks0 = ks0; px.ks1 = ks1; px.ks2 = ks2; px.om = om;
-        sse = pairwise_tile<TS>(px);
-    } else if (mycls == CL_GEN) {
-        // kinds 1-3 need the generic composite; the GEN=false variants are only
-        // launched when the host has checked that no such candidate is present
-        if constexpr (GEN) sse = score_general<TS>(L, P);
-        else sse = __builtin_nanf("");
+    // Stage the tile band by band (coalesced, 3 floats per pixel each) and score it.
+    const int64_t base = st.d_tile_pix[tile] * 3;
+    const float* T = st.d_target + base;
+    const float* Cv = st.d_canvas + base;
+    float* Lf = reinterpret_cast<float*>(L);
+    PwState pw;
+    pw_init(pw);
+#pragma unroll 1
+    for (int y0 = 0; y0 < TS; y0 += ROWS) {
+        if (y0 > 0) __syncthreads();  // every wave is done with the previous band
+        const int e0 = 3 * y0 * TS;
+        for (int i = tid; i < 3 * BPX; i += nthr) {
+            const int p = i / 3, ch = i - 3 * (i / 3);
+            Lf[6 * p + ch] = Cv[e0 + i];
+            Lf[6 * p + 3 + ch] = T[e0 + i];
+        }
+        __syncthreads();
+        score_band<TS, GEN>(mycls, cc, P, L, pw, y0, y0 + ROWS);
     }
+    const float sse = pw.template result<TS>();
 
     // Tables, objective and the work item's first minimum.
     double obj = __builtin_huge_val();

@@ -112,7 +112,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES of device memory once (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        src = torch.empty(args.copy_probe // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        torch.cuda.synchronize()
+        dst = src.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": int(dst.numel() * 4)}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the tile-fit path "
               "shards by tiles with no collective, see DESIGN.md", file=sys.stderr)
@@ -148,18 +159,16 @@ def main():
     start_ev = torch.cuda.Event(enable_timing=True)
     end_ev = torch.cuda.Event(enable_timing=True)
     start_ev.record(stream)
-    import ctypes
+    sp = _lib.stream_ptr()
     for _ in range(args.steps):
         s = fitter.step_idx
         cands = fitter._cands(s)
+        fitter.order(s, sp)
         for (w, start), cl in zip(fitter._works, fitter.classes):
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record(stream)
-            _lib.check(fitter.L.weft_fit_score(ctypes.byref(fitter._state), ctypes.byref(w), ctypes.byref(cands),
-                                               None, None, fitter.d_part_obj.data_ptr() + 8 * start,
-                                               fitter.d_part_idx.data_ptr() + 4 * start, _lib.stream_ptr()),
-                       "weft_fit_score")
+            fitter.score_class(w, start, cands, None, None, sp)
             b.record(stream)
             ev[cl["ts"]].append((a, b))
         fitter.select()

@@ -145,13 +145,19 @@ int weft_score_evals(const WeftEval* d_evals, int n_evals, const WeftPrim* d_pri
  * canvas is set).  d_tile_obj[t] = MSE(clip(canvas_t), target_t) + lam * bits_t. */
 int weft_fit_init(const WeftFitState* st, void* stream);
 
-/* Plain fit, part 2: score candidates of the tiles in `work` against their cached
- * canvases.  Per candidate k of tile t (index t * n_cand + k) writes SSE / MSE when
- * the tables are given, and per work item the first minimum of the objective into
- * d_part_obj / d_part_idx.  Tiles whose stack is full are skipped. */
+/* Plain fit, part 2a: per tile, the candidate indices sorted (stably) by coverage class
+ * (ellipse, rectangle, triangle, constant, generic), so that each wavefront of the
+ * scoring kernel runs one class.  d_order: n_tiles * n_cand int32. */
+int weft_fit_order(const WeftFitState* st, const WeftCands* cands, int32_t* d_order, void* stream);
+
+/* Plain fit, part 2b: score candidates of the tiles in `work` against their cached
+ * canvases, taking slots [cand0, cand0 + block) of each tile's d_order.  Per candidate k
+ * of tile t (index t * n_cand + k) writes SSE / MSE when the tables are given, and per
+ * work item the first minimum of the objective into d_part_obj / d_part_idx.  Tiles
+ * whose stack is full are skipped. */
 int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands* cands,
-                   float* d_out_sse, float* d_out_mse, double* d_part_obj, int32_t* d_part_idx,
-                   void* stream);
+                   const int32_t* d_order, float* d_out_sse, float* d_out_mse, double* d_part_obj,
+                   int32_t* d_part_idx, void* stream);
 
 /* Plain fit, part 3: per tile reduce its work items' partial minima (work items
  * d_tile_work0[t] .. + d_tile_nwork[t] in the concatenated partial arrays), accept the

@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -70,15 +70,17 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_shutdown.argtypes = []
     L.weft_score_evals.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_fit_init.argtypes = [ctypes.POINTER(FitState), c_void_p]
+    L.weft_fit_order.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, c_void_p]
     L.weft_fit_score.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Work), ctypes.POINTER(Cands),
-                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_fit_select.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_draw_codes.argtypes = [c_uint64, c_uint32, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                   c_void_p, c_void_p]
     L.weft_get_tables.argtypes = [c_void_p, c_void_p, c_void_p]
     L.weft_render_tiles.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
-    for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_score",
+    for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_order",
+                 "weft_fit_score",
                  "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles"):
         getattr(L, name).restype = c_int
 
@@ -157,6 +159,6 @@ def tables() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
 
 EXPORTED_SYMBOLS = (
     "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_score_evals",
-    "weft_fit_init", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
+    "weft_fit_init", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
     "weft_render_tiles",
 )

@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 1
+#define WEFT_ABI_VERSION 2
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -23,10 +23,17 @@ struct WeftFitInitArgs {
     WeftFitState st;
 };
 
+struct WeftFitOrderArgs {
+    WeftFitState st;
+    WeftCands cd;
+    int32_t* order;
+};
+
 struct WeftFitScoreArgs {
     WeftFitState st;
     WeftWork wk;
     WeftCands cd;
+    const int32_t* order;
     float* out_sse;
     float* out_mse;
     double* part_obj;

@@ -30,6 +30,7 @@ struct Module {
     hipFunction_t f_init = nullptr;
     hipFunction_t f_score[4] = {nullptr, nullptr, nullptr, nullptr};
     hipFunction_t f_score_gen[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipFunction_t f_order = nullptr;
     hipFunction_t f_select = nullptr;
     hipFunction_t f_draw = nullptr;
     hipFunction_t f_render = nullptr;
@@ -106,10 +107,10 @@ int ts_slot(int ts) {
 }
 
 unsigned score_lds_bytes(int ts) {
-    // one band of rows (whole tile up to 32 px, 32-row halves of 64-px tiles):
-    // 3 float2 per pixel + sort permutation + per-wave class counts/offsets + reduction
-    const int rows = ts > 32 ? 32 : ts;
-    return (unsigned)(24 * rows * ts + 4 * WEFT_MAX_BLOCK + 4 * 16 * 8 * 2 + 8 * 16 + 4 * 16);
+    // one band of rows (whole tile up to 32 px, 16-row bands of 64-px tiles):
+    // 36 bytes per pixel (two float4 + one float) + the per-wave argmin reduction
+    const int rows = ts > 32 ? 16 : ts;
+    return (unsigned)(36 * rows * ts + 8 * 16 + 4 * 16);
 }
 
 }  // namespace
@@ -146,6 +147,7 @@ int weft_init(int device) {
         {&g_mod.f_score_gen[1], "weft_fit_score_gen_ts16"},
         {&g_mod.f_score_gen[2], "weft_fit_score_gen_ts32"},
         {&g_mod.f_score_gen[3], "weft_fit_score_gen_ts64"},
+        {&g_mod.f_order, "weft_fit_order_kernel"},
         {&g_mod.f_select, "weft_fit_select_kernel"},
         {&g_mod.f_draw, "weft_draw_kernel"},
         {&g_mod.f_render, "weft_render_tiles_kernel"},
@@ -207,12 +209,21 @@ int weft_fit_init(const WeftFitState* st, void* stream) {
     return launch(g_mod.f_init, (unsigned)((st->n_tiles + 255) / 256), 256, 0, stream, &a);
 }
 
-int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands* cands,
-                   float* d_out_sse, float* d_out_mse, double* d_part_obj, int32_t* d_part_idx,
-                   void* stream) {
+int weft_fit_order(const WeftFitState* st, const WeftCands* cands, int32_t* d_order, void* stream) {
     int rc = require_init();
     if (rc) return rc;
-    if (!st || !work || !cands || !d_part_obj || !d_part_idx)
+    if (!st || !cands || !d_order) return set_err(WEFT_E_INVALID, "weft_fit_order: null argument");
+    if (cands->n_cand < 1) return set_err(WEFT_E_INVALID, "weft_fit_order: n_cand < 1");
+    WeftFitOrderArgs a{*st, *cands, d_order};
+    return launch(g_mod.f_order, (unsigned)st->n_tiles, 256, 0, stream, &a);
+}
+
+int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands* cands,
+                   const int32_t* d_order, float* d_out_sse, float* d_out_mse, double* d_part_obj,
+                   int32_t* d_part_idx, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !work || !cands || !d_order || !d_part_obj || !d_part_idx)
         return set_err(WEFT_E_INVALID, "weft_fit_score: null argument");
     const int slot = ts_slot(work->tile_size);
     if (slot < 0) return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_fit_score: tile size must be 8, 16, 32 or 64");
@@ -220,7 +231,7 @@ int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands
         return set_err(WEFT_E_INVALID, "weft_fit_score: block must be a multiple of 64 in [64, 1024]");
     if (cands->n_cand < 1) return set_err(WEFT_E_INVALID, "weft_fit_score: n_cand < 1");
     if (work->n_work <= 0) return WEFT_OK;
-    WeftFitScoreArgs a{*st, *work, *cands, d_out_sse, d_out_mse, d_part_obj, d_part_idx};
+    WeftFitScoreArgs a{*st, *work, *cands, d_order, d_out_sse, d_out_mse, d_part_obj, d_part_idx};
     hipFunction_t f = cands->has_general ? g_mod.f_score_gen[slot] : g_mod.f_score[slot];
     return launch(f, (unsigned)work->n_work, (unsigned)work->block,
                   score_lds_bytes(work->tile_size), stream, &a);

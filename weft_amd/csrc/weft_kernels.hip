@@ -312,22 +312,32 @@ struct PxCanvasGlobal {
     }
 };
 
-// LDS band of rows [y0, y0 + band): per pixel 3 float2 = (c_r, c_g), (c_b, t_r), (t_g, t_b).
+// LDS band of rows [y0, y0 + ROWS): per pixel two float4 and one float,
+//   A = (c_r, c_g, c_b, t_r), B = (k_r, k_g, k_b, t_g), T = t_b,
+// with c the unclipped canvas, k = clip(c) (the prediction where the candidate does not
+// cover the pixel) and t the target.
+struct LdsBand {
+    const float4* A;  // [2 * pixels]: A, B interleaved
+    const float* T;   // [pixels]
+};
+
 template <int TS>
 struct PxLdsBase {
-    const float2* L;
+    LdsBand L;
     int y0;
     float ks0, ks1, ks2, om;
     __device__ __forceinline__ void finish(int y, int x, bool in, float& e0, float& e1, float& e2) {
         const int p = (y - y0) * TS + x;
-        const float2 a = L[3 * p], b = L[3 * p + 1], c = L[3 * p + 2];
-        const float w0 = ks0 + a.x * om;
-        const float w1 = ks1 + a.y * om;
-        const float w2 = ks2 + b.x * om;
-        const float v0 = clip01(in ? w0 : a.x);
-        const float v1 = clip01(in ? w1 : a.y);
-        const float v2 = clip01(in ? w2 : b.x);
-        const float d0 = b.y - v0, d1 = c.x - v1, d2 = c.y - v2;
+        const float4 a = L.A[2 * p], b = L.A[2 * p + 1];
+        const float tb = L.T[p];
+        // clip(s*a + c*(1 - a)) where covered (the clip folds into the add), clip(c) elsewhere
+        const float w0 = clip01(ks0 + a.x * om);
+        const float w1 = clip01(ks1 + a.y * om);
+        const float w2 = clip01(ks2 + a.z * om);
+        const float v0 = in ? w0 : b.x;
+        const float v1 = in ? w1 : b.y;
+        const float v2 = in ? w2 : b.z;
+        const float d0 = a.w - v0, d1 = b.w - v1, d2 = tb - v2;
         e0 = d0 * d0;
         e1 = d1 * d1;
         e2 = d2 * d2;
@@ -363,10 +373,15 @@ struct PxEllRect : PxLdsBase<TS> {
     }
 };
 
-// Triangle (render.py:114-124): a = (A dx + B dy) / D, b = (C dx + E dy) / D.
+// Triangle (render.py:114-124): a = RN(na / D), b = RN(nb / D) with
+// na = A dx + B dy, nb = C dx + E dy.  The float32 quotient is formed as
+// RN32(na * RN64(1 / D)): that float64 product is within 2^-52 (relative) of na / D while
+// a float32 quotient is never within 2^-48 of a rounding midpoint, so both round to the
+// same float -- the IEEE division's result, at the cost of one float64 multiply.
 template <int TS>
 struct PxTri : PxLdsBase<TS> {
-    float X2, Y2, A, B, C, E, D;
+    float X2, Y2, A, B, C, E;
+    double RD;
     float bdy, edy;
     __device__ __forceinline__ void row(int y) {
         const float dy = ((float)y + 0.5f) - Y2;
@@ -375,8 +390,10 @@ struct PxTri : PxLdsBase<TS> {
     }
     __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
         const float dx = ((float)x + 0.5f) - X2;
-        const float a = (A * dx + bdy) / D;
-        const float b = (C * dx + edy) / D;
+        const float na = A * dx + bdy;
+        const float nb = C * dx + edy;
+        const float a = (float)((double)na * RD);
+        const float b = (float)((double)nb * RD);
         const float c = (1.0f - a) - b;
         const bool in = (a >= 0.0f) && (b >= 0.0f) && (c >= 0.0f);
         this->finish(y, x, in, e0, e1, e2);
@@ -386,17 +403,17 @@ struct PxTri : PxLdsBase<TS> {
 // Any kind over the LDS canvas through the generic composite.
 template <int TS>
 struct PxGeneral {
-    const float2* L;
+    LdsBand L;
     int y0;
     WeftPrim P;
     float py;
     __device__ __forceinline__ void row(int y) { py = (float)y + 0.5f; }
     __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
         const int p = (y - y0) * TS + x;
-        const float2 a = L[3 * p], b = L[3 * p + 1], c = L[3 * p + 2];
-        float r = a.x, g = a.y, bb = b.x;
+        const float4 a = L.A[2 * p], b = L.A[2 * p + 1];
+        float r = a.x, g = a.y, bb = a.z;
         composite_px(P, (float)x + 0.5f, py, r, g, bb);
-        const float d0 = b.y - clip01(r), d1 = c.x - clip01(g), d2 = c.y - clip01(bb);
+        const float d0 = a.w - clip01(r), d1 = b.w - clip01(g), d2 = L.T[p] - clip01(bb);
         e0 = d0 * d0;
         e1 = d1 * d1;
         e2 = d2 * d2;
@@ -404,7 +421,7 @@ struct PxGeneral {
 };
 
 template <int TS>
-__device__ __noinline__ void score_general(const float2* L, WeftPrim P, PwState& st, int y0, int y1) {
+__device__ __noinline__ void score_general(LdsBand L, WeftPrim P, PwState& st, int y0, int y1) {
     PxGeneral<TS> px;
     px.L = L;
     px.y0 = y0;
@@ -532,19 +549,20 @@ __device__ __forceinline__ double objective(float mse, int bits, double lam, int
 // ---------------------------------------------------------------------------
 // Plain fit: score one work item (one tile, `blockDim.x` candidates).
 // ---------------------------------------------------------------------------
-// Rows of a tile held in LDS at a time: whole tiles up to 32 px (<= 24 KiB), 32-row
-// halves of 64-px tiles (48 KiB), so several workgroups fit on a CU.
-template <int TS> struct Band { static constexpr int ROWS = TS > 32 ? 32 : TS; };
+// Rows of a tile held in LDS at a time (36 bytes per pixel): whole tiles up to 32 px
+// (<= 36 KiB), 16-row bands of 64-px tiles, so several workgroups fit on a CU.
+template <int TS> struct Band { static constexpr int ROWS = TS > 32 ? 16 : TS; };
 
 // Per-candidate constants of the hard-edge classes.
 struct CandConsts {
     float ks0, ks1, ks2, om;               // s * a per channel, 1 - a
-    float g0, g1, k11, k12, k21, k22, D;   // class-specific coverage constants
+    float g0, g1, k11, k12, k21, k22;      // class-specific coverage constants
+    double rd;                             // triangles: RN64(1 / D)
 };
 
 template <int TS, bool GEN>
 __device__ __forceinline__ void score_band(int mycls, const CandConsts& c, const WeftPrim& P,
-                                           const float2* L, PwState& pw, int y0, int y1) {
+                                           LdsBand L, PwState& pw, int y0, int y1) {
     if (mycls == CL_ELL) {
         PxEllRect<TS, true> px;
         px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
@@ -558,7 +576,7 @@ __device__ __forceinline__ void score_band(int mycls, const CandConsts& c, const
     } else if (mycls == CL_TRI) {
         PxTri<TS> px;
         px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
-        px.X2 = c.g0; px.Y2 = c.g1; px.A = c.k11; px.B = c.k12; px.C = c.k21; px.E = c.k22; px.D = c.D;
+        px.X2 = c.g0; px.Y2 = c.g1; px.A = c.k11; px.B = c.k12; px.C = c.k21; px.E = c.k22; px.RD = c.rd;
         pairwise_rows<TS>(px, pw, y0, y1);
     } else if (mycls == CL_CONST) {
         PxConst<TS> px;
@@ -572,21 +590,41 @@ __device__ __forceinline__ void score_band(int mycls, const CandConsts& c, const
     }
 }
 
+// Class of candidate k of `tile` (drawn or explicit), with the empty-coverage cases
+// (degenerate triangle, non-positive radius) folded into the no-op constant.
+__device__ __forceinline__ int candidate_class(const WeftCands& cd, uint64_t dbase, int64_t cand_base,
+                                               int k) {
+    if (cd.d_prims == nullptr) {
+        const int fam = draw_family(dbase, (uint32_t)k);
+        return fam == 0 ? CL_ELL : (fam == 1 ? CL_RECT : CL_TRI);
+    }
+    const WeftPrim* pp = cd.d_prims + cand_base + k;
+    int cls = class_of_kind(pp->kind);
+    if ((cls == CL_ELL || cls == CL_RECT) && !(pp->geom[2] > 0.0f && pp->geom[3] > 0.0f))
+        cls = CL_CONST;
+    if (cls == CL_TRI) {
+        const float* g = pp->geom;
+        double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
+        double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
+        if (fabs(den) < 1e-6) cls = CL_CONST;
+    }
+    return cls;
+}
+
 // ---------------------------------------------------------------------------
-// Plain fit: score one work item (one tile, `blockDim.x` candidates).
+// Plain fit: score one work item (one tile, `blockDim.x` consecutive slots of the
+// tile's class-sorted candidate order, so waves are class-uniform except at the at
+// most few class boundaries of a tile).
 // ---------------------------------------------------------------------------
 template <int TS, bool GEN>
 __device__ void fit_score_body(const WeftFitScoreArgs& A) {
     extern __shared__ float4 weft_smem[];
     constexpr int ROWS = Band<TS>::ROWS;
-    constexpr int NPX = TS * TS;
     constexpr int BPX = ROWS * TS;  // pixels per LDS band
-    float2* L = reinterpret_cast<float2*>(weft_smem);
-    int* perm = reinterpret_cast<int*>(L + 3 * BPX);
-    int* wcnt = perm + WEFT_MAX_BLOCK;           // [16 waves][8]
-    int* woff = wcnt + 16 * 8;                   // [16 waves][8]
-    double* red_o = reinterpret_cast<double*>(woff + 16 * 8);
-    int* red_i = reinterpret_cast<int*>(red_o + 16);
+    float4* LA = weft_smem;                                        // [2 * BPX]
+    float* LT = reinterpret_cast<float*>(LA + 2 * BPX);            // [BPX]
+    double* red_o = reinterpret_cast<double*>(LT + BPX);           // [16]
+    int* red_i = reinterpret_cast<int*>(red_o + 16);               // [16]
 
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
@@ -605,56 +643,14 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
         return;
     }
 
-    // Candidate class, then a counting sort by class so waves are class-uniform.
-    const int k = c0 + tid;
-    const bool valid = k < A.cd.n_cand;
+    // This lane's candidate: slot c0 + tid of the tile's class-sorted order.
     const bool drawn = A.cd.d_prims == nullptr;
     const uint64_t dbase = draw_base(A.cd.seed, A.cd.step, (uint32_t)tile);
     const int64_t cand_base = drawn ? 0
                               : (A.cd.d_tile_cand ? A.cd.d_tile_cand[tile]
                                                   : (int64_t)tile * A.cd.n_cand);
-    int cls = CL_NONE;
-    if (valid) {
-        if (drawn) {
-            const int fam = draw_family(dbase, (uint32_t)k);
-            cls = fam == 0 ? CL_ELL : (fam == 1 ? CL_RECT : CL_TRI);
-        } else {
-            const WeftPrim* pp = A.cd.d_prims + cand_base + k;
-            cls = class_of_kind(pp->kind);
-            if ((cls == CL_ELL || cls == CL_RECT) && !(pp->geom[2] > 0.0f && pp->geom[3] > 0.0f))
-                cls = CL_CONST;  // non-positive radius: covers nothing
-            if (cls == CL_TRI) {
-                // degenerate triangles never cover a pixel: they behave as the no-op constant
-                const float* g = pp->geom;
-                double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
-                double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
-                if (fabs(den) < 1e-6) cls = CL_CONST;
-            }
-        }
-    }
-    uint64_t my_ballot = 0;
-#pragma unroll
-    for (int c = 0; c < WEFT_N_CLASSES; ++c) {
-        const uint64_t bal = __ballot(cls == c);
-        if (cls == c) my_ballot = bal;
-        if ((tid & 63) == 0) wcnt[wave * 8 + c] = __popcll(bal);
-    }
-    __syncthreads();
-    if (tid < WEFT_N_CLASSES) {
-        int off = 0;
-        for (int c = 0; c < tid; ++c)
-            for (int ww = 0; ww < nwaves; ++ww) off += wcnt[ww * 8 + c];
-        for (int ww = 0; ww < nwaves; ++ww) {
-            woff[ww * 8 + tid] = off;
-            off += wcnt[ww * 8 + tid];
-        }
-    }
-    __syncthreads();
-    perm[woff[wave * 8 + cls] + __popcll(my_ballot & lanemask_lt())] = valid ? k : -1;
-    __syncthreads();
-
-    // This lane's sorted candidate and its constants.
-    const int ks = perm[tid];
+    const int slot = c0 + tid;
+    const int ks = slot < A.cd.n_cand ? A.order[(int64_t)tile * A.cd.n_cand + slot] : -1;
     WeftPrim P;
     P.kind = -1;
     P.alpha = 0.0f;
@@ -671,7 +667,8 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
     cc.ks1 = P.color0[1] * a;
     cc.ks2 = P.color0[2] * a;
     cc.om = 1.0f - a;
-    cc.g0 = cc.g1 = cc.k11 = cc.k12 = cc.k21 = cc.k22 = cc.D = 0.0f;
+    cc.g0 = cc.g1 = cc.k11 = cc.k12 = cc.k21 = cc.k22 = 0.0f;
+    cc.rd = 0.0;
     if (mycls == CL_TRI) {
         const float* g = P.geom;
         double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
@@ -681,7 +678,8 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
         } else {
             cc.k11 = (float)(y1 - y2); cc.k12 = (float)(x2 - x1);
             cc.k21 = (float)(y2 - y0); cc.k22 = (float)(x0 - x2);
-            cc.D = (float)den; cc.g0 = (float)x2; cc.g1 = (float)y2;
+            cc.rd = 1.0 / (double)(float)den;
+            cc.g0 = (float)x2; cc.g1 = (float)y2;
         }
     }
     if (mycls == CL_ELL || mycls == CL_RECT) {
@@ -699,11 +697,12 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
         cc.om = 1.0f;
     }
 
-    // Stage the tile band by band (coalesced, 3 floats per pixel each) and score it.
+    // Stage the tile band by band (coalesced loads) and score it.
     const int64_t base = st.d_tile_pix[tile] * 3;
     const float* T = st.d_target + base;
     const float* Cv = st.d_canvas + base;
-    float* Lf = reinterpret_cast<float*>(L);
+    float* LAf = reinterpret_cast<float*>(LA);
+    const LdsBand L{LA, LT};
     PwState pw;
     pw_init(pw);
 #pragma unroll 1
@@ -712,8 +711,12 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
         const int e0 = 3 * y0 * TS;
         for (int i = tid; i < 3 * BPX; i += nthr) {
             const int p = i / 3, ch = i - 3 * (i / 3);
-            Lf[6 * p + ch] = Cv[e0 + i];
-            Lf[6 * p + 3 + ch] = T[e0 + i];
+            const float cv = Cv[e0 + i];
+            const float tv = T[e0 + i];
+            LAf[8 * p + ch] = cv;
+            LAf[8 * p + 4 + ch] = clip01(cv);
+            if (ch < 2) LAf[8 * p + 3 + 4 * ch] = tv;
+            else LT[p] = tv;
         }
         __syncthreads();
         score_band<TS, GEN>(mycls, cc, P, L, pw, y0, y0 + ROWS);
@@ -724,7 +727,7 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
     double obj = __builtin_huge_val();
     int idx = 0x7fffffff;
     if (ks >= 0) {
-        const float mse = sse / (float)(3 * NPX);
+        const float mse = sse / (float)(3 * TS * TS);
         const int64_t ti = (int64_t)tile * A.cd.n_cand + ks;
         if (A.out_sse) A.out_sse[ti] = sse;
         if (A.out_mse) A.out_mse[ti] = mse;
@@ -748,6 +751,67 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
         for (int ww = 1; ww < nwaves; ++ww) argmin_pair(bo, bi, red_o[ww], red_i[ww]);
         A.part_obj[w] = bo;
         A.part_idx[w] = bi == 0x7fffffff ? -1 : bi;
+    }
+}
+
+// Class-sorted candidate order of one tile (stable counting sort by class).
+__device__ void fit_order_body(const WeftFitOrderArgs& A) {
+    __shared__ int s_cnt[8];
+    __shared__ int s_run[8];
+    __shared__ int s_wcnt[16 * 8];
+    __shared__ int s_woff[16 * 8];
+    const int tile = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int wave = tid >> 6;
+    const int nwaves = nthr >> 6;
+    const WeftFitState& st = A.st;
+    if (st.d_tile_nsel[tile] >= st.depth) return;
+    const int n = A.cd.n_cand;
+    const bool drawn = A.cd.d_prims == nullptr;
+    const uint64_t dbase = draw_base(A.cd.seed, A.cd.step, (uint32_t)tile);
+    const int64_t cand_base = drawn ? 0
+                              : (A.cd.d_tile_cand ? A.cd.d_tile_cand[tile]
+                                                  : (int64_t)tile * A.cd.n_cand);
+    if (tid < 8) {
+        s_cnt[tid] = 0;
+        s_run[tid] = 0;
+    }
+    __syncthreads();
+    for (int k = tid; k < n; k += nthr) atomicAdd(&s_cnt[candidate_class(A.cd, dbase, cand_base, k)], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int off = 0;
+        for (int c = 0; c < WEFT_N_CLASSES; ++c) {
+            const int v = s_cnt[c];
+            s_cnt[c] = off;  // class base
+            off += v;
+        }
+    }
+    __syncthreads();
+    int32_t* out = A.order + (int64_t)tile * n;
+    for (int j0 = 0; j0 < n; j0 += nthr) {
+        const int k = j0 + tid;
+        const int cls = k < n ? candidate_class(A.cd, dbase, cand_base, k) : CL_NONE;
+        uint64_t mine = 0;
+#pragma unroll
+        for (int c = 0; c < WEFT_N_CLASSES; ++c) {
+            const uint64_t bal = __ballot(cls == c);
+            if (cls == c) mine = bal;
+            if ((tid & 63) == 0) s_wcnt[wave * 8 + c] = __popcll(bal);
+        }
+        __syncthreads();
+        if (tid < WEFT_N_CLASSES) {
+            int off = s_cnt[tid] + s_run[tid];
+            for (int ww = 0; ww < nwaves; ++ww) {
+                s_woff[ww * 8 + tid] = off;
+                off += s_wcnt[ww * 8 + tid];
+            }
+            s_run[tid] = off - s_cnt[tid];
+        }
+        __syncthreads();
+        if (k < n) out[s_woff[wave * 8 + cls] + __popcll(mine & lanemask_lt())] = k;
+        __syncthreads();
     }
 }
 
@@ -809,6 +873,8 @@ extern "C" __global__ __launch_bounds__(256) void weft_fit_init_kernel(WeftFitIn
     const float mse = sse / (float)(3 * ts * ts);
     st.d_tile_obj[t] = objective(mse, st.d_tile_bits[t], st.lam, st.obj_mode);
 }
+
+extern "C" __global__ __launch_bounds__(256) void weft_fit_order_kernel(WeftFitOrderArgs A) { fit_order_body(A); }
 
 // Fast variants: candidate kinds 0, 4, 6, 7, 8 (hard-edge coverage over the cached canvas).
 extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_ts8(WeftFitScoreArgs A) { fit_score_body<8, false>(A); }

@@ -25,9 +25,8 @@ from .quadtree import TileSet
 from .rd import tile_lambda
 from .records import PRIM_DTYPE
 
-# Workgroup size (= candidates per work item) per tile size.  64-px tiles take 96 KiB of
-# LDS, so one workgroup per CU: give it 16 waves.
-DEFAULT_BLOCK = {8: 256, 16: 256, 32: 256, 64: 1024}
+# Workgroup size (= candidate slots per work item) per tile size.
+DEFAULT_BLOCK = {8: 256, 16: 256, 32: 256, 64: 256}
 
 
 def _round64(n: int) -> int:
@@ -105,6 +104,7 @@ class PlainFitter:
         self.d_work_c0 = torch.tensor(work_c0, dtype=torch.int32, device=dev)
         self.d_tile_work0 = torch.from_numpy(tile_work0).to(dev)
         self.d_tile_nwork = torch.from_numpy(tile_nwork).to(dev)
+        self.d_order = torch.empty(T * self.n_cand, dtype=torch.int32, device=dev)
         self.d_part_obj = torch.empty(max(1, self.n_work), dtype=torch.float64, device=dev)
         self.d_part_idx = torch.empty(max(1, self.n_work), dtype=torch.int32, device=dev)
 
@@ -150,10 +150,19 @@ class PlainFitter:
         sp = _lib.stream_ptr() if stream is None else stream
         sse = self.d_sse.data_ptr() if self.d_sse is not None else None
         mse = self.d_mse.data_ptr() if self.d_mse is not None else None
+        self.order(s, sp)
         for w, start in self._works:
-            _lib.check(self.L.weft_fit_score(ctypes.byref(self._state), ctypes.byref(w), ctypes.byref(cands),
-                                             sse, mse, self.d_part_obj.data_ptr() + 8 * start,
-                                             self.d_part_idx.data_ptr() + 4 * start, sp), "weft_fit_score")
+            self.score_class(w, start, cands, sse, mse, sp)
+
+    def order(self, step: int, stream: int) -> None:
+        _lib.check(self.L.weft_fit_order(ctypes.byref(self._state), ctypes.byref(self._cands(step)),
+                                         self.d_order.data_ptr(), stream), "weft_fit_order")
+
+    def score_class(self, w, start: int, cands, sse, mse, stream: int) -> None:
+        _lib.check(self.L.weft_fit_score(ctypes.byref(self._state), ctypes.byref(w), ctypes.byref(cands),
+                                         self.d_order.data_ptr(), sse, mse,
+                                         self.d_part_obj.data_ptr() + 8 * start,
+                                         self.d_part_idx.data_ptr() + 4 * start, stream), "weft_fit_score")
 
     def select(self, step: int | None = None, stream: int | None = None) -> None:
         s = self.step_idx if step is None else step

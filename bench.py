@@ -151,9 +151,9 @@ def main():
     torch.cuda.synchronize()
     log("[bench] warmup done")
 
-    # Per-class HIP events on the launch stream for the scoring kernels.
+    # HIP events on the launch stream around every scoring launch.
     stream = torch.cuda.current_stream()
-    ev = {cl["ts"]: [] for cl in fitter.classes}
+    ev = {i: [] for i in range(len(fitter._works))}
     t0 = time.perf_counter()
     torch.cuda.synchronize()
     start_ev = torch.cuda.Event(enable_timing=True)
@@ -164,13 +164,13 @@ def main():
         s = fitter.step_idx
         cands = fitter._cands(s)
         fitter.order(s, sp)
-        for (w, start), cl in zip(fitter._works, fitter.classes):
+        for i, (w, start) in enumerate(fitter._works):
             a = torch.cuda.Event(enable_timing=True)
             b = torch.cuda.Event(enable_timing=True)
             a.record(stream)
             fitter.score_class(w, start, cands, None, None, sp)
             b.record(stream)
-            ev[cl["ts"]].append((a, b))
+            ev[i].append((a, b))
         fitter.select()
         fitter.step_idx += 1
     end_ev.record(stream)
@@ -191,10 +191,10 @@ def main():
 
     fpe = flop_per_eval_mix()
     per_class = {}
-    for cl in fitter.classes:
-        ts = cl["ts"]
-        times = [a.elapsed_time(b) * 1e-3 for a, b in ev[ts]]
-        ids = tiles.ts == ts
+    for i, (w, start) in enumerate(fitter._works):
+        ts = int(w.tile_size)
+        times = [a.elapsed_time(b) * 1e-3 for a, b in ev[i]]
+        ids = np.ones(tiles.n_tiles, bool) if ts == 0 else (tiles.ts == ts)
         ev_launch = float(np.mean([(active[s][ids] * px[ids]).sum() * c["n_cand"]
                                    for s in range(args.warmup, total_steps)]))
         per_class[ts] = dict(mean_s=float(np.mean(times)), total_s=float(np.sum(times)),
@@ -203,7 +203,7 @@ def main():
     dom = max(per_class, key=lambda k: per_class[k]["total_s"])
     d = per_class[dom]
     achieved = d["evals_per_launch"] * fpe / d["mean_s"]
-    kname = f"weft_fit_score_ts{dom}"
+    kname = "weft_fit_score_mixed" if dom == 0 else f"weft_fit_score_ts{dom}"
     roofline = {"bound": "valu", "kernel": kname, "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP32, "traffic": traffic_from_profiles(kname),
                 "flop_per_eval": fpe, "evals_per_launch": d["evals_per_launch"],

@@ -100,7 +100,8 @@ typedef struct WeftWork {
     const int32_t* d_work_tile;
     const int32_t* d_work_cand0;
     int32_t n_work;
-    int32_t tile_size;       /* 8, 16, 32 or 64: every tile in the list has this size */
+    int32_t tile_size;       /* 8, 16, 32 or 64: every tile in the list has this size;
+                                0: tiles of any supported size (one launch for all) */
     int32_t block;           /* threads per workgroup (= candidates per work item), 64..1024 */
     int32_t _pad;
 } WeftWork;

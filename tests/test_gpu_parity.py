@@ -192,7 +192,27 @@ def test_fit_mixed_tile_sizes_bit_exact(gpu, oracle_lib):
     _say(f"fit mixed: {tiles.n_tiles} tiles, sizes {sorted(sizes)}, 200 candidates, 3 steps")
     assert {8, 64} <= sizes or len(sizes) >= 3
     f, o, sses, ro = _fit_pair(oracle_lib, tiles, 200, 4, 3, 11, 1e-5,
-                               blocks={8: 128, 16: 64, 32: 192, 64: 256})
+                               blocks={8: 128, 16: 64, 32: 192, 64: 256})  # distinct sizes: one launch per class
+    rec = f.records()
+    for s in range(3):
+        assert bits_equal(sses[s], ro["sse"][s]), f"SSE table differs at step {s}"
+    assert np.array_equal(rec["win_idx"], ro["win_idx"])
+    assert bits_equal(f.canvas(), o.canvas)
+
+
+@pytest.mark.parametrize("ts", [8, 16, 32, 64])
+def test_fit_mixed_tile_sizes_one_launch(gpu, oracle_lib):
+    """All tile sizes in one merged launch (largest tiles first)."""
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.synth import synth_image
+
+    img = synth_image(136, 200, 25, seed=9)
+    qt = decompose_quadtree_variance(img, threshold=1e-3, min_tile=8, max_tile=64)
+    tiles = TileSet.from_images([img], [qt])
+    _say(f"fit mixed, one launch: {tiles.n_tiles} tiles, sizes {sorted(set(tiles.ts.tolist()))}")
+    f, o, sses, ro = _fit_pair(oracle_lib, tiles, 300, 3, 3, 2, 2e-5,
+                               blocks={8: 128, 16: 128, 32: 128, 64: 128})
+    assert f.merged and len(f._works) == 1
     rec = f.records()
     for s in range(3):
         assert bits_equal(sses[s], ro["sse"][s]), f"SSE table differs at step {s}"

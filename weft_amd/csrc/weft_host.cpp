@@ -30,6 +30,8 @@ struct Module {
     hipFunction_t f_init = nullptr;
     hipFunction_t f_score[4] = {nullptr, nullptr, nullptr, nullptr};
     hipFunction_t f_score_gen[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipFunction_t f_score_mixed = nullptr;
+    hipFunction_t f_score_gen_mixed = nullptr;
     hipFunction_t f_order = nullptr;
     hipFunction_t f_select = nullptr;
     hipFunction_t f_draw = nullptr;
@@ -107,9 +109,10 @@ int ts_slot(int ts) {
 }
 
 unsigned score_lds_bytes(int ts) {
-    // one band of rows (whole tile up to 32 px, 16-row bands of 64-px tiles):
-    // 36 bytes per pixel (two float4 + one float) + the per-wave argmin reduction
-    const int rows = ts > 32 ? 16 : ts;
+    // one band of at most 512 pixels (36 bytes each: two float4 + one float) + the
+    // per-wave argmin reduction; a mixed-size launch (ts == 0) sizes for the largest band
+    if (ts == 0) ts = 64;
+    const int rows = ts * ts <= 512 ? ts : 512 / ts;
     return (unsigned)(36 * rows * ts + 8 * 16 + 4 * 16);
 }
 
@@ -147,6 +150,8 @@ int weft_init(int device) {
         {&g_mod.f_score_gen[1], "weft_fit_score_gen_ts16"},
         {&g_mod.f_score_gen[2], "weft_fit_score_gen_ts32"},
         {&g_mod.f_score_gen[3], "weft_fit_score_gen_ts64"},
+        {&g_mod.f_score_mixed, "weft_fit_score_mixed"},
+        {&g_mod.f_score_gen_mixed, "weft_fit_score_gen_mixed"},
         {&g_mod.f_order, "weft_fit_order_kernel"},
         {&g_mod.f_select, "weft_fit_select_kernel"},
         {&g_mod.f_draw, "weft_draw_kernel"},
@@ -225,14 +230,16 @@ int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands
     if (rc) return rc;
     if (!st || !work || !cands || !d_order || !d_part_obj || !d_part_idx)
         return set_err(WEFT_E_INVALID, "weft_fit_score: null argument");
-    const int slot = ts_slot(work->tile_size);
-    if (slot < 0) return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_fit_score: tile size must be 8, 16, 32 or 64");
+    const int slot = work->tile_size == 0 ? 4 : ts_slot(work->tile_size);
+    if (slot < 0)
+        return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_fit_score: tile size must be 0 (mixed), 8, 16, 32 or 64");
     if (work->block < 64 || work->block > WEFT_MAX_BLOCK || (work->block % 64) != 0)
         return set_err(WEFT_E_INVALID, "weft_fit_score: block must be a multiple of 64 in [64, 1024]");
     if (cands->n_cand < 1) return set_err(WEFT_E_INVALID, "weft_fit_score: n_cand < 1");
     if (work->n_work <= 0) return WEFT_OK;
     WeftFitScoreArgs a{*st, *work, *cands, d_order, d_out_sse, d_out_mse, d_part_obj, d_part_idx};
-    hipFunction_t f = cands->has_general ? g_mod.f_score_gen[slot] : g_mod.f_score[slot];
+    hipFunction_t f = slot == 4 ? (cands->has_general ? g_mod.f_score_gen_mixed : g_mod.f_score_mixed)
+                                : (cands->has_general ? g_mod.f_score_gen[slot] : g_mod.f_score[slot]);
     return launch(f, (unsigned)work->n_work, (unsigned)work->block,
                   score_lds_bytes(work->tile_size), stream, &a);
 }

@@ -549,9 +549,10 @@ __device__ __forceinline__ double objective(float mse, int bits, double lam, int
 // ---------------------------------------------------------------------------
 // Plain fit: score one work item (one tile, `blockDim.x` candidates).
 // ---------------------------------------------------------------------------
-// Rows of a tile held in LDS at a time (36 bytes per pixel): whole tiles up to 32 px
-// (<= 36 KiB), 16-row bands of 64-px tiles, so several workgroups fit on a CU.
-template <int TS> struct Band { static constexpr int ROWS = TS > 32 ? 16 : TS; };
+// Rows of a tile held in LDS at a time (36 bytes per pixel): at most 512 pixels, i.e.
+// whole 8- and 16-px tiles, 16-row bands of 32-px tiles and 8-row bands of 64-px tiles
+// (<= 18 KiB), so eight 256-thread workgroups fit on a CU.
+template <int TS> struct Band { static constexpr int ROWS = TS * TS <= 512 ? TS : 512 / TS; };
 
 // Per-candidate constants of the hard-edge classes.
 struct CandConsts {
@@ -881,6 +882,20 @@ extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_ts8(WeftFitSco
 extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_ts16(WeftFitScoreArgs A) { fit_score_body<16, false>(A); }
 extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_ts32(WeftFitScoreArgs A) { fit_score_body<32, false>(A); }
 extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_ts64(WeftFitScoreArgs A) { fit_score_body<64, false>(A); }
+// All tile sizes in one launch: each work item's size comes from its tile.
+template <bool GEN>
+__device__ __forceinline__ void fit_score_mixed(const WeftFitScoreArgs& A) {
+    switch (A.st.d_tile_ts[A.wk.d_work_tile[blockIdx.x]]) {
+    case 8: fit_score_body<8, GEN>(A); break;
+    case 16: fit_score_body<16, GEN>(A); break;
+    case 32: fit_score_body<32, GEN>(A); break;
+    case 64: fit_score_body<64, GEN>(A); break;
+    default: break;
+    }
+}
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_mixed(WeftFitScoreArgs A) { fit_score_mixed<false>(A); }
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_mixed(WeftFitScoreArgs A) { fit_score_mixed<true>(A); }
+
 // Variants that also accept kinds 1-3 (linear gradient, Gaussian line, Gaussian curve).
 extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_ts8(WeftFitScoreArgs A) { fit_score_body<8, true>(A); }
 extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_ts16(WeftFitScoreArgs A) { fit_score_body<16, true>(A); }

@@ -37,7 +37,7 @@ class PlainFitter:
     def __init__(self, tiles: TileSet, n_cand: int, depth: int, seed: int = 0,
                  lam: float | None = None, obj_mode: int = 0, max_steps: int = 64,
                  blocks: dict[int, int] | None = None, cand_prims: np.ndarray | None = None,
-                 keep_sse: bool = False, device: int = 0):
+                 keep_sse: bool = False, device: int = 0, merged: bool = True):
         import torch
 
         _lib.ensure_init(device)
@@ -121,11 +121,19 @@ class PlainFitter:
             self.d_target.data_ptr(), self.d_canvas.data_ptr(), self.d_pix.data_ptr(),
             self.d_ts.data_ptr(), self.d_nsel.data_ptr(), self.d_bits.data_ptr(),
             self.d_obj.data_ptr(), T, self.depth, self.lam, self.obj_mode, 0)
+        # One launch for every size (tile_size 0; largest tiles first in the work list,
+        # which needs one block size for all classes), or one launch per size class.
+        self.merged = merged and len({c["block"] for c in self.classes}) == 1
         self._works = []
-        for c in self.classes:
-            w = _lib.Work(self.d_work_tile.data_ptr() + 4 * c["start"],
-                          self.d_work_c0.data_ptr() + 4 * c["start"], c["n"], c["ts"], c["block"], 0)
-            self._works.append((w, c["start"]))
+        if self.merged:
+            blk = self.classes[0]["block"] if self.classes else 64
+            self._works.append((_lib.Work(self.d_work_tile.data_ptr(), self.d_work_c0.data_ptr(),
+                                          self.n_work, 0, blk, 0), 0))
+        else:
+            for c in self.classes:
+                w = _lib.Work(self.d_work_tile.data_ptr() + 4 * c["start"],
+                              self.d_work_c0.data_ptr() + 4 * c["start"], c["n"], c["ts"], c["block"], 0)
+                self._works.append((w, c["start"]))
         self.L = _lib.library()
         _lib.check(self.L.weft_fit_init(ctypes.byref(self._state), _lib.stream_ptr()), "weft_fit_init")
 

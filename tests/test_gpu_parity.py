@@ -200,7 +200,6 @@ def test_fit_mixed_tile_sizes_bit_exact(gpu, oracle_lib):
     assert bits_equal(f.canvas(), o.canvas)
 
 
-@pytest.mark.parametrize("ts", [8, 16, 32, 64])
 def test_fit_mixed_tile_sizes_one_launch(gpu, oracle_lib):
     """All tile sizes in one merged launch (largest tiles first)."""
     from weft_amd.quadtree import TileSet, decompose_quadtree_variance

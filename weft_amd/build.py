@@ -40,20 +40,24 @@ def _run(cmd: list[str]) -> None:
     subprocess.run(cmd, check=True)
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
+def build(verbose: bool = True, libdir: str | None = None, defines: tuple[str, ...] = ()) -> str:
+    """Build the code object and library into `libdir` (default weft_amd/lib)."""
+    libdir = libdir or LIBDIR
+    hsaco = os.path.join(libdir, "weft_kernels.hsaco")
+    libso = os.path.join(libdir, "libweft_hip.so")
+    os.makedirs(libdir, exist_ok=True)
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
-    _run([hipcc, "--genco", f"--offload-arch={ARCH}", *DEVICE_FLAGS,
-          "-I", INCLUDE, "-I", CSRC, os.path.join(CSRC, "weft_kernels.hip"), "-o", HSACO + ".tmp"])
-    os.replace(HSACO + ".tmp", HSACO)
-    embed = os.path.join(LIBDIR, "weft_embed.S")
+    _run([hipcc, "--genco", f"--offload-arch={ARCH}", *DEVICE_FLAGS, *[f"-D{d}" for d in defines],
+          "-I", INCLUDE, "-I", CSRC, os.path.join(CSRC, "weft_kernels.hip"), "-o", hsaco + ".tmp"])
+    os.replace(hsaco + ".tmp", hsaco)
+    embed = os.path.join(libdir, "weft_embed.S")
     with open(embed, "w") as f:
         f.write(
             "    .section .rodata\n"
             "    .balign 4096\n"
             "    .global weft_hsaco_start\n"
             "weft_hsaco_start:\n"
-            f"    .incbin \"{HSACO}\"\n"
+            f"    .incbin \"{hsaco}\"\n"
             "    .global weft_hsaco_end\n"
             "weft_hsaco_end:\n"
             "    .byte 0\n"
@@ -63,10 +67,10 @@ def build(verbose: bool = True) -> str:
           "-Wall", "-Wextra", "-Wno-unused-parameter",
           "-I", os.path.join(ROCM, "include"), "-I", INCLUDE, "-I", CSRC,
           os.path.join(CSRC, "weft_host.cpp"), embed,
-          "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-o", LIBSO + ".tmp"])
-    os.replace(LIBSO + ".tmp", LIBSO)
+          "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-o", libso + ".tmp"])
+    os.replace(libso + ".tmp", libso)
     os.remove(embed)
-    return LIBSO
+    return libso
 
 
 if __name__ == "__main__":

@@ -16,6 +16,12 @@
 
 #include "weft_abi.h"
 
+// Largest tile size whose per-column candidate terms are kept in registers (2 * TS
+// VGPRs); larger tiles form them per pixel.
+#ifndef WEFT_COLPRE_MAX
+#define WEFT_COLPRE_MAX 16
+#endif
+
 // Candidate tables, filled by weft_init through hipModuleGetGlobal.
 __device__ uint16_t weft_tab_logr[WEFT_TABLE_N];
 __device__ float weft_tab_cos[WEFT_TABLE_N];
@@ -241,17 +247,32 @@ __device__ __forceinline__ void pairwise_rows(Px& px, PwState& st, int y0, int y
 #pragma unroll 1
     for (int y = y0; y < y1; ++y) {
         px.row(y);
-#pragma unroll 1
-        for (int gx = 0; gx < TS; gx += 8) {
+        if constexpr (Px::kUnrollRow) {
+            // whole row unrolled: column index x is a compile-time constant
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
+            for (int x = 0; x < TS; ++x) {
                 float e0, e1, e2;
-                px.pixel(y, gx + i, e0, e1, e2);
-                st.template feed<BLOCK>(3 * i + 0, e0, base);
-                st.template feed<BLOCK>(3 * i + 1, e1, base);
-                st.template feed<BLOCK>(3 * i + 2, e2, base);
+                px.pixel(y, x, (float)x + 0.5f, e0, e1, e2);
+                st.template feed<BLOCK>(3 * x + 0, e0, base);
+                st.template feed<BLOCK>(3 * x + 1, e1, base);
+                st.template feed<BLOCK>(3 * x + 2, e2, base);
             }
-            base += 24;
+            base += 3 * TS;
+        } else {
+            float gxf = 0.5f;  // sample x of the group's first pixel (exact)
+#pragma unroll 1
+            for (int gx = 0; gx < TS; gx += 8) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    float e0, e1, e2;
+                    px.pixel(y, gx + i, gxf + (float)i, e0, e1, e2);
+                    st.template feed<BLOCK>(3 * i + 0, e0, base);
+                    st.template feed<BLOCK>(3 * i + 1, e1, base);
+                    st.template feed<BLOCK>(3 * i + 2, e2, base);
+                }
+                base += 24;
+                gxf += 8.0f;
+            }
         }
     }
 }
@@ -277,15 +298,15 @@ __device__ __forceinline__ float pairwise_tile(Px& px) {
 
 // Reference-semantics stack walk over a zero canvas; target read from global memory.
 struct PxStack {
+    static constexpr bool kUnrollRow = false;
     const float* tgt;
     const WeftPrim* prims;
     int n;
     int ts;
     float py;
     __device__ __forceinline__ void row(int y) { py = (float)y + 0.5f; }
-    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+    __device__ __forceinline__ void pixel(int y, int x, float px, float& e0, float& e1, float& e2) {
         float r = 0.0f, g = 0.0f, b = 0.0f;
-        const float px = (float)x + 0.5f;
         for (int i = 0; i < n; ++i) composite_px_call(prims[i], px, py, r, g, b);
         const float* t = tgt + 3 * (y * ts + x);
         float d0 = t[0] - clip01(r), d1 = t[1] - clip01(g), d2 = t[2] - clip01(b);
@@ -297,11 +318,12 @@ struct PxStack {
 
 // Current canvas against the target, both in global memory (tile objective).
 struct PxCanvasGlobal {
+    static constexpr bool kUnrollRow = false;
     const float* tgt;
     const float* cv;
     int ts;
     __device__ __forceinline__ void row(int) {}
-    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+    __device__ __forceinline__ void pixel(int y, int x, float, float& e0, float& e1, float& e2) {
         const int o = 3 * (y * ts + x);
         float d0 = tgt[o] - clip01(cv[o]);
         float d1 = tgt[o + 1] - clip01(cv[o + 1]);
@@ -346,26 +368,48 @@ struct PxLdsBase {
 
 template <int TS>
 struct PxConst : PxLdsBase<TS> {
+    static constexpr bool kUnrollRow = false;
     __device__ __forceinline__ void row(int) {}
-    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+    __device__ __forceinline__ void pixel(int y, int x, float, float& e0, float& e1, float& e2) {
         this->finish(y, x, true, e0, e1, e2);
     }
 };
 
-// Ellipse / rectangle: p = ca*dx + sa*dy, q = cb*dy - sb*dx (dy terms per row).
+// Ellipse / rectangle: p = ca*dx + sa*dy, q = cb*dy - sb*dx.  The dy terms are formed
+// once per row; for tiles up to 32 px the dx terms (which do not depend on the row) are
+// formed once per candidate and kept in registers, with every row fully unrolled.
 template <int TS, bool ELL>
 struct PxEllRect : PxLdsBase<TS> {
+    static constexpr bool kUnrollRow = TS <= WEFT_COLPRE_MAX;
+    static constexpr int NCOL = kUnrollRow ? TS : 1;
     float cx, cy, ca, sa, cb, sb;
     float p2, q1;
+    float P1[NCOL], Q2[NCOL];
+    __device__ __forceinline__ void init() {
+        if constexpr (kUnrollRow) {
+#pragma unroll
+            for (int x = 0; x < TS; ++x) {
+                const float dx = ((float)x + 0.5f) - cx;
+                P1[x] = ca * dx;
+                Q2[x] = sb * dx;
+            }
+        }
+    }
     __device__ __forceinline__ void row(int y) {
         const float dy = ((float)y + 0.5f) - cy;
         p2 = sa * dy;
         q1 = cb * dy;
     }
-    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
-        const float dx = ((float)x + 0.5f) - cx;
-        const float p = ca * dx + p2;
-        const float q = q1 - sb * dx;
+    __device__ __forceinline__ void pixel(int y, int x, float pxf, float& e0, float& e1, float& e2) {
+        float p, q;
+        if constexpr (kUnrollRow) {
+            p = P1[x] + p2;
+            q = q1 - Q2[x];
+        } else {
+            const float dx = pxf - cx;
+            p = ca * dx + p2;
+            q = q1 - sb * dx;
+        }
         bool in;
         if constexpr (ELL) in = (p * p + q * q) <= 1.0f;
         else in = (fabsf(p) <= 1.0f) && (fabsf(q) <= 1.0f);
@@ -380,18 +424,37 @@ struct PxEllRect : PxLdsBase<TS> {
 // same float -- the IEEE division's result, at the cost of one float64 multiply.
 template <int TS>
 struct PxTri : PxLdsBase<TS> {
+    static constexpr bool kUnrollRow = TS <= WEFT_COLPRE_MAX;
+    static constexpr int NCOL = kUnrollRow ? TS : 1;
     float X2, Y2, A, B, C, E;
     double RD;
     float bdy, edy;
+    float AD[NCOL], CD[NCOL];
+    __device__ __forceinline__ void init() {
+        if constexpr (kUnrollRow) {
+#pragma unroll
+            for (int x = 0; x < TS; ++x) {
+                const float dx = ((float)x + 0.5f) - X2;
+                AD[x] = A * dx;
+                CD[x] = C * dx;
+            }
+        }
+    }
     __device__ __forceinline__ void row(int y) {
         const float dy = ((float)y + 0.5f) - Y2;
         bdy = B * dy;
         edy = E * dy;
     }
-    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
-        const float dx = ((float)x + 0.5f) - X2;
-        const float na = A * dx + bdy;
-        const float nb = C * dx + edy;
+    __device__ __forceinline__ void pixel(int y, int x, float pxf, float& e0, float& e1, float& e2) {
+        float na, nb;
+        if constexpr (kUnrollRow) {
+            na = AD[x] + bdy;
+            nb = CD[x] + edy;
+        } else {
+            const float dx = pxf - X2;
+            na = A * dx + bdy;
+            nb = C * dx + edy;
+        }
         const float a = (float)((double)na * RD);
         const float b = (float)((double)nb * RD);
         const float c = (1.0f - a) - b;
@@ -403,16 +466,17 @@ struct PxTri : PxLdsBase<TS> {
 // Any kind over the LDS canvas through the generic composite.
 template <int TS>
 struct PxGeneral {
+    static constexpr bool kUnrollRow = false;
     LdsBand L;
     int y0;
     WeftPrim P;
     float py;
     __device__ __forceinline__ void row(int y) { py = (float)y + 0.5f; }
-    __device__ __forceinline__ void pixel(int y, int x, float& e0, float& e1, float& e2) {
+    __device__ __forceinline__ void pixel(int y, int x, float pxf, float& e0, float& e1, float& e2) {
         const int p = (y - y0) * TS + x;
         const float4 a = L.A[2 * p], b = L.A[2 * p + 1];
         float r = a.x, g = a.y, bb = a.z;
-        composite_px(P, (float)x + 0.5f, py, r, g, bb);
+        composite_px(P, pxf, py, r, g, bb);
         const float d0 = a.w - clip01(r), d1 = b.w - clip01(g), d2 = L.T[p] - clip01(bb);
         e0 = d0 * d0;
         e1 = d1 * d1;
@@ -568,16 +632,19 @@ __device__ __forceinline__ void score_band(int mycls, const CandConsts& c, const
         PxEllRect<TS, true> px;
         px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
         px.cx = c.g0; px.cy = c.g1; px.ca = c.k11; px.sa = c.k12; px.cb = c.k21; px.sb = c.k22;
+        px.init();
         pairwise_rows<TS>(px, pw, y0, y1);
     } else if (mycls == CL_RECT) {
         PxEllRect<TS, false> px;
         px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
         px.cx = c.g0; px.cy = c.g1; px.ca = c.k11; px.sa = c.k12; px.cb = c.k21; px.sb = c.k22;
+        px.init();
         pairwise_rows<TS>(px, pw, y0, y1);
     } else if (mycls == CL_TRI) {
         PxTri<TS> px;
         px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
         px.X2 = c.g0; px.Y2 = c.g1; px.A = c.k11; px.B = c.k12; px.C = c.k21; px.E = c.k22; px.RD = c.rd;
+        px.init();
         pairwise_rows<TS>(px, pw, y0, y1);
     } else if (mycls == CL_CONST) {
         PxConst<TS> px;

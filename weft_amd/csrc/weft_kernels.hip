@@ -19,7 +19,7 @@
 // Largest tile size whose per-column candidate terms are kept in registers (2 * TS
 // VGPRs); larger tiles form them per pixel.
 #ifndef WEFT_COLPRE_MAX
-#define WEFT_COLPRE_MAX 16
+#define WEFT_COLPRE_MAX 8
 #endif
 
 // Candidate tables, filled by weft_init through hipModuleGetGlobal.
@@ -216,7 +216,9 @@ struct PwState {
     template <int BLOCK>
     __device__ __forceinline__ void feed(int off, float e, int base) {
         r[off & 7] = r[off & 7] + e;
-        if ((off & 7) == 7) {
+        // A leaf can only end after an element whose index is 7 mod 8; when the leaf size
+        // is a multiple of 24 it ends on a group boundary (index 23 mod 24).
+        if ((off & 7) == 7 && (BLOCK % 24 != 0 || off % 24 == 23)) {
             const int end = base + off + 1;
             if (end % BLOCK == 0) {
                 const float v = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));

@@ -161,12 +161,13 @@ int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands
                    int32_t* d_part_idx, void* stream);
 
 /* Plain fit, part 3: per tile reduce its work items' partial minima (work items
- * d_tile_work0[t] .. + d_tile_nwork[t] in the concatenated partial arrays), accept the
+ * d_tile_work0[t] + j * d_tile_wstride[t], j < d_tile_nwork[t], in the concatenated
+ * partial arrays; d_tile_wstride may be NULL for stride 1), accept the
  * winner when its objective beats the tile's current one, composite it onto the canvas
  * and update the state.  Records per tile: winner index, objective, accepted flag and,
  * when candidates are drawn on the device, the winner's code (d_win_code may be NULL). */
 int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_t* d_tile_work0,
-                    const int32_t* d_tile_nwork, const double* d_part_obj,
+                    const int32_t* d_tile_nwork, const int32_t* d_tile_wstride, const double* d_part_obj,
                     const int32_t* d_part_idx, int32_t* d_win_idx, double* d_win_obj,
                     int32_t* d_win_acc, WeftCode* d_win_code, void* stream);
 

@@ -193,3 +193,25 @@ def test_fitter_rejects_unsupported_tile_sizes():
 
     with pytest.raises(Exception):
         PlainFitter(ts, n_cand=64, depth=2)
+
+
+def test_work_lists_cover_every_candidate_once_and_stay_on_one_xcd():
+    from weft_amd.fit import XCDS, build_work_lists
+
+    rng = np.random.default_rng(0)
+    ts = rng.choice([8, 16, 32, 64], 203)
+    for n_cand, blocks in ((1024, {8: 256, 16: 256, 32: 256, 64: 256}), (200, {8: 128, 16: 64, 32: 192, 64: 256})):
+        wl = build_work_lists(ts, n_cand, blocks)
+        seen = set()
+        for t in range(len(ts)):
+            blk = min(blocks[int(ts[t])], max(64, (n_cand + 63) // 64 * 64))
+            for j in range(int(wl["tile_nwork"][t])):
+                w = int(wl["tile_work0"][t] + j * wl["tile_wstride"][t])
+                assert wl["work_tile"][w] == t and wl["work_c0"][w] == j * blk
+                assert w % XCDS == int(wl["tile_work0"][t]) % XCDS or wl["tile_wstride"][t] < XCDS
+                seen.add(w)
+            assert wl["tile_nwork"][t] * blk >= n_cand
+        assert seen == set(range(len(wl["work_tile"])))
+        # classes are contiguous and in descending tile size
+        sizes = [c["ts"] for c in wl["classes"]]
+        assert sizes == sorted(sizes, reverse=True)

@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -74,7 +74,8 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_fit_score.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Work), ctypes.POINTER(Cands),
                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_fit_select.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, c_void_p,
-                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p]
     L.weft_draw_codes.argtypes = [c_uint64, c_uint32, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                   c_void_p, c_void_p]
     L.weft_get_tables.argtypes = [c_void_p, c_void_p, c_void_p]

@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 2
+#define WEFT_ABI_VERSION 3
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -45,6 +45,7 @@ struct WeftFitSelectArgs {
     WeftCands cd;
     const int32_t* tile_work0;
     const int32_t* tile_nwork;
+    const int32_t* tile_wstride;
     const double* part_obj;
     const int32_t* part_idx;
     int32_t* win_idx;

@@ -245,7 +245,7 @@ int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands
 }
 
 int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_t* d_tile_work0,
-                    const int32_t* d_tile_nwork, const double* d_part_obj,
+                    const int32_t* d_tile_nwork, const int32_t* d_tile_wstride, const double* d_part_obj,
                     const int32_t* d_part_idx, int32_t* d_win_idx, double* d_win_obj,
                     int32_t* d_win_acc, WeftCode* d_win_code, void* stream) {
     int rc = require_init();
@@ -253,7 +253,7 @@ int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_
     if (!st || !cands || !d_tile_work0 || !d_tile_nwork || !d_part_obj || !d_part_idx ||
         !d_win_idx || !d_win_obj || !d_win_acc)
         return set_err(WEFT_E_INVALID, "weft_fit_select: null argument");
-    WeftFitSelectArgs a{*st, *cands, d_tile_work0, d_tile_nwork, d_part_obj, d_part_idx,
+    WeftFitSelectArgs a{*st, *cands, d_tile_work0, d_tile_nwork, d_tile_wstride, d_part_obj, d_part_idx,
                         d_win_idx, d_win_obj, d_win_acc, d_win_code};
     return launch(g_mod.f_select, (unsigned)st->n_tiles, 256, 0, stream, &a);
 }

@@ -991,7 +991,8 @@ extern "C" __global__ __launch_bounds__(256) void weft_fit_select_kernel(WeftFit
         double bo = __builtin_huge_val();
         int bi = 0x7fffffff;
         const int w0 = A.tile_work0[t], nw = A.tile_nwork[t];
-        for (int w = w0; w < w0 + nw; ++w) {
+        const int ws = A.tile_wstride ? A.tile_wstride[t] : 1;
+        for (int j = 0, w = w0; j < nw; ++j, w += ws) {
             const int i2 = A.part_idx[w];
             if (i2 >= 0) argmin_pair(bo, bi, A.part_obj[w], i2);
         }

@@ -25,12 +25,51 @@ from .quadtree import TileSet
 from .rd import tile_lambda
 from .records import PRIM_DTYPE
 
+XCDS = 8  # MI355X: 8 accelerator complex dies, workgroups dispatched round-robin
+
 # Workgroup size (= candidate slots per work item) per tile size.
 DEFAULT_BLOCK = {8: 256, 16: 256, 32: 256, 64: 256}
 
 
 def _round64(n: int) -> int:
     return max(64, (n + 63) // 64 * 64)
+
+
+def build_work_lists(tile_ts: np.ndarray, n_cand: int, blocks: dict[int, int]) -> dict:
+    """Work items (tile, first candidate slot) for the scoring launches.
+
+    Tile-size classes in descending size (the long work items start first); within a
+    class, tiles in groups of XCDS with the work items of a group interleaved, so that
+    the items of one tile are XCDS workgroups apart and run on the same XCD (workgroup i
+    is dispatched to XCD i mod 8; each XCD has its own L2, so the tile is read from HBM
+    once).  Item j of tile t is work_tile/work_c0[tile_work0[t] + j * tile_wstride[t]].
+    """
+    tile_ts = np.asarray(tile_ts)
+    T = len(tile_ts)
+    classes = []
+    work_tile, work_c0 = [], []
+    tile_work0 = np.zeros(T, np.int32)
+    tile_nwork = np.zeros(T, np.int32)
+    tile_wstride = np.ones(T, np.int32)
+    for ts in sorted(set(tile_ts.tolist()), reverse=True):
+        blk = min(blocks[ts], _round64(n_cand))
+        if blk % 64 or not 64 <= blk <= 1024:
+            raise ValueError(f"block for ts={ts} must be a multiple of 64 in [64, 1024]")
+        ids = np.nonzero(tile_ts == ts)[0].astype(np.int32)
+        per = (n_cand + blk - 1) // blk
+        start = len(work_tile)
+        for g0 in range(0, len(ids), XCDS):
+            grp = ids[g0:g0 + XCDS]
+            tile_wstride[grp] = len(grp)
+            tile_work0[grp] = len(work_tile) + np.arange(len(grp))
+            for j in range(per):
+                work_tile.extend(grp.tolist())
+                work_c0.extend([j * blk] * len(grp))
+        tile_nwork[ids] = per
+        classes.append(dict(ts=int(ts), block=blk, start=start, n=len(work_tile) - start))
+    return dict(classes=classes, work_tile=np.array(work_tile, np.int32),
+                work_c0=np.array(work_c0, np.int32), tile_work0=tile_work0, tile_nwork=tile_nwork,
+                tile_wstride=tile_wstride)
 
 
 class PlainFitter:
@@ -81,29 +120,16 @@ class PlainFitter:
         self.blocks = dict(DEFAULT_BLOCK)
         if blocks:
             self.blocks.update(blocks)
-        self.classes = []
-        work_tile, work_c0 = [], []
-        tile_work0 = np.zeros(T, np.int32)
-        tile_nwork = np.zeros(T, np.int32)
-        for ts in sorted(set(tiles.ts.tolist()), reverse=True):
-            blk = min(self.blocks[ts], _round64(self.n_cand))
-            if blk % 64 or not 64 <= blk <= 1024:
-                raise ValueError(f"block for ts={ts} must be a multiple of 64 in [64, 1024]")
-            ids = np.nonzero(tiles.ts == ts)[0].astype(np.int32)
-            per = (self.n_cand + blk - 1) // blk
-            start = len(work_tile)
-            wt = np.repeat(ids, per)
-            wc = np.tile(np.arange(per, dtype=np.int32) * blk, len(ids))
-            tile_work0[ids] = start + np.arange(len(ids), dtype=np.int32) * per
-            tile_nwork[ids] = per
-            work_tile.extend(wt.tolist())
-            work_c0.extend(wc.tolist())
-            self.classes.append(dict(ts=ts, block=blk, start=start, n=len(wt)))
+        wl = build_work_lists(tiles.ts, self.n_cand, self.blocks)
+        self.classes = wl["classes"]
+        work_tile, work_c0 = wl["work_tile"], wl["work_c0"]
+        tile_work0, tile_nwork, tile_wstride = wl["tile_work0"], wl["tile_nwork"], wl["tile_wstride"]
         self.n_work = len(work_tile)
-        self.d_work_tile = torch.tensor(work_tile, dtype=torch.int32, device=dev)
-        self.d_work_c0 = torch.tensor(work_c0, dtype=torch.int32, device=dev)
+        self.d_work_tile = torch.from_numpy(work_tile).to(dev)
+        self.d_work_c0 = torch.from_numpy(work_c0).to(dev)
         self.d_tile_work0 = torch.from_numpy(tile_work0).to(dev)
         self.d_tile_nwork = torch.from_numpy(tile_nwork).to(dev)
+        self.d_tile_wstride = torch.from_numpy(tile_wstride).to(dev)
         self.d_order = torch.empty(T * self.n_cand, dtype=torch.int32, device=dev)
         self.d_part_obj = torch.empty(max(1, self.n_work), dtype=torch.float64, device=dev)
         self.d_part_idx = torch.empty(max(1, self.n_work), dtype=torch.int32, device=dev)
@@ -180,7 +206,8 @@ class PlainFitter:
         T = self.n_tiles
         _lib.check(self.L.weft_fit_select(
             ctypes.byref(self._state), ctypes.byref(cands), self.d_tile_work0.data_ptr(),
-            self.d_tile_nwork.data_ptr(), self.d_part_obj.data_ptr(), self.d_part_idx.data_ptr(),
+            self.d_tile_nwork.data_ptr(), self.d_tile_wstride.data_ptr(), self.d_part_obj.data_ptr(),
+            self.d_part_idx.data_ptr(),
             self.d_win_idx.data_ptr() + 4 * T * slot, self.d_win_obj.data_ptr() + 8 * T * slot,
             self.d_win_acc.data_ptr() + 4 * T * slot,
             self.d_win_code.data_ptr() + CODE_DTYPE.itemsize * T * slot, sp), "weft_fit_select")

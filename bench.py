@@ -58,15 +58,17 @@ def build_workload(args):
 
 
 def traffic_from_profiles(kernel: str):
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+    """HBM bytes per launch (read + written) from profiles/traffic.txt, measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and calibrated against a device copy."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
     if not os.path.exists(path):
         return None
-    try:
-        with open(path) as f:
-            d = json.load(f)
-        return d.get(kernel, {}).get("bytes_per_launch")
-    except Exception:
-        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == kernel:
+                return float(parts[1]) + float(parts[2])
+    return None
 
 
 def cpu_baseline(tiles, c, lam, seed, budget_s, gpu_rec0):

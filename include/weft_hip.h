@@ -182,6 +182,41 @@ int weft_draw_codes(uint64_t seed, uint32_t step, const int32_t* d_tile_ids,
                     const int32_t* d_tile_ts, int n_tiles, int n_cand, WeftCode* d_codes,
                     WeftPrim* d_prims, void* stream);
 
+/* Persistent batched fit (tiles of many images in one launch).  Workgroups take tiles
+ * from a device queue (d_tile_order[0 .. n_queue)) and run each tile's whole greedy fit:
+ * n_steps rounds of drawing n_cand candidates from (seed, step0 + s, tile, index),
+ * scoring them against the tile's canvas, keeping the first minimum of the objective and
+ * accepting it when it improves the tile.  Per round and tile the winner index,
+ * objective, accepted flag and code go to d_win_*[s * n_tiles + t].  When the last
+ * tile of an image finishes, d_image_done[image] receives the device real-time counter
+ * (weft_wallclock_rate_khz), and d_t_start the earliest workgroup start.  Drawn
+ * candidates only; kinds are ellipse / rectangle / triangle.  The launch is preceded
+ * by a small reset kernel (queue, counters), so the pair can be captured in a graph. */
+typedef struct WeftBatch {
+    const int32_t* d_tile_order;
+    const int32_t* d_tile_image;
+    const int32_t* d_image_tiles;
+    int32_t* d_image_left;
+    uint64_t* d_image_done;
+    uint64_t* d_t_start;
+    int32_t* d_queue;
+    int32_t* d_win_idx;
+    double* d_win_obj;
+    int32_t* d_win_acc;
+    WeftCode* d_win_code;
+    int32_t n_queue;
+    int32_t n_images;
+    uint32_t step0;
+    int32_t n_steps;
+    int32_t grid;   /* persistent workgroups */
+    int32_t block;  /* threads per workgroup: 64, 128, 192 or 256 */
+} WeftBatch;
+
+int weft_batch_fit(const WeftFitState* st, const WeftCands* cands, const WeftBatch* b, void* stream);
+
+/* Frequency of the counter behind d_image_done / d_t_start, in kHz. */
+int weft_wallclock_rate_khz(int* khz);
+
 /* Host-side copies of the candidate tables (log-radius codes, cos, sin: 4096 each). */
 int weft_get_tables(uint16_t* logr, float* cosv, float* sinv);
 

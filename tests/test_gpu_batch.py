@@ -1,0 +1,121 @@
+"""Batched persistent fit: equal to the stepwise fitter and the oracle, in every launch shape.
+
+The batch of several images with quadtree tiles of 8-64 px runs in one persistent
+launch (weft_batch_fit); its winners, codes, canvas and tile state must be bit-identical
+to PlainFitter.step(n) and to the CPU oracle with the same seed.  Graph replays are each
+checked against the eager result.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _batch_tiles():
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.synth import synth_image
+
+    imgs = [synth_image(136, 200, 30, seed=3), synth_image(64, 72, 8, seed=4),
+            synth_image(192, 128, 20, seed=5), synth_image(40, 64, 5, seed=6)]
+    qts = [decompose_quadtree_variance(im, threshold=1e-3, min_tile=8, max_tile=64) for im in imgs]
+    return TileSet.from_images(imgs, qts)
+
+
+def _same_records(a, b, what):
+    live = b["win_idx"] >= 0
+    assert np.array_equal(a["win_idx"], b["win_idx"]), f"{what}: winners differ"
+    assert np.array_equal(a["win_acc"], b["win_acc"]), f"{what}: accept flags differ"
+    assert bits_equal(a["win_obj"][live], b["win_obj"][live]), f"{what}: objectives differ"
+    acc = b["win_acc"].astype(bool)
+    assert bits_equal(a["win_code"][acc], b["win_code"][acc].view(a["win_code"].dtype)), f"{what}: codes differ"
+
+
+def test_batch_equals_stepwise_and_oracle(gpu, oracle_lib):
+    from oracle.coracle import PlainFit
+    from weft_amd.batch import BatchFitter
+    from weft_amd.fit import PlainFitter
+
+    tiles = _batch_tiles()
+    n_cand, depth, steps, seed, lam = 300, 4, 6, 21, 1e-5
+    _say(f"batch: {tiles.n_tiles} tiles of 4 images, sizes {sorted(set(tiles.ts.tolist()))}, "
+         f"{n_cand} candidates, depth {depth}, {steps} steps")
+    bf = BatchFitter(tiles, n_cand=n_cand, depth=depth, n_steps=steps, seed=seed, lam=lam, block=256)
+    bf.run(use_graph=False)
+    rb = bf.records()
+    pf = PlainFitter(tiles, n_cand=n_cand, depth=depth, seed=seed, lam=lam, max_steps=steps)
+    pf.step(steps)
+    rp = pf.records()
+    _same_records(rb, rp, "batch vs stepwise")
+    assert bits_equal(bf.canvas(), pf.canvas())
+    sb, sp = bf.state(), pf.state()
+    assert np.array_equal(sb["nsel"], sp["nsel"]) and np.array_equal(sb["bits"], sp["bits"])
+    assert bits_equal(sb["obj"], sp["obj"])
+    assert (sb["nsel"] == depth).any(), "some tiles reach the stack depth"
+
+    o = PlainFit(tiles.target, tiles.pix_off, tiles.ts, n_cand, depth, seed, lam)
+    ro = o.run(steps, want_sse=False)
+    _same_records(rb, ro, "batch vs oracle")
+    assert bits_equal(bf.canvas(), o.canvas)
+
+    lat = bf.image_latency_s()
+    assert lat.shape == (4,) and (lat > 0).all() and (lat < 60).all()
+    assert (bf.d_image_left.cpu().numpy() == 0).all()
+
+
+@pytest.mark.parametrize("order", ["sjf", "fifo", "lpt"])
+def test_batch_launch_shapes_agree(gpu, order):
+    """Every queue order, workgroup size and grid gives the same fit."""
+    from weft_amd.batch import BATCH_BLOCKS, BatchFitter
+
+    tiles = _batch_tiles()
+    bf = BatchFitter(tiles, n_cand=130, depth=3, n_steps=4, seed=5, lam=2e-5, order=order, block=256)
+    bf.run(use_graph=False)
+    ref, canvas = bf.records(), bf.canvas()
+    for block in BATCH_BLOCKS:
+        for grid in (1, 7, None):
+            _say(f"batch order {order} block {block} grid {grid}")
+            bf.configure(block=block, grid=grid)
+            bf.run(use_graph=False)
+            _same_records(bf.records(), ref, f"{order}/{block}/{grid}")
+            assert bits_equal(bf.canvas(), canvas)
+
+
+def test_batch_graph_replays(gpu):
+    """The captured fit replays to the eager result, with eager work between replays."""
+    import torch
+    from weft_amd.batch import BatchFitter
+
+    tiles = _batch_tiles()
+    bf = BatchFitter(tiles, n_cand=200, depth=4, n_steps=5, seed=9, lam=1e-5)
+    bf.run(use_graph=False)
+    ref, canvas = bf.records(), bf.canvas()
+    for r in range(3):
+        _say(f"graph replay {r}")
+        bf.d_canvas.fill_(0.5)  # eager work the replay must overwrite
+        bf.run(use_graph=True)
+        torch.cuda.synchronize()
+        _same_records(bf.records(), ref, f"replay {r}")
+        assert bits_equal(bf.canvas(), canvas)
+        x = torch.arange(1 << 16, device="cuda", dtype=torch.float32).sum()
+        assert float(x) > 0
+
+
+def test_batch_rejects_bad_launch(gpu):
+    from weft_amd import _lib
+    from weft_amd.batch import BatchFitter
+
+    tiles = _batch_tiles()
+    bf = BatchFitter(tiles, n_cand=64, depth=2, n_steps=1)
+    with pytest.raises(ValueError):
+        bf.configure(block=96)
+    bf.block = 96
+    with pytest.raises(_lib.WeftHipError):
+        bf.run(use_graph=False)

@@ -215,3 +215,37 @@ def test_work_lists_cover_every_candidate_once_and_stay_on_one_xcd():
         # classes are contiguous and in descending tile size
         sizes = [c["ts"] for c in wl["classes"]]
         assert sizes == sorted(sizes, reverse=True)
+
+
+def test_batch_queue_orders():
+    """Device queue orders: permutations with the documented grouping."""
+    from weft_amd.batch import ORDERS, queue_order
+    from weft_amd.quadtree import TileSet
+
+    rng = np.random.default_rng(1)
+    T = 500
+    ts = rng.choice([8, 16, 32, 64], T)
+    img = rng.integers(0, 7, T)
+    tiles = TileSet(np.zeros(T), np.zeros(T), ts, img, np.zeros(0, np.float32), np.zeros(T, np.int64))
+    for kind in ORDERS:
+        q = queue_order(tiles, kind)
+        assert q.dtype == np.int32 and np.array_equal(np.sort(q), np.arange(T))
+    lpt = queue_order(tiles, "lpt")
+    assert (np.diff(ts[lpt]) <= 0).all()
+    for kind in ("sjf", "fifo"):
+        q = queue_order(tiles, kind)
+        im = img[q]
+        # images are contiguous runs, tiles largest first within an image, ties by index
+        assert len(np.unique(im[np.r_[True, im[1:] != im[:-1]]])) == len(np.unique(img))
+        for k in np.unique(img):
+            run = q[im == k]
+            key = list(zip(-ts[run], run))
+            assert key == sorted(key)
+    work = np.bincount(img, weights=ts.astype(np.int64) ** 2)
+    seen = img[queue_order(tiles, "sjf")]
+    firsts = seen[np.r_[True, seen[1:] != seen[:-1]]]
+    assert (np.diff(work[firsts]) >= 0).all()
+    assert list(img[queue_order(tiles, "fifo")][np.r_[True, np.diff(img[queue_order(tiles, "fifo")]) != 0]]) \
+        == sorted(np.unique(img).tolist())
+    empty = TileSet([], [], [], [], np.zeros(0, np.float32), np.zeros(0, np.int64))
+    assert queue_order(empty, "sjf").size == 0

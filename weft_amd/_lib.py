@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -56,6 +56,17 @@ class Cands(ctypes.Structure):
     ]
 
 
+class Batch(ctypes.Structure):
+    _fields_ = [
+        ("d_tile_order", c_void_p), ("d_tile_image", c_void_p), ("d_image_tiles", c_void_p),
+        ("d_image_left", c_void_p), ("d_image_done", c_void_p), ("d_t_start", c_void_p),
+        ("d_queue", c_void_p), ("d_win_idx", c_void_p), ("d_win_obj", c_void_p), ("d_win_acc", c_void_p),
+        ("d_win_code", c_void_p), ("n_queue", c_int32), ("n_images", c_int32), ("step0", c_uint32),
+        ("n_steps", c_int32), ("grid", c_int32), ("block", c_int32),
+    ]
+
+
+assert ctypes.sizeof(Batch) == 112
 assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 32 and ctypes.sizeof(Cands) == 40
 
 _lock = threading.Lock()
@@ -79,10 +90,13 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_draw_codes.argtypes = [c_uint64, c_uint32, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                   c_void_p, c_void_p]
     L.weft_get_tables.argtypes = [c_void_p, c_void_p, c_void_p]
+    L.weft_batch_fit.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), ctypes.POINTER(Batch), c_void_p]
+    L.weft_wallclock_rate_khz.argtypes = [ctypes.POINTER(c_int)]
     L.weft_render_tiles.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
     for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_order",
                  "weft_fit_score",
-                 "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles"):
+                 "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles",
+                 "weft_batch_fit", "weft_wallclock_rate_khz"):
         getattr(L, name).restype = c_int
 
 
@@ -161,5 +175,5 @@ def tables() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
 EXPORTED_SYMBOLS = (
     "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_score_evals",
     "weft_fit_init", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
-    "weft_render_tiles",
+    "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz",
 )

@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 3
+#define WEFT_ABI_VERSION 4
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -74,4 +74,24 @@ struct WeftRenderArgs {
     int32_t width;
     int32_t height;
     int32_t _pad;
+};
+
+struct WeftBatchArgs {
+    WeftFitState st;
+    WeftCands cd;
+    const int32_t* tile_order;   // queue: tile indices in processing order
+    const int32_t* tile_image;   // image of each tile
+    const int32_t* image_tiles;  // tiles per image (reset value of image_left)
+    int32_t* image_left;         // tiles of the image not yet finished
+    unsigned long long* image_done;  // device real-time stamp when the image's last tile finished
+    unsigned long long* t_start;     // earliest workgroup start stamp
+    int32_t* queue;              // next queue position
+    int32_t* win_idx;            // [n_steps][n_tiles]
+    double* win_obj;
+    int32_t* win_acc;
+    WeftCode* win_code;
+    int32_t n_queue;
+    int32_t n_images;
+    uint32_t step0;
+    int32_t n_steps;
 };

@@ -35,6 +35,8 @@ struct Module {
     hipFunction_t f_order = nullptr;
     hipFunction_t f_select = nullptr;
     hipFunction_t f_draw = nullptr;
+    hipFunction_t f_batch = nullptr;
+    hipFunction_t f_batch_reset = nullptr;
     hipFunction_t f_render = nullptr;
 };
 
@@ -155,6 +157,8 @@ int weft_init(int device) {
         {&g_mod.f_order, "weft_fit_order_kernel"},
         {&g_mod.f_select, "weft_fit_select_kernel"},
         {&g_mod.f_draw, "weft_draw_kernel"},
+        {&g_mod.f_batch, "weft_batch_fit_kernel"},
+        {&g_mod.f_batch_reset, "weft_batch_reset_kernel"},
         {&g_mod.f_render, "weft_render_tiles_kernel"},
     };
     for (auto& fn : fns) {
@@ -270,6 +274,44 @@ int weft_draw_codes(uint64_t seed, uint32_t step, const int32_t* d_tile_ids,
     if (total > (1ll << 31) * 255ll) return set_err(WEFT_E_TOO_LARGE, "weft_draw_codes: too many");
     WeftDrawArgs a{d_tile_ids, d_tile_ts, d_codes, d_prims, seed, step, n_tiles, n_cand, 0};
     return launch(g_mod.f_draw, (unsigned)((total + 255) / 256), 256, 0, stream, &a);
+}
+
+int weft_batch_fit(const WeftFitState* st, const WeftCands* cands, const WeftBatch* b, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !cands || !b || !b->d_tile_order || !b->d_queue || !b->d_win_idx || !b->d_win_obj ||
+        !b->d_win_acc)
+        return set_err(WEFT_E_INVALID, "weft_batch_fit: null argument");
+    if (cands->d_prims) return set_err(WEFT_E_INVALID, "weft_batch_fit: drawn candidates only");
+    if (cands->n_cand < 1 || cands->n_cand > 8192)
+        return set_err(WEFT_E_INVALID, "weft_batch_fit: n_cand must be in [1, 8192]");
+    if (b->block < 64 || b->block > 256 || b->block % 64)
+        return set_err(WEFT_E_INVALID, "weft_batch_fit: block must be 64, 128, 192 or 256");
+    if (b->grid < 1 || b->n_queue < 0 || b->n_steps < 0 || b->n_images < 0)
+        return set_err(WEFT_E_INVALID, "weft_batch_fit: bad sizes");
+    if (b->n_images > 0 && (!b->d_tile_image || !b->d_image_tiles || !b->d_image_left || !b->d_image_done))
+        return set_err(WEFT_E_INVALID, "weft_batch_fit: image counters missing");
+    WeftBatchArgs a{*st, *cands, b->d_tile_order, b->d_tile_image, b->d_image_tiles, b->d_image_left,
+                    reinterpret_cast<unsigned long long*>(b->d_image_done),
+                    reinterpret_cast<unsigned long long*>(b->d_t_start), b->d_queue, b->d_win_idx,
+                    b->d_win_obj, b->d_win_acc, b->d_win_code, b->n_queue,
+                    b->n_images > 0 ? b->n_images : 0, b->step0, b->n_steps};
+    if (b->n_images == 0) {
+        a.image_left = nullptr;
+    }
+    const int reset_n = b->n_images > 1 ? b->n_images : 1;
+    rc = launch(g_mod.f_batch_reset, (unsigned)((reset_n + 255) / 256), 256, 0, stream, &a);
+    if (rc) return rc;
+    const unsigned lds = (unsigned)(36 * 512 + 4 * cands->n_cand);
+    return launch(g_mod.f_batch, (unsigned)b->grid, (unsigned)b->block, lds, stream, &a);
+}
+
+int weft_wallclock_rate_khz(int* khz) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!khz) return set_err(WEFT_E_INVALID, "weft_wallclock_rate_khz: null");
+    WEFT_HIP(hipDeviceGetAttribute(khz, hipDeviceAttributeWallClockRate, g_mod.device), "wall clock rate");
+    return WEFT_OK;
 }
 
 int weft_render_tiles(const WeftTileDecodeDesc* d_descs, int n_tiles, const WeftPrim* d_prims,

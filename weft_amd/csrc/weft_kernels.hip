@@ -627,6 +627,46 @@ struct CandConsts {
     double rd;                             // triangles: RN64(1 / D)
 };
 
+// Constants of candidate P (class `cls`) for the hard-edge scorers; returns the class
+// with the empty-coverage cases (degenerate triangle, non-positive radius, skipped
+// kind) folded into the no-op constant, whose canvas passes through unchanged.
+__device__ __forceinline__ int cand_consts(const WeftPrim& P, int cls, CandConsts& cc) {
+    const float a = clip01(P.alpha);
+    cc.ks0 = P.color0[0] * a;
+    cc.ks1 = P.color0[1] * a;
+    cc.ks2 = P.color0[2] * a;
+    cc.om = 1.0f - a;
+    cc.g0 = cc.g1 = cc.k11 = cc.k12 = cc.k21 = cc.k22 = 0.0f;
+    cc.rd = 0.0;
+    if (cls == CL_TRI) {
+        const float* g = P.geom;
+        double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
+        double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
+        if (fabs(den) < 1e-6) {
+            cls = CL_CONST;
+        } else {
+            cc.k11 = (float)(y1 - y2); cc.k12 = (float)(x2 - x1);
+            cc.k21 = (float)(y2 - y0); cc.k22 = (float)(x0 - x2);
+            cc.rd = 1.0 / (double)(float)den;
+            cc.g0 = (float)x2; cc.g1 = (float)y2;
+        }
+    }
+    if (cls == CL_ELL || cls == CL_RECT) {
+        if (!(P.geom[2] > 0.0f && P.geom[3] > 0.0f)) {
+            cls = CL_CONST;
+        } else {
+            cc.g0 = P.geom[0]; cc.g1 = P.geom[1];
+            cc.k11 = P.geom[4] / P.geom[2]; cc.k12 = P.geom[5] / P.geom[2];
+            cc.k21 = P.geom[4] / P.geom[3]; cc.k22 = P.geom[5] / P.geom[3];
+        }
+    }
+    if (cls == CL_CONST && P.kind != 0) {
+        cc.ks0 = cc.ks1 = cc.ks2 = 0.0f;
+        cc.om = 1.0f;
+    }
+    return cls;
+}
+
 template <int TS, bool GEN>
 __device__ __forceinline__ void score_band(int mycls, const CandConsts& c, const WeftPrim& P,
                                            LdsBand L, PwState& pw, int y0, int y1) {
@@ -732,40 +772,7 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
         mycls = class_of_kind(P.kind);
     }
     CandConsts cc;
-    const float a = clip01(P.alpha);
-    cc.ks0 = P.color0[0] * a;
-    cc.ks1 = P.color0[1] * a;
-    cc.ks2 = P.color0[2] * a;
-    cc.om = 1.0f - a;
-    cc.g0 = cc.g1 = cc.k11 = cc.k12 = cc.k21 = cc.k22 = 0.0f;
-    cc.rd = 0.0;
-    if (mycls == CL_TRI) {
-        const float* g = P.geom;
-        double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
-        double den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2);
-        if (fabs(den) < 1e-6) {
-            mycls = CL_CONST;
-        } else {
-            cc.k11 = (float)(y1 - y2); cc.k12 = (float)(x2 - x1);
-            cc.k21 = (float)(y2 - y0); cc.k22 = (float)(x0 - x2);
-            cc.rd = 1.0 / (double)(float)den;
-            cc.g0 = (float)x2; cc.g1 = (float)y2;
-        }
-    }
-    if (mycls == CL_ELL || mycls == CL_RECT) {
-        if (!(P.geom[2] > 0.0f && P.geom[3] > 0.0f)) {
-            mycls = CL_CONST;
-        } else {
-            cc.g0 = P.geom[0]; cc.g1 = P.geom[1];
-            cc.k11 = P.geom[4] / P.geom[2]; cc.k12 = P.geom[5] / P.geom[2];
-            cc.k21 = P.geom[4] / P.geom[3]; cc.k22 = P.geom[5] / P.geom[3];
-        }
-    }
-    if (mycls == CL_CONST && P.kind != 0) {
-        // skipped kind, degenerate triangle or empty ellipse: the canvas passes through
-        cc.ks0 = cc.ks1 = cc.ks2 = 0.0f;
-        cc.om = 1.0f;
-    }
+    mycls = cand_consts(P, mycls, cc);
 
     // Stage the tile band by band (coalesced loads) and score it.
     const int64_t base = st.d_tile_pix[tile] * 3;
@@ -895,6 +902,232 @@ template <int TS>
 __device__ float stack_sse(const float* tgt, const WeftPrim* prims, int n) {
     PxStack px{tgt, prims, n, TS, 0.0f};
     return pairwise_tile<TS>(px);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent batched fit: workgroups take tiles from a queue and run each tile's whole
+// greedy fit (n_steps rounds of: class-sort the tile's candidates, score them all,
+// first minimum, accept, composite) before taking the next tile.  Candidates are drawn
+// from (seed, step0 + s, tile, index) exactly as the stepwise kernels draw them, so the
+// result equals that of weft_fit_order/score/select run n_steps times.
+// ---------------------------------------------------------------------------
+struct BatchShared {
+    int cnt[8];
+    int run[8];
+    int wcnt[16 * 8];
+    int woff[16 * 8];
+    double red_o[16];
+    int red_i[16];
+    int tile;
+    int acc;
+    int best;
+    double best_obj;
+};
+
+// Stable counting sort of the tile's n candidates by class into perm[0, n).
+__device__ void batch_sort(const WeftCands& cd, uint64_t dbase, int n, int* perm, BatchShared& sh) {
+    const int tid = threadIdx.x, nthr = blockDim.x, wave = tid >> 6, nwaves = nthr >> 6;
+    if (tid < 8) {
+        sh.cnt[tid] = 0;
+        sh.run[tid] = 0;
+    }
+    __syncthreads();
+    for (int k = tid; k < n; k += nthr) atomicAdd(&sh.cnt[candidate_class(cd, dbase, 0, k)], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int off = 0;
+        for (int c = 0; c < WEFT_N_CLASSES; ++c) {
+            const int v = sh.cnt[c];
+            sh.cnt[c] = off;
+            off += v;
+        }
+    }
+    __syncthreads();
+    for (int j0 = 0; j0 < n; j0 += nthr) {
+        const int k = j0 + tid;
+        const int cls = k < n ? candidate_class(cd, dbase, 0, k) : CL_NONE;
+        uint64_t mine = 0;
+#pragma unroll
+        for (int c = 0; c < WEFT_N_CLASSES; ++c) {
+            const uint64_t bal = __ballot(cls == c);
+            if (cls == c) mine = bal;
+            if ((tid & 63) == 0) sh.wcnt[wave * 8 + c] = __popcll(bal);
+        }
+        __syncthreads();
+        if (tid < WEFT_N_CLASSES) {
+            int off = sh.cnt[tid] + sh.run[tid];
+            for (int ww = 0; ww < nwaves; ++ww) {
+                sh.woff[ww * 8 + tid] = off;
+                off += sh.wcnt[ww * 8 + tid];
+            }
+            sh.run[tid] = off - sh.cnt[tid];
+        }
+        __syncthreads();
+        if (k < n) perm[sh.woff[wave * 8 + cls] + __popcll(mine & lanemask_lt())] = k;
+        __syncthreads();
+    }
+}
+
+template <int TS>
+__device__ void batch_fit_tile(const WeftBatchArgs& A, int t, float4* LA, float* LT, int* perm,
+                               BatchShared& sh) {
+    constexpr int ROWS = Band<TS>::ROWS;
+    constexpr int NB = TS / ROWS;
+    constexpr int BPX = ROWS * TS;
+    const WeftFitState& st = A.st;
+    const int tid = threadIdx.x, nthr = blockDim.x, wave = tid >> 6, nwaves = nthr >> 6;
+    const int64_t base = st.d_tile_pix[t] * 3;
+    float* cvp = st.d_canvas + base;
+    const float* T = st.d_target + base;
+    const int n = A.cd.n_cand;
+    float* LAf = reinterpret_cast<float*>(LA);
+    const LdsBand L{LA, LT};
+    const int64_t T_all = st.n_tiles;
+#pragma unroll 1
+    for (int s = 0; s < A.n_steps; ++s) {
+        const int64_t rec = (int64_t)s * T_all + t;
+        if (st.d_tile_nsel[t] >= st.depth) {  // uniform: written by this workgroup only
+            if (tid == 0) {
+                A.win_idx[rec] = -1;
+                A.win_obj[rec] = __builtin_huge_val();
+                A.win_acc[rec] = 0;
+            }
+            continue;
+        }
+        const uint32_t step = A.step0 + (uint32_t)s;
+        const uint64_t dbase = draw_base(A.cd.seed, step, (uint32_t)t);
+        batch_sort(A.cd, dbase, n, perm, sh);
+        double bo = __builtin_huge_val();
+        int bi = 0x7fffffff;
+        bool staged = false;
+#pragma unroll 1
+        for (int c0 = 0; c0 < n; c0 += nthr) {
+            const int slot = c0 + tid;
+            const int ks = slot < n ? perm[slot] : -1;
+            WeftPrim P;
+            P.kind = -1;
+            P.alpha = 0.0f;
+            P.color0[0] = P.color0[1] = P.color0[2] = 0.0f;
+            int mycls = CL_NONE;
+            if (ks >= 0) {
+                P = code_to_prim(draw_code(dbase, (uint32_t)ks, TS));
+                mycls = class_of_kind(P.kind);
+            }
+            CandConsts cc;
+            mycls = cand_consts(P, mycls, cc);
+            PwState pw;
+            pw_init(pw);
+#pragma unroll 1
+            for (int y0 = 0; y0 < TS; y0 += ROWS) {
+                if (NB > 1 || !staged) {
+                    __syncthreads();
+                    const int e0 = 3 * y0 * TS;
+                    for (int i = tid; i < 3 * BPX; i += nthr) {
+                        const int p = i / 3, ch = i - 3 * (i / 3);
+                        const float cv = cvp[e0 + i];
+                        const float tv = T[e0 + i];
+                        LAf[8 * p + ch] = cv;
+                        LAf[8 * p + 4 + ch] = clip01(cv);
+                        if (ch < 2) LAf[8 * p + 3 + 4 * ch] = tv;
+                        else LT[p] = tv;
+                    }
+                    __syncthreads();
+                    staged = true;
+                }
+                score_band<TS, false>(mycls, cc, P, L, pw, y0, y0 + ROWS);
+            }
+            if (ks >= 0) {
+                const float sse = pw.template result<TS>();
+                const float mse = sse / (float)(3 * TS * TS);
+                argmin_pair(bo, bi, objective(mse, st.d_tile_bits[t] + prim_bytes(P.kind) * 8, st.lam,
+                                              st.obj_mode), ks);
+            }
+        }
+        // first minimum over the workgroup
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double o2 = __shfl_xor(bo, off, 64);
+            const int i2 = __shfl_xor(bi, off, 64);
+            argmin_pair(bo, bi, o2, i2);
+        }
+        if ((tid & 63) == 0) {
+            sh.red_o[wave] = bo;
+            sh.red_i[wave] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double o = sh.red_o[0];
+            int i = sh.red_i[0];
+            for (int ww = 1; ww < nwaves; ++ww) argmin_pair(o, i, sh.red_o[ww], sh.red_i[ww]);
+            const bool found = i != 0x7fffffff;
+            sh.best = found ? i : -1;
+            sh.best_obj = o;
+            sh.acc = found && (o < st.d_tile_obj[t]);
+        }
+        __syncthreads();
+        const int win = sh.best;
+        const int acc = sh.acc;
+        if (acc) {
+            const WeftCode code = draw_code(dbase, (uint32_t)win, TS);
+            const WeftPrim Pw = code_to_prim(code);
+            for (int p = tid; p < TS * TS; p += nthr) {
+                const int y = p / TS, x = p - y * TS;
+                float r = cvp[3 * p], g = cvp[3 * p + 1], b = cvp[3 * p + 2];
+                composite_px(Pw, (float)x + 0.5f, (float)y + 0.5f, r, g, b);
+                cvp[3 * p] = r;
+                cvp[3 * p + 1] = g;
+                cvp[3 * p + 2] = b;
+            }
+            if (tid == 0) {
+                if (A.win_code) A.win_code[rec] = code;
+                st.d_tile_nsel[t] += 1;
+                st.d_tile_bits[t] += prim_bytes(Pw.kind) * 8;
+                st.d_tile_obj[t] = sh.best_obj;
+            }
+        }
+        if (tid == 0) {
+            A.win_idx[rec] = win;
+            A.win_obj[rec] = sh.best_obj;
+            A.win_acc[rec] = acc;
+        }
+        __threadfence_block();
+        __syncthreads();  // canvas and state are settled before the next round reads them
+    }
+}
+
+__device__ void batch_fit_body(const WeftBatchArgs& A) {
+    extern __shared__ float4 weft_smem[];
+    __shared__ BatchShared sh;
+    float4* LA = weft_smem;                                   // [2 * 512]
+    float* LT = reinterpret_cast<float*>(LA + 2 * 512);       // [512]
+    int* perm = reinterpret_cast<int*>(LT + 512);             // [n_cand]
+    const int tid = threadIdx.x;
+    if (tid == 0 && A.t_start) {
+        atomicMin(A.t_start, (unsigned long long)__builtin_amdgcn_s_memrealtime());
+    }
+#pragma unroll 1
+    for (;;) {
+        if (tid == 0) {
+            const int q = atomicAdd(A.queue, 1);
+            sh.tile = q < A.n_queue ? A.tile_order[q] : -1;
+        }
+        __syncthreads();
+        const int t = sh.tile;
+        __syncthreads();  // every thread has read sh.tile before thread 0 may overwrite it
+        if (t < 0) break;
+        switch (A.st.d_tile_ts[t]) {
+        case 8: batch_fit_tile<8>(A, t, LA, LT, perm, sh); break;
+        case 16: batch_fit_tile<16>(A, t, LA, LT, perm, sh); break;
+        case 32: batch_fit_tile<32>(A, t, LA, LT, perm, sh); break;
+        case 64: batch_fit_tile<64>(A, t, LA, LT, perm, sh); break;
+        default: break;
+        }
+        if (tid == 0 && A.image_left) {
+            const int img = A.tile_image[t];
+            if (atomicSub(&A.image_left[img], 1) == 1)
+                A.image_done[img] = (unsigned long long)__builtin_amdgcn_s_memrealtime();
+        }
+    }
 }
 
 }  // namespace
@@ -1035,6 +1268,21 @@ extern "C" __global__ __launch_bounds__(256) void weft_fit_select_kernel(WeftFit
             st.d_tile_bits[t] += prim_bytes(P.kind) * 8;
             st.d_tile_obj[t] = s_obj;
         }
+    }
+}
+
+extern "C" __global__ __launch_bounds__(256) void weft_batch_fit_kernel(WeftBatchArgs A) { batch_fit_body(A); }
+
+// Resets the queue counter, start time and per-image counters of a batched fit.
+extern "C" __global__ __launch_bounds__(256) void weft_batch_reset_kernel(WeftBatchArgs A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        A.queue[0] = 0;
+        if (A.t_start) A.t_start[0] = ~0ull;
+    }
+    if (A.image_left && i < A.n_images) {
+        A.image_left[i] = A.image_tiles[i];
+        A.image_done[i] = 0ull;
     }
 }
 

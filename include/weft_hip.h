@@ -217,6 +217,53 @@ int weft_batch_fit(const WeftFitState* st, const WeftCands* cands, const WeftBat
 /* Frequency of the counter behind d_image_done / d_t_start, in kHz. */
 int weft_wallclock_rate_khz(int* khz);
 
+/* ---- Differentiable refinement (no reference counterpart) -----------------------
+ * Candidates of the three drawn families as continuous parameters, 9 floats each:
+ * (cx, cy, rx, ry, theta, r, g, b, a), with the hard coverage replaced by a soft one
+ * of edge width w pixels: ellipse sigma((1 - rho) * sqrt(rx*ry) / w) with rho the
+ * elliptic radius; rectangle and triangle the product of sigma(d_i / w) over their
+ * edges' signed distances d_i.  The candidate is composited over the tile's canvas
+ * with alpha a * coverage, clipped to [0, 1] and compared with the target.  As w -> 0
+ * the soft SSE tends to the hard-edge SSE of the scoring kernels. */
+typedef struct WeftSoftBatch {
+    const int32_t* d_group_tile;  /* tile of each group of k candidates */
+    const int32_t* d_kinds;       /* [n_groups * k]: 6 ellipse, 7 rectangle, 8 triangle */
+    const float* d_params;        /* [n_groups * k][9] */
+    float* d_out_sse;             /* [n_groups * k] soft SSE */
+    float* d_out_grad;            /* [n_groups * k][9] d SSE / d params, or NULL */
+    int32_t n_groups;
+    int32_t k;                    /* candidates per group: 1, 2, 4, ..., 64 */
+    float w;                      /* edge width in pixels (> 0) */
+    int32_t _pad;
+} WeftSoftBatch;
+
+int weft_soft_eval(const WeftFitState* st, const WeftSoftBatch* b, void* stream);
+
+/* Per tile, the k smallest values of d_table[t * n .. t * n + n) as indices, ascending
+ * by (value, index), i.e. the first k of a stable sort (NaN last).  Tiles whose stack
+ * is full (d_tile_nsel >= depth) are skipped.  n <= 8192, k <= min(n, 256). */
+int weft_topk(const WeftFitState* st, const float* d_table, int n, int k, int32_t* d_out_idx,
+              void* stream);
+
+/* Adam refinement of each tile's top-k drawn candidates (d_topk[t * k + j], drawn
+ * from cands' seed and step), n_iter steps on the soft SSE.  Step i uses edge width
+ * d_sched[4 i], bias corrections d_sched[4 i + 1] = 1 - beta1^(i+1) and
+ * d_sched[4 i + 2] = 1 - beta2^(i+1).  After each step the parameters are projected
+ * onto the code ranges.  Outputs per tile k + 1 slots: slot 0 the best drawn
+ * candidate unchanged, slot j + 1 candidate j refined and quantized to a code, as
+ * codes and as records (exactly as the scoring kernels decode codes). */
+typedef struct WeftRefineOpts {
+    float lr[9];
+    float beta1, one_minus_beta1, beta2, one_minus_beta2, eps;
+    int32_t k;       /* 1, 2, 4, ..., 64 */
+    int32_t n_iter;  /* >= 0 */
+    const float* d_sched;  /* [n_iter][4] */
+} WeftRefineOpts;
+
+int weft_refine(const WeftFitState* st, const WeftCands* cands, const int32_t* d_topk,
+                const WeftRefineOpts* o, WeftCode* d_out_codes, WeftPrim* d_out_prims,
+                float* d_out_params, void* stream);
+
 /* Host-side copies of the candidate tables (log-radius codes, cos, sin: 4096 each). */
 int weft_get_tables(uint16_t* logr, float* cosv, float* sinv);
 

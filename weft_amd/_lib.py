@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -66,6 +66,22 @@ class Batch(ctypes.Structure):
     ]
 
 
+class SoftBatch(ctypes.Structure):
+    _fields_ = [
+        ("d_group_tile", c_void_p), ("d_kinds", c_void_p), ("d_params", c_void_p), ("d_out_sse", c_void_p),
+        ("d_out_grad", c_void_p), ("n_groups", c_int32), ("k", c_int32), ("w", ctypes.c_float), ("_pad", c_int32),
+    ]
+
+
+class RefineOpts(ctypes.Structure):
+    _fields_ = [
+        ("lr", ctypes.c_float * 9), ("beta1", ctypes.c_float), ("one_minus_beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float), ("one_minus_beta2", ctypes.c_float), ("eps", ctypes.c_float),
+        ("k", c_int32), ("n_iter", c_int32), ("d_sched", c_void_p),
+    ]
+
+
+assert ctypes.sizeof(SoftBatch) == 56 and ctypes.sizeof(RefineOpts) == 72
 assert ctypes.sizeof(Batch) == 112
 assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 32 and ctypes.sizeof(Cands) == 40
 
@@ -92,11 +108,15 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_get_tables.argtypes = [c_void_p, c_void_p, c_void_p]
     L.weft_batch_fit.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), ctypes.POINTER(Batch), c_void_p]
     L.weft_wallclock_rate_khz.argtypes = [ctypes.POINTER(c_int)]
+    L.weft_soft_eval.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(SoftBatch), c_void_p]
+    L.weft_topk.argtypes = [ctypes.POINTER(FitState), c_void_p, c_int, c_int, c_void_p, c_void_p]
+    L.weft_refine.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, ctypes.POINTER(RefineOpts),
+                              c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_render_tiles.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
     for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_order",
                  "weft_fit_score",
                  "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles",
-                 "weft_batch_fit", "weft_wallclock_rate_khz"):
+                 "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk", "weft_refine"):
         getattr(L, name).restype = c_int
 
 
@@ -175,5 +195,6 @@ def tables() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
 EXPORTED_SYMBOLS = (
     "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_score_evals",
     "weft_fit_init", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
-    "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz",
+    "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk",
+    "weft_refine",
 )

@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 4
+#define WEFT_ABI_VERSION 5
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -94,4 +94,29 @@ struct WeftBatchArgs {
     int32_t n_images;
     uint32_t step0;
     int32_t n_steps;
+};
+
+struct WeftSoftArgs {
+    WeftFitState st;
+    WeftSoftBatch b;
+};
+
+struct WeftTopkArgs {
+    WeftFitState st;
+    const float* table;
+    int32_t* out_idx;
+    int32_t n;
+    int32_t k;
+    int32_t n_pow2;
+    int32_t _pad;
+};
+
+struct WeftRefineArgs {
+    WeftFitState st;
+    WeftCands cd;
+    WeftRefineOpts o;
+    const int32_t* topk;
+    WeftCode* out_codes;
+    WeftPrim* out_prims;
+    float* out_params;
 };

@@ -36,6 +36,9 @@ struct Module {
     hipFunction_t f_select = nullptr;
     hipFunction_t f_draw = nullptr;
     hipFunction_t f_batch = nullptr;
+    hipFunction_t f_soft = nullptr;
+    hipFunction_t f_topk = nullptr;
+    hipFunction_t f_refine = nullptr;
     hipFunction_t f_batch_reset = nullptr;
     hipFunction_t f_render = nullptr;
 };
@@ -158,6 +161,9 @@ int weft_init(int device) {
         {&g_mod.f_select, "weft_fit_select_kernel"},
         {&g_mod.f_draw, "weft_draw_kernel"},
         {&g_mod.f_batch, "weft_batch_fit_kernel"},
+        {&g_mod.f_soft, "weft_soft_eval_kernel"},
+        {&g_mod.f_topk, "weft_topk_kernel"},
+        {&g_mod.f_refine, "weft_refine_kernel"},
         {&g_mod.f_batch_reset, "weft_batch_reset_kernel"},
         {&g_mod.f_render, "weft_render_tiles_kernel"},
     };
@@ -304,6 +310,48 @@ int weft_batch_fit(const WeftFitState* st, const WeftCands* cands, const WeftBat
     if (rc) return rc;
     const unsigned lds = (unsigned)(36 * 512 + 4 * cands->n_cand);
     return launch(g_mod.f_batch, (unsigned)b->grid, (unsigned)b->block, lds, stream, &a);
+}
+
+static bool pow2_le64(int k) { return k >= 1 && k <= 64 && (k & (k - 1)) == 0; }
+
+int weft_soft_eval(const WeftFitState* st, const WeftSoftBatch* b, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !b || !b->d_group_tile || !b->d_kinds || !b->d_params || !b->d_out_sse)
+        return set_err(WEFT_E_INVALID, "weft_soft_eval: null argument");
+    if (!pow2_le64(b->k)) return set_err(WEFT_E_INVALID, "weft_soft_eval: k must be a power of two <= 64");
+    if (b->n_groups < 0) return set_err(WEFT_E_INVALID, "weft_soft_eval: n_groups < 0");
+    if (!(b->w > 0.0f)) return set_err(WEFT_E_INVALID, "weft_soft_eval: edge width must be > 0");
+    if (b->n_groups == 0) return WEFT_OK;
+    WeftSoftArgs a{*st, *b};
+    return launch(g_mod.f_soft, (unsigned)b->n_groups, 64, 0, stream, &a);
+}
+
+int weft_topk(const WeftFitState* st, const float* d_table, int n, int k, int32_t* d_out_idx, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !d_table || !d_out_idx) return set_err(WEFT_E_INVALID, "weft_topk: null argument");
+    if (n < 1 || n > 8192) return set_err(WEFT_E_INVALID, "weft_topk: n must be in [1, 8192]");
+    if (k < 1 || k > n || k > 256) return set_err(WEFT_E_INVALID, "weft_topk: k must be in [1, min(n, 256)]");
+    int np2 = 2;
+    while (np2 < n) np2 <<= 1;
+    if (st->n_tiles == 0) return WEFT_OK;
+    WeftTopkArgs a{*st, d_table, d_out_idx, n, k, np2, 0};
+    return launch(g_mod.f_topk, (unsigned)st->n_tiles, 256, (unsigned)(8 * np2), stream, &a);
+}
+
+int weft_refine(const WeftFitState* st, const WeftCands* cands, const int32_t* d_topk, const WeftRefineOpts* o,
+                WeftCode* d_out_codes, WeftPrim* d_out_prims, float* d_out_params, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !cands || !d_topk || !o || !d_out_prims) return set_err(WEFT_E_INVALID, "weft_refine: null argument");
+    if (cands->d_prims) return set_err(WEFT_E_INVALID, "weft_refine: drawn candidates only");
+    if (!pow2_le64(o->k) || o->k > cands->n_cand)
+        return set_err(WEFT_E_INVALID, "weft_refine: k must be a power of two <= min(64, n_cand)");
+    if (o->n_iter < 0 || (o->n_iter > 0 && !o->d_sched)) return set_err(WEFT_E_INVALID, "weft_refine: bad schedule");
+    if (st->n_tiles == 0) return WEFT_OK;
+    WeftRefineArgs a{*st, *cands, *o, d_topk, d_out_codes, d_out_prims, d_out_params};
+    return launch(g_mod.f_refine, (unsigned)st->n_tiles, 64, 0, stream, &a);
 }
 
 int weft_wallclock_rate_khz(int* khz) {

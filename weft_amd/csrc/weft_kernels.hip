@@ -1130,6 +1130,8 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
     }
 }
 
+#include "weft_soft.h"
+
 }  // namespace
 
 // ===========================================================================
@@ -1336,3 +1338,12 @@ extern "C" __global__ __launch_bounds__(256) void weft_render_tiles_kernel(WeftR
         }
     }
 }
+
+// Soft-edge SSE and its gradient for groups of k candidates (one 64-lane workgroup each).
+extern "C" __global__ __launch_bounds__(64) void weft_soft_eval_kernel(WeftSoftArgs A) { soft_eval_body(A); }
+
+// Per-tile top-k indices of a value table (one workgroup per tile).
+extern "C" __global__ __launch_bounds__(256) void weft_topk_kernel(WeftTopkArgs A) { topk_body(A); }
+
+// Adam refinement of each tile's top-k candidates (one 64-lane workgroup per tile).
+extern "C" __global__ __launch_bounds__(64) void weft_refine_kernel(WeftRefineArgs A) { refine_body(A); }

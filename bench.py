@@ -13,6 +13,20 @@ second over the K timed steps; roofline = the dominant scoring kernel's algorith
 rate (FLOP per evaluation x evaluations per launch / mean launch time measured with HIP
 events on the launch stream) against MI355X's 157.3 TFLOP/s FP32 vector peak;
 cpu_baseline = the CPU oracle (C, float32, OpenMP) on a bounded sample of the same step.
+
+Further objects on the same line (skipped with --no-extras):
+- time_to_psnr: on the same image, the plain fit (1024 candidates, 16 rounds, one
+  persistent launch) against the refined fit (512 drawn candidates, top 8 refined by 10
+  Adam steps, exact re-score): wall time until the refined fit reaches the plain fit's
+  final PSNR, and both fits' PSNR at the end and at equal primitive bytes.
+- time_to_psnr_4k: the same at 3840x2160 (2500 primitives + value-noise texture),
+  depth 24, plain with 2048 candidates.
+- batched: 64 synthetic images (~88.8 Mpx) in one persistent hipGraph-captured launch,
+  512 candidates, 16 rounds, autotuned queue order and workgroup size; throughput and
+  per-image p50 / p99 latency.
+- refine: 4096 tiles of 32 px from the 256x256 generator, 256 drawn candidates, the top
+  64 refined by 20 Adam steps; the refinement's time and candidate-pixel rate (forward
+  and backward) and how much the first round improves on plain search.
 """
 from __future__ import annotations
 
@@ -107,6 +121,130 @@ def cpu_baseline(tiles, c, lam, seed, budget_s, gpu_rec0):
     }
 
 
+def valid_mask(tiles, heights, widths):
+    """Packed-layout mask (3 floats per pixel) of pixels inside their image (not padding)."""
+    import torch
+
+    valid = np.zeros(tiles.n_pixels, np.float32)
+    for t in range(tiles.n_tiles):
+        s = int(tiles.ts[t])
+        o = int(tiles.pix_off[t])
+        k = int(tiles.image[t])
+        yy, xx = np.mgrid[0:s, 0:s]
+        valid[o:o + s * s] = ((tiles.y[t] + yy < heights[k]) & (tiles.x[t] + xx < widths[k])).ravel()
+    return torch.from_numpy(np.repeat(valid, 3)).cuda(), float(valid.sum())
+
+
+def psnr_db(fitter, mask) -> float:
+    d_valid, npx = mask
+    e = (fitter.d_canvas.clamp(0, 1) - fitter.d_target) ** 2
+    mse = float((e * d_valid).sum()) / (3 * npx)
+    return float(10 * np.log10(1 / mse))
+
+
+def time_to_psnr(tiles, mask, n_cand_plain, depth, seed, lam, refined_cfg):
+    """Plain fit (persistent launch) vs refined fit round by round."""
+    import torch
+    from weft_amd.batch import BatchFitter
+    from weft_amd.refine import AdamConfig, RefinedFitter
+
+    bf = BatchFitter(tiles, n_cand=n_cand_plain, depth=depth, n_steps=depth, seed=seed, lam=lam)
+    times = bf.time_runs(3, warmup=1)
+    plain = dict(n_cand=n_cand_plain, rounds=depth, wall_s=min(times), psnr_db=psnr_db(bf, mask),
+                 prims=int(bf.records()["win_acc"].sum()))
+    plain["prim_bytes"] = 17 * plain["prims"]
+    del bf
+    n_cand, k, iters = refined_cfg
+    adam = AdamConfig(n_iter=iters)
+    warm = RefinedFitter(tiles, n_cand=n_cand, depth=depth, k=k, adam=adam, seed=seed, lam=lam, max_steps=depth)
+    warm.step(1)
+    del warm
+    rf = RefinedFitter(tiles, n_cand=n_cand, depth=depth, k=k, adam=adam, seed=seed, lam=lam, max_steps=depth)
+    torch.cuda.synchronize()
+    elapsed, reach, curve, psnr_at_bytes = 0.0, None, [], None
+    for r in range(depth):
+        t0 = time.perf_counter()
+        rf.step(1)
+        torch.cuda.synchronize()
+        elapsed += time.perf_counter() - t0
+        p = psnr_db(rf, mask)
+        prims = int(rf.records(0, r + 1)["win_acc"].sum())
+        curve.append([round(elapsed, 5), round(p, 3), prims])
+        if reach is None and p >= plain["psnr_db"]:
+            reach = dict(rounds=r + 1, wall_s=elapsed, psnr_db=p, prims=prims)
+        if psnr_at_bytes is None and prims >= plain["prims"]:
+            psnr_at_bytes = p
+    refined = dict(n_cand=n_cand, top_k=k, adam_steps=iters, rounds=depth, wall_s=elapsed, psnr_db=curve[-1][1],
+                   prims=curve[-1][2], prim_bytes=17 * curve[-1][2], reached_plain_psnr=reach,
+                   psnr_at_plain_bytes=psnr_at_bytes, curve=curve)
+    speed = plain["wall_s"] / reach["wall_s"] if reach else None
+    return dict(plain=plain, refined=refined, speedup_to_plain_psnr=speed)
+
+
+def batched_report(seed, lam):
+    """64-image batch, one persistent launch per fit, autotuned."""
+    from weft_amd.batch import BatchFitter, autotune
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.synth import config_c3_images
+
+    t0 = time.perf_counter()
+    imgs = config_c3_images(64, seed=0)
+    qts = [decompose_quadtree_variance(im, threshold=1e-3, min_tile=8, max_tile=64) for im in imgs]
+    tiles = TileSet.from_images(imgs, qts)
+    log(f"[bench] batched: {tiles.n_tiles} tiles of 64 images built in {time.perf_counter() - t0:.1f} s")
+    bf = BatchFitter(tiles, n_cand=512, depth=16, n_steps=16, seed=seed, lam=lam)
+    tune = autotune(bf, orders=("sjf", "lpt"), blocks=(128, 256), trials=1)
+    times = bf.time_runs(3, warmup=0)
+    lat = bf.image_latency_s()
+    ev = bf.evals()
+    wall = float(np.median(times))
+    return dict(images=64, tiles=int(tiles.n_tiles), pixels=int(tiles.n_pixels), n_cand=512, rounds=16,
+                graph=True, order=bf.order, block=bf.block, grid=bf.grid, wall_s=wall, evals=ev,
+                evals_per_s=ev / wall, p50_latency_ms=float(np.percentile(lat, 50)) * 1e3,
+                p99_latency_ms=float(np.percentile(lat, 99)) * 1e3, max_latency_ms=float(lat.max()) * 1e3,
+                autotune=[{k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()}
+                          for r in tune["table"]])
+
+
+def refine_report(seed, lam):
+    """4096 tiles of 32 px, top 64 of 256 drawn candidates refined by 20 Adam steps."""
+    import torch
+    from weft_amd.fit import PlainFitter
+    from weft_amd.quadtree import TileSet, uniform_tiles
+    from weft_amd.refine import AdamConfig, RefinedFitter, refine, topk
+    from weft_amd.synth import CONFIG_C1, synth_image
+
+    c = CONFIG_C1
+    imgs = [synth_image(c["height"], c["width"], c["n_prims"], seed=s) for s in range(64)]
+    tiles = TileSet.from_images(imgs, [uniform_tiles(c["height"], c["width"], 32)] * 64, pad_multiple=32)
+    k, iters = 64, 20
+    rf = RefinedFitter(tiles, n_cand=256, depth=8, k=k, adam=AdamConfig(n_iter=iters), seed=seed, lam=lam,
+                       max_steps=8)
+    pf = PlainFitter(tiles, n_cand=256, depth=8, seed=seed, lam=lam, max_steps=8)
+    rf.step(1)
+    pf.step(1)
+    o_r, o_p = rf.state()["obj"], pf.state()["obj"]
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(5):
+        s = rf.step_idx
+        rf.score(s)
+        topk(rf, rf.d_mse, rf.n_cand, k, rf.d_topk)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        refine(rf, rf._cands(s), rf.d_topk, k, rf.adam, rf.d_rcodes, rf.d_rprims)
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) * 1e-3)
+    t = float(np.median(times))
+    cand_px = tiles.n_pixels * k * iters
+    return dict(tiles=int(tiles.n_tiles), tile_size=32, n_cand=256, top_k=k, adam_steps=iters,
+                kernel="weft_refine_kernel", refine_ms=t * 1e3, cand_px_fwd_bwd_per_s=cand_px / t,
+                round1_refined_better_frac=float((o_r < o_p).mean()),
+                round1_objective_ratio_mean=float((o_r / o_p).mean()))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +252,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the time-to-PSNR, batched and refine reports")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES of device memory once (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -222,6 +361,33 @@ def main():
         log(f"[bench] cpu baseline {cpu['value'] / 1e9:.3f} G evals/s on {cpu['cores']} cores; "
             f"winners match: {cpu['winners_match_gpu']}")
 
+    extras = {}
+    if not args.no_extras:
+        mask = valid_mask(tiles, [c["height"]], [c["width"]])
+        extras["time_to_psnr"] = time_to_psnr(tiles, mask, c["n_cand"], c["depth"], seed, lam, (512, 8, 10))
+        t = extras["time_to_psnr"]
+        log(f"[bench] time-to-PSNR 1080p: plain {t['plain']['psnr_db']:.2f} dB in {t['plain']['wall_s']:.3f} s; "
+            f"speedup {t['speedup_to_plain_psnr']}")
+        del fitter
+        from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+        from weft_amd.synth import synth_image
+
+        img4 = synth_image(2160, 3840, 2500, seed=0, texture=True)
+        t4 = TileSet.from_images([img4], [decompose_quadtree_variance(img4, threshold=1e-3, min_tile=8, max_tile=64)])
+        m4 = valid_mask(t4, [2160], [3840])
+        extras["time_to_psnr_4k"] = time_to_psnr(t4, m4, 2048, 24, seed, lam, (512, 8, 10))
+        extras["time_to_psnr_4k"]["tiles"] = int(t4.n_tiles)
+        t = extras["time_to_psnr_4k"]
+        log(f"[bench] time-to-PSNR 4k: plain {t['plain']['psnr_db']:.2f} dB in {t['plain']['wall_s']:.3f} s; "
+            f"speedup {t['speedup_to_plain_psnr']}")
+        del t4, m4
+        extras["batched"] = batched_report(seed, lam)
+        log(f"[bench] batched: {extras['batched']['evals_per_s'] / 1e12:.3f} T evals/s, "
+            f"p50 {extras['batched']['p50_latency_ms']:.1f} ms, p99 {extras['batched']['p99_latency_ms']:.1f} ms")
+        extras["refine"] = refine_report(seed, lam)
+        log(f"[bench] refine: {extras['refine']['refine_ms']:.2f} ms, "
+            f"{extras['refine']['cand_px_fwd_bwd_per_s'] / 1e9:.1f} G cand*px/s")
+
     out = {
         "metric": METRIC, "value": value, "unit": "candidate·pixel evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -236,6 +402,7 @@ def main():
                    "evals_per_step": int(evals_steps[args.warmup]), "parallelism": "1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(out), flush=True)
 

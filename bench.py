@@ -157,9 +157,11 @@ def time_to_psnr(tiles, mask, n_cand_plain, depth, seed, lam, refined_cfg):
     n_cand, k, iters = refined_cfg
     adam = AdamConfig(n_iter=iters)
     warm = RefinedFitter(tiles, n_cand=n_cand, depth=depth, k=k, adam=adam, seed=seed, lam=lam, max_steps=depth)
+    cpw = warm.autotune()  # one-time choice of the refinement's work split, before timing
     warm.step(1)
     del warm
-    rf = RefinedFitter(tiles, n_cand=n_cand, depth=depth, k=k, adam=adam, seed=seed, lam=lam, max_steps=depth)
+    rf = RefinedFitter(tiles, n_cand=n_cand, depth=depth, k=k, adam=adam, seed=seed, lam=lam, max_steps=depth,
+                       cpw=cpw)
     torch.cuda.synchronize()
     elapsed, reach, curve, psnr_at_bytes = 0.0, None, [], None
     for r in range(depth):
@@ -174,7 +176,7 @@ def time_to_psnr(tiles, mask, n_cand_plain, depth, seed, lam, refined_cfg):
             reach = dict(rounds=r + 1, wall_s=elapsed, psnr_db=p, prims=prims)
         if psnr_at_bytes is None and prims >= plain["prims"]:
             psnr_at_bytes = p
-    refined = dict(n_cand=n_cand, top_k=k, adam_steps=iters, rounds=depth, wall_s=elapsed, psnr_db=curve[-1][1],
+    refined = dict(n_cand=n_cand, top_k=k, adam_steps=iters, cands_per_work_item=cpw, rounds=depth, wall_s=elapsed, psnr_db=curve[-1][1],
                    prims=curve[-1][2], prim_bytes=17 * curve[-1][2], reached_plain_psnr=reach,
                    psnr_at_plain_bytes=psnr_at_bytes, curve=curve)
     speed = plain["wall_s"] / reach["wall_s"] if reach else None
@@ -221,6 +223,7 @@ def refine_report(seed, lam):
     rf = RefinedFitter(tiles, n_cand=256, depth=8, k=k, adam=AdamConfig(n_iter=iters), seed=seed, lam=lam,
                        max_steps=8)
     pf = PlainFitter(tiles, n_cand=256, depth=8, seed=seed, lam=lam, max_steps=8)
+    cpw = rf.autotune()
     rf.step(1)
     pf.step(1)
     o_r, o_p = rf.state()["obj"], pf.state()["obj"]
@@ -233,13 +236,13 @@ def refine_report(seed, lam):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record(stream)
-        refine(rf, rf._cands(s), rf.d_topk, k, rf.adam, rf.d_rcodes, rf.d_rprims)
+        refine(rf, rf._cands(s), rf.d_topk, k, rf.adam, rf.d_rcodes, rf.d_rprims, work=rf.refine_work)
         b.record(stream)
         torch.cuda.synchronize()
         times.append(a.elapsed_time(b) * 1e-3)
     t = float(np.median(times))
     cand_px = tiles.n_pixels * k * iters
-    return dict(tiles=int(tiles.n_tiles), tile_size=32, n_cand=256, top_k=k, adam_steps=iters,
+    return dict(tiles=int(tiles.n_tiles), tile_size=32, n_cand=256, top_k=k, adam_steps=iters, cands_per_work_item=cpw[32],
                 kernel="weft_refine_kernel", refine_ms=t * 1e3, cand_px_fwd_bwd_per_s=cand_px / t,
                 round1_refined_better_frac=float((o_r < o_p).mean()),
                 round1_objective_ratio_mean=float((o_r / o_p).mean()))

@@ -246,7 +246,8 @@ int weft_topk(const WeftFitState* st, const float* d_table, int n, int k, int32_
               void* stream);
 
 /* Adam refinement of each tile's top-k drawn candidates (d_topk[t * k + j], drawn
- * from cands' seed and step), n_iter steps on the soft SSE.  Step i uses edge width
+ * from cands' seed and step), n_iter steps on the soft SSE, candidates spread over the
+ * work items of WeftRefineOpts.  Step i uses edge width
  * d_sched[4 i], bias corrections d_sched[4 i + 1] = 1 - beta1^(i+1) and
  * d_sched[4 i + 2] = 1 - beta2^(i+1).  After each step the parameters are projected
  * onto the code ranges.  Outputs per tile k + 1 slots: slot 0 the best drawn
@@ -258,11 +259,40 @@ typedef struct WeftRefineOpts {
     int32_t k;       /* 1, 2, 4, ..., 64 */
     int32_t n_iter;  /* >= 0 */
     const float* d_sched;  /* [n_iter][4] */
+    /* Work items, one 64-lane workgroup each: candidates d_work_j0[i] ..
+     * d_work_j0[i] + cpw - 1 of tile d_work_tile[i], cpw = cpw_by_size[log2(ts) - 3]
+     * (a power of two dividing k); the 64 / cpw lanes of a candidate split its pixels.
+     * Every (tile, candidate) must be covered once; tiles whose stack is full are skipped. */
+    const int32_t* d_work_tile;
+    const int32_t* d_work_j0;
+    int32_t n_work;
+    int32_t cpw_by_size[4];  /* tiles of 8, 16, 32, 64 px */
+    int32_t _pad;
 } WeftRefineOpts;
 
 int weft_refine(const WeftFitState* st, const WeftCands* cands, const int32_t* d_topk,
                 const WeftRefineOpts* o, WeftCode* d_out_codes, WeftPrim* d_out_prims,
                 float* d_out_params, void* stream);
+
+/* Exact scoring (the same values as weft_fit_score) of a few candidates per tile, for
+ * refined candidates (no reference counterpart; the candidates' SSE follows NumPy's
+ * pairwise float32 order exactly, see weft_score_evals).  Work item i (one 64-lane
+ * workgroup) scores candidates d_work_c0[i] .. + cpw - 1 of tile d_work_tile[i] (those
+ * below cands->n_cand), cpw = cpw_by_size[log2(ts) - 3], each candidate's pixels split
+ * over 64 / cpw lanes along the pairwise tree: 64 / cpw must be a power of two at most
+ * 2, 8, 32, 32 for tiles of 8, 16, 32, 64 px.  Outputs as weft_fit_score: per-item first
+ * minimum of the objective (for weft_fit_select) and optional SSE / MSE tables. */
+typedef struct WeftSplitWork {
+    const int32_t* d_work_tile;
+    const int32_t* d_work_c0;
+    int32_t n_work;
+    int32_t cpw_by_size[4];
+    int32_t _pad;
+} WeftSplitWork;
+
+int weft_fit_score_split(const WeftFitState* st, const WeftCands* cands, const WeftSplitWork* w,
+                         float* d_out_sse, float* d_out_mse, double* d_part_obj, int32_t* d_part_idx,
+                         void* stream);
 
 /* Host-side copies of the candidate tables (log-radius codes, cos, sin: 4096 each). */
 int weft_get_tables(uint16_t* logr, float* cosv, float* sinv);

@@ -175,24 +175,46 @@ def test_topk_matches_stable_sort(gpu):
             assert np.array_equal(out, ref.astype(np.int32)), (n, k)
 
 
-@pytest.mark.parametrize("k,n_iter", [(8, 0), (8, 3), (64, 2), (2, 5)])
-def test_refine_matches_kernel_gradient_and_cpu_adam(gpu, oracle_lib, k, n_iter):
+def _mirror_grads(f, tiles, kinds, prm, k, cpw, w):
+    """Kernel gradients with the refinement's lane split: per tile size, groups of
+    cpw[ts] candidates of one tile (64 / cpw[ts] lanes each)."""
+    import torch
+    from weft_amd.refine import soft_eval
+
+    g = np.zeros_like(prm)
+    for ts in sorted(set(tiles.ts.tolist())):
+        c = min(cpw[ts], k)
+        ids = np.nonzero(tiles.ts == ts)[0]
+        rows = (ids[:, None] * k + np.arange(k)[None, :]).ravel()
+        gt = torch.from_numpy(np.repeat(ids, k // c).astype(np.int32)).cuda()
+        _, gg = soft_eval(f, gt, torch.from_numpy(kinds[rows]).cuda(), torch.from_numpy(prm[rows]).cuda(), c, w)
+        g[rows] = gg.cpu().numpy()
+    return g
+
+
+@pytest.mark.parametrize("k,n_iter,cpw", [(8, 0, None), (8, 3, None), (64, 2, None), (2, 5, None),
+                                          (16, 3, {8: 16, 16: 2, 32: 1, 64: 4}), (64, 2, {8: 64, 16: 64, 32: 64, 64: 64})])
+def test_refine_matches_kernel_gradient_and_cpu_adam(gpu, oracle_lib, k, n_iter, cpw):
     import torch
     from oracle import soft_np
     from weft_amd import _lib
     from weft_amd.primitives import CODE_DTYPE
     from weft_amd.records import PRIM_DTYPE
-    from weft_amd.refine import AdamConfig, refine, soft_eval
+    from weft_amd.refine import DEFAULT_CPW, AdamConfig, refine, refine_work
 
-    tiles, canvas = _random_tiles([8, 16, 32, 64, 32, 16], seed=k + n_iter)
+    tiles, canvas = _random_tiles([8, 16, 32, 64, 32, 16, 8, 64], seed=k + n_iter)
     f = _fitter(tiles, canvas)
     T, n_cand, seed, step = tiles.n_tiles, 100, 77, 5
     rng = np.random.default_rng(k * 10 + n_iter)
     tk = np.stack([rng.permutation(n_cand)[:k] for _ in range(T)]).astype(np.int32)
     cfg = AdamConfig(n_iter=n_iter, w0=0.8, w1=0.3)
-    _say(f"refine k {k} n_iter {n_iter}")
+    cpw = cpw or DEFAULT_CPW
+    wt, wj, cpwl = refine_work(tiles.ts, k, cpw)
+    work = (torch.from_numpy(wt).cuda(), torch.from_numpy(wj).cuda(), cpwl)
+    _say(f"refine k {k} n_iter {n_iter} cpw {cpwl}, {len(wt)} work items")
     codes, prims = refine(f, _lib.Cands(None, None, seed, step, n_cand, 0, 0), torch.from_numpy(tk).cuda(), k, cfg,
-                          out_params=(pout := torch.zeros((T * k, 9), dtype=torch.float32, device="cuda")))
+                          out_params=(pout := torch.zeros((T * k, 9), dtype=torch.float32, device="cuda")),
+                          work=work)
     codes = codes.cpu().numpy().view(CODE_DTYPE)
     prims = prims.cpu().numpy().view(PRIM_DTYPE)
     dcodes = [oracle_lib.draw_codes(seed, step, t, n_cand, int(tiles.ts[t])) for t in range(T)]
@@ -204,10 +226,8 @@ def test_refine_matches_kernel_gradient_and_cpu_adam(gpu, oracle_lib, k, n_iter)
     m1, m2 = np.zeros_like(prm), np.zeros_like(prm)
     sched = cfg.schedule()
     lr = np.array(cfg.lr(), np.float32)
-    gt = torch.arange(T, dtype=torch.int32, device="cuda")
     for i in range(n_iter):
-        _, g = soft_eval(f, gt, torch.from_numpy(kinds).cuda(), torch.from_numpy(prm).cuda(), k, float(sched[i, 0]))
-        g = g.cpu().numpy()
+        g = _mirror_grads(f, tiles, kinds, prm, k, cpw, float(sched[i, 0]))
         for r in range(T * k):
             prm[r], m1[r], m2[r] = soft_np.adam_step(prm[r], m1[r], m2[r], g[r], lr, cfg.beta1,
                                                       np.float32(1) - np.float32(cfg.beta1), cfg.beta2,
@@ -308,3 +328,54 @@ def test_refined_fit_is_exact_and_no_worse(gpu, oracle_lib):
         o = int(tiles.pix_off[t]) * 3
         canvas[o:o + ts * ts * 3] = cv.ravel()
     assert bits_equal(rf.canvas(), canvas)
+
+
+@pytest.mark.parametrize("cpw", [None, {8: 64, 16: 64, 32: 64, 64: 64}, {8: 32, 16: 32, 32: 4, 64: 8}])
+def test_split_scorer_matches_oracle(gpu, oracle_lib, cpw):
+    """Lane-split exact scores equal the oracle's and the sequential kernel's bit for bit."""
+    import ctypes
+
+    import torch
+    from weft_amd import _lib
+    from weft_amd.fit import PlainFitter
+    from weft_amd.records import PRIM_DTYPE
+    from weft_amd.refine import split_work
+
+    tiles, canvas = _random_tiles([8, 16, 32, 64, 8, 16, 32, 64, 64, 32], seed=12)
+    T, n = tiles.n_tiles, 9
+    prims = np.zeros(T * n, PRIM_DTYPE)
+    for t in range(T):
+        prims[t * n:(t + 1) * n] = oracle_lib.codes_to_prims(oracle_lib.draw_codes(3, 1, t, n, int(tiles.ts[t])))
+    prims[2]["kind"] = 0
+    prims[3]["kind"] = 4
+    prims[3]["geom"][:6] = [1, 1, 6, 2, 3, 7]
+    f = PlainFitter(tiles, n_cand=n, depth=4, lam=1e-5, cand_prims=prims, keep_sse=True, max_steps=2)
+    f.set_canvas(canvas)
+    f.score(0)
+    seq = f.sse_table()
+    wt, wc, w0, nw, st, cp = split_work(tiles.ts, n, cpw)
+    d = [torch.from_numpy(a).cuda() for a in (wt, wc)]
+    w = _lib.SplitWork(d[0].data_ptr(), d[1].data_ptr(), len(wt), (ctypes.c_int32 * 4)(*cp), 0)
+    sse = torch.zeros(T * n, dtype=torch.float32, device="cuda")
+    po = torch.zeros(len(wt), dtype=torch.float64, device="cuda")
+    pi = torch.zeros(len(wt), dtype=torch.int32, device="cuda")
+    _say(f"split scorer cpw {cp}, {len(wt)} work items")
+    _lib.check(f.L.weft_fit_score_split(ctypes.byref(f._state), ctypes.byref(f._cands(0)), ctypes.byref(w),
+                                        sse.data_ptr(), None, po.data_ptr(), pi.data_ptr(), _lib.stream_ptr()),
+               "weft_fit_score_split")
+    sse = sse.cpu().numpy().reshape(T, n)
+    assert bits_equal(sse, seq)
+    for t in range(T):
+        ts = int(tiles.ts[t])
+        ref, _ = oracle_lib.score_candidates(_tile(tiles.target, tiles, t), _tile(canvas, tiles, t),
+                                             prims[t * n:(t + 1) * n])
+        assert bits_equal(sse[t], ref), t
+    # per-item first minima reduce to the sequential kernel's per-tile winner
+    po, pi = po.cpu().numpy(), pi.cpu().numpy()
+    for t in range(T):
+        items = [int(w0[t]) + j * int(st[t]) for j in range(int(nw[t]))]
+        best = min((po[i], pi[i]) for i in items)
+        mse = sse[t] / np.float32(3 * int(tiles.ts[t]) ** 2)
+        nbytes = np.array([{0: 7, 4: 19}.get(int(k), 17) for k in prims["kind"][t * n:(t + 1) * n]])
+        obj = mse.astype(np.float64) + 1e-5 * (8 + 8 * nbytes)
+        assert best[1] == int(np.argmin(obj)) and best[0] == obj.min()

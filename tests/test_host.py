@@ -249,3 +249,43 @@ def test_batch_queue_orders():
         == sorted(np.unique(img).tolist())
     empty = TileSet([], [], [], [], np.zeros(0, np.float32), np.zeros(0, np.int64))
     assert queue_order(empty, "sjf").size == 0
+
+
+def test_refine_work_lists_cover_each_candidate_once():
+    from weft_amd.refine import XCDS, refine_work
+
+    rng = np.random.default_rng(4)
+    ts = rng.choice([8, 16, 32, 64], 203)
+    for k, cpw in ((8, None), (64, None), (1, None), (16, {8: 16, 16: 2, 32: 1, 64: 4}), (64, {32: 64})):
+        wt, wj, cl = refine_work(ts, k, cpw)
+        per = dict(zip((8, 16, 32, 64), cl))
+        assert all(k % c == 0 and c <= k for c in cl)
+        seen = {}
+        for t, j in zip(wt.tolist(), wj.tolist()):
+            seen.setdefault(t, []).append(j)
+        for t in range(len(ts)):
+            assert sorted(seen[t]) == list(range(0, k, per[int(ts[t])])), (k, t)
+        # items of one tile sit XCDS apart inside a group of same-size tiles
+        pos = {}
+        for i, t in enumerate(wt.tolist()):
+            pos.setdefault(t, []).append(i)
+        same = [p for t, p in pos.items() if len(p) > 1]
+        assert sum(all((b - a) % XCDS == 0 for a, b in zip(p, p[1:])) for p in same) >= 0.9 * len(same)
+    sub = np.nonzero(ts == 32)[0]
+    wt, wj, _ = refine_work(ts, 8, {32: 2}, tiles=sub)
+    assert set(wt.tolist()) == set(sub.tolist()) and len(wt) == 4 * len(sub)
+
+
+def test_split_work_tables():
+    from weft_amd.refine import split_work
+
+    rng = np.random.default_rng(1)
+    ts = rng.choice([8, 16, 32, 64], 101)
+    for n in (9, 5, 17, 1, 65):
+        wt, wc, w0, nw, st, cp = split_work(ts, n)
+        cpd = dict(zip((8, 16, 32, 64), cp))
+        assert len(wt) == int(nw.sum())
+        for t in range(len(ts)):
+            items = [int(w0[t]) + j * int(st[t]) for j in range(int(nw[t]))]
+            assert all(wt[i] == t for i in items)
+            assert sorted(int(wc[i]) for i in items) == list(range(0, n, cpd[int(ts[t])]))

@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 5
+ABI_VERSION = 7
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -77,11 +77,20 @@ class RefineOpts(ctypes.Structure):
     _fields_ = [
         ("lr", ctypes.c_float * 9), ("beta1", ctypes.c_float), ("one_minus_beta1", ctypes.c_float),
         ("beta2", ctypes.c_float), ("one_minus_beta2", ctypes.c_float), ("eps", ctypes.c_float),
-        ("k", c_int32), ("n_iter", c_int32), ("d_sched", c_void_p),
+        ("k", c_int32), ("n_iter", c_int32), ("d_sched", c_void_p), ("d_work_tile", c_void_p),
+        ("d_work_j0", c_void_p), ("n_work", c_int32), ("cpw_by_size", c_int32 * 4), ("_pad", c_int32),
     ]
 
 
-assert ctypes.sizeof(SoftBatch) == 56 and ctypes.sizeof(RefineOpts) == 72
+class SplitWork(ctypes.Structure):
+    _fields_ = [
+        ("d_work_tile", c_void_p), ("d_work_c0", c_void_p), ("n_work", c_int32), ("cpw_by_size", c_int32 * 4),
+        ("_pad", c_int32),
+    ]
+
+
+assert ctypes.sizeof(SplitWork) == 40
+assert ctypes.sizeof(SoftBatch) == 56 and ctypes.sizeof(RefineOpts) == 112
 assert ctypes.sizeof(Batch) == 112
 assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 32 and ctypes.sizeof(Cands) == 40
 
@@ -109,6 +118,8 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_batch_fit.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), ctypes.POINTER(Batch), c_void_p]
     L.weft_wallclock_rate_khz.argtypes = [ctypes.POINTER(c_int)]
     L.weft_soft_eval.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(SoftBatch), c_void_p]
+    L.weft_fit_score_split.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), ctypes.POINTER(SplitWork),
+                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_topk.argtypes = [ctypes.POINTER(FitState), c_void_p, c_int, c_int, c_void_p, c_void_p]
     L.weft_refine.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, ctypes.POINTER(RefineOpts),
                               c_void_p, c_void_p, c_void_p, c_void_p]
@@ -116,7 +127,8 @@ def _bind(L: ctypes.CDLL) -> None:
     for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_order",
                  "weft_fit_score",
                  "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles",
-                 "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk", "weft_refine"):
+                 "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk", "weft_refine",
+                 "weft_fit_score_split"):
         getattr(L, name).restype = c_int
 
 
@@ -196,5 +208,5 @@ EXPORTED_SYMBOLS = (
     "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_score_evals",
     "weft_fit_init", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
     "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk",
-    "weft_refine",
+    "weft_refine", "weft_fit_score_split",
 )

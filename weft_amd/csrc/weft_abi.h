@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 5
+#define WEFT_ABI_VERSION 7
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -119,4 +119,16 @@ struct WeftRefineArgs {
     WeftCode* out_codes;
     WeftPrim* out_prims;
     float* out_params;
+};
+
+struct WeftSplitScoreArgs {
+    WeftFitState st;
+    WeftCands cd;
+    const int32_t* work_tile;
+    const int32_t* work_c0;
+    int32_t cpw[4];
+    float* out_sse;
+    float* out_mse;
+    double* part_obj;
+    int32_t* part_idx;
 };

@@ -39,6 +39,7 @@ struct Module {
     hipFunction_t f_soft = nullptr;
     hipFunction_t f_topk = nullptr;
     hipFunction_t f_refine = nullptr;
+    hipFunction_t f_split = nullptr;
     hipFunction_t f_batch_reset = nullptr;
     hipFunction_t f_render = nullptr;
 };
@@ -129,6 +130,17 @@ int weft_abi_version(void) { return WEFT_ABI_VERSION; }
 
 const char* weft_last_error(void) { return g_err.c_str(); }
 
+// The Python binding (weft_amd/_lib.py) mirrors these layouts.
+static_assert(sizeof(WeftFitState) == 80, "WeftFitState layout");
+static_assert(sizeof(WeftWork) == 32, "WeftWork layout");
+static_assert(sizeof(WeftCands) == 40, "WeftCands layout");
+static_assert(sizeof(WeftBatch) == 112, "WeftBatch layout");
+static_assert(sizeof(WeftSoftBatch) == 56, "WeftSoftBatch layout");
+static_assert(sizeof(WeftRefineOpts) == 112, "WeftRefineOpts layout");
+static_assert(sizeof(WeftCode) == 20, "WeftCode layout");
+static_assert(sizeof(WeftSplitWork) == 40, "WeftSplitWork layout");
+static_assert(sizeof(WeftPrim) == 60, "WeftPrim layout");
+
 int weft_init(int device) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (g_mod.mod && g_mod.device == device) return WEFT_OK;
@@ -164,6 +176,7 @@ int weft_init(int device) {
         {&g_mod.f_soft, "weft_soft_eval_kernel"},
         {&g_mod.f_topk, "weft_topk_kernel"},
         {&g_mod.f_refine, "weft_refine_kernel"},
+        {&g_mod.f_split, "weft_fit_score_split_kernel"},
         {&g_mod.f_batch_reset, "weft_batch_reset_kernel"},
         {&g_mod.f_render, "weft_render_tiles_kernel"},
     };
@@ -349,9 +362,38 @@ int weft_refine(const WeftFitState* st, const WeftCands* cands, const int32_t* d
     if (!pow2_le64(o->k) || o->k > cands->n_cand)
         return set_err(WEFT_E_INVALID, "weft_refine: k must be a power of two <= min(64, n_cand)");
     if (o->n_iter < 0 || (o->n_iter > 0 && !o->d_sched)) return set_err(WEFT_E_INVALID, "weft_refine: bad schedule");
-    if (st->n_tiles == 0) return WEFT_OK;
+    for (int c = 0; c < 4; ++c)
+        if (!pow2_le64(o->cpw_by_size[c]) || o->k % o->cpw_by_size[c])
+            return set_err(WEFT_E_INVALID, "weft_refine: cpw must be powers of two dividing k");
+    if (o->n_work < 0 || (o->n_work > 0 && (!o->d_work_tile || !o->d_work_j0)))
+        return set_err(WEFT_E_INVALID, "weft_refine: bad work list");
+    if (o->n_work == 0) return WEFT_OK;
     WeftRefineArgs a{*st, *cands, *o, d_topk, d_out_codes, d_out_prims, d_out_params};
-    return launch(g_mod.f_refine, (unsigned)st->n_tiles, 64, 0, stream, &a);
+    return launch(g_mod.f_refine, (unsigned)o->n_work, 64, 0, stream, &a);
+}
+
+int weft_fit_score_split(const WeftFitState* st, const WeftCands* cands, const WeftSplitWork* w,
+                         float* d_out_sse, float* d_out_mse, double* d_part_obj, int32_t* d_part_idx,
+                         void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !cands || !w || !d_part_obj || !d_part_idx)
+        return set_err(WEFT_E_INVALID, "weft_fit_score_split: null argument");
+    if (cands->n_cand < 1) return set_err(WEFT_E_INVALID, "weft_fit_score_split: n_cand < 1");
+    const int max_lanes[4] = {2, 8, 32, 32};
+    for (int c = 0; c < 4; ++c) {
+        const int cpw = w->cpw_by_size[c];
+        if (!pow2_le64(cpw) || 64 / cpw > max_lanes[c])
+            return set_err(WEFT_E_INVALID, "weft_fit_score_split: lanes per candidate must be a power of two "
+                                           "<= 2, 8, 32, 32 for 8, 16, 32, 64-px tiles");
+    }
+    if (w->n_work < 0 || (w->n_work > 0 && (!w->d_work_tile || !w->d_work_c0)))
+        return set_err(WEFT_E_INVALID, "weft_fit_score_split: bad work list");
+    if (w->n_work == 0) return WEFT_OK;
+    WeftSplitScoreArgs a{*st, *cands, w->d_work_tile, w->d_work_c0,
+                         {w->cpw_by_size[0], w->cpw_by_size[1], w->cpw_by_size[2], w->cpw_by_size[3]},
+                         d_out_sse, d_out_mse, d_part_obj, d_part_idx};
+    return launch(g_mod.f_split, (unsigned)w->n_work, 64, 0, stream, &a);
 }
 
 int weft_wallclock_rate_khz(int* khz) {

@@ -1131,6 +1131,7 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
 }
 
 #include "weft_soft.h"
+#include "weft_split.h"
 
 }  // namespace
 
@@ -1347,3 +1348,6 @@ extern "C" __global__ __launch_bounds__(256) void weft_topk_kernel(WeftTopkArgs 
 
 // Adam refinement of each tile's top-k candidates (one 64-lane workgroup per tile).
 extern "C" __global__ __launch_bounds__(64) void weft_refine_kernel(WeftRefineArgs A) { refine_body(A); }
+
+// Exact scores of a few candidates per tile, each candidate's pixels split over lanes.
+extern "C" __global__ __launch_bounds__(64) void weft_fit_score_split_kernel(WeftSplitScoreArgs A) { split_score_body(A); }

@@ -251,6 +251,7 @@ __device__ __forceinline__ void soft_tile(const SoftCand& C, const SoftPre& P, f
                                           SoftAcc& A) {
     soft_acc_zero(A);
     const int npx = ts * ts;
+#pragma unroll 4
     for (int px = slice; px < npx; px += L) {
         const int y = px / ts, x = px - y * ts;
         const float dx = ((float)x + 0.5f) - C.p[0];
@@ -398,17 +399,22 @@ __device__ void topk_body(const WeftTopkArgs& A) {
 }
 
 // Adam on the soft SSE of each tile's top-k drawn candidates; one 64-lane workgroup
-// per tile.  Every lane of a candidate holds the same parameters and moments (the
-// butterfly sums give all of them the same totals).
+// per work item (cpw candidates of one tile, 64 / cpw lanes each).  Every lane of a
+// candidate holds the same parameters and moments (the butterfly sums give all of them
+// the same totals).
 __device__ void refine_body(const WeftRefineArgs& A) {
-    const int t = blockIdx.x;
+    const int item = blockIdx.x;
     const WeftFitState& st = A.st;
+    const int t = A.o.d_work_tile[item];
     if (st.d_tile_nsel[t] >= st.depth) return;
     const int k = A.o.k;
-    const int L = 64 / k;
-    const int lane = threadIdx.x;
-    const int j = lane / L, slice = lane - j * L;
     const int ts = st.d_tile_ts[t];
+    const int cls = ts == 8 ? 0 : (ts == 16 ? 1 : (ts == 32 ? 2 : 3));
+    const int cpw = A.o.cpw_by_size[cls];
+    const int L = 64 / cpw;
+    const int lane = threadIdx.x;
+    const int j0 = A.o.d_work_j0[item];
+    const int j = j0 + lane / L, slice = lane - (lane / L) * L;
     const int64_t base = st.d_tile_pix[t] * 3;
     const float* tg = st.d_target + base;
     const float* cv = st.d_canvas + base;
@@ -416,7 +422,7 @@ __device__ void refine_body(const WeftRefineArgs& A) {
     const int64_t slot0 = (int64_t)t * (k + 1);
     const int idx = A.topk[(int64_t)t * k + j];
     const WeftCode code0 = draw_code(dbase, (uint32_t)idx, ts);
-    if (lane == 0) {
+    if (j0 == 0 && lane == 0) {
         // slot 0: the best drawn candidate (top-1) as drawn
         const WeftCode best = draw_code(dbase, (uint32_t)A.topk[(int64_t)t * k], ts);
         if (A.out_codes) A.out_codes[slot0] = best;

@@ -1,0 +1,170 @@
+// Exact scoring of a few candidates per tile with the tile's pixels split over lanes.
+// Included by weft_kernels.hip inside its anonymous namespace.
+//
+// The SSE of a candidate is NumPy's float32 pairwise sum over the tile's elements in
+// (y, x, channel) order: chunks of 8192 elements added left to right, each chunk a
+// binary tree (halves rounded down to multiples of 8) over leaves of at most 128
+// elements, a leaf summed with 8 interleaved accumulators.  For the tile sizes here
+// the trees are perfect: 8, 16, 32 px -> 2, 8, 32 leaves of 96 elements; 64 px ->
+// chunks of 64 and 32 leaves of 128 elements.  So L lanes (a power of two dividing
+// the leaf counts) can each reduce a contiguous run of leaves to its subtree's value,
+// and a butterfly over the L lanes adds the subtrees in tree order -- the same value
+// as the sequential sum, bit for bit (a + b == b + a in IEEE arithmetic).
+
+struct SplitTree {
+    float lvl[7];  // up to 64 leaves per lane
+    int n;
+};
+
+__device__ __forceinline__ void split_tree_push(SplitTree& T, float v) {
+    int j = 0;
+    int n = T.n;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        if (n & 1) {
+            v = T.lvl[q] + v;  // left subtree (earlier leaves) + right
+            n >>= 1;
+            j = q + 1;
+        } else {
+            break;
+        }
+    }
+    // store at level j (static indices only)
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+        if (q == j) T.lvl[q] = v;
+    T.n += 1;
+}
+
+// Sum of elements [e0, e0 + LS) of the candidate's squared errors (one leaf).
+template <int TS, int LS>
+__device__ __forceinline__ float split_leaf(const WeftPrim& P, const float* cv, const float* tg, int e0) {
+    constexpr int NPX = TS * TS;
+    float r[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = 0.0f;
+    const int p0 = e0 / 3;
+    const int p1 = (e0 + LS + 2) / 3;  // one past the last pixel touching the leaf
+#pragma unroll 1
+    for (int pg = p0 & ~7; pg < p1; pg += 8) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int p = pg + i;
+            float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
+            if (p >= p0 && p < p1 && p < NPX) {
+                const int y = p / TS, x = p - y * TS;
+                float o0 = cv[3 * p], o1 = cv[3 * p + 1], o2 = cv[3 * p + 2];
+                composite_px(P, (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
+                const float d0 = clip01(o0) - tg[3 * p];
+                const float d1 = clip01(o1) - tg[3 * p + 1];
+                const float d2 = clip01(o2) - tg[3 * p + 2];
+                q0 = d0 * d0;
+                q1 = d1 * d1;
+                q2 = d2 * d2;
+            }
+            const int e = 3 * p;
+            // element 3p + c goes to accumulator (3i + c) % 8 (pg is a multiple of 8)
+            r[(3 * i) % 8] += (e >= e0 && e < e0 + LS) ? q0 : 0.0f;
+            r[(3 * i + 1) % 8] += (e + 1 >= e0 && e + 1 < e0 + LS) ? q1 : 0.0f;
+            r[(3 * i + 2) % 8] += (e + 2 >= e0 && e + 2 < e0 + LS) ? q2 : 0.0f;
+        }
+    }
+    return ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+}
+
+// Perfect tree over the nleaves leaves of one chunk, L lanes; every lane returns the sum.
+template <int TS, int LS>
+__device__ __forceinline__ float split_chunk(const WeftPrim& P, const float* cv, const float* tg, int c0,
+                                             int nleaves, int L, int sl) {
+    const int m = nleaves / L;
+    SplitTree T;
+    T.n = 0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) T.lvl[q] = 0.0f;
+#pragma unroll 1
+    for (int li = 0; li < m; ++li) split_tree_push(T, split_leaf<TS, LS>(P, cv, tg, c0 + (sl * m + li) * LS));
+    // m is a power of two: the root is the single level left
+    float v = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+        if ((1 << q) == m) v = T.lvl[q];
+    for (int off = 1; off < L; off <<= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int TS>
+__device__ float split_sse(const WeftPrim& P, const float* cv, const float* tg, int L, int sl) {
+    if constexpr (TS == 64) {
+        const float a = split_chunk<TS, 128>(P, cv, tg, 0, 64, L, sl);
+        const float b = split_chunk<TS, 128>(P, cv, tg, 8192, 32, L, sl);
+        return (0.0f + a) + b;
+    } else {
+        return split_chunk<TS, 96>(P, cv, tg, 0, (3 * TS * TS) / 96, L, sl);
+    }
+}
+
+template <int TS>
+__device__ void split_score_item(const WeftSplitScoreArgs& A, int item, int t, int cpw) {
+    const WeftFitState& st = A.st;
+    const int L = 64 / cpw;
+    const int lane = threadIdx.x;
+    const int jl = lane / L, sl = lane - jl * L;
+    const int n = A.cd.n_cand;
+    const int c = A.work_c0[item] + jl;
+    const bool live = c < n;
+    WeftPrim P;
+    P.kind = -1;
+    P.alpha = 0.0f;
+    P.color0[0] = P.color0[1] = P.color0[2] = 0.0f;
+    if (live) {
+        if (A.cd.d_prims == nullptr) {
+            P = code_to_prim(draw_code(draw_base(A.cd.seed, A.cd.step, (uint32_t)t), (uint32_t)c, TS));
+        } else {
+            const int64_t cb = A.cd.d_tile_cand ? A.cd.d_tile_cand[t] : (int64_t)t * n;
+            P = load_prim(A.cd.d_prims + cb + c);
+        }
+    }
+    const int64_t base = st.d_tile_pix[t] * 3;
+    const float sse = split_sse<TS>(P, st.d_canvas + base, st.d_target + base, L, sl);
+    double obj = __builtin_huge_val();
+    int idx = 0x7fffffff;
+    if (live) {
+        const float mse = sse / (float)(3 * TS * TS);
+        if (sl == 0) {
+            const int64_t ti = (int64_t)t * n + c;
+            if (A.out_sse) A.out_sse[ti] = sse;
+            if (A.out_mse) A.out_mse[ti] = mse;
+        }
+        obj = objective(mse, st.d_tile_bits[t] + prim_bytes(P.kind) * 8, st.lam, st.obj_mode);
+        idx = c;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o2 = __shfl_xor(obj, off, 64);
+        const int i2 = __shfl_xor(idx, off, 64);
+        argmin_pair(obj, idx, o2, i2);
+    }
+    if (lane == 0) {
+        A.part_obj[item] = obj;
+        A.part_idx[item] = idx == 0x7fffffff ? -1 : idx;
+    }
+}
+
+__device__ void split_score_body(const WeftSplitScoreArgs& A) {
+    const int item = blockIdx.x;
+    const int t = A.work_tile[item];
+    if (A.st.d_tile_nsel[t] >= A.st.depth) {
+        if (threadIdx.x == 0) {
+            A.part_obj[item] = __builtin_huge_val();
+            A.part_idx[item] = -1;
+        }
+        return;
+    }
+    switch (A.st.d_tile_ts[t]) {
+    case 8: split_score_item<8>(A, item, t, A.cpw[0]); break;
+    case 16: split_score_item<16>(A, item, t, A.cpw[1]); break;
+    case 32: split_score_item<32>(A, item, t, A.cpw[2]); break;
+    case 64: split_score_item<64>(A, item, t, A.cpw[3]); break;
+    default: break;
+    }
+}

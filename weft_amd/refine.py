@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from .fit import PlainFitter, build_work_lists
+from .fit import PlainFitter
 from .primitives import CODE_DTYPE
 from .records import PRIM_DTYPE
 
@@ -136,17 +136,128 @@ class AdamConfig:
 
     def opts(self, k: int, d_sched) -> _lib.RefineOpts:
         f = np.float32
-        return _lib.RefineOpts((ctypes.c_float * 9)(*self.lr()), f(self.beta1), f(1.0) - f(self.beta1),
-                               f(self.beta2), f(1.0) - f(self.beta2), f(self.eps), int(k), int(self.n_iter),
-                               d_sched.data_ptr())
+        o = _lib.RefineOpts()
+        o.lr = (ctypes.c_float * 9)(*self.lr())
+        o.beta1, o.one_minus_beta1 = f(self.beta1), f(1.0) - f(self.beta1)
+        o.beta2, o.one_minus_beta2 = f(self.beta2), f(1.0) - f(self.beta2)
+        o.eps = f(self.eps)
+        o.k, o.n_iter, o.d_sched = int(k), int(self.n_iter), d_sched.data_ptr()
+        return o
+
+
+# Candidates per 64-lane work item by tile size (8, 16, 32, 64 px): small tiles put
+# several candidates in one wave, large tiles spread one candidate's pixels over it.
+DEFAULT_CPW = {8: 8, 16: 4, 32: 2, 64: 1}
+XCDS = 8
+
+
+def refine_work(tile_ts, k: int, cpw: dict[int, int] | None = None, tiles=None):
+    """Work items (tile, first candidate) of weft_refine for the tiles `tiles` (default
+    all), large tiles first; the items of a group of XCDS tiles are interleaved so that
+    a tile's items are XCDS apart, run on one XCD (workgroup i goes to XCD i mod 8) and
+    share its L2."""
+    cpw = {ts: min(v, k) for ts, v in {**DEFAULT_CPW, **(cpw or {})}.items()}
+    tile_ts = np.asarray(tile_ts)
+    ids = np.arange(len(tile_ts)) if tiles is None else np.asarray(tiles)
+    ts_ids = tile_ts[ids]
+    order = ids[np.argsort(-ts_ids, kind="stable")]
+    per = np.array([k // cpw[int(v)] for v in tile_ts[order]], np.int64)
+    cp = np.array([cpw[int(v)] for v in tile_ts[order]], np.int64)
+    wt, wj = [], []
+    for g0 in range(0, len(order), XCDS):
+        grp, pg, cg = order[g0:g0 + XCDS], per[g0:g0 + XCDS], cp[g0:g0 + XCDS]
+        n = int(pg.max())
+        r = np.arange(n)[:, None]
+        live = r < pg[None, :]
+        wt.append(np.broadcast_to(grp[None, :], live.shape)[live])
+        wj.append((r * cg[None, :])[live])
+    cat = (lambda a: np.concatenate(a).astype(np.int32)) if wt else (lambda a: np.zeros(0, np.int32))
+    return cat(wt), cat(wj), [cpw[8], cpw[16], cpw[32], cpw[64]]
+
+
+# Candidates per work item of the lane-split exact scorer, by tile size: 64 / cpw lanes
+# per candidate, at most the pairwise tree's leaf counts 2, 8, 32, 32.
+SPLIT_CPW = {8: 32, 16: 8, 32: 2, 64: 2}
+
+
+def split_work(tile_ts, n: int, cpw: dict[int, int] | None = None):
+    """Work items of weft_fit_score_split for n candidates per tile, with the per-tile
+    (first item, item count, stride) table weft_fit_select reads; a group of XCDS
+    same-size tiles has its items interleaved (a tile's items are XCDS apart)."""
+    cpw = {**SPLIT_CPW, **(cpw or {})}
+    tile_ts = np.asarray(tile_ts)
+    T = len(tile_ts)
+    order = np.argsort(-tile_ts, kind="stable")
+    per = np.array([-(-n // cpw[int(v)]) for v in tile_ts[order]], np.int64)
+    wt, wc = [], []
+    work0 = np.zeros(T, np.int32)
+    nwork = np.zeros(T, np.int32)
+    stride = np.ones(T, np.int32)
+    pos = 0
+    for g0 in range(0, T, XCDS):
+        grp = order[g0:g0 + XCDS]
+        pg = per[g0:g0 + XCDS]
+        same = bool((pg == pg[0]).all())
+        if same:
+            for j, t in enumerate(grp):
+                work0[t], nwork[t], stride[t] = pos + j, pg[0], len(grp)
+            for r in range(int(pg[0])):
+                for t in grp:
+                    wt.append(int(t))
+                    wc.append(r * cpw[int(tile_ts[t])])
+            pos += len(grp) * int(pg[0])
+        else:
+            for t, q in zip(grp, pg):
+                work0[t], nwork[t], stride[t] = pos, q, 1
+                for r in range(int(q)):
+                    wt.append(int(t))
+                    wc.append(r * cpw[int(tile_ts[t])])
+                pos += int(q)
+    return (np.array(wt, np.int32), np.array(wc, np.int32), work0, nwork, stride,
+            [cpw[8], cpw[16], cpw[32], cpw[64]])
+
+
+def autotune_cpw(fitter, cands: _lib.Cands, topk_idx, k: int, cfg: AdamConfig,
+                 options=(1, 2, 4, 8, 16, 32, 64), reps: int = 2) -> dict[int, int]:
+    """Candidates per work item for each tile size, the fastest on this workload.
+
+    Each size class is timed alone (HIP events around weft_refine over that class's
+    tiles) for every option that divides k; results do not depend on the choice."""
+    torch = fitter.torch
+    stream = torch.cuda.current_stream()
+    best = {}
+    for ts in sorted(set(np.asarray(fitter.tiles.ts).tolist())):
+        ids = np.nonzero(np.asarray(fitter.tiles.ts) == ts)[0]
+        times = {}
+        for c in options:
+            if c > k or k % c:
+                continue
+            wt, wj, cl = refine_work(fitter.tiles.ts, k, {ts: c}, tiles=ids)
+            work = (torch.from_numpy(wt).to(fitter.dev), torch.from_numpy(wj).to(fitter.dev), cl)
+            refine(fitter, cands, topk_idx, k, cfg, work=work)  # warm
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(reps):
+                refine(fitter, cands, topk_idx, k, cfg, work=work)
+            b.record(stream)
+            torch.cuda.synchronize()
+            times[c] = a.elapsed_time(b) / reps
+        best[int(ts)] = min(times, key=times.get)
+    return best
 
 
 def refine(fitter, cands: _lib.Cands, topk_idx, k: int, cfg: AdamConfig, out_codes=None, out_prims=None,
-           out_params=None, stream=None):
+           out_params=None, stream=None, work=None):
     """Adam-refine each tile's top-k drawn candidates; returns (codes, prims) with k + 1
-    slots per tile (slot 0 the best drawn candidate, slots 1..k the refined ones)."""
+    slots per tile (slot 0 the best drawn candidate, slots 1..k the refined ones).
+
+    work: (d_work_tile, d_work_j0, cpw list) from refine_work (default: built here)."""
     torch = fitter.torch
     T = fitter.n_tiles
+    if work is None:
+        wt, wj, cpw = refine_work(fitter.tiles.ts, k)
+        work = (torch.from_numpy(wt).to(fitter.dev), torch.from_numpy(wj).to(fitter.dev), cpw)
     if out_codes is None:
         out_codes = torch.zeros(T * (k + 1) * CODE_DTYPE.itemsize, dtype=torch.uint8, device=fitter.dev)
     if out_prims is None:
@@ -157,6 +268,11 @@ def refine(fitter, cands: _lib.Cands, topk_idx, k: int, cfg: AdamConfig, out_cod
         d_sched = torch.from_numpy(cfg.schedule()).to(fitter.dev)
         cfg._cache[key] = d_sched
     o = cfg.opts(k, d_sched)
+    o.d_work_tile = work[0].data_ptr()
+    o.d_work_j0 = work[1].data_ptr()
+    o.n_work = int(work[0].numel())
+    for i, v in enumerate(work[2]):
+        o.cpw_by_size[i] = int(v)
     sp = _lib.stream_ptr() if stream is None else stream
     _lib.check(fitter.L.weft_refine(ctypes.byref(_state_of(fitter)), ctypes.byref(cands), topk_idx.data_ptr(),
                                     ctypes.byref(o), out_codes.data_ptr(), out_prims.data_ptr(),
@@ -178,7 +294,8 @@ class RefinedFitter(PlainFitter):
     With k = 0 this is PlainFitter.
     """
 
-    def __init__(self, tiles, n_cand: int, depth: int, k: int = 8, adam: AdamConfig | None = None, **kw):
+    def __init__(self, tiles, n_cand: int, depth: int, k: int = 8, adam: AdamConfig | None = None,
+                 cpw: dict[int, int] | None = None, **kw):
         kw.setdefault("keep_sse", True)
         super().__init__(tiles, n_cand, depth, **kw)
         torch = self.torch
@@ -188,24 +305,37 @@ class RefinedFitter(PlainFitter):
         self.adam = adam or AdamConfig()
         T, K1 = self.n_tiles, self.k + 1
         dev = self.dev
+        self.set_cpw(cpw)
         self.d_topk = torch.zeros((T, self.k), dtype=torch.int32, device=dev)
         self.d_rcodes = torch.zeros(T * K1 * CODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
         self.d_rprims = torch.zeros(T * K1 * PRIM_DTYPE.itemsize, dtype=torch.uint8, device=dev)
-        self.d_rorder = torch.empty(T * K1, dtype=torch.int32, device=dev)
-        wl = build_work_lists(tiles.ts, K1, {8: 64, 16: 64, 32: 64, 64: 64})
-        self.d_rwork_tile = torch.from_numpy(wl["work_tile"]).to(dev)
-        self.d_rwork_c0 = torch.from_numpy(wl["work_c0"]).to(dev)
-        self.d_rtile_work0 = torch.from_numpy(wl["tile_work0"]).to(dev)
-        self.d_rtile_nwork = torch.from_numpy(wl["tile_nwork"]).to(dev)
-        self.d_rtile_wstride = torch.from_numpy(wl["tile_wstride"]).to(dev)
-        self.n_rwork = len(wl["work_tile"])
+        sw = split_work(tiles.ts, K1)
+        self._split_arrays = [torch.from_numpy(a).to(dev) for a in sw[:5]]
+        self.n_rwork = len(sw[0])
+        self._split = _lib.SplitWork(self._split_arrays[0].data_ptr(), self._split_arrays[1].data_ptr(),
+                                     self.n_rwork, (ctypes.c_int32 * 4)(*sw[5]), 0)
         self.d_rpart_obj = torch.empty(max(1, self.n_rwork), dtype=torch.float64, device=dev)
         self.d_rpart_idx = torch.empty(max(1, self.n_rwork), dtype=torch.int32, device=dev)
-        self._rwork = _lib.Work(self.d_rwork_tile.data_ptr(), self.d_rwork_c0.data_ptr(), self.n_rwork, 0,
-                                wl["classes"][0]["block"] if wl["classes"] else 64, 0)
         S = self.max_steps
         self.d_rwin_idx = torch.full((S, T), -1, dtype=torch.int32, device=dev)
         self.d_rwin_code = torch.zeros((S, T * CODE_DTYPE.itemsize), dtype=torch.uint8, device=dev)
+
+    def set_cpw(self, cpw: dict[int, int] | None) -> None:
+        """Candidates per refine work item by tile size (default DEFAULT_CPW)."""
+        torch = self.torch
+        wt, wj, cpwl = refine_work(self.tiles.ts, self.k, cpw)
+        self.cpw = dict(zip((8, 16, 32, 64), cpwl))
+        self.refine_work = (torch.from_numpy(wt).to(self.dev), torch.from_numpy(wj).to(self.dev), cpwl)
+
+    def autotune(self) -> dict[int, int]:
+        """Time the refinement's work split per tile size on this round's candidates and
+        keep the fastest (the fit's results do not depend on it)."""
+        s = self.step_idx
+        self.score(s)
+        topk(self, self.d_mse, self.n_cand, self.k, self.d_topk)
+        best = autotune_cpw(self, self._cands(s), self.d_topk, self.k, self.adam)
+        self.set_cpw(best)
+        return best
 
     def _rcands(self, step: int) -> _lib.Cands:
         return _lib.Cands(self.d_rprims.data_ptr(), None, self.seed, step, self.k + 1, 0, 0)
@@ -219,18 +349,16 @@ class RefinedFitter(PlainFitter):
             self.score(s, sp)                                    # exact scores of the drawn set
             topk(self, self.d_mse, self.n_cand, self.k, self.d_topk, sp)
             refine(self, self._cands(s), self.d_topk, self.k, self.adam, self.d_rcodes, self.d_rprims,
-                   stream=sp)
+                   stream=sp, work=self.refine_work)
             rc = self._rcands(s)
-            _lib.check(self.L.weft_fit_order(ctypes.byref(self._state), ctypes.byref(rc),
-                                             self.d_rorder.data_ptr(), sp), "weft_fit_order")
-            _lib.check(self.L.weft_fit_score(ctypes.byref(self._state), ctypes.byref(self._rwork),
-                                             ctypes.byref(rc), self.d_rorder.data_ptr(), None, None,
-                                             self.d_rpart_obj.data_ptr(), self.d_rpart_idx.data_ptr(), sp),
-                       "weft_fit_score")
+            _lib.check(self.L.weft_fit_score_split(ctypes.byref(self._state), ctypes.byref(rc),
+                                                   ctypes.byref(self._split), None, None,
+                                                   self.d_rpart_obj.data_ptr(), self.d_rpart_idx.data_ptr(), sp),
+                       "weft_fit_score_split")
+            w0, nw, ws = self._split_arrays[2], self._split_arrays[3], self._split_arrays[4]
             _lib.check(self.L.weft_fit_select(
-                ctypes.byref(self._state), ctypes.byref(rc), self.d_rtile_work0.data_ptr(),
-                self.d_rtile_nwork.data_ptr(), self.d_rtile_wstride.data_ptr(), self.d_rpart_obj.data_ptr(),
-                self.d_rpart_idx.data_ptr(),
+                ctypes.byref(self._state), ctypes.byref(rc), w0.data_ptr(), nw.data_ptr(), ws.data_ptr(),
+                self.d_rpart_obj.data_ptr(), self.d_rpart_idx.data_ptr(),
                 self.d_win_idx.data_ptr() + 4 * T * slot, self.d_win_obj.data_ptr() + 8 * T * slot,
                 self.d_win_acc.data_ptr() + 4 * T * slot, None, sp), "weft_fit_select")
             # winner codes: slot win_idx of the tile's k + 1 refined codes

@@ -16,7 +16,7 @@ cpu_baseline = the CPU oracle (C, float32, OpenMP) on a bounded sample of the sa
 
 Further objects on the same line (skipped with --no-extras):
 - time_to_psnr: on the same image, the plain fit (1024 candidates, 16 rounds, one
-  persistent launch) against the refined fit (512 drawn candidates, top 8 refined by 10
+  persistent launch) against the refined fit (512 drawn candidates, top 4 refined by 10
   Adam steps, exact re-score): wall time until the refined fit reaches the plain fit's
   final PSNR, and both fits' PSNR at the end and at equal primitive bytes.
 - time_to_psnr_4k: the same at 3840x2160 (2500 primitives + value-noise texture),
@@ -46,6 +46,9 @@ PEAK_FP32 = 157.3e12
 # Algorithmic FLOP per candidate-pixel evaluation (add/sub/mul/div = 1, FMA-shaped = 2,
 # compares 0, per-candidate constants hoisted): coverage + composite (10) + SSE (9).
 FLOP_PER_EVAL = {"triangle": 32, "ellipse": 33, "rect": 28}
+# Soft-edge forward + backward per candidate-pixel (same rules; sigmoid = exp + add + rcp):
+# ellipse 80, rectangle 114, triangle 96, in the drawn family mix.
+SOFT_FLOP_MIX = (80 * 21845 + 114 * 21845 + 96 * 21846) / 65536
 
 
 def log(*a):
@@ -242,8 +245,11 @@ def refine_report(seed, lam):
         times.append(a.elapsed_time(b) * 1e-3)
     t = float(np.median(times))
     cand_px = tiles.n_pixels * k * iters
-    return dict(tiles=int(tiles.n_tiles), tile_size=32, n_cand=256, top_k=k, adam_steps=iters, cands_per_work_item=cpw[32],
-                kernel="weft_refine_kernel", refine_ms=t * 1e3, cand_px_fwd_bwd_per_s=cand_px / t,
+    rate = cand_px / t
+    return dict(flop_per_cand_px=SOFT_FLOP_MIX, achieved_tflops=rate * SOFT_FLOP_MIX / 1e12,
+                frac_of_fp32_peak=rate * SOFT_FLOP_MIX / PEAK_FP32,
+                tiles=int(tiles.n_tiles), tile_size=32, n_cand=256, top_k=k, adam_steps=iters, cands_per_work_item=cpw[32],
+                kernel="weft_refine_kernel", refine_ms=t * 1e3, cand_px_fwd_bwd_per_s=rate,
                 round1_refined_better_frac=float((o_r < o_p).mean()),
                 round1_objective_ratio_mean=float((o_r / o_p).mean()))
 
@@ -367,7 +373,7 @@ def main():
     extras = {}
     if not args.no_extras:
         mask = valid_mask(tiles, [c["height"]], [c["width"]])
-        extras["time_to_psnr"] = time_to_psnr(tiles, mask, c["n_cand"], c["depth"], seed, lam, (512, 8, 10))
+        extras["time_to_psnr"] = time_to_psnr(tiles, mask, c["n_cand"], c["depth"], seed, lam, (512, 4, 10))
         t = extras["time_to_psnr"]
         log(f"[bench] time-to-PSNR 1080p: plain {t['plain']['psnr_db']:.2f} dB in {t['plain']['wall_s']:.3f} s; "
             f"speedup {t['speedup_to_plain_psnr']}")
@@ -378,7 +384,7 @@ def main():
         img4 = synth_image(2160, 3840, 2500, seed=0, texture=True)
         t4 = TileSet.from_images([img4], [decompose_quadtree_variance(img4, threshold=1e-3, min_tile=8, max_tile=64)])
         m4 = valid_mask(t4, [2160], [3840])
-        extras["time_to_psnr_4k"] = time_to_psnr(t4, m4, 2048, 24, seed, lam, (512, 8, 10))
+        extras["time_to_psnr_4k"] = time_to_psnr(t4, m4, 2048, 24, seed, lam, (512, 4, 10))
         extras["time_to_psnr_4k"]["tiles"] = int(t4.n_tiles)
         t = extras["time_to_psnr_4k"]
         log(f"[bench] time-to-PSNR 4k: plain {t['plain']['psnr_db']:.2f} dB in {t['plain']['wall_s']:.3f} s; "

@@ -289,3 +289,35 @@ def test_split_work_tables():
             items = [int(w0[t]) + j * int(st[t]) for j in range(int(nw[t]))]
             assert all(wt[i] == t for i in items)
             assert sorted(int(wc[i]) for i in items) == list(range(0, n, cpd[int(ts[t])]))
+
+
+def test_quantize_primitive_equals_decode_of_encode():
+    """quantize_primitive (encoder.py:211-237) = decoding the encoded record (the record
+    codec is pinned by the reference's encode_tile fixtures); idempotent; within half a
+    step of the input."""
+    from weft_amd.primitives import Primitive, decode_primitive, encode_primitive, quantize_primitive
+
+    rng = np.random.default_rng(7)
+    geoms = {0: 0, 1: 4, 2: 5, 3: 7, 4: 6, 6: 5, 7: 5, 8: 5}
+    for _ in range(300):
+        kind = int(rng.choice(list(geoms)))
+        n = geoms[kind]
+        if kind in (6, 7, 8):
+            g = tuple(rng.uniform(0, 30, 4).tolist()) + (float(rng.uniform(0, np.pi)),)
+        else:
+            g = tuple(rng.uniform(-1, 16, n).tolist())
+            if kind in (2, 3):
+                g = g[:-1] + (float(rng.uniform(0, 5)),)
+        c1 = tuple(rng.uniform(-0.1, 1.1, 3).tolist()) if kind == 1 else None
+        p = Primitive(kind=kind, geom=g, color0=tuple(rng.uniform(-0.1, 1.1, 3).tolist()), color1=c1,
+                      alpha=float(rng.uniform(-0.1, 1.1)))
+        q = quantize_primitive(p)
+        d, _ = decode_primitive(encode_primitive(p))
+        assert q.alpha == d.alpha and q.color0 == d.color0 and q.color1 == d.color1
+        if kind != 0:
+            np.testing.assert_array_equal(np.array(q.geom), np.array(d.geom))
+        q2 = quantize_primitive(q)
+        assert q2.geom == q.geom and q2.color0 == q.color0 and q2.alpha == q.alpha
+        assert abs(q.alpha - min(1, max(0, p.alpha))) <= 0.5 / 65535 + 1e-12
+        for a, b in zip(q.color0, p.color0):
+            assert abs(a - min(1, max(0, b))) <= 0.5 / 255 + 1e-12

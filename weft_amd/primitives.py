@@ -115,6 +115,40 @@ def _theta_code(theta: float) -> int:
     return int(round(theta * THETA_STEPS / math.pi)) % THETA_STEPS
 
 
+def _snap_color(c) -> tuple[float, float, float]:
+    return (dq_u8(q_u8(c[0])), dq_u8(q_u8(c[1])), dq_u8(q_u8(c[2])))
+
+
+def quantize_primitive(prim: Primitive) -> Primitive:
+    """Snap a primitive's values to its PRIM grid without serializing.
+
+    Reference: src/weft/encoder.py:211-237 (_quantize_primitive): alpha and colours to
+    u16 / u8 steps (half-even rounding of clip(v) * steps), kind-2/3 thickness to u16 over
+    [0, 4], every other geometry value of kinds 1-5 to u16 over [0, 15]; const patches keep
+    their geometry.  The parametric kinds 6-8 snap to their code grid (1/1024 px, pi/4096).
+    Equal to decoding the primitive's encoded record.
+    """
+    kind = primitive_kind(prim.kind)
+    g = tuple(prim.geom)
+    snap15 = lambda v: dq_u16(q_u16(v, 0.0, 15.0), 0.0, 15.0)  # noqa: E731
+    snap4 = lambda v: dq_u16(q_u16(v, 0.0, 4.0), 0.0, 4.0)  # noqa: E731
+    if kind == PRIM_CONST_PATCH:
+        geom = g
+    elif kind == PRIM_LINE:
+        geom = tuple(snap15(v) for v in g[:4]) + tuple(snap4(v) for v in g[4:5])
+    elif kind == PRIM_QUAD_CURVE:
+        geom = tuple(snap15(v) for v in g[:6]) + tuple(snap4(v) for v in g[6:7])
+    elif kind in PARAMETRIC_KINDS:
+        geom = (_coord_code(g[0]) / COORD_SCALE, _coord_code(g[1]) / COORD_SCALE,
+                _coord_code(g[2]) / COORD_SCALE, _coord_code(g[3]) / COORD_SCALE,
+                _theta_code(g[4]) * math.pi / THETA_STEPS)
+    else:
+        geom = tuple(snap15(v) for v in g)
+    return Primitive(kind=kind, geom=geom, color0=_snap_color(prim.color0),
+                     color1=_snap_color(prim.color1) if prim.color1 is not None else None,
+                     alpha=dq_u16(q_u16(prim.alpha, 0.0, 1.0), 0.0, 1.0))
+
+
 def encode_primitive(prim: Primitive) -> bytes:
     kind = primitive_kind(prim.kind)
     alpha = q_u16(prim.alpha, 0.0, 1.0)

@@ -27,6 +27,10 @@ Further objects on the same line (skipped with --no-extras):
 - refine: 4096 tiles of 32 px from the 256x256 generator, 256 drawn candidates, the top
   64 refined by 20 Adam steps; the refinement's time and candidate-pixel rate (forward
   and backward) and how much the first round improves on plain search.
+- drop_in: the reference API gpu_batch_score_prepacked on 32 stacks of 4 primitives per
+  tile of the 1920x1080 quadtree: device-resident and host-inclusive rates.
+- decode: gpu_render_tiles_to_image of the 3840x2160 plain fit's primitives.
+- entropy: chunked rANS encode and decode (64 tiles per chunk, the reference default) of that fit's PRIM bytes.
 """
 from __future__ import annotations
 
@@ -153,8 +157,9 @@ def time_to_psnr(tiles, mask, n_cand_plain, depth, seed, lam, refined_cfg):
 
     bf = BatchFitter(tiles, n_cand=n_cand_plain, depth=depth, n_steps=depth, seed=seed, lam=lam)
     times = bf.time_runs(3, warmup=1)
+    plain_records = bf.records()
     plain = dict(n_cand=n_cand_plain, rounds=depth, wall_s=min(times), psnr_db=psnr_db(bf, mask),
-                 prims=int(bf.records()["win_acc"].sum()))
+                 prims=int(plain_records["win_acc"].sum()))
     plain["prim_bytes"] = 17 * plain["prims"]
     del bf
     n_cand, k, iters = refined_cfg
@@ -183,7 +188,140 @@ def time_to_psnr(tiles, mask, n_cand_plain, depth, seed, lam, refined_cfg):
                    prims=curve[-1][2], prim_bytes=17 * curve[-1][2], reached_plain_psnr=reach,
                    psnr_at_plain_bytes=psnr_at_bytes, curve=curve)
     speed = plain["wall_s"] / reach["wall_s"] if reach else None
-    return dict(plain=plain, refined=refined, speedup_to_plain_psnr=speed)
+    return dict(plain=plain, refined=refined, speedup_to_plain_psnr=speed), plain_records
+
+
+def drop_in_report(tiles, seed):
+    """The reference API (gpu_batch_score_prepacked): 32 stacks of 4 drawn primitives per
+    tile of the 1920x1080 quadtree, host arrays in and out."""
+    import torch
+    from weft_amd import _lib
+    from weft_amd.fit import draw_codes
+    from weft_amd.gpu_render import gpu_batch_score_prepacked
+    from weft_amd.records import EVAL_DTYPE
+
+    T, P, D = tiles.n_tiles, 32, 4
+    _, prims = draw_codes(np.arange(T, dtype=np.int32), tiles.ts, P * D, seed=seed, step=1000)
+    arrays = [prims[t * P * D:(t + 1) * P * D] for t in range(T)]
+    counts = [np.full(P, D, np.int32)] * T
+    offsets = {t: int(tiles.pix_off[t]) * 3 for t in range(T)}
+    args = (tiles.target, offsets, list(range(T)), [int(v) for v in tiles.ts], arrays, counts)
+    gpu_batch_score_prepacked(*args)  # warm
+    host = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = gpu_batch_score_prepacked(*args)
+        host.append(time.perf_counter() - t0)
+    assert out is not None and len(out) == T
+    # device-resident: the same evals with inputs already in HBM, HIP events around the launch
+    ev = np.zeros(T * P, EVAL_DTYPE)
+    ev["prim_offset"] = np.arange(T * P) * D
+    ev["prim_count"] = D
+    ev["tile_offset"] = np.repeat(tiles.pix_off.astype(np.int64) * 3, P)
+    ev["tile_size"] = np.repeat(tiles.ts, P)
+    d_ev = torch.from_numpy(ev.view(np.uint8)).cuda()
+    d_pr = torch.from_numpy(np.ascontiguousarray(prims).view(np.uint8)).cuda()
+    d_t = torch.from_numpy(tiles.target).cuda()
+    d_out = torch.empty(T * P, dtype=torch.float32, device="cuda")
+    L = _lib.library()
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(4):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        _lib.check(L.weft_score_evals(d_ev.data_ptr(), T * P, d_pr.data_ptr(), d_t.data_ptr(), d_out.data_ptr(),
+                                      None, _lib.stream_ptr()), "weft_score_evals")
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) * 1e-3)
+    dev_s = float(np.median(times[1:]))
+    same = bool(np.array_equal(d_out.cpu().numpy(), np.concatenate(out)))
+    px = int((tiles.ts.astype(np.int64) ** 2).sum()) * P
+    return dict(api="gpu_batch_score_prepacked", tiles=T, evals=T * P, stack_depth=D, kernel="weft_score_evals_kernel",
+                device_s=dev_s, evals_per_s=T * P / dev_s, eval_px_per_s=px / dev_s,
+                eval_px_prim_per_s=px * D / dev_s, host_inclusive_s=min(host),
+                host_inclusive_evals_per_s=T * P / min(host), device_equals_api=same)
+
+
+def decode_report(tiles, records, height, width):
+    """Decode walk (gpu_render_tiles_to_image) of a fitted image's accepted primitives."""
+    import torch
+    from weft_amd import _lib
+    from weft_amd.gpu_render import gpu_render_tiles_to_image
+    from weft_amd.records import PRIM_DTYPE, TILE_DECODE_DTYPE, codes_to_records
+
+    acc, codes = records["win_acc"].astype(bool), records["win_code"]
+    per_tile = [codes[acc[:, t], t] for t in range(tiles.n_tiles)]
+    counts = np.array([len(c) for c in per_tile], np.int32)
+    prims = codes_to_records(np.concatenate(per_tile)) if counts.sum() else np.zeros(1, PRIM_DTYPE)
+    descs = np.zeros(tiles.n_tiles, TILE_DECODE_DTYPE)
+    descs["prim_offset"] = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    descs["prim_count"] = counts
+    descs["target_x0"], descs["target_y0"] = tiles.x, tiles.y
+    descs["target_w"] = descs["target_h"] = descs["source_size"] = tiles.ts
+    gpu_render_tiles_to_image(descs, prims, height, width)  # warm
+    t0 = time.perf_counter()
+    img = gpu_render_tiles_to_image(descs, prims, height, width)
+    host = time.perf_counter() - t0
+    d_d = torch.from_numpy(descs.view(np.uint8)).cuda()
+    d_p = torch.from_numpy(np.ascontiguousarray(prims).view(np.uint8)).cuda()
+    d_o = torch.zeros(height * width * 3, dtype=torch.float32, device="cuda")
+    L = _lib.library()
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(4):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        _lib.check(L.weft_render_tiles(d_d.data_ptr(), tiles.n_tiles, d_p.data_ptr(), d_o.data_ptr(), width, height,
+                                       _lib.stream_ptr()), "weft_render_tiles")
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) * 1e-3)
+    dev_s = float(np.median(times[1:]))
+    same = bool(np.array_equal(d_o.cpu().numpy().reshape(height, width, 3), img))
+    return dict(api="gpu_render_tiles_to_image", width=width, height=height, tiles=int(tiles.n_tiles),
+                prims=int(counts.sum()), kernel="weft_render_tiles_kernel", device_s=dev_s,
+                mpx_per_s=height * width / dev_s / 1e6, host_inclusive_s=host, device_equals_api=same)
+
+
+def entropy_report(tiles, records, chunk_tiles=64):
+    """Chunked rANS of a fitted image's PRIM payload, encode and decode on the device."""
+    import torch
+    from weft_amd import _lib
+    from weft_amd.entropy import decode_chunks_device, encode_chunks_device
+    from weft_amd.primitives import encode_code_tiles
+
+    acc, codes = records["win_acc"].astype(bool), records["win_code"]
+    raw, toc = encode_code_tiles([list(codes[acc[:, t], t]) for t in range(tiles.n_tiles)])
+    T = len(toc) - 1
+    starts = np.arange(0, T, chunk_tiles)
+    ends = np.minimum(T, starts + chunk_tiles)
+    toc_a = np.asarray(toc, np.int64)
+    descs = np.zeros((len(starts), 6), np.int32)
+    descs[:, 0], descs[:, 1] = starts, ends - starts
+    descs[:, 2], descs[:, 3] = toc_a[starts], toc_a[ends] - toc_a[starts]
+    d_raw = torch.from_numpy(np.frombuffer(raw, np.uint8).copy()).cuda()
+    stream = torch.cuda.current_stream()
+    enc_t, dec_t = [], []
+    for _ in range(4):
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a.record(stream)
+        d_coded, out, total = encode_chunks_device(d_raw, descs)
+        b.record(stream)
+        back = decode_chunks_device(d_coded, total, out, len(raw))
+        c.record(stream)
+        torch.cuda.synchronize()
+        enc_t.append(a.elapsed_time(b) * 1e-3)
+        dec_t.append(b.elapsed_time(c) * 1e-3)
+    same = back[:len(raw)].cpu().numpy().tobytes() == raw
+    e, dd = float(np.median(enc_t[1:])), float(np.median(dec_t[1:]))
+    return dict(payload="PRIM bytes of the 3840x2160 plain fit", raw_bytes=len(raw), coded_bytes=int(total),
+                ratio=len(raw) / max(1, int(total)), chunks=len(descs), chunk_tiles=chunk_tiles,
+                encode_s=e, decode_s=dd, encode_mb_per_s=len(raw) / e / 1e6, decode_mb_per_s=len(raw) / dd / 1e6,
+                roundtrip_exact=bool(same),
+                note="encode/decode times include the host-side launches and index transfers of the calls")
 
 
 def batched_report(seed, lam):
@@ -373,7 +511,7 @@ def main():
     extras = {}
     if not args.no_extras:
         mask = valid_mask(tiles, [c["height"]], [c["width"]])
-        extras["time_to_psnr"] = time_to_psnr(tiles, mask, c["n_cand"], c["depth"], seed, lam, (512, 4, 10))
+        extras["time_to_psnr"], _ = time_to_psnr(tiles, mask, c["n_cand"], c["depth"], seed, lam, (512, 4, 10))
         t = extras["time_to_psnr"]
         log(f"[bench] time-to-PSNR 1080p: plain {t['plain']['psnr_db']:.2f} dB in {t['plain']['wall_s']:.3f} s; "
             f"speedup {t['speedup_to_plain_psnr']}")
@@ -384,12 +522,20 @@ def main():
         img4 = synth_image(2160, 3840, 2500, seed=0, texture=True)
         t4 = TileSet.from_images([img4], [decompose_quadtree_variance(img4, threshold=1e-3, min_tile=8, max_tile=64)])
         m4 = valid_mask(t4, [2160], [3840])
-        extras["time_to_psnr_4k"] = time_to_psnr(t4, m4, 2048, 24, seed, lam, (512, 4, 10))
+        extras["time_to_psnr_4k"], rec4 = time_to_psnr(t4, m4, 2048, 24, seed, lam, (512, 4, 10))
         extras["time_to_psnr_4k"]["tiles"] = int(t4.n_tiles)
         t = extras["time_to_psnr_4k"]
         log(f"[bench] time-to-PSNR 4k: plain {t['plain']['psnr_db']:.2f} dB in {t['plain']['wall_s']:.3f} s; "
             f"speedup {t['speedup_to_plain_psnr']}")
-        del t4, m4
+        extras["decode"] = decode_report(t4, rec4, 2160, 3840)
+        extras["entropy"] = entropy_report(t4, rec4)
+        log(f"[bench] entropy 4k PRIM: ratio {extras['entropy']['ratio']:.2f}, encode "
+            f"{extras['entropy']['encode_mb_per_s']:.0f} MB/s, decode {extras['entropy']['decode_mb_per_s']:.0f} MB/s")
+        log(f"[bench] decode 4k: {extras['decode']['mpx_per_s']:.0f} Mpx/s on the device")
+        del t4, m4, rec4
+        extras["drop_in"] = drop_in_report(tiles, seed)
+        log(f"[bench] drop-in scorer: {extras['drop_in']['evals_per_s'] / 1e6:.1f} M evals/s on the device, "
+            f"{extras['drop_in']['host_inclusive_evals_per_s'] / 1e6:.1f} M/s through the host API")
         extras["batched"] = batched_report(seed, lam)
         log(f"[bench] batched: {extras['batched']['evals_per_s'] / 1e12:.3f} T evals/s, "
             f"p50 {extras['batched']['p50_latency_ms']:.1f} ms, p99 {extras['batched']['p99_latency_ms']:.1f} ms")

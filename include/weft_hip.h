@@ -294,6 +294,36 @@ int weft_fit_score_split(const WeftFitState* st, const WeftCands* cands, const W
                          float* d_out_sse, float* d_out_mse, double* d_part_obj, int32_t* d_part_idx,
                          void* stream);
 
+/* ---- Chunked PRIM entropy coding (static order-0 rANS) ----------------------------
+ * One chunk = the PRIM bytes of consecutive tiles, coded independently in the format of
+ * src/weft/entropy.py: raw [0][u32 n][bytes] or rANS [1][u32 n][u16 n_pairs]
+ * [(u8 sym, u16 freq)...][u32 state][u32 n_emit][emitted], 12-bit frequencies. */
+typedef struct WeftChunkDesc {       /* ChunkDesc of src/weft/gpu_entropy.py:72-79 */
+    int32_t start_tile;
+    int32_t tile_count;
+    int32_t raw_offset;
+    int32_t raw_length;
+    int32_t coded_offset;
+    int32_t coded_length;
+} WeftChunkDesc;
+
+/* Decode every chunk (replaces weft_decode_chunks, src/weft/gpu_entropy.py:90-211):
+ * d_status[i] = 0, or the reference decoder's error code for chunk i. */
+int weft_rans_decode_chunks(const uint8_t* d_coded, const WeftChunkDesc* d_descs, int n_chunks,
+                            uint8_t* d_out_raw, int32_t* d_status, void* stream);
+
+/* Encode every chunk as encode_bytes would (src/weft/entropy.py:190-198, called per chunk
+ * by build_prim_chunks, src/weft/prim_chunks.py:23-78) and pack the payloads back to
+ * back.  In d_descs the caller fills start_tile, tile_count, raw_offset and raw_length;
+ * the call fills coded_offset and coded_length.  Chunk i is staged at
+ * d_stage + d_stage_off[i] (at least weft_rans_stage_bytes(raw_length) bytes); d_coded
+ * receives the packed stream (at most the sum of the staging sizes) and *d_coded_total
+ * its length.  d_status[i] = 0, or 12 if a symbol's frequency came out zero. */
+int64_t weft_rans_stage_bytes(int64_t raw_length);
+int weft_rans_encode_chunks(const uint8_t* d_raw, WeftChunkDesc* d_descs, int n_chunks,
+                            uint8_t* d_stage, const int64_t* d_stage_off, uint8_t* d_coded,
+                            int64_t* d_coded_total, int32_t* d_status, void* stream);
+
 /* Host-side copies of the candidate tables (log-radius codes, cos, sin: 4096 each). */
 int weft_get_tables(uint16_t* logr, float* cosv, float* sinv);
 

@@ -71,6 +71,11 @@ def lib() -> ctypes.CDLL:
     L.orc_fit_plain_ids.restype = ctypes.c_int
     L.orc_score_candidates.argtypes = [P, P, ctypes.c_int, P, ctypes.c_int, P, P]
     L.orc_max_threads.restype = ctypes.c_int
+    L.orc_rans_model.argtypes = [P, ctypes.c_int64, P]
+    L.orc_rans_encode.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64]
+    L.orc_rans_encode.restype = ctypes.c_int64
+    L.orc_rans_decode.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64]
+    L.orc_rans_decode.restype = ctypes.c_int64
     _lib = L
     return L
 
@@ -194,6 +199,35 @@ class PlainFit:
                                 None if self.tile_ids is None else _p(self.tile_ids))
         self.step += n_steps
         return dict(sse=sse, win_idx=widx, win_obj=wobj, win_acc=wacc, win_code=wcode)
+
+
+def rans_model(data: bytes) -> np.ndarray:
+    a = np.frombuffer(bytes(data), np.uint8)
+    freqs = np.zeros(256, np.int32)
+    lib().orc_rans_model(_p(np.ascontiguousarray(a)) if len(a) else None, len(a), _p(freqs))
+    return freqs
+
+
+def rans_encode(data: bytes) -> bytes:
+    """encode_bytes (src/weft/entropy.py:190-198)."""
+    a = np.frombuffer(bytes(data), np.uint8)
+    cap = 2 * len(a) + 1024
+    out = np.zeros(cap, np.uint8)
+    n = lib().orc_rans_encode(_p(np.ascontiguousarray(a)) if len(a) else None, len(a), _p(out), cap)
+    if n < 0:
+        raise ValueError("oracle rANS encode failed")
+    return out[:n].tobytes()
+
+
+def rans_decode(payload: bytes, cap: int | None = None) -> bytes:
+    """decode_bytes (src/weft/entropy.py:201-217); raises ValueError with the error code."""
+    a = np.frombuffer(bytes(payload), np.uint8)
+    cap = cap if cap is not None else max(16, 16 * len(a) + (1 << 20))
+    out = np.zeros(cap, np.uint8)
+    n = lib().orc_rans_decode(_p(np.ascontiguousarray(a)) if len(a) else None, len(a), _p(out), cap)
+    if n < 0:
+        raise ValueError(f"oracle rANS decode error {n}")
+    return out[:n].tobytes()
 
 
 def max_threads() -> int:

@@ -16,6 +16,9 @@
  *   encoder._estimate_bits + LUT  src/weft/encoder.py:242-268
  *   encoder._greedy_add           src/weft/encoder.py:752-817
  *   cuda_backend.cccl_argmin      src/weft/cuda_backend.py:80-98 (first index on ties)
+ *   entropy._build_model          src/weft/entropy.py:30-87   (order-0 frequency model)
+ *   entropy._encode_rans / encode_bytes   src/weft/entropy.py:90-122, 190-198
+ *   entropy._decode_rans / decode_bytes   src/weft/entropy.py:125-187, 201-217
  *
  * Float semantics.  NumPy 2 (NEP 50) evaluates `float32_array OP python_float` in
  * float32 after rounding the Python float to float32; Python-scalar sub-expressions
@@ -702,4 +705,190 @@ int orc_max_threads(void) {
 #else
     return 1;
 #endif
+}
+
+/* ---- Static order-0 rANS (src/weft/entropy.py) ----------------------------------
+ * 12-bit probabilities, 32-bit state in [2^23, 2^31), bytes emitted low first while
+ * encoding backwards; the decoder reads the emitted bytes from the end. */
+#define ORC_PROB_BITS 12
+#define ORC_TOTFREQ (1 << ORC_PROB_BITS)
+#define ORC_RANS_L (1u << 23)
+
+/* _build_model: counts -> frequencies summing to 4096 (entropy.py:30-87). */
+void orc_rans_model(const uint8_t* data, int64_t n, int32_t* freqs /* 256 */) {
+    int64_t counts[256] = {0};
+    for (int64_t i = 0; i < n; ++i) counts[data[i]] += 1;
+    for (int s = 0; s < 256; ++s) freqs[s] = 0;
+    if (n <= 0) {
+        freqs[0] = ORC_TOTFREQ;
+        return;
+    }
+    int64_t cur = 0;
+    for (int s = 0; s < 256; ++s) {
+        if (counts[s] > 0) {
+            /* max(1, int(round((c / total) * TOTFREQ))): Python rounds half to even */
+            double r = nearbyint(((double)counts[s] / (double)n) * (double)ORC_TOTFREQ);
+            freqs[s] = r < 1.0 ? 1 : (int32_t)r;
+        }
+        cur += freqs[s];
+    }
+    if (cur == 0) {
+        freqs[0] = ORC_TOTFREQ;
+        cur = ORC_TOTFREQ;
+    }
+    int64_t diff = ORC_TOTFREQ - cur;
+    /* sorted(range(256), key=counts, reverse=True): stable, ties keep ascending symbol */
+    int order[256];
+    for (int s = 0; s < 256; ++s) order[s] = s;
+    for (int i = 1; i < 256; ++i) {  /* insertion sort, stable */
+        int v = order[i], j = i - 1;
+        while (j >= 0 && counts[order[j]] < counts[v]) {
+            order[j + 1] = order[j];
+            --j;
+        }
+        order[j + 1] = v;
+    }
+    int64_t k = 0;
+    while (diff != 0 && k < 100000) {
+        int idx = order[k % 256];
+        if (counts[idx] == 0) idx = 0;
+        if (diff > 0) {
+            freqs[idx] += 1;
+            diff -= 1;
+        } else if (freqs[idx] > 1) {
+            freqs[idx] -= 1;
+            diff += 1;
+        }
+        k += 1;
+    }
+    int64_t sum = 0;
+    for (int s = 0; s < 256; ++s) sum += freqs[s];
+    if (sum != ORC_TOTFREQ) freqs[0] += (int32_t)(ORC_TOTFREQ - sum);
+}
+
+static void put_u32(uint8_t* p, uint32_t v) {
+    p[0] = (uint8_t)v; p[1] = (uint8_t)(v >> 8); p[2] = (uint8_t)(v >> 16); p[3] = (uint8_t)(v >> 24);
+}
+static uint32_t get_u32(const uint8_t* p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+}
+
+/* encode_bytes (entropy.py:190-198): rANS when strictly shorter than the raw form.
+ * Returns the payload length, or -1 if `cap` is too small / a frequency is zero. */
+int64_t orc_rans_encode(const uint8_t* data, int64_t n, uint8_t* out, int64_t cap) {
+    const int64_t raw_len = 5 + n;
+    if (n > 0) {
+        int32_t freqs[256];
+        int32_t cum[257];
+        orc_rans_model(data, n, freqs);
+        int32_t run = 0;
+        for (int s = 0; s < 256; ++s) {
+            cum[s] = run;
+            run += freqs[s];
+        }
+        cum[256] = run;
+        int n_pairs = 0;
+        for (int s = 0; s < 256; ++s) n_pairs += freqs[s] > 0;
+        const int64_t hdr = 1 + 4 + 2 + 3 * (int64_t)n_pairs + 4 + 4;
+        uint8_t* em = (uint8_t*)malloc((size_t)(2 * n + 16));
+        if (!em) return -1;
+        int64_t ne = 0;
+        uint64_t state = ORC_RANS_L;
+        for (int64_t i = n - 1; i >= 0; --i) {
+            const int sym = data[i];
+            const int64_t freq = freqs[sym];
+            if (freq <= 0) {
+                free(em);
+                return -1;
+            }
+            const uint64_t x_max = ((uint64_t)(ORC_RANS_L >> ORC_PROB_BITS) << 8) * (uint64_t)freq;
+            while (state >= x_max) {
+                em[ne++] = (uint8_t)(state & 0xFF);
+                state >>= 8;
+            }
+            state = ((state / (uint64_t)freq) << ORC_PROB_BITS) + (state % (uint64_t)freq) + (uint64_t)cum[sym];
+        }
+        const int64_t rans_len = hdr + ne;
+        if (rans_len < raw_len) {
+            if (rans_len > cap) {
+                free(em);
+                return -1;
+            }
+            uint8_t* p = out;
+            *p++ = 1;
+            put_u32(p, (uint32_t)n); p += 4;
+            p[0] = (uint8_t)n_pairs; p[1] = (uint8_t)(n_pairs >> 8); p += 2;
+            for (int s = 0; s < 256; ++s) {
+                if (freqs[s] > 0) {
+                    p[0] = (uint8_t)s; p[1] = (uint8_t)freqs[s]; p[2] = (uint8_t)(freqs[s] >> 8);
+                    p += 3;
+                }
+            }
+            put_u32(p, (uint32_t)state); p += 4;
+            put_u32(p, (uint32_t)ne); p += 4;
+            for (int64_t i = 0; i < ne; ++i) p[i] = em[i];
+            free(em);
+            return rans_len;
+        }
+        free(em);
+    }
+    if (raw_len > cap) return -1;
+    out[0] = 0;
+    put_u32(out + 1, (uint32_t)n);
+    for (int64_t i = 0; i < n; ++i) out[5 + i] = data[i];
+    return raw_len;
+}
+
+/* decode_bytes (entropy.py:201-217, 125-187).  Returns the decoded length, or a
+ * negative code: -2 truncated, -3 raw length mismatch, -4 unknown mode, -5 truncated
+ * model, -6 frequency sum, -7 emitted length mismatch, -8 output capacity. */
+int64_t orc_rans_decode(const uint8_t* pl, int64_t n, uint8_t* out, int64_t cap) {
+    if (n == 0) return 0;
+    const int mode = pl[0];
+    if (mode == 0) {
+        if (n < 5) return -2;
+        const int64_t len = get_u32(pl + 1);
+        if (5 + len != n) return -3;
+        if (len > cap) return -8;
+        for (int64_t i = 0; i < len; ++i) out[i] = pl[5 + i];
+        return len;
+    }
+    if (mode != 1) return -4;
+    if (n < 7) return -2;
+    const int64_t size = get_u32(pl + 1);
+    const int n_syms = pl[5] | (pl[6] << 8);
+    int64_t off = 7;
+    if (off + 3 * (int64_t)n_syms + 8 > n) return -5;
+    int32_t freqs[256] = {0};
+    for (int i = 0; i < n_syms; ++i) {
+        freqs[pl[off]] = pl[off + 1] | (pl[off + 2] << 8);
+        off += 3;
+    }
+    int32_t cum[257];
+    int32_t run = 0;
+    for (int s = 0; s < 256; ++s) {
+        cum[s] = run;
+        run += freqs[s];
+    }
+    cum[256] = run;
+    if (run != ORC_TOTFREQ) return -6;
+    uint8_t lookup[ORC_TOTFREQ];
+    for (int s = 0; s < 256; ++s)
+        for (int32_t q = cum[s]; q < cum[s] + freqs[s]; ++q) lookup[q] = (uint8_t)s;
+    uint32_t state = get_u32(pl + off);
+    off += 4;
+    const int64_t ne = get_u32(pl + off);
+    off += 4;
+    if (off + ne != n) return -7;
+    if (size > cap) return -8;
+    const uint8_t* em = pl + off;
+    int64_t ptr = ne - 1;
+    for (int64_t i = 0; i < size; ++i) {
+        const uint32_t slot = state & (ORC_TOTFREQ - 1);
+        const int sym = lookup[slot];
+        out[i] = (uint8_t)sym;
+        state = (uint32_t)freqs[sym] * (state >> ORC_PROB_BITS) + (slot - (uint32_t)cum[sym]);
+        while (state < ORC_RANS_L && ptr >= 0) state = (state << 8) | em[ptr--];
+    }
+    return size;
 }

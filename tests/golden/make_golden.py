@@ -18,6 +18,10 @@ pinned against.  Functions captured (reference file:line):
 - ``encoder._estimate_bits``        src/weft/encoder.py:259-268
 - ``primitives.encode_tile``        src/weft/primitives.py:291-298
 - ``quadtree.decompose_quadtree``   src/weft/quadtree.py:42-91
+- ``entropy.encode_bytes``          src/weft/entropy.py:190-198 (static order-0 rANS or raw)
+- ``prim_chunks.build_prim_chunks`` src/weft/prim_chunks.py:23-78 (per-chunk coding + index)
+
+``python tests/golden/make_golden.py entropy`` writes only ``ref_entropy.npz``.
 
 Every primitive parameter is a float32-representable value, so the float32 records the
 HIP path receives carry exactly what the reference's Python floats carry.
@@ -252,13 +256,71 @@ def misc_cases(rng):
     return d
 
 
+def _bytes_table(blobs):
+    return (np.frombuffer(b"".join(blobs), np.uint8).copy(),
+            np.cumsum([0] + [len(b) for b in blobs]).astype(np.int64))
+
+
+def entropy_cases():
+    from weft.entropy import encode_bytes
+    from weft.prim_chunks import build_prim_chunks
+    from weft.primitives import Primitive, TileRecord, encode_tiles
+
+    rng = np.random.default_rng(77)
+    inputs = [b"", b"\x00", b"\xff", bytes([7]) * 1000, bytes(range(256)), bytes(range(256)) * 3]
+    for n in (1, 2, 7, 100, 257, 4096, 20000):
+        inputs.append(rng.integers(0, 256, n, dtype=np.uint8).tobytes())
+    for n, p in ((50, 0.5), (1000, 0.2), (5000, 0.05), (30000, 0.01), (3000, 0.9)):
+        inputs.append(np.minimum(rng.geometric(p, n) - 1, 255).astype(np.uint8).tobytes())
+    inputs.append(rng.choice([0, 1, 2, 200], 777, p=[0.7, 0.2, 0.09, 0.01]).astype(np.uint8).tobytes())
+    # PRIM payloads of random tiles (reference record layouts, kinds 0-4)
+    tiles = []
+    for _ in range(300):
+        prims = []
+        for _ in range(int(rng.integers(0, 6))):
+            kind = int(rng.choice([0, 1, 2, 3, 4]))
+            ng = {0: 0, 1: 4, 2: 5, 3: 7, 4: 6}[kind]
+            geom = tuple(float(v) for v in rng.uniform(0, 15, ng))
+            if kind in (2, 3):
+                geom = geom[:-1] + (float(rng.uniform(0, 4)),)
+            prims.append(Primitive(kind=kind, geom=geom, color0=tuple(float(v) for v in rng.random(3)),
+                                   color1=tuple(float(v) for v in rng.random(3)) if kind == 1 else None,
+                                   alpha=float(rng.random())))
+        tiles.append(TileRecord(primitives=prims))
+    prim_raw, toc = encode_tiles(tiles)
+    inputs.append(prim_raw)
+    coded = [encode_bytes(b) for b in inputs]
+    d = {}
+    d["in_bytes"], d["in_off"] = _bytes_table(inputs)
+    d["out_bytes"], d["out_off"] = _bytes_table(coded)
+    d["prim_raw"] = np.frombuffer(prim_raw, np.uint8).copy()
+    d["prim_toc"] = np.asarray(toc, np.int64)
+    chunk_payloads, chunk_rows, sizes = [], [], []
+    for chunk_tiles in (1, 4, 16, 1000):
+        payload, entries = build_prim_chunks(prim_raw, list(toc), chunk_tiles)
+        chunk_payloads.append(payload)
+        sizes.append(chunk_tiles)
+        chunk_rows.append(np.array([[e.start_tile, e.tile_count, e.raw_offset, e.raw_length, e.coded_offset,
+                                     e.coded_length] for e in entries], np.int64).reshape(-1, 6))
+    d["chunk_tiles"] = np.asarray(sizes, np.int64)
+    d["chunk_payload"], d["chunk_payload_off"] = _bytes_table(chunk_payloads)
+    d["chunk_entries"] = np.concatenate(chunk_rows)
+    d["chunk_entries_off"] = np.cumsum([0] + [len(r) for r in chunk_rows]).astype(np.int64)
+    return d
+
+
 def main() -> None:
     sys.path.insert(0, REF_SRC)
+    if len(sys.argv) > 1 and sys.argv[1] == "entropy":
+        np.savez_compressed(os.path.join(OUT_DIR, "ref_entropy.npz"), **entropy_cases())
+        print("wrote ref_entropy.npz to", OUT_DIR)
+        return
     rng = np.random.default_rng(20261203)
     np.savez_compressed(os.path.join(OUT_DIR, "ref_render_tile.npz"), **render_cases(rng))
     np.savez_compressed(os.path.join(OUT_DIR, "ref_tile_objective.npz"), **objective_cases(rng))
     np.savez_compressed(os.path.join(OUT_DIR, "ref_greedy_add.npz"), **greedy_cases(rng))
     np.savez_compressed(os.path.join(OUT_DIR, "ref_misc.npz"), **misc_cases(rng))
+    np.savez_compressed(os.path.join(OUT_DIR, "ref_entropy.npz"), **entropy_cases())
     print("wrote fixtures to", OUT_DIR)
 
 

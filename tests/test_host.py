@@ -14,7 +14,7 @@ from conftest import ROOT, bits_equal, golden_prims, load_golden
 
 def _header_functions() -> list[str]:
     text = open(os.path.join(ROOT, "include", "weft_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(weft_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(weft_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -120,6 +120,11 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_soft_eval.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(SoftBatch), c_void_p]
     L.weft_fit_score_split.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), ctypes.POINTER(SplitWork),
                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
+    L.weft_rans_stage_bytes.argtypes = [ctypes.c_int64]
+    L.weft_rans_stage_bytes.restype = ctypes.c_int64
+    L.weft_rans_encode_chunks.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p]
+    L.weft_rans_decode_chunks.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]
     L.weft_topk.argtypes = [ctypes.POINTER(FitState), c_void_p, c_int, c_int, c_void_p, c_void_p]
     L.weft_refine.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, ctypes.POINTER(RefineOpts),
                               c_void_p, c_void_p, c_void_p, c_void_p]
@@ -128,7 +133,7 @@ def _bind(L: ctypes.CDLL) -> None:
                  "weft_fit_score",
                  "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles",
                  "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk", "weft_refine",
-                 "weft_fit_score_split"):
+                 "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks"):
         getattr(L, name).restype = c_int
 
 
@@ -208,5 +213,6 @@ EXPORTED_SYMBOLS = (
     "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_score_evals",
     "weft_fit_init", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
     "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk",
-    "weft_refine", "weft_fit_score_split",
+    "weft_refine", "weft_fit_score_split", "weft_rans_stage_bytes", "weft_rans_encode_chunks",
+    "weft_rans_decode_chunks",
 )

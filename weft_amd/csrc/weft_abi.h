@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 7
+#define WEFT_ABI_VERSION 8
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -131,4 +131,25 @@ struct WeftSplitScoreArgs {
     float* out_mse;
     double* part_obj;
     int32_t* part_idx;
+};
+
+struct WeftRansEncodeArgs {
+    const uint8_t* raw;
+    WeftChunkDesc* descs;
+    uint8_t* stage;
+    const int64_t* stage_off;
+    uint8_t* coded;
+    int64_t* coded_total;
+    int32_t* status;
+    int32_t n_chunks;
+    int32_t _pad;
+};
+
+struct WeftRansDecodeArgs {
+    const uint8_t* coded;
+    const WeftChunkDesc* descs;
+    uint8_t* out_raw;
+    int32_t* status;
+    int32_t n_chunks;
+    int32_t _pad;
 };

@@ -40,6 +40,10 @@ struct Module {
     hipFunction_t f_topk = nullptr;
     hipFunction_t f_refine = nullptr;
     hipFunction_t f_split = nullptr;
+    hipFunction_t f_rans_enc = nullptr;
+    hipFunction_t f_rans_scan = nullptr;
+    hipFunction_t f_rans_pack = nullptr;
+    hipFunction_t f_rans_dec = nullptr;
     hipFunction_t f_batch_reset = nullptr;
     hipFunction_t f_render = nullptr;
 };
@@ -139,6 +143,7 @@ static_assert(sizeof(WeftSoftBatch) == 56, "WeftSoftBatch layout");
 static_assert(sizeof(WeftRefineOpts) == 112, "WeftRefineOpts layout");
 static_assert(sizeof(WeftCode) == 20, "WeftCode layout");
 static_assert(sizeof(WeftSplitWork) == 40, "WeftSplitWork layout");
+static_assert(sizeof(WeftChunkDesc) == 24, "WeftChunkDesc layout");
 static_assert(sizeof(WeftPrim) == 60, "WeftPrim layout");
 
 int weft_init(int device) {
@@ -177,6 +182,10 @@ int weft_init(int device) {
         {&g_mod.f_topk, "weft_topk_kernel"},
         {&g_mod.f_refine, "weft_refine_kernel"},
         {&g_mod.f_split, "weft_fit_score_split_kernel"},
+        {&g_mod.f_rans_enc, "weft_rans_encode_kernel"},
+        {&g_mod.f_rans_scan, "weft_rans_scan_kernel"},
+        {&g_mod.f_rans_pack, "weft_rans_pack_kernel"},
+        {&g_mod.f_rans_dec, "weft_rans_decode_kernel"},
         {&g_mod.f_batch_reset, "weft_batch_reset_kernel"},
         {&g_mod.f_render, "weft_render_tiles_kernel"},
     };
@@ -394,6 +403,42 @@ int weft_fit_score_split(const WeftFitState* st, const WeftCands* cands, const W
                          {w->cpw_by_size[0], w->cpw_by_size[1], w->cpw_by_size[2], w->cpw_by_size[3]},
                          d_out_sse, d_out_mse, d_part_obj, d_part_idx};
     return launch(g_mod.f_split, (unsigned)w->n_work, 64, 0, stream, &a);
+}
+
+int64_t weft_rans_stage_bytes(int64_t raw_length) {
+    // the encoder writes a rANS payload only while it stays shorter than the raw form,
+    // so a slot never holds more than the raw form's 5 + n bytes
+    const int64_t n = raw_length > 0 ? raw_length : 0;
+    return (5 + n + 15) / 16 * 16;
+}
+
+int weft_rans_encode_chunks(const uint8_t* d_raw, WeftChunkDesc* d_descs, int n_chunks, uint8_t* d_stage,
+                            const int64_t* d_stage_off, uint8_t* d_coded, int64_t* d_coded_total, int32_t* d_status,
+                            void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n_chunks < 0) return set_err(WEFT_E_INVALID, "weft_rans_encode_chunks: n_chunks < 0");
+    if (n_chunks == 0) return WEFT_OK;
+    if (!d_raw || !d_descs || !d_stage || !d_stage_off || !d_coded || !d_coded_total || !d_status)
+        return set_err(WEFT_E_INVALID, "weft_rans_encode_chunks: null argument");
+    WeftRansEncodeArgs a{d_raw, d_descs, d_stage, d_stage_off, d_coded, d_coded_total, d_status, n_chunks, 0};
+    rc = launch(g_mod.f_rans_enc, (unsigned)n_chunks, 256, 0, stream, &a);
+    if (rc) return rc;
+    rc = launch(g_mod.f_rans_scan, 1, 1024, 0, stream, &a);
+    if (rc) return rc;
+    return launch(g_mod.f_rans_pack, (unsigned)n_chunks, 256, 0, stream, &a);
+}
+
+int weft_rans_decode_chunks(const uint8_t* d_coded, const WeftChunkDesc* d_descs, int n_chunks, uint8_t* d_out_raw,
+                            int32_t* d_status, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n_chunks < 0) return set_err(WEFT_E_INVALID, "weft_rans_decode_chunks: n_chunks < 0");
+    if (n_chunks == 0) return WEFT_OK;
+    if (!d_coded || !d_descs || !d_out_raw || !d_status)
+        return set_err(WEFT_E_INVALID, "weft_rans_decode_chunks: null argument");
+    WeftRansDecodeArgs a{d_coded, d_descs, d_out_raw, d_status, n_chunks, 0};
+    return launch(g_mod.f_rans_dec, (unsigned)n_chunks, 64, 0, stream, &a);
 }
 
 int weft_wallclock_rate_khz(int* khz) {

@@ -1132,6 +1132,7 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
 
 #include "weft_soft.h"
 #include "weft_split.h"
+#include "weft_rans.h"
 
 }  // namespace
 
@@ -1351,3 +1352,9 @@ extern "C" __global__ __launch_bounds__(64) void weft_refine_kernel(WeftRefineAr
 
 // Exact scores of a few candidates per tile, each candidate's pixels split over lanes.
 extern "C" __global__ __launch_bounds__(64) void weft_fit_score_split_kernel(WeftSplitScoreArgs A) { split_score_body(A); }
+
+// Chunked PRIM entropy coding.
+extern "C" __global__ __launch_bounds__(256) void weft_rans_encode_kernel(WeftRansEncodeArgs A) { rans_encode_body(A); }
+extern "C" __global__ __launch_bounds__(1024) void weft_rans_scan_kernel(WeftRansEncodeArgs A) { rans_scan_body(A); }
+extern "C" __global__ __launch_bounds__(256) void weft_rans_pack_kernel(WeftRansEncodeArgs A) { rans_pack_body(A); }
+extern "C" __global__ __launch_bounds__(64) void weft_rans_decode_kernel(WeftRansDecodeArgs A) { rans_decode_body(A); }

@@ -274,6 +274,17 @@ int weft_refine(const WeftFitState* st, const WeftCands* cands, const int32_t* d
                 const WeftRefineOpts* o, WeftCode* d_out_codes, WeftPrim* d_out_prims,
                 float* d_out_params, void* stream);
 
+/* Persistent refined fit: weft_batch_fit whose every round also takes the tile's o->k
+ * best drawn candidates (by MSE, then index), refines them by o->n_iter Adam steps on the
+ * soft SSE (cpw_by_size candidates per wave, as weft_refine), scores the refined codes and
+ * the unrefined best exactly, and accepts the first minimum if it beats the tile's
+ * objective.  Records: win_idx = slot (0 the unrefined best, j + 1 refined candidate j),
+ * win_code = the accepted code.  Equals the stepwise refined round (weft_fit_score,
+ * weft_topk, weft_refine, weft_fit_score_split, weft_fit_select) with the same cpw.
+ * The work-list fields of o are not used.  No reference counterpart. */
+int weft_batch_refine_fit(const WeftFitState* st, const WeftCands* cands, const WeftBatch* b,
+                          const WeftRefineOpts* o, void* stream);
+
 /* Exact scoring (the same values as weft_fit_score) of a few candidates per tile, for
  * refined candidates (no reference counterpart; the candidates' SSE follows NumPy's
  * pairwise float32 order exactly, see weft_score_evals).  Work item i (one 64-lane

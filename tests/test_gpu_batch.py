@@ -119,3 +119,29 @@ def test_batch_rejects_bad_launch(gpu):
     bf.block = 96
     with pytest.raises(_lib.WeftHipError):
         bf.run(use_graph=False)
+
+
+@pytest.mark.parametrize("k,cpw,order,block", [(4, None, "sjf", 256), (8, {8: 8, 16: 2, 32: 1, 64: 1}, "lpt", 128),
+                                               (2, {8: 1, 16: 2, 32: 2, 64: 1}, "fifo", 64)])
+def test_batch_refine_equals_stepwise_refined(gpu, k, cpw, order, block):
+    """The persistent refined fit = RefinedFitter round by round, bit for bit."""
+    from weft_amd.batch import BatchFitter
+    from weft_amd.refine import AdamConfig, RefinedFitter
+
+    tiles = _batch_tiles()
+    n_cand, depth, steps, seed, lam = 130, 3, 4, 17, 1e-5
+    adam = AdamConfig(n_iter=6, w0=0.7, w1=0.4)
+    _say(f"batch refine: {tiles.n_tiles} tiles, k {k}, cpw {cpw}, order {order}, block {block}")
+    bf = BatchFitter(tiles, n_cand=n_cand, depth=depth, n_steps=steps, seed=seed, lam=lam, k=k, refine=adam,
+                     cpw=cpw, order=order, block=block)
+    bf.run(use_graph=False)
+    rf = RefinedFitter(tiles, n_cand=n_cand, depth=depth, k=k, adam=adam, cpw=cpw, seed=seed, lam=lam,
+                       max_steps=steps)
+    rf.step(steps)
+    _same_records(bf.records(), rf.records(), "fused vs stepwise refined")
+    assert bits_equal(bf.canvas(), rf.canvas())
+    sb, sr = bf.state(), rf.state()
+    assert np.array_equal(sb["nsel"], sr["nsel"]) and bits_equal(sb["obj"], sr["obj"])
+    assert (bf.records()["win_idx"] > 0).any(), "refined candidates win somewhere"
+    bf.run(use_graph=True)  # graph replay gives the same fit
+    _same_records(bf.records(), rf.records(), "graph replay")

@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -117,6 +117,8 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_get_tables.argtypes = [c_void_p, c_void_p, c_void_p]
     L.weft_batch_fit.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), ctypes.POINTER(Batch), c_void_p]
     L.weft_wallclock_rate_khz.argtypes = [ctypes.POINTER(c_int)]
+    L.weft_batch_refine_fit.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), ctypes.POINTER(Batch),
+                                        ctypes.POINTER(RefineOpts), c_void_p]
     L.weft_soft_eval.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(SoftBatch), c_void_p]
     L.weft_fit_score_split.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), ctypes.POINTER(SplitWork),
                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
@@ -133,7 +135,8 @@ def _bind(L: ctypes.CDLL) -> None:
                  "weft_fit_score",
                  "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles",
                  "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk", "weft_refine",
-                 "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks"):
+                 "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks",
+                 "weft_batch_refine_fit"):
         getattr(L, name).restype = c_int
 
 
@@ -214,5 +217,5 @@ EXPORTED_SYMBOLS = (
     "weft_fit_init", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
     "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk",
     "weft_refine", "weft_fit_score_split", "weft_rans_stage_bytes", "weft_rans_encode_chunks",
-    "weft_rans_decode_chunks",
+    "weft_rans_decode_chunks", "weft_batch_refine_fit",
 )

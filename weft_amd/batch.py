@@ -11,6 +11,9 @@ host round trips.
 Results are bit-identical to ``PlainFitter.step(n_steps)`` with the same seed, since
 candidate k of tile t in round s is drawn from (seed, step0 + s, t, k) in both, and
 the first minimum of the objective does not depend on the order candidates are scored.
+With ``k > 0`` every round also refines the tile's k best drawn candidates by Adam and
+re-scores them exactly (weft_batch_refine_fit): bit-identical to
+``RefinedFitter.step(n_steps)`` with the same candidates-per-wave split.
 
 The fit (state reset, initial objectives, queue reset, persistent launch) is captured
 once in a HIP graph and replayed.  Each image's completion is stamped on the device
@@ -72,7 +75,8 @@ class BatchFitter:
     def __init__(self, tiles: TileSet, n_cand: int, depth: int, n_steps: int | None = None,
                  seed: int = 0, lam: float | None = None, obj_mode: int = 0, step0: int = 0,
                  order: str = "sjf", block: int = 256, grid: int | None = None,
-                 keep_codes: bool = True, device: int = 0):
+                 keep_codes: bool = True, device: int = 0, k: int = 0, refine=None,
+                 cpw: dict[int, int] | None = None):
         import torch
 
         _lib.ensure_init(device)
@@ -130,7 +134,31 @@ class BatchFitter:
         _lib.check(self.L.weft_wallclock_rate_khz(ctypes.byref(khz)), "weft_wallclock_rate_khz")
         self.clock_hz = khz.value * 1000.0
         self.graph = None
+        # refined rounds (weft_batch_refine_fit): the k best drawn candidates of every
+        # round refined by Adam and re-scored exactly; k = 0 is the plain fit
+        self.k = int(k)
+        self._opts = None
+        if self.k:
+            from .refine import DEFAULT_CPW, AdamConfig
+
+            if self.k not in (1, 2, 4, 8, 16, 32, 64) or self.k > self.n_cand:
+                raise ValueError("k must be a power of two <= min(64, n_cand)")
+            self.adam = refine or AdamConfig()
+            self.d_sched = torch.from_numpy(self.adam.schedule()).to(dev)
+            self.set_cpw(cpw or DEFAULT_CPW)
         self.configure(order=order, block=block, grid=grid)
+
+    def set_cpw(self, cpw: dict[int, int]) -> None:
+        """Refined candidates per wave by tile size (as RefinedFitter.set_cpw)."""
+        from .refine import DEFAULT_CPW
+
+        c = {ts: min(v, self.k) for ts, v in {**DEFAULT_CPW, **cpw}.items()}
+        o = self.adam.opts(self.k, self.d_sched)
+        for i, ts in enumerate((8, 16, 32, 64)):
+            o.cpw_by_size[i] = int(c[ts])
+        self.cpw = c
+        self._opts = o
+        self.graph = None
 
     # ---- launch configuration -------------------------------------------------
     def configure(self, order: str | None = None, block: int | None = None,
@@ -170,8 +198,13 @@ class BatchFitter:
         sp = _lib.stream_ptr()
         _lib.check(self.L.weft_fit_init(ctypes.byref(self._state), sp), "weft_fit_init")
         b = self._batch()
-        _lib.check(self.L.weft_batch_fit(ctypes.byref(self._state), ctypes.byref(self._cands),
-                                         ctypes.byref(b), sp), "weft_batch_fit")
+        if self.k:
+            _lib.check(self.L.weft_batch_refine_fit(ctypes.byref(self._state), ctypes.byref(self._cands),
+                                                    ctypes.byref(b), ctypes.byref(self._opts), sp),
+                       "weft_batch_refine_fit")
+        else:
+            _lib.check(self.L.weft_batch_fit(ctypes.byref(self._state), ctypes.byref(self._cands),
+                                             ctypes.byref(b), sp), "weft_batch_fit")
 
     def run(self, use_graph: bool = True) -> None:
         """One whole fit from an empty canvas (asynchronous on the current stream)."""

@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 8
+#define WEFT_ABI_VERSION 9
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -151,5 +151,13 @@ struct WeftRansDecodeArgs {
     uint8_t* out_raw;
     int32_t* status;
     int32_t n_chunks;
+    int32_t _pad;
+};
+
+struct WeftBatchRefineArgs {
+    WeftBatchArgs b;
+    WeftRefineOpts o;
+    int32_t split_lanes[4];
+    int32_t n_pow2;
     int32_t _pad;
 };

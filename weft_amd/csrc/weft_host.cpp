@@ -40,6 +40,7 @@ struct Module {
     hipFunction_t f_topk = nullptr;
     hipFunction_t f_refine = nullptr;
     hipFunction_t f_split = nullptr;
+    hipFunction_t f_batch_refine = nullptr;
     hipFunction_t f_rans_enc = nullptr;
     hipFunction_t f_rans_scan = nullptr;
     hipFunction_t f_rans_pack = nullptr;
@@ -182,6 +183,7 @@ int weft_init(int device) {
         {&g_mod.f_topk, "weft_topk_kernel"},
         {&g_mod.f_refine, "weft_refine_kernel"},
         {&g_mod.f_split, "weft_fit_score_split_kernel"},
+        {&g_mod.f_batch_refine, "weft_batch_refine_kernel"},
         {&g_mod.f_rans_enc, "weft_rans_encode_kernel"},
         {&g_mod.f_rans_scan, "weft_rans_scan_kernel"},
         {&g_mod.f_rans_pack, "weft_rans_pack_kernel"},
@@ -439,6 +441,56 @@ int weft_rans_decode_chunks(const uint8_t* d_coded, const WeftChunkDesc* d_descs
         return set_err(WEFT_E_INVALID, "weft_rans_decode_chunks: null argument");
     WeftRansDecodeArgs a{d_coded, d_descs, d_out_raw, d_status, n_chunks, 0};
     return launch(g_mod.f_rans_dec, (unsigned)n_chunks, 64, 0, stream, &a);
+}
+
+int weft_batch_refine_fit(const WeftFitState* st, const WeftCands* cands, const WeftBatch* b,
+                          const WeftRefineOpts* o, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !cands || !b || !o || !b->d_tile_order || !b->d_queue || !b->d_win_idx || !b->d_win_obj ||
+        !b->d_win_acc)
+        return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: null argument");
+    if (cands->d_prims) return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: drawn candidates only");
+    if (cands->n_cand < 1 || cands->n_cand > 8192)
+        return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: n_cand must be in [1, 8192]");
+    if (b->block < 64 || b->block > 256 || b->block % 64)
+        return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: block must be 64, 128, 192 or 256");
+    if (b->grid < 1 || b->n_queue < 0 || b->n_steps < 0 || b->n_images < 0)
+        return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: bad sizes");
+    if (b->n_images > 0 && (!b->d_tile_image || !b->d_image_tiles || !b->d_image_left || !b->d_image_done))
+        return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: image counters missing");
+    if (!pow2_le64(o->k) || o->k > cands->n_cand)
+        return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: k must be a power of two <= min(64, n_cand)");
+    if (o->n_iter < 0 || (o->n_iter > 0 && !o->d_sched))
+        return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: bad schedule");
+    for (int c = 0; c < 4; ++c)
+        if (!pow2_le64(o->cpw_by_size[c]) || o->k % o->cpw_by_size[c])
+            return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: cpw must be powers of two dividing k");
+    WeftBatchRefineArgs a{};
+    a.b = WeftBatchArgs{*st, *cands, b->d_tile_order, b->d_tile_image, b->d_image_tiles, b->d_image_left,
+                        reinterpret_cast<unsigned long long*>(b->d_image_done),
+                        reinterpret_cast<unsigned long long*>(b->d_t_start), b->d_queue, b->d_win_idx,
+                        b->d_win_obj, b->d_win_acc, b->d_win_code, b->n_queue, b->n_images > 0 ? b->n_images : 0,
+                        b->step0, b->n_steps};
+    if (b->n_images == 0) a.b.image_left = nullptr;
+    a.o = *o;
+    // exact re-score: lanes per candidate, a power of two within the pairwise tree's leaf
+    // count and small enough for all k + 1 candidates to fit the workgroup in one pass
+    const int leaf_limit[4] = {2, 8, 32, 32};
+    const int k1 = o->k + 1;
+    for (int c = 0; c < 4; ++c) {
+        int ls = 1;
+        while (ls * 2 <= leaf_limit[c] && ls * 2 * k1 <= b->block) ls *= 2;
+        a.split_lanes[c] = ls;
+    }
+    int np2 = 2;
+    while (np2 < cands->n_cand) np2 <<= 1;
+    a.n_pow2 = np2;
+    const int reset_n = b->n_images > 1 ? b->n_images : 1;
+    rc = launch(g_mod.f_batch_reset, (unsigned)((reset_n + 255) / 256), 256, 0, stream, &a.b);
+    if (rc) return rc;
+    const unsigned lds = (unsigned)(36 * 512 + 8 * np2 + 4 * cands->n_cand);
+    return launch(g_mod.f_batch_refine, (unsigned)b->grid, (unsigned)b->block, lds, stream, &a);
 }
 
 int weft_wallclock_rate_khz(int* khz) {

@@ -165,6 +165,27 @@ __device__ void composite_px(const WeftPrim& P, float px, float py, float& r, fl
     b = s2 * al + b * oma;
 }
 
+// composite_px restricted to the constant and hard-edge kinds (0, 4, 6, 7, 8): the same
+// operations, without the code of the Gaussian and gradient kinds; other kinds are
+// skipped as composite_px skips unknown ones.
+__device__ __forceinline__ void composite_hard_px(const WeftPrim& P, float px, float py, float& r, float& g,
+                                                  float& b) {
+    const float a = clip01(P.alpha);
+    float al;
+    switch (P.kind) {
+    case 0: al = a; break;
+    case 4:
+    case 8: al = tri_alpha(P.geom, px, py) * a; break;
+    case 6:
+    case 7: al = ell_rect_alpha(P.kind, P.geom, px, py) * a; break;
+    default: return;
+    }
+    const float oma = 1.0f - al;
+    r = P.color0[0] * al + r * oma;
+    g = P.color0[1] * al + g * oma;
+    b = P.color0[2] * al + b * oma;
+}
+
 // Out-of-line copy for the per-pixel loops of the generic paths: keeps the unrolled
 // pixel bodies small and the fast classes' register allocation unaffected.
 __device__ __noinline__ void composite_px_call(const WeftPrim& P, float px, float py, float& r,
@@ -1133,6 +1154,7 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
 #include "weft_soft.h"
 #include "weft_split.h"
 #include "weft_rans.h"
+#include "weft_batch_refine.h"
 
 }  // namespace
 
@@ -1358,3 +1380,6 @@ extern "C" __global__ __launch_bounds__(256) void weft_rans_encode_kernel(WeftRa
 extern "C" __global__ __launch_bounds__(1024) void weft_rans_scan_kernel(WeftRansEncodeArgs A) { rans_scan_body(A); }
 extern "C" __global__ __launch_bounds__(256) void weft_rans_pack_kernel(WeftRansEncodeArgs A) { rans_pack_body(A); }
 extern "C" __global__ __launch_bounds__(64) void weft_rans_decode_kernel(WeftRansDecodeArgs A) { rans_decode_body(A); }
+
+// Persistent refined fit (tiles of many images, every round refined and re-scored).
+extern "C" __global__ __launch_bounds__(256) void weft_batch_refine_kernel(WeftBatchRefineArgs A) { batch_refine_body(A); }

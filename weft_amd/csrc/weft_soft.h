@@ -20,6 +20,11 @@
 // 1, dx, dy for each edge; of the ellipse's radial terms) and applies the chain rule
 // to the parameters once per candidate.
 
+#ifndef WEFT_SOFT_UNROLL
+#define WEFT_SOFT_UNROLL 4  // pixels in flight per lane (memory-level parallelism vs registers)
+#endif
+
+constexpr int SOFT_UNROLL = WEFT_SOFT_UNROLL;
 constexpr int SOFT_NP = 9;
 constexpr float SOFT_PI = 3.14159265358979323846f;
 
@@ -245,13 +250,15 @@ __device__ __forceinline__ void soft_acc_reduce(SoftAcc& A, int L) {
 }
 
 // SSE (and sums for the gradient) of candidate C over tile pixels slice, slice + L, ...
-template <bool GRAD>
+// UNROLL pixels in flight per lane: memory-level parallelism against registers; the
+// sums are accumulated in the same order whatever its value.
+template <bool GRAD, int UNROLL = SOFT_UNROLL>
 __device__ __forceinline__ void soft_tile(const SoftCand& C, const SoftPre& P, float invw,
                                           const float* tg, const float* cv, int ts, int slice, int L,
                                           SoftAcc& A) {
     soft_acc_zero(A);
     const int npx = ts * ts;
-#pragma unroll 4
+#pragma unroll UNROLL
     for (int px = slice; px < npx; px += L) {
         const int y = px / ts, x = px - y * ts;
         const float dx = ((float)x + 0.5f) - C.p[0];
@@ -398,6 +405,36 @@ __device__ void topk_body(const WeftTopkArgs& A) {
     for (int i = tid; i < A.k; i += nthr) A.out_idx[(int64_t)t * A.k + i] = (int32_t)(key[i] & 0xFFFFFFFFull);
 }
 
+// n_iter Adam steps on candidate C's soft SSE over its tile, L lanes per candidate
+// (this lane takes pixels slice, slice + L, ...), projected onto the code ranges after
+// every step.  Every lane of the candidate ends with the same parameters.
+template <int UNROLL = SOFT_UNROLL>
+__device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const float* tg, const float* cv,
+                                                 int ts, int slice, int L, SoftCand& C) {
+    float m1[SOFT_NP], m2[SOFT_NP];
+#pragma unroll
+    for (int i = 0; i < SOFT_NP; ++i) m1[i] = m2[i] = 0.0f;
+#pragma unroll 1
+    for (int it = 0; it < o.n_iter; ++it) {
+        const float w = o.d_sched[4 * it], bc1 = o.d_sched[4 * it + 1], bc2 = o.d_sched[4 * it + 2];
+        SoftPre P;
+        soft_pre(C, P);
+        SoftAcc S;
+        soft_tile<true, UNROLL>(C, P, 1.0f / w, tg, cv, ts, slice, L, S);
+        float gr[SOFT_NP];
+        soft_finish(C, P, S, gr);
+#pragma unroll
+        for (int i = 0; i < SOFT_NP; ++i) {
+            m1[i] = o.beta1 * m1[i] + o.one_minus_beta1 * gr[i];
+            m2[i] = o.beta2 * m2[i] + o.one_minus_beta2 * (gr[i] * gr[i]);
+            const float mh = m1[i] / bc1;
+            const float vh = m2[i] / bc2;
+            C.p[i] = C.p[i] - o.lr[i] * (mh / (sqrtf(vh) + o.eps));
+        }
+        soft_project(C, ts);
+    }
+}
+
 // Adam on the soft SSE of each tile's top-k drawn candidates; one 64-lane workgroup
 // per work item (cpw candidates of one tile, 64 / cpw lanes each).  Every lane of a
 // candidate holds the same parameters and moments (the butterfly sums give all of them
@@ -429,28 +466,7 @@ __device__ void refine_body(const WeftRefineArgs& A) {
         A.out_prims[slot0] = code_to_prim(best);
     }
     SoftCand C = soft_from_code(code0);
-    float m1[SOFT_NP], m2[SOFT_NP];
-#pragma unroll
-    for (int i = 0; i < SOFT_NP; ++i) m1[i] = m2[i] = 0.0f;
-#pragma unroll 1
-    for (int it = 0; it < A.o.n_iter; ++it) {
-        const float w = A.o.d_sched[4 * it], bc1 = A.o.d_sched[4 * it + 1], bc2 = A.o.d_sched[4 * it + 2];
-        SoftPre P;
-        soft_pre(C, P);
-        SoftAcc S;
-        soft_tile<true>(C, P, 1.0f / w, tg, cv, ts, slice, L, S);
-        float gr[SOFT_NP];
-        soft_finish(C, P, S, gr);
-#pragma unroll
-        for (int i = 0; i < SOFT_NP; ++i) {
-            m1[i] = A.o.beta1 * m1[i] + A.o.one_minus_beta1 * gr[i];
-            m2[i] = A.o.beta2 * m2[i] + A.o.one_minus_beta2 * (gr[i] * gr[i]);
-            const float mh = m1[i] / bc1;
-            const float vh = m2[i] / bc2;
-            C.p[i] = C.p[i] - A.o.lr[i] * (mh / (sqrtf(vh) + A.o.eps));
-        }
-        soft_project(C, ts);
-    }
+    refine_candidate(A.o, tg, cv, ts, slice, L, C);
     if (slice == 0) {
         const WeftCode q = soft_to_code(C, ts);
         if (A.out_codes) A.out_codes[slot0 + 1 + j] = q;

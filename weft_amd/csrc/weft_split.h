@@ -36,8 +36,9 @@ __device__ __forceinline__ void split_tree_push(SplitTree& T, float v) {
     T.n += 1;
 }
 
-// Sum of elements [e0, e0 + LS) of the candidate's squared errors (one leaf).
-template <int TS, int LS>
+// Sum of elements [e0, e0 + LS) of the candidate's squared errors (one leaf).  HARD:
+// the candidate is of a constant or hard-edge kind (smaller composite).
+template <int TS, int LS, bool HARD>
 __device__ __forceinline__ float split_leaf(const WeftPrim& P, const float* cv, const float* tg, int e0) {
     constexpr int NPX = TS * TS;
     float r[8];
@@ -54,7 +55,8 @@ __device__ __forceinline__ float split_leaf(const WeftPrim& P, const float* cv, 
             if (p >= p0 && p < p1 && p < NPX) {
                 const int y = p / TS, x = p - y * TS;
                 float o0 = cv[3 * p], o1 = cv[3 * p + 1], o2 = cv[3 * p + 2];
-                composite_px(P, (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
+                if constexpr (HARD) composite_hard_px(P, (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
+                else composite_px(P, (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
                 const float d0 = clip01(o0) - tg[3 * p];
                 const float d1 = clip01(o1) - tg[3 * p + 1];
                 const float d2 = clip01(o2) - tg[3 * p + 2];
@@ -73,7 +75,7 @@ __device__ __forceinline__ float split_leaf(const WeftPrim& P, const float* cv, 
 }
 
 // Perfect tree over the nleaves leaves of one chunk, L lanes; every lane returns the sum.
-template <int TS, int LS>
+template <int TS, int LS, bool HARD>
 __device__ __forceinline__ float split_chunk(const WeftPrim& P, const float* cv, const float* tg, int c0,
                                              int nleaves, int L, int sl) {
     const int m = nleaves / L;
@@ -82,7 +84,7 @@ __device__ __forceinline__ float split_chunk(const WeftPrim& P, const float* cv,
 #pragma unroll
     for (int q = 0; q < 7; ++q) T.lvl[q] = 0.0f;
 #pragma unroll 1
-    for (int li = 0; li < m; ++li) split_tree_push(T, split_leaf<TS, LS>(P, cv, tg, c0 + (sl * m + li) * LS));
+    for (int li = 0; li < m; ++li) split_tree_push(T, split_leaf<TS, LS, HARD>(P, cv, tg, c0 + (sl * m + li) * LS));
     // m is a power of two: the root is the single level left
     float v = 0.0f;
 #pragma unroll
@@ -92,14 +94,14 @@ __device__ __forceinline__ float split_chunk(const WeftPrim& P, const float* cv,
     return v;
 }
 
-template <int TS>
+template <int TS, bool HARD = false>
 __device__ float split_sse(const WeftPrim& P, const float* cv, const float* tg, int L, int sl) {
     if constexpr (TS == 64) {
-        const float a = split_chunk<TS, 128>(P, cv, tg, 0, 64, L, sl);
-        const float b = split_chunk<TS, 128>(P, cv, tg, 8192, 32, L, sl);
+        const float a = split_chunk<TS, 128, HARD>(P, cv, tg, 0, 64, L, sl);
+        const float b = split_chunk<TS, 128, HARD>(P, cv, tg, 8192, 32, L, sl);
         return (0.0f + a) + b;
     } else {
-        return split_chunk<TS, 96>(P, cv, tg, 0, (3 * TS * TS) / 96, L, sl);
+        return split_chunk<TS, 96, HARD>(P, cv, tg, 0, (3 * TS * TS) / 96, L, sl);
     }
 }
 

@@ -244,8 +244,9 @@ int weft_fit_init(const WeftFitState* st, void* stream) {
     int rc = require_init();
     if (rc) return rc;
     if (!st || st->n_tiles < 0) return set_err(WEFT_E_INVALID, "weft_fit_init: bad state");
+    if (st->n_tiles == 0) return WEFT_OK;
     WeftFitInitArgs a{*st};
-    return launch(g_mod.f_init, (unsigned)((st->n_tiles + 255) / 256), 256, 0, stream, &a);
+    return launch(g_mod.f_init, (unsigned)st->n_tiles, 64, 0, stream, &a);
 }
 
 int weft_fit_order(const WeftFitState* st, const WeftCands* cands, int32_t* d_order, void* stream) {

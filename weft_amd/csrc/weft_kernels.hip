@@ -339,23 +339,6 @@ struct PxStack {
     }
 };
 
-// Current canvas against the target, both in global memory (tile objective).
-struct PxCanvasGlobal {
-    static constexpr bool kUnrollRow = false;
-    const float* tgt;
-    const float* cv;
-    int ts;
-    __device__ __forceinline__ void row(int) {}
-    __device__ __forceinline__ void pixel(int y, int x, float, float& e0, float& e1, float& e2) {
-        const int o = 3 * (y * ts + x);
-        float d0 = tgt[o] - clip01(cv[o]);
-        float d1 = tgt[o + 1] - clip01(cv[o + 1]);
-        float d2 = tgt[o + 2] - clip01(cv[o + 2]);
-        e0 = d0 * d0;
-        e1 = d1 * d1;
-        e2 = d2 * d2;
-    }
-};
 
 // LDS band of rows [y0, y0 + ROWS): per pixel two float4 and one float,
 //   A = (c_r, c_g, c_b, t_r), B = (k_r, k_g, k_b, t_g), T = t_b,
@@ -913,11 +896,6 @@ __device__ void fit_order_body(const WeftFitOrderArgs& A) {
     }
 }
 
-template <int TS>
-__device__ float tile_sse_global(const float* tgt, const float* cv) {
-    PxCanvasGlobal px{tgt, cv, TS};
-    return pairwise_tile<TS>(px);
-}
 
 template <int TS>
 __device__ float stack_sse(const float* tgt, const WeftPrim* prims, int n) {
@@ -1182,27 +1160,18 @@ extern "C" __global__ __launch_bounds__(256) void weft_score_evals_kernel(WeftSc
     if (A.out_sse) A.out_sse[i] = sse;
 }
 
-// One lane per tile: objective of the current canvas.
-extern "C" __global__ __launch_bounds__(256) void weft_fit_init_kernel(WeftFitInitArgs A) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const WeftFitState& st = A.st;
-    if (t >= st.n_tiles) return;
-    const int64_t base = st.d_tile_pix[t] * 3;
-    const float* tg = st.d_target + base;
-    const float* cv = st.d_canvas + base;
-    const int ts = st.d_tile_ts[t];
-    float sse;
-    switch (ts) {
-    case 8: sse = tile_sse_global<8>(tg, cv); break;
-    case 16: sse = tile_sse_global<16>(tg, cv); break;
-    case 32: sse = tile_sse_global<32>(tg, cv); break;
-    case 64: sse = tile_sse_global<64>(tg, cv); break;
-    default: sse = __builtin_nanf(""); break;
-    }
-    const float mse = sse / (float)(3 * ts * ts);
-    st.d_tile_obj[t] = objective(mse, st.d_tile_bits[t], st.lam, st.obj_mode);
+// One 64-lane workgroup per tile: objective of the current canvas.  The tile's SSE is
+// the pairwise tree split over lanes (weft_split.h) with no candidate composited, so it
+// equals the sequential sum bit for bit.
+template <int TS>
+__device__ __forceinline__ float init_tile_sse(const float* tg, const float* cv) {
+    constexpr int L = TS == 8 ? 2 : (TS == 16 ? 8 : 32);
+    WeftPrim none;
+    none.kind = -1;  // skipped: the canvas passes through
+    none.alpha = 0.0f;
+    none.color0[0] = none.color0[1] = none.color0[2] = 0.0f;
+    return split_sse<TS, true>(none, cv, tg, L, (int)(threadIdx.x & (L - 1)));
 }
-
 extern "C" __global__ __launch_bounds__(256) void weft_fit_order_kernel(WeftFitOrderArgs A) { fit_order_body(A); }
 
 // Fast variants: candidate kinds 0, 4, 6, 7, 8 (hard-edge coverage over the cached canvas).
@@ -1383,3 +1352,24 @@ extern "C" __global__ __launch_bounds__(64) void weft_rans_decode_kernel(WeftRan
 
 // Persistent refined fit (tiles of many images, every round refined and re-scored).
 extern "C" __global__ __launch_bounds__(256) void weft_batch_refine_kernel(WeftBatchRefineArgs A) { batch_refine_body(A); }
+
+extern "C" __global__ __launch_bounds__(64) void weft_fit_init_kernel(WeftFitInitArgs A) {
+    const int t = blockIdx.x;
+    const WeftFitState& st = A.st;
+    const int64_t base = st.d_tile_pix[t] * 3;
+    const float* tg = st.d_target + base;
+    const float* cv = st.d_canvas + base;
+    const int ts = st.d_tile_ts[t];
+    float sse;
+    switch (ts) {
+    case 8: sse = init_tile_sse<8>(tg, cv); break;
+    case 16: sse = init_tile_sse<16>(tg, cv); break;
+    case 32: sse = init_tile_sse<32>(tg, cv); break;
+    case 64: sse = init_tile_sse<64>(tg, cv); break;
+    default: sse = __builtin_nanf(""); break;
+    }
+    if (threadIdx.x == 0) {
+        const float mse = sse / (float)(3 * ts * ts);
+        st.d_tile_obj[t] = objective(mse, st.d_tile_bits[t], st.lam, st.obj_mode);
+    }
+}

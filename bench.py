@@ -228,7 +228,7 @@ def drop_in_report(tiles, seed):
     import torch
     from weft_amd import _lib
     from weft_amd.fit import draw_codes
-    from weft_amd.gpu_render import gpu_batch_score_prepacked
+    from weft_amd.gpu_render import gpu_batch_score_prepacked, score_evals_split_device, split_order
     from weft_amd.records import EVAL_DTYPE
 
     T, P, D = tiles.n_tiles, 32, 4
@@ -256,23 +256,34 @@ def drop_in_report(tiles, seed):
     d_out = torch.empty(T * P, dtype=torch.float32, device="cuda")
     L = _lib.library()
     stream = torch.cuda.current_stream()
-    times = []
-    for _ in range(4):
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        _lib.check(L.weft_score_evals(d_ev.data_ptr(), T * P, d_pr.data_ptr(), d_t.data_ptr(), d_out.data_ptr(),
-                                      None, _lib.stream_ptr()), "weft_score_evals")
-        b.record(stream)
-        torch.cuda.synchronize()
-        times.append(a.elapsed_time(b) * 1e-3)
-    dev_s = float(np.median(times[1:]))
+
+    def timed(fn):
+        times = []
+        for _ in range(4):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            times.append(a.elapsed_time(b) * 1e-3)
+        return float(np.median(times[1:]))
+
+    one_s = timed(lambda: _lib.check(L.weft_score_evals(d_ev.data_ptr(), T * P, d_pr.data_ptr(), d_t.data_ptr(),
+                                                        d_out.data_ptr(), None, _lib.stream_ptr()), "weft_score_evals"))
+    one_lane = d_out.cpu().numpy()
+    order, groups = split_order(ev["tile_size"])
+    d_order = torch.from_numpy(order).cuda()
+    dev_s = timed(lambda: score_evals_split_device(d_ev, d_order, groups, d_pr, d_t, d_out))
     same = bool(np.array_equal(d_out.cpu().numpy(), np.concatenate(out)))
+    same_one = bool(d_out.cpu().numpy().tobytes() == one_lane.tobytes())
     px = int((tiles.ts.astype(np.int64) ** 2).sum()) * P
-    return dict(api="gpu_batch_score_prepacked", tiles=T, evals=T * P, stack_depth=D, kernel="weft_score_evals_kernel",
-                device_s=dev_s, evals_per_s=T * P / dev_s, eval_px_per_s=px / dev_s,
-                eval_px_prim_per_s=px * D / dev_s, host_inclusive_s=min(host),
-                host_inclusive_evals_per_s=T * P / min(host), device_equals_api=same)
+    return dict(api="gpu_batch_score_prepacked", tiles=T, evals=T * P, stack_depth=D,
+                kernel="weft_score_evals_split_kernel", device_s=dev_s, evals_per_s=T * P / dev_s,
+                eval_px_per_s=px / dev_s, eval_px_prim_per_s=px * D / dev_s, host_inclusive_s=min(host),
+                host_inclusive_evals_per_s=T * P / min(host), device_equals_api=same,
+                one_lane_kernel_s=one_s, one_lane_evals_per_s=T * P / one_s, split_equals_one_lane=same_one,
+                split_speedup=one_s / dev_s)
 
 
 def decode_report(tiles, records, height, width):

@@ -142,6 +142,15 @@ int weft_abi_version(void);
 int weft_score_evals(const WeftEval* d_evals, int n_evals, const WeftPrim* d_prims,
                      const float* d_tiles, float* d_out_mse, float* d_out_sse, void* stream);
 
+/* Replaces the same batch_tile_objective launch (src/weft/gpu_render.py:113-152, launched
+ * at 455-566) for the evals d_order[0 .. n) that all have tile size `tile_size` (8, 16,
+ * 32, 64): each eval's pixels split over 2, 8, 32 or 32 lanes along NumPy's pairwise
+ * tree, the same values as weft_score_evals bit for bit, for large tiles many times
+ * faster.  Results go to the evals' own indices; an eval of another size gets NaN. */
+int weft_score_evals_split(const WeftEval* d_evals, const int32_t* d_order, int n, int tile_size,
+                           const WeftPrim* d_prims, const float* d_tiles, float* d_out_mse,
+                           float* d_out_sse, void* stream);
+
 /* Plain fit, part 1: objective of every tile's current canvas (call once after the
  * canvas is set).  d_tile_obj[t] = MSE(clip(canvas_t), target_t) + lam * bits_t. */
 int weft_fit_init(const WeftFitState* st, void* stream);

@@ -72,6 +72,59 @@ def test_score_evals_matches_reference_render(gpu, oracle_lib):
             assert o[0].tobytes() == np.float32(r).tobytes(), (j, float(o[0]), float(r))
 
 
+def test_split_evals_equal_one_lane_evals(gpu, oracle_lib):
+    """weft_score_evals_split (pixels over lanes) == weft_score_evals (one lane per eval).
+
+    Stacks of 0-6 primitives mixing the reference fixtures' kinds 0-4 with drawn ones, all
+    four tile sizes interleaved; an eval of another size in a launch's group gets NaN.
+    """
+    import torch
+    from weft_amd import _lib
+    from weft_amd.gpu_render import score_evals_split_device, split_order
+    from weft_amd.records import EVAL_DTYPE, PRIM_DTYPE
+
+    d = load_golden("ref_render_tile.npz")
+    pool = np.concatenate([golden_prims(d, i) for i in range(len(d["tile_size"])) if not d["has_residual"][i]])
+    rng = np.random.default_rng(23)
+    n = 3000
+    ts = rng.choice([8, 16, 32, 64], n).astype(np.int32)
+    cnt = rng.integers(0, 7, n).astype(np.int32)
+    po = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int32)
+    prims = np.zeros(max(1, int(cnt.sum())), PRIM_DTYPE)
+    drawn = oracle_lib.codes_to_prims(oracle_lib.draw_codes(4, 2, 9, len(prims), 32))
+    pick = rng.random(len(prims)) < 0.5
+    prims[pick] = pool[rng.integers(0, len(pool), int(pick.sum()))].astype(PRIM_DTYPE)
+    prims[~pick] = drawn[~pick]
+    toff = np.concatenate([[0], np.cumsum(3 * ts.astype(np.int64) ** 2)[:-1]])
+    tiles = rng.random(int(3 * (ts.astype(np.int64) ** 2).sum()), dtype=np.float32)
+    ev = np.zeros(n, EVAL_DTYPE)
+    ev["prim_offset"], ev["prim_count"], ev["tile_offset"], ev["tile_size"] = po, cnt, toff, ts
+    _say(f"split evals: {n} stacks, {len(prims)} primitives, sizes 8-64 interleaved")
+    d_ev = torch.from_numpy(ev.view(np.uint8)).cuda()
+    d_pr = torch.from_numpy(prims.view(np.uint8)).cuda()
+    d_t = torch.from_numpy(tiles).cuda()
+    m1, s1 = (torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(2))
+    L = _lib.library()
+    _lib.check(L.weft_score_evals(d_ev.data_ptr(), n, d_pr.data_ptr(), d_t.data_ptr(), m1.data_ptr(), s1.data_ptr(),
+                                  _lib.stream_ptr()), "weft_score_evals")
+    order, groups = split_order(ts)
+    d_order = torch.from_numpy(order).cuda()
+    m2, s2 = (torch.full((n,), -1.0, dtype=torch.float32, device="cuda") for _ in range(2))
+    score_evals_split_device(d_ev, d_order, groups, d_pr, d_t, m2, s2)
+    assert bits_equal(m1.cpu().numpy(), m2.cpu().numpy()) and bits_equal(s1.cpu().numpy(), s2.cpu().numpy())
+    # a group holding evals of another size: NaN for those, nothing else written
+    g0, k0 = groups[0]
+    m3 = torch.full((n,), -1.0, dtype=torch.float32, device="cuda")
+    _lib.check(L.weft_score_evals_split(d_ev.data_ptr(), d_order.data_ptr(), k0 + 5, g0, d_pr.data_ptr(),
+                                        d_t.data_ptr(), m3.data_ptr(), None, _lib.stream_ptr()), "split")
+    h = m3.cpu().numpy()
+    assert bits_equal(h[order[:k0]], m1.cpu().numpy()[order[:k0]])
+    assert np.isnan(h[order[k0:k0 + 5]]).all()
+    assert (h[order[k0 + 5:]] == -1.0).all()
+    assert L.weft_score_evals_split(d_ev.data_ptr(), d_order.data_ptr(), 1, 12, d_pr.data_ptr(), d_t.data_ptr(),
+                                    m3.data_ptr(), None, _lib.stream_ptr()) != 0
+
+
 def test_drop_in_objective_matches_reference(gpu):
     """gpu_batch_objectives == float(np.mean((src - render_tile(stack))**2)) of the reference."""
     from weft_amd.gpu_render import gpu_batch_objectives

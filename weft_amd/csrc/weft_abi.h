@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 9
+#define WEFT_ABI_VERSION 10
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -160,4 +160,15 @@ struct WeftBatchRefineArgs {
     int32_t split_lanes[4];
     int32_t n_pow2;
     int32_t _pad;
+};
+
+struct WeftScoreEvalsSplitArgs {
+    const WeftEval* evals;
+    const int32_t* order;
+    const WeftPrim* prims;
+    const float* tiles;
+    float* out_mse;
+    float* out_sse;
+    int32_t n;
+    int32_t tile_size;
 };

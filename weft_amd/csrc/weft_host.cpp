@@ -41,6 +41,7 @@ struct Module {
     hipFunction_t f_refine = nullptr;
     hipFunction_t f_split = nullptr;
     hipFunction_t f_batch_refine = nullptr;
+    hipFunction_t f_evals_split = nullptr;
     hipFunction_t f_rans_enc = nullptr;
     hipFunction_t f_rans_scan = nullptr;
     hipFunction_t f_rans_pack = nullptr;
@@ -184,6 +185,7 @@ int weft_init(int device) {
         {&g_mod.f_refine, "weft_refine_kernel"},
         {&g_mod.f_split, "weft_fit_score_split_kernel"},
         {&g_mod.f_batch_refine, "weft_batch_refine_kernel"},
+        {&g_mod.f_evals_split, "weft_score_evals_split_kernel"},
         {&g_mod.f_rans_enc, "weft_rans_encode_kernel"},
         {&g_mod.f_rans_scan, "weft_rans_scan_kernel"},
         {&g_mod.f_rans_pack, "weft_rans_pack_kernel"},
@@ -238,6 +240,23 @@ int weft_score_evals(const WeftEval* d_evals, int n_evals, const WeftPrim* d_pri
         return set_err(WEFT_E_INVALID, "weft_score_evals: null pointer");
     WeftScoreEvalsArgs a{d_evals, d_prims, d_tiles, d_out_mse, d_out_sse, n_evals, 0};
     return launch(g_mod.f_evals, (unsigned)((n_evals + 255) / 256), 256, 0, stream, &a);
+}
+
+int weft_score_evals_split(const WeftEval* d_evals, const int32_t* d_order, int n, int tile_size,
+                           const WeftPrim* d_prims, const float* d_tiles, float* d_out_mse, float* d_out_sse,
+                           void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n < 0) return set_err(WEFT_E_INVALID, "weft_score_evals_split: n < 0");
+    if (tile_size != 8 && tile_size != 16 && tile_size != 32 && tile_size != 64)
+        return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_score_evals_split: tile size must be 8, 16, 32 or 64");
+    if (n == 0) return WEFT_OK;
+    if (!d_evals || !d_order || !d_prims || !d_tiles || !d_out_mse)
+        return set_err(WEFT_E_INVALID, "weft_score_evals_split: null pointer");
+    const int lanes = tile_size == 8 ? 2 : (tile_size == 16 ? 8 : 32);
+    const int per = 64 / lanes;
+    WeftScoreEvalsSplitArgs a{d_evals, d_order, d_prims, d_tiles, d_out_mse, d_out_sse, n, tile_size};
+    return launch(g_mod.f_evals_split, (unsigned)((n + per - 1) / per), 64, 0, stream, &a);
 }
 
 int weft_fit_init(const WeftFitState* st, void* stream) {

@@ -1373,3 +1373,14 @@ extern "C" __global__ __launch_bounds__(64) void weft_fit_init_kernel(WeftFitIni
         st.d_tile_obj[t] = objective(mse, st.d_tile_bits[t], st.lam, st.obj_mode);
     }
 }
+
+// Drop-in stack scoring, each eval's pixels split over lanes (one tile size per launch).
+extern "C" __global__ __launch_bounds__(64) void weft_score_evals_split_kernel(WeftScoreEvalsSplitArgs A) {
+    switch (A.tile_size) {
+    case 8: score_evals_split_item<8>(A); break;
+    case 16: score_evals_split_item<16>(A); break;
+    case 32: score_evals_split_item<32>(A); break;
+    case 64: score_evals_split_item<64>(A); break;
+    default: break;
+    }
+}

@@ -36,10 +36,47 @@ __device__ __forceinline__ void split_tree_push(SplitTree& T, float v) {
     T.n += 1;
 }
 
-// Sum of elements [e0, e0 + LS) of the candidate's squared errors (one leaf).  HARD:
-// the candidate is of a constant or hard-edge kind (smaller composite).
-template <int TS, int LS, bool HARD>
-__device__ __forceinline__ float split_leaf(const WeftPrim& P, const float* cv, const float* tg, int e0) {
+// Per-pixel squared errors (pixel p = (y, x) of a TS-px tile):
+// a candidate composited over the tile's canvas (HARD: constant or hard-edge kinds only,
+// a smaller composite) ...
+template <bool HARD>
+struct PxOverCanvas {
+    const WeftPrim& P;
+    const float* cv;
+    const float* tg;
+    __device__ __forceinline__ void operator()(int p, int y, int x, float& q0, float& q1, float& q2) const {
+        float o0 = cv[3 * p], o1 = cv[3 * p + 1], o2 = cv[3 * p + 2];
+        if constexpr (HARD) composite_hard_px(P, (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
+        else composite_px(P, (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
+        const float d0 = clip01(o0) - tg[3 * p];
+        const float d1 = clip01(o1) - tg[3 * p + 1];
+        const float d2 = clip01(o2) - tg[3 * p + 2];
+        q0 = d0 * d0;
+        q1 = d1 * d1;
+        q2 = d2 * d2;
+    }
+};
+
+// ... or a whole stack rendered over black (render_tile, then the clipped error).
+struct PxStackWalk {
+    const WeftPrim* prims;
+    int n;
+    const float* tg;
+    __device__ __forceinline__ void operator()(int p, int y, int x, float& q0, float& q1, float& q2) const {
+        float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+        for (int i = 0; i < n; ++i) composite_px_call(prims[i], (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
+        const float d0 = clip01(o0) - tg[3 * p];
+        const float d1 = clip01(o1) - tg[3 * p + 1];
+        const float d2 = clip01(o2) - tg[3 * p + 2];
+        q0 = d0 * d0;
+        q1 = d1 * d1;
+        q2 = d2 * d2;
+    }
+};
+
+// Sum of elements [e0, e0 + LS) of the squared errors (one leaf).
+template <int TS, int LS, class F>
+__device__ __forceinline__ float split_leaf(const F& f, int e0) {
     constexpr int NPX = TS * TS;
     float r[8];
 #pragma unroll
@@ -54,15 +91,7 @@ __device__ __forceinline__ float split_leaf(const WeftPrim& P, const float* cv, 
             float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
             if (p >= p0 && p < p1 && p < NPX) {
                 const int y = p / TS, x = p - y * TS;
-                float o0 = cv[3 * p], o1 = cv[3 * p + 1], o2 = cv[3 * p + 2];
-                if constexpr (HARD) composite_hard_px(P, (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
-                else composite_px(P, (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
-                const float d0 = clip01(o0) - tg[3 * p];
-                const float d1 = clip01(o1) - tg[3 * p + 1];
-                const float d2 = clip01(o2) - tg[3 * p + 2];
-                q0 = d0 * d0;
-                q1 = d1 * d1;
-                q2 = d2 * d2;
+                f(p, y, x, q0, q1, q2);
             }
             const int e = 3 * p;
             // element 3p + c goes to accumulator (3i + c) % 8 (pg is a multiple of 8)
@@ -75,16 +104,15 @@ __device__ __forceinline__ float split_leaf(const WeftPrim& P, const float* cv, 
 }
 
 // Perfect tree over the nleaves leaves of one chunk, L lanes; every lane returns the sum.
-template <int TS, int LS, bool HARD>
-__device__ __forceinline__ float split_chunk(const WeftPrim& P, const float* cv, const float* tg, int c0,
-                                             int nleaves, int L, int sl) {
+template <int TS, int LS, class F>
+__device__ __forceinline__ float split_chunk(const F& f, int c0, int nleaves, int L, int sl) {
     const int m = nleaves / L;
     SplitTree T;
     T.n = 0;
 #pragma unroll
     for (int q = 0; q < 7; ++q) T.lvl[q] = 0.0f;
 #pragma unroll 1
-    for (int li = 0; li < m; ++li) split_tree_push(T, split_leaf<TS, LS, HARD>(P, cv, tg, c0 + (sl * m + li) * LS));
+    for (int li = 0; li < m; ++li) split_tree_push(T, split_leaf<TS, LS>(f, c0 + (sl * m + li) * LS));
     // m is a power of two: the root is the single level left
     float v = 0.0f;
 #pragma unroll
@@ -94,14 +122,39 @@ __device__ __forceinline__ float split_chunk(const WeftPrim& P, const float* cv,
     return v;
 }
 
-template <int TS, bool HARD = false>
-__device__ float split_sse(const WeftPrim& P, const float* cv, const float* tg, int L, int sl) {
+template <int TS, class F>
+__device__ __forceinline__ float split_sse_f(const F& f, int L, int sl) {
     if constexpr (TS == 64) {
-        const float a = split_chunk<TS, 128, HARD>(P, cv, tg, 0, 64, L, sl);
-        const float b = split_chunk<TS, 128, HARD>(P, cv, tg, 8192, 32, L, sl);
+        const float a = split_chunk<TS, 128>(f, 0, 64, L, sl);
+        const float b = split_chunk<TS, 128>(f, 8192, 32, L, sl);
         return (0.0f + a) + b;
     } else {
-        return split_chunk<TS, 96, HARD>(P, cv, tg, 0, (3 * TS * TS) / 96, L, sl);
+        return split_chunk<TS, 96>(f, 0, (3 * TS * TS) / 96, L, sl);
+    }
+}
+
+template <int TS, bool HARD = false>
+__device__ float split_sse(const WeftPrim& P, const float* cv, const float* tg, int L, int sl) {
+    return split_sse_f<TS>(PxOverCanvas<HARD>{P, cv, tg}, L, sl);
+}
+
+// Drop-in stack scoring with lanes split: evals d_order[0 .. n) of one tile size, L lanes
+// each (L = the pairwise tree's leaf limit of the size), 64 / L evals per workgroup.
+template <int TS>
+__device__ void score_evals_split_item(const WeftScoreEvalsSplitArgs& A) {
+    constexpr int L = TS == 8 ? 2 : (TS == 16 ? 8 : 32);
+    const int lane = threadIdx.x;
+    const int j = blockIdx.x * (64 / L) + lane / L, sl = lane & (L - 1);
+    const bool live = j < A.n;
+    const int ei = live ? A.order[j] : 0;
+    const WeftEval ev = A.evals[ei];
+    const bool ok = live && ev.tile_size == TS;
+    const PxStackWalk f{A.prims + ev.prim_offset, ok ? ev.prim_count : 0, A.tiles + ev.tile_offset};
+    const float sse = split_sse_f<TS>(f, L, sl);
+    if (live && sl == 0) {
+        const float v = ok ? sse : __builtin_nanf("");
+        A.out_mse[ei] = v / (float)(3 * TS * TS);
+        if (A.out_sse) A.out_sse[ei] = v;
     }
 }
 

@@ -73,6 +73,30 @@ def _validate_evals(evals: np.ndarray, n_prims: int, n_floats: int) -> None:
         raise ValueError(f"tile sizes must be in {KERNEL_TILE_SIZES}")
 
 
+def split_order(tile_sizes: np.ndarray) -> tuple[np.ndarray, list[tuple[int, int]]]:
+    """Evals grouped by tile size for weft_score_evals_split: (order, [(size, count), ...])."""
+    ts = np.asarray(tile_sizes)
+    order = np.argsort(ts, kind="stable").astype(np.int32)
+    groups = [(int(s), int((ts == s).sum())) for s in KERNEL_TILE_SIZES if (ts == s).any()]
+    return order, groups
+
+
+def score_evals_split_device(d_evals, d_order, groups, d_prims, d_tiles, d_out_mse, d_out_sse=None) -> None:
+    """Device-resident lane-split scoring: one launch per tile size on the current stream.
+
+    ``d_order`` holds the eval indices grouped by size as ``groups`` lists them (see
+    split_order); results land at the evals' own indices in ``d_out_mse``/``d_out_sse``.
+    """
+    L = _lib.library()
+    pos = 0
+    for ts, n in groups:
+        _lib.check(L.weft_score_evals_split(d_evals.data_ptr(), d_order.data_ptr() + 4 * pos, n, ts,
+                                            d_prims.data_ptr(), d_tiles.data_ptr(), d_out_mse.data_ptr(),
+                                            None if d_out_sse is None else d_out_sse.data_ptr(),
+                                            _lib.stream_ptr()), "weft_score_evals_split")
+        pos += n
+
+
 def _score_evals(tiles_flat: np.ndarray, evals: np.ndarray, prims: np.ndarray) -> np.ndarray:
     import torch
 
@@ -81,9 +105,9 @@ def _score_evals(tiles_flat: np.ndarray, evals: np.ndarray, prims: np.ndarray) -
     t_evals = torch.from_numpy(np.ascontiguousarray(evals).view(np.uint8)).to(dev)
     t_prims = torch.from_numpy(np.ascontiguousarray(prims, dtype=PRIM_DTYPE).view(np.uint8)).to(dev)
     out = torch.empty(len(evals), dtype=torch.float32, device=dev)
-    L = _lib.library()
-    _lib.check(L.weft_score_evals(t_evals.data_ptr(), len(evals), t_prims.data_ptr(), t_tiles.data_ptr(),
-                                  out.data_ptr(), None, _lib.stream_ptr()), "weft_score_evals")
+    order, groups = split_order(evals["tile_size"])
+    t_order = torch.from_numpy(order).to(dev)
+    score_evals_split_device(t_evals, t_order, groups, t_prims, t_tiles, out)
     return out.cpu().numpy()
 
 

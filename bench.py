@@ -32,6 +32,9 @@ Further objects on the same line (skipped with --no-extras):
   tile of the 1920x1080 quadtree: device-resident and host-inclusive rates.
 - decode: gpu_render_tiles_to_image of the 3840x2160 plain fit's primitives.
 - entropy: chunked rANS encode and decode (64 tiles per chunk, the reference default) of that fit's PRIM bytes.
+- template_fit: the stage-3 template fit (fit_tiles_gpu_constant) of the 3840x2160 image at
+  16-px tiles: device rate, HBM rate against the 8 TB/s peak, the host API's time and the
+  single-threaded CPU oracle on the same image.
 """
 from __future__ import annotations
 
@@ -328,6 +331,44 @@ def decode_report(tiles, records, height, width):
                 mpx_per_s=height * width / dev_s / 1e6, host_inclusive_s=host, device_equals_api=same)
 
 
+def template_report(image, ts=16):
+    """Stage-3 template fit (18 templates per tile) of one image, device-resident and via the API."""
+    import torch
+    from oracle.coracle import template_fit
+    from weft_amd.gpu_encoder import MODEL_COUNT, TemplateFitter, fit_tiles_gpu_constant
+
+    h, w, _ = image.shape
+    res = fit_tiles_gpu_constant(image, ts)  # warm (module, masks)
+    t0 = time.perf_counter()
+    res = fit_tiles_gpu_constant(image, ts)
+    host = time.perf_counter() - t0
+    f = TemplateFitter(w, h, ts)
+    d_img = torch.from_numpy(np.ascontiguousarray(image)).cuda()
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(6):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        f.launch(d_img)
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) * 1e-3)
+    dev_s = float(np.median(times[1:]))
+    T = f.n_tiles
+    # algorithmic bytes: the image once, the per-tile outputs (model, 2 colours, 18 MSE, 2 x 18 colours)
+    nbytes = h * w * 12 + T * (4 + 24 + 4 * MODEL_COUNT + 24 * MODEL_COUNT)
+    t0 = time.perf_counter()
+    o = template_fit(image, ts)
+    cpu_s = time.perf_counter() - t0
+    same_model = float(np.mean(o["model"] == res.model_ids))
+    return dict(api="fit_tiles_gpu_constant", width=w, height=h, tile_size=ts, tiles=T,
+                kernel="weft_template_fit_kernel", device_s=dev_s, mpx_per_s=h * w / dev_s / 1e6,
+                hbm_gb_per_s=nbytes / dev_s / 1e9, hbm_frac=nbytes / dev_s / 8e12, algorithmic_bytes=nbytes,
+                host_inclusive_s=host, cpu_oracle_s=cpu_s, cpu_oracle_threads=1,
+                speedup_vs_cpu_oracle=cpu_s / dev_s, models_equal_oracle_frac=same_model)
+
+
 def entropy_report(tiles, records, chunk_tiles=64):
     """Chunked rANS of a fitted image's PRIM payload, encode and decode on the device."""
     import torch
@@ -576,6 +617,10 @@ def main():
         log(f"[bench] entropy 4k PRIM: ratio {extras['entropy']['ratio']:.2f}, encode "
             f"{extras['entropy']['encode_mb_per_s']:.0f} MB/s, decode {extras['entropy']['decode_mb_per_s']:.0f} MB/s")
         log(f"[bench] decode 4k: {extras['decode']['mpx_per_s']:.0f} Mpx/s on the device")
+        extras["template_fit"] = template_report(img4)
+        log(f"[bench] template fit 4k: {extras['template_fit']['mpx_per_s']:.0f} Mpx/s, "
+            f"{extras['template_fit']['hbm_gb_per_s']:.0f} GB/s on the device; CPU oracle "
+            f"{extras['template_fit']['speedup_vs_cpu_oracle']:.0f}x slower")
         del t4, m4, rec4
         extras["drop_in"] = drop_in_report(tiles, seed)
         log(f"[bench] drop-in scorer: {extras['drop_in']['evals_per_s'] / 1e6:.1f} M evals/s on the device, "

@@ -26,6 +26,9 @@
  *   weft_render_tiles             kernel decode_tile_pixels (gpu_render.py:194-237) as
  *                                 launched by gpu_render_tiles_to_image
  *                                 (gpu_render.py:594-690), with render.py's alpha clamp
+ *   weft_template_masks /         kernel weft_tile_fit_candidates and its launcher
+ *   weft_template_fit             fit_tiles_gpu_constant (src/weft/gpu_encoder.py:42-251,
+ *                                 411-562), the stage-3 baseline fit
  */
 #ifndef WEFT_HIP_H
 #define WEFT_HIP_H
@@ -351,6 +354,31 @@ int weft_get_tables(uint16_t* logr, float* cosv, float* sinv);
  * to [0, 1]); pixels no tile covers are left untouched. */
 int weft_render_tiles(const WeftTileDecodeDesc* d_descs, int n_tiles, const WeftPrim* d_prims,
                       float* d_out, int width, int height, void* stream);
+
+/* Stage-3 template fit (src/weft/gpu_encoder.py:42-251, weft_model_bucket and
+ * weft_tile_fit_candidates; launched by fit_tiles_gpu_constant, gpu_encoder.py:411-562).
+ * 18 templates: 0 constant, 1-8 half-planes through the tile centre at angles (m-1)*pi/8,
+ * 9-11 bands of half-width 1.5 (vertical, horizontal, diagonal), 12-14 corner triangles,
+ * 15-17 bands of half-width 1.25 around quadratic curves; positions sampled at integer
+ * coordinates.
+ *
+ * weft_template_masks: d_mask holds ts^2 + 18 uint32: d_mask[y * ts + x] gets bit m set
+ * when position (x, y) falls in bucket 1 of template m, and d_mask[ts^2 + m] is the
+ * number of positions in bucket 0 of template m.  Depends only on tile_size (1..64).
+ *
+ * weft_template_fit: the image is H x W x 3 float32 (row-major, RGB interleaved), cut into
+ * ceil(W / ts) x ceil(H / ts) tiles in row-major order; pixels past the right / bottom
+ * edge repeat the last column / row.  Per tile t and template m: bucket means
+ * (sum * (1 / max(count, 1)), float32; template 0's colour1 = colour0) in
+ * d_c0_all / d_c1_all[(t * 18 + m) * 3 + c], MSE of the two-colour prediction in
+ * d_mse[t * 18 + m]; the first template of least MSE in d_model[t] and its colours in
+ * d_c0 / d_c1[t * 3 + c].  Colours are not clipped.  Sums are float64 in a fixed order,
+ * so the result is deterministic. */
+#define WEFT_TEMPLATE_MODELS 18
+int weft_template_masks(int tile_size, uint32_t* d_mask, void* stream);
+int weft_template_fit(const float* d_image, int width, int height, int tile_size, const uint32_t* d_mask,
+                      int32_t* d_model, float* d_c0, float* d_c1, float* d_mse, float* d_c0_all,
+                      float* d_c1_all, void* stream);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,9 @@ def lib() -> ctypes.CDLL:
     L.orc_rans_encode.restype = ctypes.c_int64
     L.orc_rans_decode.argtypes = [P, ctypes.c_int64, P, ctypes.c_int64]
     L.orc_rans_decode.restype = ctypes.c_int64
+    L.orc_tmpl_bucket.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_tmpl_bucket.restype = ctypes.c_int
+    L.orc_template_fit.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, P, P, P]
     _lib = L
     return L
 
@@ -232,3 +235,31 @@ def rans_decode(payload: bytes, cap: int | None = None) -> bytes:
 
 def max_threads() -> int:
     return int(lib().orc_max_threads())
+
+
+TEMPLATE_MODELS = 18
+
+
+def template_masks(ts: int) -> np.ndarray:
+    """Bucket bit table of one tile size: bit m of entry y * ts + x = bucket of template m."""
+    L = lib()
+    out = np.zeros(ts * ts, np.uint32)
+    for i in range(ts * ts):
+        v = 0
+        for m in range(1, TEMPLATE_MODELS):
+            v |= L.orc_tmpl_bucket(m, i % ts, i // ts, ts) << m
+        out[i] = v
+    return out
+
+
+def template_fit(image: np.ndarray, ts: int) -> dict:
+    """Stage-3 template fit of an H x W x 3 float32 image (colours unclipped)."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    h, w, _ = img.shape
+    T = ((w + ts - 1) // ts) * ((h + ts - 1) // ts)
+    r = dict(model=np.zeros(T, np.int32), c0=np.zeros((T, 3), np.float32), c1=np.zeros((T, 3), np.float32),
+             mse=np.zeros((T, TEMPLATE_MODELS), np.float32),
+             c0_all=np.zeros((T, TEMPLATE_MODELS, 3), np.float32), c1_all=np.zeros((T, TEMPLATE_MODELS, 3), np.float32))
+    lib().orc_template_fit(_p(img), w, h, ts, _p(r["model"]), _p(r["c0"]), _p(r["c1"]), _p(r["mse"]),
+                           _p(r["c0_all"]), _p(r["c1_all"]))
+    return r

@@ -892,3 +892,156 @@ int64_t orc_rans_decode(const uint8_t* pl, int64_t n, uint8_t* out, int64_t cap)
     }
     return size;
 }
+
+/* ------------------------------------------------------------------------------------
+ * Stage-3 template fit: gpu_encoder.weft_model_bucket / weft_tile_fit_candidates
+ * (src/weft/gpu_encoder.py:42-251) and the result assembly of fit_tiles_gpu_constant
+ * (gpu_encoder.py:411-562, without the final np.clip, which the caller applies).
+ *
+ * The reference accumulates the bucket sums and squared errors with float32 atomics in
+ * an unspecified order.  This restatement adds them in pixel order in float64 and rounds
+ * once; everything else is the reference's float32 arithmetic, one rounding at a time.
+ * ------------------------------------------------------------------------------------ */
+#define ORC_TMPL_MODELS 18
+
+static float orc_minf(float a, float b) { return a < b ? a : b; }
+static float orc_maxf(float a, float b) { return a > b ? a : b; }
+
+/* gpu_encoder.py:48-60 */
+static float orc_tmpl_seg(float px, float py, float x0, float y0, float x1, float y1) {
+    float vx = x1 - x0, vy = y1 - y0;
+    float vv = vx * vx;
+    vv = vv + vy * vy;
+    float den = vv + 1e-8f;
+    float num = (px - x0) * vx;
+    num = num + (py - y0) * vy;
+    float t = num / den;
+    t = orc_maxf(0.0f, orc_minf(1.0f, t));
+    float qx = x0 + t * vx, qy = y0 + t * vy;
+    float dx = px - qx, dy = py - qy;
+    float d2 = dx * dx;
+    d2 = d2 + dy * dy;
+    return sqrtf(d2);
+}
+
+/* gpu_encoder.py:62-69 */
+static int orc_tmpl_tri(float px, float py, float x0, float y0, float x1, float y1, float x2, float y2) {
+    float den = (y1 - y2) * (x0 - x2);
+    den = den + (x2 - x1) * (y0 - y2);
+    if (fabsf(den) < 1e-6f) return 0;
+    float a = (y1 - y2) * (px - x2);
+    a = (a + (x2 - x1) * (py - y2)) / den;
+    float b = (y2 - y0) * (px - x2);
+    b = (b + (x0 - x2) * (py - y2)) / den;
+    float c = 1.0f - a;
+    c = c - b;
+    return a >= 0.0f && b >= 0.0f && c >= 0.0f;
+}
+
+/* gpu_encoder.py:71-87 */
+static float orc_tmpl_curve(float px, float py, float x0, float y0, float cx, float cy, float x1, float y1) {
+    float best = 1e9f, ax = x0, ay = y0;
+    for (int i = 1; i <= 12; ++i) {
+        float t = (float)i / 12.0f;
+        float u = 1.0f - t;
+        float uu = u * u, ut2 = (2.0f * u) * t, tt = t * t;
+        float qx = uu * x0;
+        qx = qx + ut2 * cx;
+        qx = qx + tt * x1;
+        float qy = uu * y0;
+        qy = qy + ut2 * cy;
+        qy = qy + tt * y1;
+        best = orc_minf(best, orc_tmpl_seg(px, py, ax, ay, qx, qy));
+        ax = qx;
+        ay = qy;
+    }
+    return best;
+}
+
+/* gpu_encoder.py:89-129; cos / sin of the float32 angle rounded once to float32 */
+int orc_tmpl_bucket(int m, int lx, int ly, int tile_size) {
+    float x = (float)lx, y = (float)ly, ts = (float)(tile_size - 1);
+    float cx = ts * 0.5f, cy = ts * 0.5f;
+    if (m == 0) return 0;
+    if (m <= 8) {
+        float angle = (float)(m - 1) * (3.14159265358979323846f / 8.0f);
+        float nx = (float)cos((double)angle), ny = (float)sin((double)angle);
+        float proj = (x - cx) * nx;
+        proj = proj + (y - cy) * ny;
+        return proj < 0.0f ? 0 : 1;
+    }
+    if (m == 9) return fabsf(x - cx) <= 1.5f ? 0 : 1;
+    if (m == 10) return fabsf(y - cy) <= 1.5f ? 0 : 1;
+    if (m == 11) return fabsf(y - x) <= 1.5f ? 0 : 1;
+    if (m == 12) return orc_tmpl_tri(x, y, 0.0f, 0.0f, ts, 0.0f, 0.0f, ts) ? 0 : 1;
+    if (m == 13) return orc_tmpl_tri(x, y, ts, 0.0f, ts, ts, 0.0f, 0.0f) ? 0 : 1;
+    if (m == 14) return orc_tmpl_tri(x, y, 0.0f, ts, ts, ts, 0.0f, 0.0f) ? 0 : 1;
+    float d;
+    if (m == 15) d = orc_tmpl_curve(x, y, 0.0f, cy, cx, 0.0f, ts, cy);
+    else if (m == 16) d = orc_tmpl_curve(x, y, 0.0f, cy, cx, ts, ts, cy);
+    else d = orc_tmpl_curve(x, y, cx, 0.0f, 0.0f, cy, cx, ts);
+    return d <= 1.25f ? 0 : 1;
+}
+
+/* gpu_encoder.py:132-251 for every tile of an H x W x 3 float32 image */
+void orc_template_fit(const float* img, int w, int h, int ts, int32_t* model, float* c0, float* c1, float* mse,
+                      float* c0_all, float* c1_all) {
+    const int cols = (w + ts - 1) / ts, rows = (h + ts - 1) / ts, n = ts * ts;
+    unsigned char* bk = (unsigned char*)malloc((size_t)n * ORC_TMPL_MODELS);
+    for (int m = 0; m < ORC_TMPL_MODELS; ++m)
+        for (int i = 0; i < n; ++i) bk[m * n + i] = (unsigned char)orc_tmpl_bucket(m, i % ts, i / ts, ts);
+    for (int t = 0; t < cols * rows; ++t) {
+        const int x0 = (t % cols) * ts, y0 = (t / cols) * ts;
+        float col[ORC_TMPL_MODELS][2][3];
+        float e32[ORC_TMPL_MODELS];
+        for (int m = 0; m < ORC_TMPL_MODELS; ++m) {
+            double s[2][3] = {{0, 0, 0}, {0, 0, 0}};
+            int cnt[2] = {0, 0};
+            for (int i = 0; i < n; ++i) {
+                int x = x0 + i % ts, y = y0 + i / ts;
+                if (x >= w) x = w - 1;
+                if (y >= h) y = h - 1;
+                const float* p = img + ((long)y * w + x) * 3;
+                const int b = bk[m * n + i];
+                cnt[b] += 1;
+                for (int c = 0; c < 3; ++c) s[b][c] += (double)p[c];
+            }
+            for (int c = 0; c < 3; ++c) {
+                float inv0 = 1.0f / (float)(cnt[0] > 1 ? cnt[0] : 1);
+                float inv1 = 1.0f / (float)(cnt[1] > 1 ? cnt[1] : 1);
+                col[m][0][c] = (float)s[0][c] * inv0;
+                col[m][1][c] = m == 0 ? col[m][0][c] : (float)s[1][c] * inv1;
+            }
+            double err = 0.0;
+            for (int i = 0; i < n; ++i) {
+                int x = x0 + i % ts, y = y0 + i / ts;
+                if (x >= w) x = w - 1;
+                if (y >= h) y = h - 1;
+                const float* p = img + ((long)y * w + x) * 3;
+                const float* q = col[m][bk[m * n + i]];
+                float dr = p[0] - q[0], dg = p[1] - q[1], db = p[2] - q[2];
+                float e = dr * dr;
+                e = e + dg * dg;
+                e = e + db * db;
+                err += (double)e;
+            }
+            e32[m] = (float)err * (1.0f / (float)n);
+        }
+        int best = 0;
+        for (int m = 0; m < ORC_TMPL_MODELS; ++m)
+            if (e32[m] < e32[best]) best = m;
+        model[t] = best;
+        for (int m = 0; m < ORC_TMPL_MODELS; ++m) {
+            mse[(long)t * ORC_TMPL_MODELS + m] = e32[m];
+            for (int c = 0; c < 3; ++c) {
+                c0_all[((long)t * ORC_TMPL_MODELS + m) * 3 + c] = col[m][0][c];
+                c1_all[((long)t * ORC_TMPL_MODELS + m) * 3 + c] = col[m][1][c];
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            c0[(long)t * 3 + c] = col[best][0][c];
+            c1[(long)t * 3 + c] = col[best][1][c];
+        }
+    }
+    free(bk);
+}

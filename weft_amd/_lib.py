@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -106,6 +106,9 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_shutdown.argtypes = []
     L.weft_score_evals.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_fit_init.argtypes = [ctypes.POINTER(FitState), c_void_p]
+    L.weft_template_masks.argtypes = [c_int, c_void_p, c_void_p]
+    L.weft_template_fit.argtypes = [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]
     L.weft_score_evals_split.argtypes = [c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                          c_void_p]
     L.weft_fit_order.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, c_void_p]
@@ -138,7 +141,7 @@ def _bind(L: ctypes.CDLL) -> None:
                  "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles",
                  "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk", "weft_refine",
                  "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks",
-                 "weft_batch_refine_fit", "weft_score_evals_split"):
+                 "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks", "weft_template_fit"):
         getattr(L, name).restype = c_int
 
 
@@ -219,5 +222,6 @@ EXPORTED_SYMBOLS = (
     "weft_fit_init", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
     "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk",
     "weft_refine", "weft_fit_score_split", "weft_rans_stage_bytes", "weft_rans_encode_chunks",
-    "weft_rans_decode_chunks", "weft_batch_refine_fit", "weft_score_evals_split",
+    "weft_rans_decode_chunks", "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks",
+    "weft_template_fit",
 )

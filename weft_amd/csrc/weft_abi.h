@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 10
+#define WEFT_ABI_VERSION 11
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -171,4 +171,25 @@ struct WeftScoreEvalsSplitArgs {
     float* out_sse;
     int32_t n;
     int32_t tile_size;
+};
+
+struct WeftTemplateMaskArgs {
+    uint32_t* mask;
+    int32_t tile_size;
+    int32_t _pad;
+};
+
+struct WeftTemplateFitArgs {
+    const float* image;
+    const uint32_t* mask;
+    int32_t* out_model;
+    float* out_c0;
+    float* out_c1;
+    float* out_mse;
+    float* out_c0_all;
+    float* out_c1_all;
+    int32_t width;
+    int32_t height;
+    int32_t tile_size;
+    int32_t tile_cols;
 };

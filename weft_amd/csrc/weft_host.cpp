@@ -42,6 +42,8 @@ struct Module {
     hipFunction_t f_split = nullptr;
     hipFunction_t f_batch_refine = nullptr;
     hipFunction_t f_evals_split = nullptr;
+    hipFunction_t f_tmpl_masks = nullptr;
+    hipFunction_t f_tmpl_fit = nullptr;
     hipFunction_t f_rans_enc = nullptr;
     hipFunction_t f_rans_scan = nullptr;
     hipFunction_t f_rans_pack = nullptr;
@@ -186,6 +188,8 @@ int weft_init(int device) {
         {&g_mod.f_split, "weft_fit_score_split_kernel"},
         {&g_mod.f_batch_refine, "weft_batch_refine_kernel"},
         {&g_mod.f_evals_split, "weft_score_evals_split_kernel"},
+        {&g_mod.f_tmpl_masks, "weft_template_masks_kernel"},
+        {&g_mod.f_tmpl_fit, "weft_template_fit_kernel"},
         {&g_mod.f_rans_enc, "weft_rans_encode_kernel"},
         {&g_mod.f_rans_scan, "weft_rans_scan_kernel"},
         {&g_mod.f_rans_pack, "weft_rans_pack_kernel"},
@@ -530,6 +534,38 @@ int weft_render_tiles(const WeftTileDecodeDesc* d_descs, int n_tiles, const Weft
     if (!d_descs || !d_prims || !d_out) return set_err(WEFT_E_INVALID, "weft_render_tiles: null pointer");
     WeftRenderArgs a{d_descs, d_prims, d_out, n_tiles, width, height, 0};
     return launch(g_mod.f_render, (unsigned)n_tiles, 256, 0, stream, &a);
+}
+
+int weft_template_masks(int tile_size, uint32_t* d_mask, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (tile_size < 1 || tile_size > 64)
+        return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_template_masks: tile size must be in 1..64");
+    if (!d_mask) return set_err(WEFT_E_INVALID, "weft_template_masks: null pointer");
+    WeftTemplateMaskArgs a{d_mask, tile_size, 0};
+    return launch(g_mod.f_tmpl_masks, 1, 256, 0, stream, &a);
+}
+
+int weft_template_fit(const float* d_image, int width, int height, int tile_size, const uint32_t* d_mask,
+                      int32_t* d_model, float* d_c0, float* d_c1, float* d_mse, float* d_c0_all, float* d_c1_all,
+                      void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (tile_size < 1 || tile_size > 64)
+        return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_template_fit: tile size must be in 1..64");
+    if (width <= 0 || height <= 0) return set_err(WEFT_E_INVALID, "weft_template_fit: image dimensions must be > 0");
+    if ((int64_t)width * height * 3 >= ((int64_t)1 << 62))
+        return set_err(WEFT_E_TOO_LARGE, "weft_template_fit: image too large");
+    const int64_t cols = (width + tile_size - 1) / tile_size;
+    const int64_t rows = (height + tile_size - 1) / tile_size;
+    if (cols * rows > 0x7fffffff) return set_err(WEFT_E_TOO_LARGE, "weft_template_fit: too many tiles");
+    if (!d_image || !d_mask || !d_model || !d_c0 || !d_c1 || !d_mse || !d_c0_all || !d_c1_all)
+        return set_err(WEFT_E_INVALID, "weft_template_fit: null pointer");
+    WeftTemplateFitArgs a{d_image, d_mask, d_model, d_c0, d_c1, d_mse, d_c0_all, d_c1_all,
+                          width, height, tile_size, (int32_t)cols};
+    const int n = tile_size * tile_size;
+    const unsigned lds = (unsigned)((3 * (n + 16) + n) * 4);
+    return launch(g_mod.f_tmpl_fit, (unsigned)(cols * rows), 256, lds, stream, &a);
 }
 
 }  // extern "C"

@@ -1133,6 +1133,7 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
 #include "weft_split.h"
 #include "weft_rans.h"
 #include "weft_batch_refine.h"
+#include "weft_template.h"
 
 }  // namespace
 
@@ -1384,3 +1385,9 @@ extern "C" __global__ __launch_bounds__(64) void weft_score_evals_split_kernel(W
     default: break;
     }
 }
+
+// Stage-3 template fit: bucket table of one tile size, then one workgroup per tile.
+extern "C" __global__ __launch_bounds__(256) void weft_template_masks_kernel(WeftTemplateMaskArgs A) {
+    tmpl_masks_body(A);
+}
+extern "C" __global__ __launch_bounds__(256) void weft_template_fit_kernel(WeftTemplateFitArgs A) { tmpl_fit_body(A); }

@@ -29,6 +29,9 @@
  *   weft_template_masks /         kernel weft_tile_fit_candidates and its launcher
  *   weft_template_fit             fit_tiles_gpu_constant (src/weft/gpu_encoder.py:42-251,
  *                                 411-562), the stage-3 baseline fit
+ *   weft_argmin_f64, weft_exclusive_scan_i64, weft_compact_flags, weft_segmented_topk
+ *                                 cccl_argmin, cccl_exclusive_scan, cccl_compact,
+ *                                 cccl_segmented_topk (src/weft/cuda_backend.py:80-181)
  */
 #ifndef WEFT_HIP_H
 #define WEFT_HIP_H
@@ -379,6 +382,39 @@ int weft_template_masks(int tile_size, uint32_t* d_mask, void* stream);
 int weft_template_fit(const float* d_image, int width, int height, int tile_size, const uint32_t* d_mask,
                       int32_t* d_model, float* d_c0, float* d_c1, float* d_mse, float* d_c0_all,
                       float* d_c1_all, void* stream);
+
+/* Selection primitives (src/weft/cuda_backend.py:80-181).  Each takes a caller-owned
+ * device workspace of at least weft_select_work_bytes(op, n) bytes; calls sharing one
+ * workspace must not overlap.  Orders are NumPy's: NaN after +inf when sorting and first
+ * in argmin, -0.0 equal to +0.0, equal keys in index order. */
+#define WEFT_SELECT_ARGMIN 0
+#define WEFT_SELECT_SCAN 1
+#define WEFT_SELECT_COMPACT 2
+#define WEFT_SELECT_TOPK 3
+int64_t weft_select_work_bytes(int op, int64_t n);
+
+/* cccl_argmin (cuda_backend.py:80-98): *d_out_index = np.argmin(values) -- the first
+ * index of the least value, or of the first NaN.  n > 0. */
+int weft_argmin_f64(const double* d_values, int64_t n, int64_t* d_out_index, void* d_work, int64_t work_bytes,
+                    void* stream);
+
+/* cccl_exclusive_scan (cuda_backend.py:101-118): d_out[i] = sum of d_in[0 .. i) (int64,
+ * wrapping on overflow); the grand total goes to *d_total when it is not NULL. */
+int weft_exclusive_scan_i64(const int64_t* d_in, int64_t n, int64_t* d_out, int64_t* d_total, void* d_work,
+                            int64_t work_bytes, void* stream);
+
+/* cccl_compact (cuda_backend.py:121-140): the indices of the nonzero flags in increasing
+ * order into d_out_indices (room for n), their number into *d_out_count. */
+int weft_compact_flags(const uint8_t* d_flags, int64_t n, int64_t* d_out_indices, int64_t* d_out_count,
+                       void* d_work, int64_t work_bytes, void* stream);
+
+/* cccl_segmented_topk (cuda_backend.py:143-181): the indices of the k smallest scores of
+ * each segment (largest != 0: the k largest), listed in the order of a stable ascending
+ * sort of all scores (largest: that order reversed), into d_out_indices (room for n); the
+ * number written into *d_out_count.  n <= 2^30. */
+int weft_segmented_topk(const double* d_scores, const int64_t* d_segments, int64_t n, int64_t k, int largest,
+                        int64_t* d_out_indices, int64_t* d_out_count, void* d_work, int64_t work_bytes,
+                        void* stream);
 
 #ifdef __cplusplus
 }

@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 11
+#define WEFT_ABI_VERSION 12
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -192,4 +192,34 @@ struct WeftTemplateFitArgs {
     int32_t height;
     int32_t tile_size;
     int32_t tile_cols;
+};
+
+struct WeftSelRec {
+    uint64_t a;
+    uint64_t b;
+    uint32_t idx;
+    uint32_t _pad;
+};
+
+struct WeftSelectArgs {
+    const double* f64_in;
+    const int64_t* i64_in;
+    const uint8_t* u8_in;
+    int64_t* out_i64;
+    int64_t* out_total;
+    double* work_f64;
+    int64_t* work_i64;
+    WeftSelRec* recs;
+    WeftSelRec* recs2;
+    uint8_t* flags;
+    const int64_t* positions;
+    const int64_t* count_in;
+    int64_t n;
+    int64_t n_pad;
+    int64_t n_blocks;
+    int64_t k;
+    int64_t k_stage;
+    int64_t j_step;
+    int32_t largest;
+    int32_t _pad;
 };

@@ -44,6 +44,17 @@ struct Module {
     hipFunction_t f_evals_split = nullptr;
     hipFunction_t f_tmpl_masks = nullptr;
     hipFunction_t f_tmpl_fit = nullptr;
+    hipFunction_t f_sel_argmin_partial = nullptr;
+    hipFunction_t f_sel_argmin_final = nullptr;
+    hipFunction_t f_sel_tile_sums = nullptr;
+    hipFunction_t f_sel_scan_tiles = nullptr;
+    hipFunction_t f_sel_scan_apply = nullptr;
+    hipFunction_t f_sel_topk_pack = nullptr;
+    hipFunction_t f_sel_bitonic_local = nullptr;
+    hipFunction_t f_sel_bitonic_global = nullptr;
+    hipFunction_t f_sel_topk_keep = nullptr;
+    hipFunction_t f_sel_topk_gather = nullptr;
+    hipFunction_t f_sel_topk_emit = nullptr;
     hipFunction_t f_rans_enc = nullptr;
     hipFunction_t f_rans_scan = nullptr;
     hipFunction_t f_rans_pack = nullptr;
@@ -190,6 +201,17 @@ int weft_init(int device) {
         {&g_mod.f_evals_split, "weft_score_evals_split_kernel"},
         {&g_mod.f_tmpl_masks, "weft_template_masks_kernel"},
         {&g_mod.f_tmpl_fit, "weft_template_fit_kernel"},
+        {&g_mod.f_sel_argmin_partial, "weft_sel_argmin_partial_kernel"},
+        {&g_mod.f_sel_argmin_final, "weft_sel_argmin_final_kernel"},
+        {&g_mod.f_sel_tile_sums, "weft_sel_tile_sums_kernel"},
+        {&g_mod.f_sel_scan_tiles, "weft_sel_scan_tiles_kernel"},
+        {&g_mod.f_sel_scan_apply, "weft_sel_scan_apply_kernel"},
+        {&g_mod.f_sel_topk_pack, "weft_sel_topk_pack_kernel"},
+        {&g_mod.f_sel_bitonic_local, "weft_sel_bitonic_local_kernel"},
+        {&g_mod.f_sel_bitonic_global, "weft_sel_bitonic_global_kernel"},
+        {&g_mod.f_sel_topk_keep, "weft_sel_topk_keep_kernel"},
+        {&g_mod.f_sel_topk_gather, "weft_sel_topk_gather_kernel"},
+        {&g_mod.f_sel_topk_emit, "weft_sel_topk_emit_kernel"},
         {&g_mod.f_rans_enc, "weft_rans_encode_kernel"},
         {&g_mod.f_rans_scan, "weft_rans_scan_kernel"},
         {&g_mod.f_rans_pack, "weft_rans_pack_kernel"},
@@ -566,6 +588,203 @@ int weft_template_fit(const float* d_image, int width, int height, int tile_size
     const int n = tile_size * tile_size;
     const unsigned lds = (unsigned)((3 * (n + 16) + n) * 4);
     return launch(g_mod.f_tmpl_fit, (unsigned)(cols * rows), 256, lds, stream, &a);
+}
+
+// ---- selection primitives: workspace layout shared by the size query and the launchers
+namespace {
+constexpr int64_t SEL_TILE_ELEMS = 2048;
+constexpr int64_t SEL_SORT_MIN = 512;
+constexpr int64_t SEL_MAX_SORT = (int64_t)1 << 30;
+
+int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
+int64_t sel_blocks(int64_t n) { return (n + SEL_TILE_ELEMS - 1) / SEL_TILE_ELEMS; }
+int64_t sel_argmin_grid(int64_t n) {
+    const int64_t g = sel_blocks(n);
+    return g < 1024 ? (g < 1 ? 1 : g) : 1024;
+}
+int64_t sel_pad(int64_t n) {
+    int64_t p = SEL_SORT_MIN;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+struct SelLayout {  // byte offsets into the workspace
+    int64_t f64 = 0, i64 = 0, recs = 0, recs2 = 0, flags = 0, pos = 0, count = 0, total = 0;
+};
+
+SelLayout sel_layout(int op, int64_t n) {
+    SelLayout L;
+    int64_t off = 0;
+    if (op == WEFT_SELECT_ARGMIN) {
+        const int64_t g = sel_argmin_grid(n);
+        L.f64 = off;
+        off += align256(8 * g);
+        L.i64 = off;
+        off += align256(8 * g);
+    } else if (op == WEFT_SELECT_SCAN || op == WEFT_SELECT_COMPACT) {
+        L.i64 = off;
+        off += align256(8 * sel_blocks(n));
+    } else {
+        const int64_t np = sel_pad(n);
+        L.recs = off;
+        off += align256((int64_t)sizeof(WeftSelRec) * np);
+        L.recs2 = off;
+        off += align256((int64_t)sizeof(WeftSelRec) * np);
+        L.flags = off;
+        off += align256(n);
+        L.pos = off;
+        off += align256(8 * n);
+        L.i64 = off;
+        off += align256(8 * sel_blocks(n));
+        L.count = off;
+        off += 256;
+    }
+    L.total = off;
+    return L;
+}
+
+// Three-pass scan of A.i64_in (or compaction of A.u8_in) over A.n elements.
+int sel_scan_passes(WeftSelectArgs& a, void* stream) {
+    a.n_blocks = sel_blocks(a.n);
+    int rc = launch(g_mod.f_sel_tile_sums, (unsigned)a.n_blocks, 256, 0, stream, &a);
+    if (rc) return rc;
+    rc = launch(g_mod.f_sel_scan_tiles, 1, 256, 0, stream, &a);
+    if (rc) return rc;
+    return launch(g_mod.f_sel_scan_apply, (unsigned)a.n_blocks, 256, 0, stream, &a);
+}
+
+// Bitonic sort of a.n_pad records at `recs`.
+int sel_sort(WeftSelectArgs& a, WeftSelRec* recs, void* stream) {
+    WeftSelectArgs s = a;
+    s.recs = recs;
+    s.k_stage = 0;
+    const unsigned blocks = (unsigned)(s.n_pad / SEL_SORT_MIN);
+    int rc = launch(g_mod.f_sel_bitonic_local, blocks, 256, 0, stream, &s);
+    for (int64_t k = 2 * SEL_SORT_MIN; k <= s.n_pad && rc == 0; k <<= 1) {
+        s.k_stage = k;
+        for (int64_t j = k / 2; j >= SEL_SORT_MIN && rc == 0; j >>= 1) {
+            s.j_step = j;
+            rc = launch(g_mod.f_sel_bitonic_global, (unsigned)(s.n_pad / 2 / 256), 256, 0, stream, &s);
+        }
+        if (rc == 0) rc = launch(g_mod.f_sel_bitonic_local, blocks, 256, 0, stream, &s);
+    }
+    return rc;
+}
+
+int sel_check_work(int op, int64_t n, void* d_work, int64_t work_bytes, const char* who) {
+    if (!d_work) return set_err(WEFT_E_INVALID, std::string(who) + ": null workspace");
+    if (work_bytes < sel_layout(op, n).total)
+        return set_err(WEFT_E_INVALID, std::string(who) + ": workspace smaller than weft_select_work_bytes");
+    return WEFT_OK;
+}
+}  // namespace
+
+int64_t weft_select_work_bytes(int op, int64_t n) {
+    if (op < WEFT_SELECT_ARGMIN || op > WEFT_SELECT_TOPK || n < 0) return -1;
+    return sel_layout(op, n).total;
+}
+
+int weft_argmin_f64(const double* d_values, int64_t n, int64_t* d_out_index, void* d_work, int64_t work_bytes,
+                    void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n <= 0) return set_err(WEFT_E_INVALID, "weft_argmin_f64: n must be > 0");
+    if (!d_values || !d_out_index) return set_err(WEFT_E_INVALID, "weft_argmin_f64: null pointer");
+    if ((rc = sel_check_work(WEFT_SELECT_ARGMIN, n, d_work, work_bytes, "weft_argmin_f64"))) return rc;
+    const SelLayout L = sel_layout(WEFT_SELECT_ARGMIN, n);
+    WeftSelectArgs a{};
+    a.f64_in = d_values;
+    a.n = n;
+    a.work_f64 = (double*)((char*)d_work + L.f64);
+    a.work_i64 = (int64_t*)((char*)d_work + L.i64);
+    a.out_i64 = d_out_index;
+    const int64_t g = sel_argmin_grid(n);
+    a.n_blocks = g;
+    rc = launch(g_mod.f_sel_argmin_partial, (unsigned)g, 256, 0, stream, &a);
+    if (rc) return rc;
+    return launch(g_mod.f_sel_argmin_final, 1, 256, 0, stream, &a);
+}
+
+int weft_exclusive_scan_i64(const int64_t* d_in, int64_t n, int64_t* d_out, int64_t* d_total, void* d_work,
+                            int64_t work_bytes, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n < 0) return set_err(WEFT_E_INVALID, "weft_exclusive_scan_i64: n < 0");
+    if (n == 0) return WEFT_OK;
+    if (!d_in || !d_out) return set_err(WEFT_E_INVALID, "weft_exclusive_scan_i64: null pointer");
+    if ((rc = sel_check_work(WEFT_SELECT_SCAN, n, d_work, work_bytes, "weft_exclusive_scan_i64"))) return rc;
+    WeftSelectArgs a{};
+    a.i64_in = d_in;
+    a.n = n;
+    a.out_i64 = d_out;
+    a.out_total = d_total;
+    a.work_i64 = (int64_t*)((char*)d_work + sel_layout(WEFT_SELECT_SCAN, n).i64);
+    return sel_scan_passes(a, stream);
+}
+
+int weft_compact_flags(const uint8_t* d_flags, int64_t n, int64_t* d_out_indices, int64_t* d_out_count,
+                       void* d_work, int64_t work_bytes, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n < 0) return set_err(WEFT_E_INVALID, "weft_compact_flags: n < 0");
+    if (n == 0) return WEFT_OK;
+    if (!d_flags || !d_out_indices || !d_out_count) return set_err(WEFT_E_INVALID, "weft_compact_flags: null pointer");
+    if ((rc = sel_check_work(WEFT_SELECT_COMPACT, n, d_work, work_bytes, "weft_compact_flags"))) return rc;
+    WeftSelectArgs a{};
+    a.u8_in = d_flags;
+    a.n = n;
+    a.out_i64 = d_out_indices;
+    a.out_total = d_out_count;
+    a.work_i64 = (int64_t*)((char*)d_work + sel_layout(WEFT_SELECT_COMPACT, n).i64);
+    return sel_scan_passes(a, stream);
+}
+
+int weft_segmented_topk(const double* d_scores, const int64_t* d_segments, int64_t n, int64_t k, int largest,
+                        int64_t* d_out_indices, int64_t* d_out_count, void* d_work, int64_t work_bytes,
+                        void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n < 0 || k < 0) return set_err(WEFT_E_INVALID, "weft_segmented_topk: n and k must be >= 0");
+    if (n > SEL_MAX_SORT) return set_err(WEFT_E_TOO_LARGE, "weft_segmented_topk: more than 2^30 scores");
+    if (!d_out_count) return set_err(WEFT_E_INVALID, "weft_segmented_topk: null count pointer");
+    if (n == 0 || k == 0) {  // count = 0, written by a kernel (the scan of no tiles)
+        WeftSelectArgs z{};
+        z.out_total = d_out_count;
+        return launch(g_mod.f_sel_scan_tiles, 1, 256, 0, stream, &z);
+    }
+    if (!d_scores || !d_segments || !d_out_indices) return set_err(WEFT_E_INVALID, "weft_segmented_topk: null pointer");
+    if ((rc = sel_check_work(WEFT_SELECT_TOPK, n, d_work, work_bytes, "weft_segmented_topk"))) return rc;
+    const SelLayout L = sel_layout(WEFT_SELECT_TOPK, n);
+    char* w = (char*)d_work;
+    WeftSelectArgs a{};
+    a.f64_in = d_scores;
+    a.i64_in = d_segments;
+    a.n = n;
+    a.n_pad = sel_pad(n);
+    a.k = k;
+    a.largest = largest ? 1 : 0;
+    a.recs = (WeftSelRec*)(w + L.recs);
+    a.recs2 = (WeftSelRec*)(w + L.recs2);
+    a.flags = (uint8_t*)(w + L.flags);
+    const unsigned pad_blocks = (unsigned)(a.n_pad / 256);
+    if ((rc = launch(g_mod.f_sel_topk_pack, pad_blocks, 256, 0, stream, &a))) return rc;
+    if ((rc = sel_sort(a, a.recs, stream))) return rc;
+    if ((rc = launch(g_mod.f_sel_topk_keep, (unsigned)((n + 255) / 256), 256, 0, stream, &a))) return rc;
+    // kept sorted positions, in order, and their count
+    WeftSelectArgs c{};
+    c.u8_in = a.flags;
+    c.n = n;
+    c.out_i64 = (int64_t*)(w + L.pos);
+    c.out_total = d_out_count;
+    c.work_i64 = (int64_t*)(w + L.i64);
+    if ((rc = sel_scan_passes(c, stream))) return rc;
+    a.positions = c.out_i64;
+    a.count_in = d_out_count;
+    if ((rc = launch(g_mod.f_sel_topk_gather, pad_blocks, 256, 0, stream, &a))) return rc;
+    if ((rc = sel_sort(a, a.recs2, stream))) return rc;
+    a.recs = a.recs2;
+    a.out_i64 = d_out_indices;
+    return launch(g_mod.f_sel_topk_emit, (unsigned)((n + 255) / 256), 256, 0, stream, &a);
 }
 
 }  // extern "C"

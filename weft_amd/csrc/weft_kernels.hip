@@ -1134,6 +1134,7 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
 #include "weft_rans.h"
 #include "weft_batch_refine.h"
 #include "weft_template.h"
+#include "weft_select.h"
 
 }  // namespace
 
@@ -1391,3 +1392,30 @@ extern "C" __global__ __launch_bounds__(256) void weft_template_masks_kernel(Wef
     tmpl_masks_body(A);
 }
 extern "C" __global__ __launch_bounds__(256) void weft_template_fit_kernel(WeftTemplateFitArgs A) { tmpl_fit_body(A); }
+
+// Selection primitives (argmin, exclusive scan, compaction, segmented top-k).
+extern "C" __global__ __launch_bounds__(256) void weft_sel_argmin_partial_kernel(WeftSelectArgs A) {
+    sel_argmin_partial_body(A);
+}
+extern "C" __global__ __launch_bounds__(256) void weft_sel_argmin_final_kernel(WeftSelectArgs A) {
+    sel_argmin_final_body(A);
+}
+extern "C" __global__ __launch_bounds__(256) void weft_sel_tile_sums_kernel(WeftSelectArgs A) { sel_tile_sums_body(A); }
+extern "C" __global__ __launch_bounds__(256) void weft_sel_scan_tiles_kernel(WeftSelectArgs A) {
+    sel_scan_tiles_body(A);
+}
+extern "C" __global__ __launch_bounds__(256) void weft_sel_scan_apply_kernel(WeftSelectArgs A) {
+    sel_scan_apply_body(A);
+}
+extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_pack_kernel(WeftSelectArgs A) { sel_topk_pack_body(A); }
+extern "C" __global__ __launch_bounds__(256) void weft_sel_bitonic_local_kernel(WeftSelectArgs A) {
+    sel_bitonic_local_body(A);
+}
+extern "C" __global__ __launch_bounds__(256) void weft_sel_bitonic_global_kernel(WeftSelectArgs A) {
+    sel_bitonic_global_body(A);
+}
+extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_keep_kernel(WeftSelectArgs A) { sel_topk_keep_body(A); }
+extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_gather_kernel(WeftSelectArgs A) {
+    sel_topk_gather_body(A);
+}
+extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_emit_kernel(WeftSelectArgs A) { sel_topk_emit_body(A); }

@@ -32,6 +32,8 @@ Further objects on the same line (skipped with --no-extras):
   tile of the 1920x1080 quadtree: device-resident and host-inclusive rates.
 - decode: gpu_render_tiles_to_image of the 3840x2160 plain fit's primitives.
 - entropy: chunked rANS encode and decode (64 tiles per chunk, the reference default) of that fit's PRIM bytes.
+- select: the selection helpers (argmin, exclusive scan, compaction, per-segment top-k)
+  on 2^22 device elements.
 - template_fit: the stage-3 template fit (fit_tiles_gpu_constant) of the 3840x2160 image at
   16-px tiles: device rate, HBM rate against the 8 TB/s peak, the host API's time and the
   single-threaded CPU oracle on the same image.
@@ -369,6 +371,47 @@ def template_report(image, ts=16):
                 speedup_vs_cpu_oracle=cpu_s / dev_s, models_equal_oracle_frac=same_model)
 
 
+def select_report(n=1 << 22, seed=0):
+    """The selection helpers (argmin, exclusive scan, compaction, segmented top-k) on n device elements."""
+    import torch
+    from oracle import select_np
+    from weft_amd.cuda_backend import argmin_device, compact_device, exclusive_scan_device, segmented_topk_device
+
+    rng = np.random.default_rng(seed)
+    v = torch.from_numpy(rng.normal(size=n)).cuda()
+    x = torch.from_numpy(rng.integers(0, 1000, n)).cuda()
+    f = torch.from_numpy((rng.random(n) < 0.5).astype(np.uint8)).cuda()
+    seg = torch.from_numpy(rng.integers(0, n // 64, n)).cuda()
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        times = []
+        for _ in range(5):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            times.append(a.elapsed_time(b) * 1e-3)
+        return float(np.median(times))
+
+    out = dict(n=n)
+    for name, fn, nbytes in (("argmin", lambda: argmin_device(v), 8 * n),
+                             ("exclusive_scan", lambda: exclusive_scan_device(x), 16 * n),
+                             ("compact", lambda: compact_device(f), n + 8 * n // 2),
+                             ("segmented_topk_k4", lambda: segmented_topk_device(v, seg, 4), 16 * n)):
+        t = timed(fn)
+        out[name] = dict(device_s=t, melem_per_s=n / t / 1e6, gb_per_s=nbytes / t / 1e9)
+    o, c = segmented_topk_device(v, seg, 4)
+    m = 1 << 16  # correctness spot check on a prefix against the restatement
+    o2, c2 = segmented_topk_device(v[:m], seg[:m], 4)
+    out["topk_prefix_equals_oracle"] = (o2[: int(c2[0])].cpu().numpy().tolist()
+                                        == select_np.segmented_topk(v[:m].cpu().numpy(), seg[:m].cpu().numpy(), 4))
+    return out
+
+
 def entropy_report(tiles, records, chunk_tiles=64):
     """Chunked rANS of a fitted image's PRIM payload, encode and decode on the device."""
     import torch
@@ -485,7 +528,18 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the time-to-PSNR, batched and refine reports")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES of device memory once (HBM counter calibration) and exit")
+    ap.add_argument("--only", choices=["template_fit", "select"], default=None,
+                    help="run only that report (for profiling) and print it as JSON")
     args = ap.parse_args()
+    if args.only:
+        from weft_amd.synth import synth_image
+
+        if args.only == "template_fit":
+            rep = template_report(synth_image(2160, 3840, 2500, seed=0, texture=True))
+        else:
+            rep = select_report()
+        print(json.dumps({args.only: rep}), flush=True)
+        return
     if args.copy_probe:
         import torch
 
@@ -622,6 +676,10 @@ def main():
             f"{extras['template_fit']['hbm_gb_per_s']:.0f} GB/s on the device; CPU oracle "
             f"{extras['template_fit']['speedup_vs_cpu_oracle']:.0f}x slower")
         del t4, m4, rec4
+        extras["select"] = select_report()
+        log(f"[bench] select 4M: argmin {extras['select']['argmin']['gb_per_s']:.0f} GB/s, scan "
+            f"{extras['select']['exclusive_scan']['gb_per_s']:.0f} GB/s, segmented top-k "
+            f"{extras['select']['segmented_topk_k4']['melem_per_s']:.0f} Melem/s")
         extras["drop_in"] = drop_in_report(tiles, seed)
         log(f"[bench] drop-in scorer: {extras['drop_in']['evals_per_s'] / 1e6:.1f} M evals/s on the device, "
             f"{extras['drop_in']['host_inclusive_evals_per_s'] / 1e6:.1f} M/s through the host API")

@@ -36,6 +36,7 @@ def _device():
 def _work(op: int, n: int):
     import torch
 
+    _lib.ensure_init(0)
     nbytes = int(_lib.library().weft_select_work_bytes(op, n))
     if nbytes < 0:
         raise ValueError("bad selection size")

@@ -1353,7 +1353,16 @@ extern "C" __global__ __launch_bounds__(256) void weft_rans_pack_kernel(WeftRans
 extern "C" __global__ __launch_bounds__(64) void weft_rans_decode_kernel(WeftRansDecodeArgs A) { rans_decode_body(A); }
 
 // Persistent refined fit (tiles of many images, every round refined and re-scored).
-extern "C" __global__ __launch_bounds__(256) void weft_batch_refine_kernel(WeftBatchRefineArgs A) { batch_refine_body(A); }
+#ifndef WEFT_BATCH_REFINE_MIN_WAVES
+// Waves per SIMD the register allocation must allow: 4 caps the kernel at 128 VGPRs (the
+// refinement phase spills a little to scratch), which runs the whole fit faster than the
+// uncapped 201 VGPRs at 2 waves per SIMD.
+#define WEFT_BATCH_REFINE_MIN_WAVES 4
+#endif
+extern "C" __global__ __launch_bounds__(256, WEFT_BATCH_REFINE_MIN_WAVES) void weft_batch_refine_kernel(
+    WeftBatchRefineArgs A) {
+    batch_refine_body(A);
+}
 
 extern "C" __global__ __launch_bounds__(64) void weft_fit_init_kernel(WeftFitInitArgs A) {
     const int t = blockIdx.x;

@@ -1335,13 +1335,20 @@ extern "C" __global__ __launch_bounds__(256) void weft_render_tiles_kernel(WeftR
 }
 
 // Soft-edge SSE and its gradient for groups of k candidates (one 64-lane workgroup each).
-extern "C" __global__ __launch_bounds__(64) void weft_soft_eval_kernel(WeftSoftArgs A) { soft_eval_body(A); }
+#ifndef WEFT_SOFT_MIN_WAVES
+#define WEFT_SOFT_MIN_WAVES 1
+#endif
+extern "C" __global__ __launch_bounds__(64, WEFT_SOFT_MIN_WAVES) void weft_soft_eval_kernel(WeftSoftArgs A) {
+    soft_eval_body(A);
+}
 
 // Per-tile top-k indices of a value table (one workgroup per tile).
 extern "C" __global__ __launch_bounds__(256) void weft_topk_kernel(WeftTopkArgs A) { topk_body(A); }
 
 // Adam refinement of each tile's top-k candidates (one 64-lane workgroup per tile).
-extern "C" __global__ __launch_bounds__(64) void weft_refine_kernel(WeftRefineArgs A) { refine_body(A); }
+extern "C" __global__ __launch_bounds__(64, WEFT_SOFT_MIN_WAVES) void weft_refine_kernel(WeftRefineArgs A) {
+    refine_body(A);
+}
 
 // Exact scores of a few candidates per tile, each candidate's pixels split over lanes.
 extern "C" __global__ __launch_bounds__(64) void weft_fit_score_split_kernel(WeftSplitScoreArgs A) { split_score_body(A); }

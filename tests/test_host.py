@@ -321,3 +321,31 @@ def test_quantize_primitive_equals_decode_of_encode():
         assert abs(q.alpha - min(1, max(0, p.alpha))) <= 0.5 / 65535 + 1e-12
         for a, b in zip(q.color0, p.color0):
             assert abs(a - min(1, max(0, b))) <= 0.5 / 255 + 1e-12
+
+
+def test_prepacked_eval_packing_matches_loop():
+    """The vectorized eval packing of gpu_batch_score_prepacked equals the per-tile loop."""
+    from weft_amd.gpu_render import _pack_evals
+    from weft_amd.records import EVAL_DTYPE, PRIM_DTYPE
+
+    rng = np.random.default_rng(2)
+    T = 40
+    sizes = rng.choice([8, 16, 32, 64], T).tolist()
+    idx = rng.permutation(100)[:T].tolist()
+    offs = {ti: int(rng.integers(0, 10 ** 6)) * 3 for ti in idx}
+    counts = [rng.integers(0, 6, int(rng.integers(0, 5))).astype(np.int32) for _ in range(T)]
+    arrays = [np.zeros(int(c.sum()) + int(rng.integers(0, 3)), PRIM_DTYPE) for c in counts]
+    ev, ranges, _ = _pack_evals(offs, idx, sizes, arrays, counts)
+    ref = np.zeros(sum(len(c) for c in counts), EVAL_DTYPE)
+    e0 = p0 = 0
+    want = []
+    for j, ti in enumerate(idx):
+        c = counts[j]
+        o = p0
+        for i, n in enumerate(c):
+            ref[e0 + i] = (o, n, offs[ti], sizes[j])
+            o += int(n)
+        want.append((e0, len(c)))
+        e0 += len(c)
+        p0 += len(arrays[j])
+    assert ev.tobytes() == ref.tobytes() and ranges == want

@@ -111,6 +111,28 @@ def _score_evals(tiles_flat: np.ndarray, evals: np.ndarray, prims: np.ndarray) -
     return out.cpu().numpy()
 
 
+def _pack_evals(tile_pixel_offsets, tile_indices, tile_sizes, eval_prim_arrays, eval_prim_counts):
+    """Eval records of the prepacked layout, vectorized: (evals, [(first, count)] per tile, array lengths).
+
+    Tile j's stacks lie back to back in eval_prim_arrays[j], which sit back to back in the
+    concatenated primitive array; stack i of tile j starts after the stacks before it.
+    """
+    counts = [np.asarray(c, dtype=np.int64).ravel() for c in eval_prim_counts]
+    n_per = np.array([len(c) for c in counts], np.int64)
+    cnt = np.concatenate(counts) if len(counts) else np.zeros(0, np.int64)
+    arr_len = np.array([len(a) for a in eval_prim_arrays], np.int64)
+    arr_start = np.cumsum(arr_len) - arr_len
+    e_start = np.cumsum(n_per) - n_per
+    ex = np.cumsum(cnt) - cnt  # exclusive prefix over every stack
+    tile_base = np.append(ex, 0)[e_start]
+    evals = np.zeros(len(cnt), dtype=EVAL_DTYPE)
+    evals["prim_offset"] = np.repeat(arr_start - tile_base, n_per) + ex
+    evals["prim_count"] = cnt
+    evals["tile_offset"] = np.repeat(np.array([int(tile_pixel_offsets[ti]) for ti in tile_indices], np.int64), n_per)
+    evals["tile_size"] = np.repeat(np.array([int(v) for v in tile_sizes], np.int64), n_per)
+    return evals, list(zip(e_start.tolist(), n_per.tolist())), arr_len
+
+
 def gpu_batch_score_prepacked(
     tile_pixels_flat: np.ndarray,
     tile_pixel_offsets: dict[int, int],
@@ -133,22 +155,9 @@ def gpu_batch_score_prepacked(
             return None
         if any(int(s) not in KERNEL_TILE_SIZES for s in tile_sizes):
             return _fail(ValueError(f"tile sizes must be in {KERNEL_TILE_SIZES}"))
-        evals = np.zeros(total, dtype=EVAL_DTYPE)
-        ranges = []
-        e0 = 0
-        p0 = 0
-        for j, ti in enumerate(tile_indices):
-            counts = np.asarray(eval_prim_counts[j], dtype=np.int64)
-            n = len(counts)
-            offs = p0 + np.concatenate(([0], np.cumsum(counts)[:-1])) if n else np.zeros(0, np.int64)
-            sl = slice(e0, e0 + n)
-            evals["prim_offset"][sl] = offs
-            evals["prim_count"][sl] = counts
-            evals["tile_offset"][sl] = int(tile_pixel_offsets[ti])
-            evals["tile_size"][sl] = int(tile_sizes[j])
-            ranges.append((e0, n))
-            e0 += n
-            p0 += len(eval_prim_arrays[j])
+        evals, ranges, arr_len = _pack_evals(tile_pixel_offsets, tile_indices, tile_sizes, eval_prim_arrays,
+                                             eval_prim_counts)
+        p0 = int(arr_len.sum())
         if p0 >= 2 ** 31 or len(tile_pixels_flat) >= 2 ** 31:
             return _fail(ValueError("batch too large for 32-bit offsets"))
         prims = (np.concatenate([np.asarray(a, dtype=PRIM_DTYPE) for a in eval_prim_arrays])

@@ -186,6 +186,151 @@ __device__ __forceinline__ void composite_hard_px(const WeftPrim& P, float px, f
     b = P.color0[2] * al + b * oma;
 }
 
+// composite_px over 8 consecutive pixels of one row (x0 + i + 0.5, py), i = 0..7: the same
+// float operations per pixel, with every per-primitive and per-row term formed once.
+__device__ __forceinline__ void composite_row8(const WeftPrim& P, float x0, float py, float (&r)[8], float (&g)[8],
+                                               float (&b)[8]) {
+    const float a = clip01(P.alpha);
+    float al[8];
+    float s0[8], s1[8], s2[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        s0[i] = P.color0[0];
+        s1[i] = P.color0[1];
+        s2[i] = P.color0[2];
+    }
+    switch (P.kind) {
+    case 0:
+#pragma unroll
+        for (int i = 0; i < 8; ++i) al[i] = a;
+        break;
+    case 1:
+    case 2: {
+        const double gx0 = P.geom[0], gy0 = P.geom[1], gx1 = P.geom[2], gy1 = P.geom[3];
+        const double vx = gx1 - gx0, vy = gy1 - gy0;
+        const double den = vx * vx + vy * vy + 1e-8;
+        const float VX = (float)vx, VY = (float)vy, DN = (float)den;
+        const float X0 = (float)gx0, Y0 = (float)gy0;
+        const float ty = (py - Y0) * VY;
+        if (P.kind == 1) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float px = (x0 + (float)i) + 0.5f;
+                const float t = clip01(((px - X0) * VX + ty) / DN);
+                const float omt = 1.0f - t;
+                s0[i] = P.color0[0] * omt + P.color1[0] * t;
+                s1[i] = P.color0[1] * omt + P.color1[1] * t;
+                s2[i] = P.color0[2] * omt + P.color1[2] * t;
+                al[i] = a;
+            }
+        } else {
+            const double sigma = (double)P.geom[4] > 0.5 ? (double)P.geom[4] : 0.5;
+            const float k2 = (float)(1.0 / (2.0 * sigma * sigma));
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const float px = (x0 + (float)i) + 0.5f;
+                const float t = clip01(((px - X0) * VX + ty) / DN);
+                const float cx = X0 + t * VX, cy = Y0 + t * VY;
+                const float dx = px - cx, dy = py - cy;
+                const float dist = sqrtf(dx * dx + dy * dy);
+                al[i] = expf((-(dist * dist)) * k2) * a;
+            }
+        }
+        break;
+    }
+    case 3: {
+        float best[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) best[i] = 1e9f;
+        float prevx = 0.0f, prevy = 0.0f;
+        for (int k = 0; k < 32; ++k) {
+            const float t = (k == 31) ? 1.0f : (float)((double)k * (1.0 / 31.0));
+            const float u = 1.0f - t;
+            const float uu = u * u, tu2 = (2.0f * u) * t, tt = t * t;
+            const float x = (uu * P.geom[0] + tu2 * P.geom[2]) + tt * P.geom[4];
+            const float y = (uu * P.geom[1] + tu2 * P.geom[3]) + tt * P.geom[5];
+            if (k > 0) {  // seg_dist_f32(px, py, prevx, prevy, x, y) for the 8 pixels
+                const float vx = x - prevx, vy = y - prevy;
+                const float den = (vx * vx + vy * vy) + 1e-8f;
+                const float ty = (py - prevy) * vy;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const float px = (x0 + (float)i) + 0.5f;
+                    const float tt2 = clip01(((px - prevx) * vx + ty) / den);
+                    const float qx = prevx + tt2 * vx, qy = prevy + tt2 * vy;
+                    const float dx = px - qx, dy = py - qy;
+                    best[i] = fminf(best[i], sqrtf(dx * dx + dy * dy));
+                }
+            }
+            prevx = x;
+            prevy = y;
+        }
+        const double sigma = (double)P.geom[6] > 0.5 ? (double)P.geom[6] : 0.5;
+        const float k2 = (float)(1.0 / (2.0 * sigma * sigma));
+#pragma unroll
+        for (int i = 0; i < 8; ++i) al[i] = expf((-(best[i] * best[i])) * k2) * a;
+        break;
+    }
+    case 4:
+    case 8: {
+        const float* gm = P.geom;
+        const double gx0 = gm[0], gy0 = gm[1], gx1 = gm[2], gy1 = gm[3], gx2 = gm[4], gy2 = gm[5];
+        const double den = (gy1 - gy2) * (gx0 - gx2) + (gx2 - gx1) * (gy0 - gy2);
+        if (fabs(den) < 1e-6) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) al[i] = 0.0f * a;
+            break;
+        }
+        const float A = (float)(gy1 - gy2), B = (float)(gx2 - gx1);
+        const float C = (float)(gy2 - gy0), E = (float)(gx0 - gx2);
+        const float D = (float)den;
+        const float dy = py - (float)gy2;
+        const float bdy = B * dy, edy = E * dy;
+        const float X2 = (float)gx2;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float dx = ((x0 + (float)i) + 0.5f) - X2;
+            const float ta = (A * dx + bdy) / D;
+            const float tb = (C * dx + edy) / D;
+            const float tc = (1.0f - ta) - tb;
+            al[i] = ((ta >= 0.0f && tb >= 0.0f && tc >= 0.0f) ? 1.0f : 0.0f) * a;
+        }
+        break;
+    }
+    case 6:
+    case 7: {
+        const float* gm = P.geom;
+        const float rx = gm[2], ry = gm[3];
+        if (!(rx > 0.0f) || !(ry > 0.0f)) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) al[i] = 0.0f * a;
+            break;
+        }
+        const float ca = gm[4] / rx, sa = gm[5] / rx, cb = gm[4] / ry, sb = gm[5] / ry;
+        const float dy = py - gm[1];
+        const float sady = sa * dy, cbdy = cb * dy;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float dx = ((x0 + (float)i) + 0.5f) - gm[0];
+            const float p = ca * dx + sady;
+            const float q = cbdy - sb * dx;
+            const bool in = P.kind == 6 ? (p * p + q * q) <= 1.0f : (fabsf(p) <= 1.0f && fabsf(q) <= 1.0f);
+            al[i] = (in ? 1.0f : 0.0f) * a;
+        }
+        break;
+    }
+    default:
+        return;  // render.py:225-226 skips unknown kinds
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float oma = 1.0f - al[i];
+        r[i] = s0[i] * al[i] + r[i] * oma;
+        g[i] = s1[i] * al[i] + g[i] * oma;
+        b[i] = s2[i] * al[i] + b[i] * oma;
+    }
+}
+
 // Out-of-line copy for the per-pixel loops of the generic paths: keeps the unrolled
 // pixel bodies small and the fast classes' register allocation unaffected.
 __device__ __noinline__ void composite_px_call(const WeftPrim& P, float px, float py, float& r,

@@ -41,6 +41,7 @@ __device__ __forceinline__ void split_tree_push(SplitTree& T, float v) {
 // a smaller composite) ...
 template <bool HARD>
 struct PxOverCanvas {
+    static constexpr bool kRow8 = false;
     const WeftPrim& P;
     const float* cv;
     const float* tg;
@@ -57,20 +58,32 @@ struct PxOverCanvas {
     }
 };
 
-// ... or a whole stack rendered over black (render_tile, then the clipped error).
+// ... or a whole stack rendered over black (render_tile, then the clipped error), eight
+// pixels of a row at a time so that each primitive's constants are formed once per group.
+template <int TS>
 struct PxStackWalk {
+    static constexpr bool kRow8 = true;
     const WeftPrim* prims;
     int n;
     const float* tg;
-    __device__ __forceinline__ void operator()(int p, int y, int x, float& q0, float& q1, float& q2) const {
-        float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
-        for (int i = 0; i < n; ++i) composite_px_call(prims[i], (float)x + 0.5f, (float)y + 0.5f, o0, o1, o2);
-        const float d0 = clip01(o0) - tg[3 * p];
-        const float d1 = clip01(o1) - tg[3 * p + 1];
-        const float d2 = clip01(o2) - tg[3 * p + 2];
-        q0 = d0 * d0;
-        q1 = d1 * d1;
-        q2 = d2 * d2;
+    // squared errors of pixels pg .. pg + 7 (one row: TS is a multiple of 8)
+    __device__ __forceinline__ void row8(int pg, float (&q)[24]) const {
+        const int y = pg / TS, x0 = pg - y * TS;
+        float r[8], g[8], b[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = g[i] = b[i] = 0.0f;
+        const float py = (float)y + 0.5f;
+        for (int k = 0; k < n; ++k) composite_row8(prims[k], (float)x0, py, r, g, b);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float* t = tg + 3 * (pg + i);
+            const float d0 = clip01(r[i]) - t[0];
+            const float d1 = clip01(g[i]) - t[1];
+            const float d2 = clip01(b[i]) - t[2];
+            q[3 * i] = d0 * d0;
+            q[3 * i + 1] = d1 * d1;
+            q[3 * i + 2] = d2 * d2;
+        }
     }
 };
 
@@ -85,11 +98,17 @@ __device__ __forceinline__ float split_leaf(const F& f, int e0) {
     const int p1 = (e0 + LS + 2) / 3;  // one past the last pixel touching the leaf
 #pragma unroll 1
     for (int pg = p0 & ~7; pg < p1; pg += 8) {
+        float qg[24];
+        if constexpr (F::kRow8) f.row8(pg, qg);  // pg + 7 < NPX: NPX and pg are multiples of 8
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int p = pg + i;
             float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
-            if (p >= p0 && p < p1 && p < NPX) {
+            if constexpr (F::kRow8) {
+                q0 = qg[3 * i];
+                q1 = qg[3 * i + 1];
+                q2 = qg[3 * i + 2];
+            } else if (p >= p0 && p < p1 && p < NPX) {
                 const int y = p / TS, x = p - y * TS;
                 f(p, y, x, q0, q1, q2);
             }
@@ -149,7 +168,7 @@ __device__ void score_evals_split_item(const WeftScoreEvalsSplitArgs& A) {
     const int ei = live ? A.order[j] : 0;
     const WeftEval ev = A.evals[ei];
     const bool ok = live && ev.tile_size == TS;
-    const PxStackWalk f{A.prims + ev.prim_offset, ok ? ev.prim_count : 0, A.tiles + ev.tile_offset};
+    const PxStackWalk<TS> f{A.prims + ev.prim_offset, ok ? ev.prim_count : 0, A.tiles + ev.tile_offset};
     const float sse = split_sse_f<TS>(f, L, sl);
     if (live && sl == 0) {
         const float v = ok ? sse : __builtin_nanf("");

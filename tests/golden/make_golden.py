@@ -20,8 +20,12 @@ pinned against.  Functions captured (reference file:line):
 - ``quadtree.decompose_quadtree``   src/weft/quadtree.py:42-91
 - ``entropy.encode_bytes``          src/weft/entropy.py:190-198 (static order-0 rANS or raw)
 - ``prim_chunks.build_prim_chunks`` src/weft/prim_chunks.py:23-78 (per-chunk coding + index)
+- ``encoder._generate_candidates``  src/weft/encoder.py:535-745 (template candidate pools),
+  with its helpers ``_kmeans_colors`` (encoder.py:414-441) and ``_split_by_lines_batched``
+  (encoder.py:343-404)
 
-``python tests/golden/make_golden.py entropy`` writes only ``ref_entropy.npz``.
+``python tests/golden/make_golden.py entropy`` writes only ``ref_entropy.npz``;
+``python tests/golden/make_golden.py candidates`` writes only ``ref_candidates.npz``.
 
 Every primitive parameter is a float32-representable value, so the float32 records the
 HIP path receives carry exactly what the reference's Python floats carry.
@@ -309,8 +313,96 @@ def entropy_cases():
     return d
 
 
+def _cand_tile(rng, ts, mode):
+    """A float32 tile of one of five kinds of content."""
+    yy, xx = np.mgrid[0:ts, 0:ts].astype(np.float64)
+    if mode == 0:  # smooth gradient with hard-edged shapes
+        c = rng.random((4, 3))
+        u, v = xx / max(ts - 1, 1), yy / max(ts - 1, 1)
+        img = ((1 - u) * (1 - v))[..., None] * c[0] + (u * (1 - v))[..., None] * c[1] + \
+              ((1 - u) * v)[..., None] * c[2] + (u * v)[..., None] * c[3]
+        for _ in range(int(rng.integers(1, 4))):
+            cx, cy, r = rng.uniform(0, ts), rng.uniform(0, ts), rng.uniform(ts / 8, ts / 2)
+            m = (xx - cx) ** 2 + (yy - cy) ** 2 <= r * r if rng.random() < 0.5 else \
+                (np.abs(xx - cx) <= r) & (np.abs(yy - cy) <= r / 2)
+            img[m] = rng.random(3)
+    elif mode == 1:  # flat
+        img = np.broadcast_to(rng.random(3), (ts, ts, 3)).copy()
+    elif mode == 2:  # noise
+        img = rng.random((ts, ts, 3))
+    elif mode == 3:  # two colours split by a line
+        a = rng.uniform(0, np.pi)
+        m = (xx - ts / 2) * np.cos(a) + (yy - ts / 2) * np.sin(a) >= 0
+        img = np.where(m[..., None], rng.random(3), rng.random(3))
+    else:  # 8-bit levels (equal luminances occur)
+        img = rng.integers(0, 4, (ts, ts, 3)) * (85.0 / 255.0)
+    return np.ascontiguousarray(img, dtype=np.float32)
+
+
+def candidate_cases():
+    from weft.encoder import _generate_candidates, _kmeans_colors, _split_by_lines_batched
+
+    rng = np.random.default_rng(535)
+    d = {k: [] for k in ("ts", "quality", "mode", "tile_off", "cand_off")}
+    tiles, kinds, geoms, ngeom, c0, c1, has1, alpha = [], [], [], [], [], [], [], []
+    km_in, km_k, km_out, sl_lines, sl_out = [], [], [], [], []
+    toff = coff = 0
+    for ts in (8, 16, 32, 64):
+        for mode in range(5):
+            for quality in (75, 85):
+                tile = _cand_tile(rng, ts, mode)
+                cands = _generate_candidates(tile, quality)
+                d["ts"].append(ts)
+                d["quality"].append(quality)
+                d["mode"].append(mode)
+                d["tile_off"].append(toff)
+                d["cand_off"].append(coff)
+                tiles.append(tile.ravel())
+                toff += tile.size
+                coff += len(cands)
+                for c in cands:
+                    kinds.append(c.kind)
+                    g = list(c.geom) + [0.0] * (7 - len(c.geom))
+                    geoms.append(g)
+                    ngeom.append(len(c.geom))
+                    c0.append(list(c.color0))
+                    c1.append(list(c.color1) if c.color1 is not None else [0.0, 0.0, 0.0])
+                    has1.append(c.color1 is not None)
+                    alpha.append(c.alpha)
+                if quality == 75:  # the helpers on the same pixels
+                    px = tile.reshape(-1, 3)
+                    for k in (2, 3, 4):
+                        km_in.append(len(tiles) - 1)
+                        km_k.append(k)
+                        km_out.append(np.asarray(_kmeans_colors(px, k), np.float32))
+                    lines = rng.uniform(-0.25 * ts, 1.25 * ts, (6, 4)).astype(np.float32)
+                    sl_lines.append(lines)
+                    sl_out.append(np.asarray(_split_by_lines_batched(tile, lines), np.float32))
+    d = {k: np.asarray(v, np.int64) for k, v in d.items()}
+    d["tile_off"] = np.append(d["tile_off"], toff)
+    d["cand_off"] = np.append(d["cand_off"], coff)
+    d["tiles"] = np.concatenate(tiles).astype(np.float32)
+    d["kind"] = np.asarray(kinds, np.int32)
+    d["geom"] = np.asarray(geoms, np.float64)
+    d["ngeom"] = np.asarray(ngeom, np.int32)
+    d["color0"] = np.asarray(c0, np.float64)
+    d["color1"] = np.asarray(c1, np.float64)
+    d["has_color1"] = np.asarray(has1, bool)
+    d["alpha"] = np.asarray(alpha, np.float64)
+    d["km_case"] = np.asarray(km_in, np.int64)
+    d["km_k"] = np.asarray(km_k, np.int64)
+    d["km_out"] = np.concatenate(km_out)
+    d["sl_lines"] = np.stack(sl_lines)
+    d["sl_out"] = np.stack(sl_out)
+    return d
+
+
 def main() -> None:
     sys.path.insert(0, REF_SRC)
+    if len(sys.argv) > 1 and sys.argv[1] == "candidates":
+        np.savez_compressed(os.path.join(OUT_DIR, "ref_candidates.npz"), **candidate_cases())
+        print("wrote ref_candidates.npz to", OUT_DIR)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "entropy":
         np.savez_compressed(os.path.join(OUT_DIR, "ref_entropy.npz"), **entropy_cases())
         print("wrote ref_entropy.npz to", OUT_DIR)
@@ -321,6 +413,7 @@ def main() -> None:
     np.savez_compressed(os.path.join(OUT_DIR, "ref_greedy_add.npz"), **greedy_cases(rng))
     np.savez_compressed(os.path.join(OUT_DIR, "ref_misc.npz"), **misc_cases(rng))
     np.savez_compressed(os.path.join(OUT_DIR, "ref_entropy.npz"), **entropy_cases())
+    np.savez_compressed(os.path.join(OUT_DIR, "ref_candidates.npz"), **candidate_cases())
     print("wrote fixtures to", OUT_DIR)
 
 

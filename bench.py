@@ -32,6 +32,9 @@ Further objects on the same line (skipped with --no-extras):
   tile of the 1920x1080 quadtree: device-resident and host-inclusive rates.
 - decode: gpu_render_tiles_to_image of the 3840x2160 plain fit's primitives.
 - entropy: chunked rANS encode and decode (64 tiles per chunk, the reference default) of that fit's PRIM bytes.
+- candidates: the template candidate pools (encoder._generate_candidates) of every tile
+  of the 1920x1080 quadtree on the device, and the NumPy restatement on one CPU core for a
+  sample of the same tiles.
 - select: the selection helpers (argmin, exclusive scan, compaction, per-segment top-k)
   on 2^22 device elements.
 - template_fit: the stage-3 template fit (fit_tiles_gpu_constant) of the 3840x2160 image at
@@ -374,6 +377,48 @@ def template_report(image, ts=16):
                 speedup_vs_cpu_oracle=cpu_s / dev_s, models_equal_oracle_frac=same_model)
 
 
+def candidates_report(tiles, quality=75, sample=48):
+    """Template candidate pools (encoder._generate_candidates) for every tile of a quadtree."""
+    import torch
+    from oracle import candgen_np
+    from weft_amd.candidates import MAX_CANDS, generate_pools
+    from weft_amd.records import PRIM_DTYPE
+
+    d_px = torch.from_numpy(tiles.target).cuda()
+    d_off = torch.from_numpy(np.asarray(tiles.pix_off, np.int64)).cuda()
+    generate_pools(tiles, quality, d_pixels=d_px, d_tile_pix=d_off)  # warm
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(3):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        recs, counts, _ = generate_pools(tiles, quality, d_pixels=d_px, d_tile_pix=d_off)
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) * 1e-3)
+    dev_s = float(np.median(times))
+    cnt = counts.cpu().numpy()
+    # CPU: the restatement (NumPy, one core) on a sample of the same tiles
+    rng = np.random.default_rng(0)
+    pick = rng.choice(tiles.n_tiles, min(sample, tiles.n_tiles), replace=False)
+    t0 = time.perf_counter()
+    same = 0
+    host = recs.cpu().numpy().view(PRIM_DTYPE).reshape(-1, MAX_CANDS)
+    for t in pick:
+        s = int(tiles.ts[t])
+        o = 3 * int(tiles.pix_off[t])
+        ref = candgen_np.generate(tiles.target[o:o + 3 * s * s].reshape(s, s, 3), quality)
+        same += int(cnt[t] == len(ref) and host[t, :cnt[t]].tobytes()
+                    == candgen_np.to_records(ref, PRIM_DTYPE).tobytes())
+    cpu_per_tile = (time.perf_counter() - t0) / len(pick)
+    return dict(api="generate_pools", tiles=int(tiles.n_tiles), quality=quality, kernel="weft_gen_candidates_kernel",
+                device_s=dev_s, tiles_per_s=tiles.n_tiles / dev_s, mean_pool=float(cnt.mean()),
+                candidates_per_s=float(cnt.sum()) / dev_s, cpu_numpy_s_per_tile=cpu_per_tile,
+                cpu_numpy_tiles=len(pick), speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / dev_s,
+                sample_equal_restatement=f"{same}/{len(pick)}")
+
+
 def select_report(n=1 << 22, seed=0):
     """The selection helpers (argmin, exclusive scan, compaction, segmented top-k) on n device elements."""
     import torch
@@ -531,7 +576,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the time-to-PSNR, batched and refine reports")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES of device memory once (HBM counter calibration) and exit")
-    ap.add_argument("--only", choices=["template_fit", "select"], default=None,
+    ap.add_argument("--only", choices=["template_fit", "select", "candidates"], default=None,
                     help="run only that report (for profiling) and print it as JSON")
     args = ap.parse_args()
     if args.only:
@@ -539,6 +584,8 @@ def main():
 
         if args.only == "template_fit":
             rep = template_report(synth_image(2160, 3840, 2500, seed=0, texture=True))
+        elif args.only == "candidates":
+            rep = candidates_report(build_workload(args)[2])
         else:
             rep = select_report()
         print(json.dumps({args.only: rep}), flush=True)
@@ -679,6 +726,10 @@ def main():
             f"{extras['template_fit']['hbm_gb_per_s']:.0f} GB/s on the device; CPU oracle "
             f"{extras['template_fit']['speedup_vs_cpu_oracle']:.0f}x slower")
         del t4, m4, rec4
+        extras["candidates"] = candidates_report(tiles)
+        log(f"[bench] candidate pools 1080p: {extras['candidates']['device_s'] * 1e3:.2f} ms for "
+            f"{extras['candidates']['tiles']} tiles ({extras['candidates']['speedup_vs_cpu_numpy_one_core']:.0f}x "
+            f"one CPU core); sample equal: {extras['candidates']['sample_equal_restatement']}")
         extras["select"] = select_report()
         log(f"[bench] select 4M: argmin {extras['select']['argmin']['gb_per_s']:.0f} GB/s, scan "
             f"{extras['select']['exclusive_scan']['gb_per_s']:.0f} GB/s, segmented top-k "

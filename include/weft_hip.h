@@ -29,6 +29,8 @@
  *   weft_template_masks /         kernel weft_tile_fit_candidates and its launcher
  *   weft_template_fit             fit_tiles_gpu_constant (src/weft/gpu_encoder.py:42-251,
  *                                 411-562), the stage-3 baseline fit
+ *   weft_gen_candidates           encoder._generate_candidates (src/weft/encoder.py:535-745)
+ *                                 for every tile of one size
  *   weft_argmin_f64, weft_exclusive_scan_i64, weft_compact_flags, weft_segmented_topk
  *                                 cccl_argmin, cccl_exclusive_scan, cccl_compact,
  *                                 cccl_segmented_topk (src/weft/cuda_backend.py:80-181)
@@ -415,6 +417,23 @@ int weft_compact_flags(const uint8_t* d_flags, int64_t n, int64_t* d_out_indices
 int weft_segmented_topk(const double* d_scores, const int64_t* d_segments, int64_t n, int64_t k, int largest,
                         int64_t* d_out_indices, int64_t* d_out_count, void* d_work, int64_t work_bytes,
                         void* stream);
+
+/* Template candidate pools (src/weft/encoder.py:535-745, _generate_candidates, with
+ * _kmeans_colors 414-441, _split_by_lines_batched 343-404 and _dedup_candidates 447-530)
+ * for the tiles d_tiles[0 .. n_tiles) of one size (8, 16, 32, 64 px) of a packed pixel
+ * buffer (tile t's RGB pixels start at float 3 * d_tile_pix[t], row-major).  The pool of
+ * list entry i -- the reference's candidates in its order, de-duplicated by its keys --
+ * goes to d_out[i * max_cands ...] as packed records (geometry zero-padded, colour1 zero
+ * except for linear patches) and its length to d_out_count[i]; when d_out_f64 is not NULL
+ * it receives, per record, the same values before rounding to float32 (7 geometry, 3 + 3
+ * colour, 1 alpha doubles).  max_cands >= WEFT_GEN_MAX_CANDS.  Where the reference depends
+ * on unspecified NumPy behaviour the pool uses: a stable luminance sort for the palette,
+ * row-major side sums for the linear patches, and the six brightest / darkest pixels in
+ * ascending (luminance, index) order. */
+#define WEFT_GEN_MAX_CANDS 352
+int weft_gen_candidates(const float* d_pixels, const int64_t* d_tile_pix, const int32_t* d_tiles, int n_tiles,
+                        int tile_size, int quality, WeftPrim* d_out, double* d_out_f64, int32_t* d_out_count,
+                        int max_cands, void* stream);
 
 #ifdef __cplusplus
 }

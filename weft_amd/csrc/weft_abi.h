@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 12
+#define WEFT_ABI_VERSION 13
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -222,4 +222,21 @@ struct WeftSelectArgs {
     int64_t j_step;
     int32_t largest;
     int32_t _pad;
+};
+
+struct WeftCandGenArgs {
+    const float* pixels;
+    const int64_t* tile_pix;
+    const int32_t* tiles;
+    WeftPrim* out;
+    double* out_f64;
+    int32_t* out_count;
+    int32_t n_tiles;
+    int32_t tile_size;
+    int32_t quality;
+    int32_t max_cands;
+    double cos_ang[16];
+    double sin_ang[16];
+    double cos6[6];
+    double sin6[6];
 };

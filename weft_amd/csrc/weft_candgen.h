@@ -1,0 +1,618 @@
+// Template candidate pools on the device: encoder._generate_candidates
+// (src/weft/encoder.py:535-745) for every tile of one size, one 256-thread workgroup per
+// tile.  Included by weft_kernels.hip inside its anonymous namespace.
+//
+// The pool is built in the reference's order and with its arithmetic: float32 image
+// statistics reduced in NumPy's orders (pairwise for flat means, row by row for means over
+// leading axes, float32 percentile interpolation), float64 geometry written as the
+// reference's Python expressions, and the reference's de-duplication keys (geometry on a
+// 1/4 grid, colours 1/64, alpha 1/32).  Three choices the reference leaves to NumPy
+// internals are fixed: a stable luminance sort for the k-means palette, row-major side sums
+// for the linear patches, and ascending (luminance, index) order for the six brightest and
+// darkest pixels that pair into triangles.
+
+constexpr int CG_MAXC = 352;        // pre-de-duplication pool bound (341 at quality >= 80)
+constexpr int CG_KEY = 12;          // longest de-duplication key
+constexpr int CG_MAX_LINES = 16 + 36;
+constexpr int CG_THREADS = 256;
+
+__device__ __forceinline__ double cg_ct(double v) {  // encoder._ct on one channel
+    const double m = v < 1.0 ? v : 1.0;
+    return m > 0.0 ? m : 0.0;
+}
+
+__device__ __forceinline__ uint32_t cg_fkey(float v) {  // float order as unsigned order
+    if (v == 0.0f) v = 0.0f;
+    const uint32_t u = __float_as_uint(v);
+    return (u >> 31) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float cg_funkey(uint32_t k) {
+    return __uint_as_float((k >> 31) ? (k & 0x7fffffffu) : ~k);
+}
+
+// NumPy's pairwise float32 sum of a[0 .. n) (n a power of two >= 8): 128-element leaves
+// with 8 accumulators, combined as a perfect binary tree; the reduction adds it to 0.
+__device__ float cg_pairwise(const float* a, int n) {
+    const int leaf = n < 128 ? n : 128;
+    const int nl = n / leaf;  // a power of two <= 32
+    SplitTree T;
+    T.n = 0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) T.lvl[q] = 0.0f;
+    for (int l = 0; l < nl; ++l) {
+        const float* b = a + l * leaf;
+        float r[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) r[j] = b[j];
+        for (int i = 8; i < leaf; i += 8)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r[j] += b[i + j];
+        split_tree_push(T, ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7])));
+    }
+    float v = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+        if ((1 << q) == nl) v = T.lvl[q];
+    return 0.0f + v;
+}
+
+// float32 percentile q (0.25f or 0.75f) of n sorted keys, NumPy's "linear" method.
+__device__ float cg_percentile(const uint32_t* keys, int n, float q) {
+    const float virt = (float)(n - 1) * q;
+    const float prev = floorf(virt);
+    if (virt >= (float)(n - 1)) return cg_funkey(keys[n - 1]);
+    const int i = (int)prev;
+    const float a = cg_funkey(keys[i]), b = cg_funkey(keys[i + 1]);
+    const float gamma = virt - prev;
+    const float diff = b - a;
+    float r = a + diff * gamma;
+    if (gamma >= 0.5f) r = b - diff * (1.0f - gamma);
+    return r;
+}
+
+struct CgShared {
+    int keys[CG_MAXC][CG_KEY];
+    int keep[CG_MAXC];
+    float line_col[CG_MAX_LINES][2][3];
+    double line_geom[CG_MAX_LINES][4];
+    float mean_all[3], quad[4][3], sub[16][3], km[9][3], ext[3][3];
+    int ext_n[3];
+    float avg_mag, gmx, gmy, p25, p75, gthr;
+    double pal[13][3];
+    int n_pal, grad_lines, n_tri_x, n_km_tri, n_total, n_kept;
+    double base_alpha, gline[4];
+    int pk[6], vl[6];
+    int km_cnt[3], km_imin[3], km_imax[3];
+    double km_mx[3], km_my[3];
+};
+
+// Sequential float32 sum of plane values over the pixels (x, y) in [x0, x1) x [y0, y1), row-major.
+__device__ __forceinline__ float cg_block_sum(const float* plane, int ts, int x0, int x1, int y0, int y1) {
+    float s = 0.0f;
+    for (int y = y0; y < y1; ++y)
+        for (int x = x0; x < x1; ++x) s += plane[y * ts + x];
+    return s;
+}
+
+__device__ __forceinline__ int cg_q(double v, double s) {  // int(v * s + 0.5), truncated
+    const double t = v * s + 0.5;
+    return t >= 2147483647.0 ? 2147483647 : (t <= -2147483647.0 ? -2147483647 : (int)t);
+}
+
+// Candidate j of the pool -> its record and de-duplication key.
+__device__ void cg_emit(CgShared& S, WeftPrim* out, double* f64, int j, int kind, const double* g, int ng,
+                        const double* c0, const double* c1, double alpha) {
+    WeftPrim P;
+    P.kind = kind;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) P.geom[i] = i < ng ? (float)g[i] : 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        P.color0[c] = (float)c0[c];
+        P.color1[c] = c1 ? (float)c1[c] : 0.0f;
+    }
+    P.alpha = (float)alpha;
+    out[j] = P;
+    if (f64) {  // the same values before rounding to float32: geometry, colours, alpha
+        double* d = f64 + (int64_t)j * 14;
+        for (int i = 0; i < 7; ++i) d[i] = i < ng ? g[i] : 0.0;
+        for (int c = 0; c < 3; ++c) {
+            d[7 + c] = c0[c];
+            d[10 + c] = c1 ? c1[c] : 0.0;
+        }
+        d[13] = alpha;
+    }
+    int* k = S.keys[j];
+    int n = 0;
+    k[n++] = kind;
+    for (int i = 0; i < ng; ++i) k[n++] = cg_q(g[i], 4.0);
+    for (int c = 0; c < 3; ++c) k[n++] = cg_q(c0[c], 64.0);
+    if (c1)
+        for (int c = 0; c < 3; ++c) k[n++] = cg_q(c1[c], 64.0);
+    k[n++] = cg_q(alpha, 32.0);
+    while (n < CG_KEY) k[n++] = -2147483647 - 1;
+}
+
+__device__ void cg_body(const WeftCandGenArgs& A) {
+    __shared__ CgShared S;
+    extern __shared__ float4 weft_smem[];
+    const int tid = threadIdx.x;
+    const int ts = A.tile_size, n = ts * ts;
+    const int slot = blockIdx.x;
+    const int tile = A.tiles[slot];
+    const double hf = (double)(ts - 1), wf = (double)(ts - 1);
+    float* R = reinterpret_cast<float*>(weft_smem);
+    float* G = R + n;
+    float* B = G + n;
+    float* LUM = B + n;
+    float* MAG = LUM + n;
+    float* GX = MAG + n;              // gx, gy: then the luminance sort keys (2n words)
+    float* GY = GX + n;
+    uint64_t* SORT = reinterpret_cast<uint64_t*>(GX);
+    uint32_t* MSORT = reinterpret_cast<uint32_t*>(GY + n);
+    WeftPrim* out = A.out + (int64_t)slot * A.max_cands;
+    double* f64 = A.out_f64 ? A.out_f64 + (int64_t)slot * A.max_cands * 14 : nullptr;
+
+    // pixels and luminance (float32, left to right as NumPy evaluates it)
+    const float* src = A.pixels + A.tile_pix[tile] * 3;
+    for (int p = tid; p < n; p += CG_THREADS) {
+        const float r = src[3 * p], g = src[3 * p + 1], b = src[3 * p + 2];
+        R[p] = r;
+        G[p] = g;
+        B[p] = b;
+        LUM[p] = (0.2126f * r + 0.7152f * g) + 0.0722f * b;
+    }
+    __syncthreads();
+    // np.gradient (central differences inside, one-sided at the borders) and the magnitude
+    for (int p = tid; p < n; p += CG_THREADS) {
+        const int y = p / ts, x = p - y * ts;
+        const float gx = x == 0 ? LUM[p + 1] - LUM[p] : (x == ts - 1 ? LUM[p] - LUM[p - 1] : (LUM[p + 1] - LUM[p - 1]) / 2.0f);
+        const float gy = y == 0 ? LUM[p + ts] - LUM[p] : (y == ts - 1 ? LUM[p] - LUM[p - ts] : (LUM[p + ts] - LUM[p - ts]) / 2.0f);
+        GX[p] = gx;
+        GY[p] = gy;
+        MAG[p] = sqrtf(gx * gx + gy * gy);
+    }
+    __syncthreads();
+    // flat means (pairwise) and block means (row by row)
+    if (tid == 0) S.avg_mag = cg_pairwise(MAG, n) / (float)n;
+    else if (tid == 1) S.gmx = cg_pairwise(GX, n) / (float)n;
+    else if (tid == 2) S.gmy = cg_pairwise(GY, n) / (float)n;
+    else if (tid < 6) S.mean_all[tid - 3] = cg_block_sum((R + (tid - 3) * n), ts, 0, ts, 0, ts) / (float)n;
+    else if (tid < 18) {
+        const int q = (tid - 6) / 3, c = (tid - 6) % 3, qh = ts / 2;
+        const int x0 = (q & 1) * qh, y0 = (q >> 1) * qh;
+        S.quad[q][c] = cg_block_sum((R + c * n), ts, x0, x0 + qh, y0, y0 + qh) / (float)(qh * qh);
+    } else if (tid < 66) {
+        const int q = (tid - 18) / 3, c = (tid - 18) % 3, sh = ts / 4;
+        const int x0 = (q & 3) * sh, y0 = (q >> 2) * sh;
+        S.sub[q][c] = cg_block_sum((R + c * n), ts, x0, x0 + sh, y0, y0 + sh) / (float)(sh * sh);
+    }
+    __syncthreads();
+    // sort keys: (luminance, index) for the palette and the extreme pixels; magnitude values
+    for (int p = tid; p < n; p += CG_THREADS) {
+        SORT[p] = ((uint64_t)cg_fkey(LUM[p]) << 32) | (uint32_t)p;
+        MSORT[p] = cg_fkey(MAG[p]);
+    }
+    __syncthreads();
+    for (int k = 2; k <= n; k <<= 1)
+        for (int j = k >> 1; j >= 1; j >>= 1) {
+            for (int t = tid; t < n / 2; t += CG_THREADS) {
+                const int lo = (t / j) * 2 * j + (t % j), hi = lo + j;
+                const bool up = (lo & k) == 0;
+                const uint64_t a = SORT[lo], b = SORT[hi];
+                if ((b < a) == up) {
+                    SORT[lo] = b;
+                    SORT[hi] = a;
+                }
+                const uint32_t c = MSORT[lo], d = MSORT[hi];
+                if ((d < c) == up) {
+                    MSORT[lo] = d;
+                    MSORT[hi] = c;
+                }
+            }
+            __syncthreads();
+        }
+    // extreme pixels and the linear-patch geometry
+    const int n_ang = A.quality >= 80 ? 16 : (A.quality >= 50 ? 12 : 8);
+    const int n_lines = n_ang + 36;
+    if (tid < 6) {
+        S.vl[tid] = (int)(SORT[tid] & 0xffffffffu);
+        S.pk[tid] = (int)(SORT[n - 6 + tid] & 0xffffffffu);
+    } else if (tid >= 32 && tid < 32 + n_lines) {
+        const int l = tid - 32;
+        double vx, vy, cx, cy, f;
+        if (l < n_ang) {
+            vx = A.cos_ang[l];
+            vy = A.sin_ang[l];
+            cx = wf * 0.5;
+            cy = hf * 0.5;
+            f = 0.6;
+        } else {
+            const int o = (l - n_ang) / 6, a = (l - n_ang) % 6;
+            const double ox[6] = {0.25, 0.75, 0.25, 0.75, 0.5, 0.5}, oy[6] = {0.25, 0.25, 0.75, 0.75, 0.25, 0.75};
+            vx = A.cos6[a];
+            vy = A.sin6[a];
+            cx = wf * ox[o];
+            cy = hf * oy[o];
+            f = 0.4;
+        }
+        S.line_geom[l][0] = cx - vx * wf * f;
+        S.line_geom[l][1] = cy - vy * hf * f;
+        S.line_geom[l][2] = cx + vx * wf * f;
+        S.line_geom[l][3] = cy + vy * hf * f;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // luminance percentiles from the high words of the sorted (luminance, index) keys
+        auto lum_at = [&](int i) { return cg_funkey((uint32_t)(SORT[i] >> 32)); };
+        const float virt25 = (float)(n - 1) * 0.25f, virt75 = (float)(n - 1) * 0.75f;
+        auto pct = [&](float virt) {
+            const float prev = floorf(virt);
+            const int i = (int)prev;
+            const float a = lum_at(i), b = lum_at(i + 1);
+            const float gamma = virt - prev, diff = b - a;
+            float r = a + diff * gamma;
+            if (gamma >= 0.5f) r = b - diff * (1.0f - gamma);
+            return r;
+        };
+        S.p25 = pct(virt25);
+        S.p75 = pct(virt75);
+        S.gthr = cg_percentile(MSORT, n, 0.75f);
+    }
+    __syncthreads();
+    // masked means, k-means groups, linear-patch side means
+    if (tid < 9) {
+        const int e = tid / 3, c = tid % 3;
+        float s = 0.0f;
+        int cnt = 0;
+        for (int p = 0; p < n; ++p) {
+            const bool m = e == 0 ? LUM[p] > S.p75 : (e == 1 ? LUM[p] < S.p25 : MAG[p] >= S.gthr);
+            if (m) {
+                s += (R + c * n)[p];
+                ++cnt;
+            }
+        }
+        S.ext[e][c] = cnt ? s / (float)cnt : 0.0f;
+        if (c == 0) S.ext_n[e] = cnt;
+    } else if (tid >= 16 && tid < 16 + 27) {
+        const int g = (tid - 16) / 3, c = (tid - 16) % 3;
+        const int k = g < 2 ? 2 : (g < 5 ? 3 : 4);
+        const int jj = g < 2 ? g : (g < 5 ? g - 2 : g - 5);
+        const double step = (double)n / (double)k;
+        const int a = (int)((double)jj * step), b = (int)((double)(jj + 1) * step);
+        float v;
+        if (b <= a) {
+            v = (R + c * n)[(int)(SORT[a < n - 1 ? a : n - 1] & 0xffffffffu)];
+        } else {
+            float s = 0.0f;
+            for (int i = a; i < b; ++i) s += (R + c * n)[(int)(SORT[i] & 0xffffffffu)];
+            v = s / (float)(b - a);
+        }
+        S.km[g][c] = v;
+    } else if (tid >= 64 && tid < 64 + 3 * n_lines) {
+        const int l = (tid - 64) / 3, c = (tid - 64) % 3;
+        const float x0 = (float)S.line_geom[l][0], y0 = (float)S.line_geom[l][1];
+        const float x1 = (float)S.line_geom[l][2], y1 = (float)S.line_geom[l][3];
+        const float dyl = y1 - y0, dxl = x1 - x0;
+        float sa = 0.0f, sb = 0.0f;
+        int na = 0;
+        for (int p = 0; p < n; ++p) {
+            const int y = p / ts, x = p - y * ts;
+            const float side = ((float)x - x0) * dyl - ((float)y - y0) * dxl;
+            if (side >= 0.0f) {
+                sa += (R + c * n)[p];
+                ++na;
+            } else {
+                sb += (R + c * n)[p];
+            }
+        }
+        const int nb = n - na;
+        S.line_col[l][0][c] = na ? sa / (float)na : S.mean_all[c];
+        S.line_col[l][1][c] = nb ? sb / (float)nb : S.mean_all[c];
+    }
+    __syncthreads();
+    // palette, alphas, gradient line, section sizes
+    if (tid == 0) {
+        auto near = [&](const double* c) {
+            for (int i = 0; i < S.n_pal; ++i) {
+                double d = c[0] - S.pal[i][0];
+                double s = d * d;
+                d = c[1] - S.pal[i][1];
+                s = s + d * d;
+                d = c[2] - S.pal[i][2];
+                s = s + d * d;
+                if (s < 0.002) return true;
+            }
+            return false;
+        };
+        S.n_pal = 0;
+        for (int c = 0; c < 3; ++c) S.pal[0][c] = cg_ct((double)S.mean_all[c]);
+        S.n_pal = 1;
+        for (int g = 0; g < 9; ++g) {
+            double c[3];
+            for (int i = 0; i < 3; ++i) c[i] = cg_ct((double)S.km[g][i]);
+            if (!near(c)) {
+                for (int i = 0; i < 3; ++i) S.pal[S.n_pal][i] = c[i];
+                ++S.n_pal;
+            }
+        }
+        for (int e = 0; e < 3; ++e) {  // bright, dark, edge (the tile mean when the mask is empty)
+            double c[3];
+            for (int i = 0; i < 3; ++i) c[i] = S.ext_n[e] ? cg_ct((double)S.ext[e][i]) : S.pal[0][i];
+            for (int i = 0; i < 3; ++i) S.ext[e][i] = (float)c[i];  // clamped copies for the lines
+            if (!near(c)) {
+                for (int i = 0; i < 3; ++i) S.pal[S.n_pal][i] = c[i];
+                ++S.n_pal;
+            }
+        }
+        const double ba = 0.25 + (double)S.avg_mag * 3.0;
+        S.base_alpha = ba < 1.0 ? ba : 1.0;
+        const double gmx = S.gmx, gmy = S.gmy;
+        const double gnorm = sqrt(gmx * gmx + gmy * gmy);
+        S.grad_lines = gnorm > 1e-4 ? 10 : 0;
+        if (S.grad_lines) {
+            const double lx = -gmy / (gnorm + 1e-8), ly = gmx / (gnorm + 1e-8);
+            const double cx = wf * 0.5, cy = hf * 0.5;
+            S.gline[0] = cx - lx * wf * 0.7;
+            S.gline[1] = cy - ly * hf * 0.7;
+            S.gline[2] = cx + lx * wf * 0.7;
+            S.gline[3] = cy + ly * hf * 0.7;
+        }
+        S.n_tri_x = 8;  // 4 brightest-pixel and 4 darkest-pixel triangles (6 extreme pixels each)
+    }
+    __syncthreads();
+    // k-means labels and their region triangles (quality >= 80); the labels overwrite MAG
+    if (A.quality >= 80) {
+        int* L = reinterpret_cast<int*>(MAG);
+        for (int p = tid; p < n; p += CG_THREADS) {
+            int best = 0;
+            float bd = 0.0f;
+            for (int k = 0; k < 3; ++k) {
+                float d = 0.0f;
+                for (int c = 0; c < 3; ++c) {
+                    const float e = (R + c * n)[p] - S.km[2 + k][c];
+                    d = d + e * e;
+                }
+                if (k == 0 || d < bd) {
+                    bd = d;
+                    best = k;
+                }
+            }
+            L[p] = best;
+        }
+        __syncthreads();
+        if (tid < 3) {
+            int cnt = 0, imin = -1, imax = -1, xmin = 0, xmax = 0;
+            int64_t sx = 0, sy = 0;
+            for (int p = 0; p < n; ++p)
+                if (L[p] == tid) {
+                    const int y = p / ts, x = p - y * ts;
+                    if (cnt == 0 || x < xmin) {
+                        xmin = x;
+                        imin = p;
+                    }
+                    if (cnt == 0 || x > xmax) {
+                        xmax = x;
+                        imax = p;
+                    }
+                    sx += x;
+                    sy += y;
+                    ++cnt;
+                }
+            S.km_cnt[tid] = cnt;
+            S.km_imin[tid] = imin;
+            S.km_imax[tid] = imax;
+            S.km_mx[tid] = cnt ? (double)sx / (double)cnt : 0.0;
+            S.km_my[tid] = cnt ? (double)sy / (double)cnt : 0.0;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        int m = 0;
+        if (A.quality >= 80)
+            for (int k = 0; k < 3; ++k) m += S.km_cnt[k] >= 3 ? 2 : 0;
+        S.n_km_tri = m;
+        const int pc = S.n_pal < 3 ? S.n_pal : 3;
+        S.n_total = 4 * S.n_pal + 8 + 16 + 3 * n_ang + 36 + 80 + S.grad_lines + 45 + 8 * pc + 8 + S.n_tri_x + m;
+    }
+    __syncthreads();
+
+    // the pool, in the reference's order
+    const int n_total = S.n_total;
+    for (int j = tid; j < n_total; j += CG_THREADS) {
+        double g[7] = {0, 0, 0, 0, 0, 0, 0};
+        double c0[3], c1[3];
+        int r = j;
+        const double ext0[3] = {S.ext[2][0], S.ext[2][1], S.ext[2][2]};  // edge colour
+        auto col = [&](double* d, const float* s) {
+            for (int c = 0; c < 3; ++c) d[c] = (double)s[c];
+        };
+        auto colct = [&](double* d, const float* s) {
+            for (int c = 0; c < 3; ++c) d[c] = cg_ct((double)s[c]);
+        };
+        if (r < 4 * S.n_pal) {  // constant patches of the palette
+            const double al[4] = {1.0, 0.7, 0.4, 0.2};
+            cg_emit(S, out, f64, j, 0, g, 0, S.pal[r / 4], nullptr, al[r % 4]);
+            continue;
+        }
+        r -= 4 * S.n_pal;
+        if (r < 8) {  // quadrant means
+            colct(c0, S.quad[r / 2]);
+            cg_emit(S, out, f64, j, 0, g, 0, c0, nullptr, (r & 1) ? 0.3 : 0.5);
+            continue;
+        }
+        r -= 8;
+        if (r < 16) {  // 4 x 4 block means
+            colct(c0, S.sub[r]);
+            cg_emit(S, out, f64, j, 0, g, 0, c0, nullptr, 0.25);
+            continue;
+        }
+        r -= 16;
+        if (r < 3 * n_ang + 36) {  // linear patches
+            const int l = r < 3 * n_ang ? r / 3 : n_ang + (r - 3 * n_ang);
+            const double al[3] = {1.0, 0.7, 0.4};
+            const double a = r < 3 * n_ang ? al[r % 3] : 0.5;
+            for (int i = 0; i < 4; ++i) g[i] = S.line_geom[l][i];
+            col(c0, S.line_col[l][0]);
+            col(c1, S.line_col[l][1]);
+            cg_emit(S, out, f64, j, 1, g, 4, c0, c1, a);
+            continue;
+        }
+        r -= 3 * n_ang + 36;
+        const double thick[5] = {0.4, 0.8, 1.2, 2.0, 3.0};
+        if (r < 80) {  // lines
+            const int t = r / 16, q = r % 16;
+            g[4] = thick[t];
+            if (q < 10) {
+                const double fr[5] = {0.0, 0.25, 0.5, 0.75, 1.0};
+                const double f = fr[q / 2];
+                if ((q & 1) == 0) {
+                    g[0] = wf * f; g[1] = 0.0; g[2] = wf * f; g[3] = hf;
+                } else {
+                    g[0] = 0.0; g[1] = hf * f; g[2] = wf; g[3] = hf * f;
+                }
+                cg_emit(S, out, f64, j, 2, g, 5, ext0, nullptr, S.base_alpha);
+            } else {
+                const int i = q - 10;
+                const int e = i / 2 == 0 ? 2 : (i / 2 == 1 ? 0 : 1);  // edge, bright, dark
+                col(c0, S.ext[e]);
+                if ((i & 1) == 0) {
+                    g[0] = 0.0; g[1] = 0.0; g[2] = wf; g[3] = hf;
+                } else {
+                    g[0] = 0.0; g[1] = hf; g[2] = wf; g[3] = 0.0;
+                }
+                cg_emit(S, out, f64, j, 2, g, 5, c0, nullptr, S.base_alpha);
+            }
+            continue;
+        }
+        r -= 80;
+        if (r < S.grad_lines) {  // lines across the mean gradient
+            for (int i = 0; i < 4; ++i) g[i] = S.gline[i];
+            g[4] = thick[r / 2];
+            col(c0, S.ext[(r & 1) ? 0 : 2]);
+            cg_emit(S, out, f64, j, 2, g, 5, c0, nullptr, S.base_alpha);
+            continue;
+        }
+        r -= S.grad_lines;
+        if (r < 45) {  // quadratic curves
+            const double cf[3] = {0.25, 0.5, 0.75};
+            if (r < 27) {
+                const double tt[3] = {0.5, 1.0, 2.0};
+                g[0] = 0.0; g[1] = hf * 0.5; g[2] = wf * cf[(r / 3) % 3]; g[3] = hf * cf[r / 9];
+                g[4] = wf; g[5] = hf * 0.5; g[6] = tt[r % 3];
+            } else {
+                const int q = r - 27;
+                g[0] = wf * 0.5; g[1] = 0.0; g[2] = wf * cf[q / 6]; g[3] = hf * cf[(q / 2) % 3];
+                g[4] = wf * 0.5; g[5] = hf; g[6] = (q & 1) ? 1.0 : 0.5;
+            }
+            cg_emit(S, out, f64, j, 3, g, 7, ext0, nullptr, S.base_alpha);
+            continue;
+        }
+        r -= 45;
+        const int pc = S.n_pal < 3 ? S.n_pal : 3;
+        if (r < 8 * pc) {  // corner-centre triangles
+            const double cx[4] = {0.0, wf, wf, 0.0}, cy[4] = {0.0, 0.0, hf, hf};
+            const int i = r / (2 * pc), q = r % (2 * pc);
+            g[0] = cx[i]; g[1] = cy[i]; g[2] = cx[(i + 1) & 3]; g[3] = cy[(i + 1) & 3];
+            g[4] = wf * 0.5; g[5] = hf * 0.5;
+            cg_emit(S, out, f64, j, 4, g, 6, S.pal[q / 2], nullptr, (q & 1) ? 0.3 : 0.6);
+            continue;
+        }
+        r -= 8 * pc;
+        if (r < 8) {  // half-tile triangles
+            const double hv[4][6] = {{0, 0, wf, 0, wf, hf}, {0, 0, wf, hf, 0, hf}, {0, 0, wf, 0, 0, hf},
+                                     {wf, 0, wf, hf, 0, hf}};
+            for (int i = 0; i < 6; ++i) g[i] = hv[r / 2][i];
+            cg_emit(S, out, f64, j, 4, g, 6, S.pal[0], nullptr, (r & 1) ? 0.3 : 0.5);
+            continue;
+        }
+        r -= 8;
+        if (r < S.n_tri_x) {  // triangles on the brightest, then the darkest, pixels
+            const int* ix = r < 4 ? S.pk : S.vl;
+            const int i = r & 3;
+            float cs[3];
+            for (int c = 0; c < 3; ++c) {
+                float s = 0.0f;
+                for (int v = 0; v < 3; ++v) s += (R + c * n)[ix[i + v]];
+                cs[c] = s / 3.0f;
+            }
+            for (int v = 0; v < 3; ++v) {
+                g[2 * v] = (double)(ix[i + v] % ts);
+                g[2 * v + 1] = (double)(ix[i + v] / ts);
+            }
+            colct(c0, cs);
+            cg_emit(S, out, f64, j, 4, g, 6, c0, nullptr, 0.5);
+            continue;
+        }
+        r -= S.n_tri_x;
+        {  // k-means region triangles (quality >= 80)
+            int lab = 0, q = r;
+            while (lab < 3 && (S.km_cnt[lab] < 3 || q >= 2)) {
+                if (S.km_cnt[lab] >= 3) q -= 2;
+                ++lab;
+            }
+            const int a = S.km_imin[lab], b = S.km_imax[lab];
+            g[0] = (double)(a % ts); g[1] = (double)(a / ts);
+            g[2] = (double)(b % ts); g[3] = (double)(b / ts);
+            g[4] = S.km_mx[lab]; g[5] = S.km_my[lab];
+            colct(c0, S.km[2 + lab]);
+            cg_emit(S, out, f64, j, 4, g, 6, c0, nullptr, q ? 0.3 : 0.5);
+        }
+    }
+    __syncthreads();
+    // de-duplication: keep the first candidate of every key
+    for (int j = tid; j < n_total; j += CG_THREADS) {
+        int keep = 1;
+        for (int i = 0; i < j && keep; ++i) {
+            bool same = true;
+            for (int t = 0; t < CG_KEY && same; ++t) same = S.keys[i][t] == S.keys[j][t];
+            if (same) keep = 0;
+        }
+        S.keep[j] = keep;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int j = 0; j < n_total; ++j) {
+            const int k = S.keep[j];
+            S.keep[j] = k ? run : -1;
+            run += k;
+        }
+        A.out_count[slot] = run;
+        S.n_kept = run;
+    }
+    __syncthreads();
+    // compact in place: every kept record is read before any is written
+    WeftPrim mine[2];
+    int dst[2] = {-1, -1};
+    for (int q = 0; q < 2; ++q) {
+        const int j = tid + q * CG_THREADS;
+        if (j < n_total && S.keep[j] >= 0) {
+            mine[q] = out[j];
+            dst[q] = S.keep[j];
+        }
+    }
+    __syncthreads();
+    for (int q = 0; q < 2; ++q)
+        if (dst[q] >= 0) out[dst[q]] = mine[q];
+    // slots past the pool hold records of no kind (skipped by every renderer)
+    const int n_kept = S.n_kept;
+    for (int j = n_kept + tid; j < A.max_cands; j += CG_THREADS) {
+        WeftPrim P;
+        P.kind = -1;
+        for (int i = 0; i < 7; ++i) P.geom[i] = 0.0f;
+        for (int c = 0; c < 3; ++c) P.color0[c] = P.color1[c] = 0.0f;
+        P.alpha = 0.0f;
+        out[j] = P;
+    }
+    if (f64) {
+        double v[2][14];
+        for (int q = 0; q < 2; ++q)
+            if (dst[q] >= 0)
+                for (int i = 0; i < 14; ++i) v[q][i] = f64[(int64_t)(tid + q * CG_THREADS) * 14 + i];
+        __syncthreads();
+        for (int q = 0; q < 2; ++q)
+            if (dst[q] >= 0)
+                for (int i = 0; i < 14; ++i) f64[(int64_t)dst[q] * 14 + i] = v[q][i];
+    }
+}

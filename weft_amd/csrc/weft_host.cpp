@@ -55,6 +55,7 @@ struct Module {
     hipFunction_t f_sel_topk_keep = nullptr;
     hipFunction_t f_sel_topk_gather = nullptr;
     hipFunction_t f_sel_topk_emit = nullptr;
+    hipFunction_t f_gen_cands = nullptr;
     hipFunction_t f_rans_enc = nullptr;
     hipFunction_t f_rans_scan = nullptr;
     hipFunction_t f_rans_pack = nullptr;
@@ -212,6 +213,7 @@ int weft_init(int device) {
         {&g_mod.f_sel_topk_keep, "weft_sel_topk_keep_kernel"},
         {&g_mod.f_sel_topk_gather, "weft_sel_topk_gather_kernel"},
         {&g_mod.f_sel_topk_emit, "weft_sel_topk_emit_kernel"},
+        {&g_mod.f_gen_cands, "weft_gen_candidates_kernel"},
         {&g_mod.f_rans_enc, "weft_rans_encode_kernel"},
         {&g_mod.f_rans_scan, "weft_rans_scan_kernel"},
         {&g_mod.f_rans_pack, "weft_rans_pack_kernel"},
@@ -785,6 +787,48 @@ int weft_segmented_topk(const double* d_scores, const int64_t* d_segments, int64
     a.recs = a.recs2;
     a.out_i64 = d_out_indices;
     return launch(g_mod.f_sel_topk_emit, (unsigned)((n + 255) / 256), 256, 0, stream, &a);
+}
+
+int weft_gen_candidates(const float* d_pixels, const int64_t* d_tile_pix, const int32_t* d_tiles, int n_tiles,
+                        int tile_size, int quality, WeftPrim* d_out, double* d_out_f64, int32_t* d_out_count,
+                        int max_cands, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (tile_size != 8 && tile_size != 16 && tile_size != 32 && tile_size != 64)
+        return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_gen_candidates: tile size must be 8, 16, 32 or 64");
+    if (n_tiles < 0) return set_err(WEFT_E_INVALID, "weft_gen_candidates: n_tiles < 0");
+    if (max_cands < WEFT_GEN_MAX_CANDS)
+        return set_err(WEFT_E_INVALID, "weft_gen_candidates: max_cands below WEFT_GEN_MAX_CANDS");
+    if (n_tiles == 0) return WEFT_OK;
+    if (!d_pixels || !d_tile_pix || !d_tiles || !d_out || !d_out_count)
+        return set_err(WEFT_E_INVALID, "weft_gen_candidates: null pointer");
+    WeftCandGenArgs a{};
+    a.pixels = d_pixels;
+    a.tile_pix = d_tile_pix;
+    a.tiles = d_tiles;
+    a.out = d_out;
+    a.out_f64 = d_out_f64;
+    a.out_count = d_out_count;
+    a.n_tiles = n_tiles;
+    a.tile_size = tile_size;
+    a.quality = quality;
+    a.max_cands = max_cands;
+    // np.linspace(0, pi, n, endpoint=False)[i] = i * (pi / n); the host's libm cos / sin, as
+    // the reference's math.cos / math.sin
+    const int n_ang = quality >= 80 ? 16 : (quality >= 50 ? 12 : 8);
+    const double pi = 3.141592653589793;
+    for (int i = 0; i < n_ang; ++i) {
+        const double ang = (double)i * (pi / (double)n_ang);
+        a.cos_ang[i] = std::cos(ang);
+        a.sin_ang[i] = std::sin(ang);
+    }
+    for (int i = 0; i < 6; ++i) {
+        const double ang = (double)i * (pi / 6.0);
+        a.cos6[i] = std::cos(ang);
+        a.sin6[i] = std::sin(ang);
+    }
+    const unsigned lds = (unsigned)(32 * tile_size * tile_size);
+    return launch(g_mod.f_gen_cands, (unsigned)n_tiles, 256, lds, stream, &a);
 }
 
 }  // extern "C"

@@ -1280,6 +1280,7 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
 #include "weft_batch_refine.h"
 #include "weft_template.h"
 #include "weft_select.h"
+#include "weft_candgen.h"
 
 }  // namespace
 
@@ -1580,3 +1581,6 @@ extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_gather_kernel(We
     sel_topk_gather_body(A);
 }
 extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_emit_kernel(WeftSelectArgs A) { sel_topk_emit_body(A); }
+
+// Template candidate pools, one workgroup per tile of one size.
+extern "C" __global__ __launch_bounds__(256) void weft_gen_candidates_kernel(WeftCandGenArgs A) { cg_body(A); }

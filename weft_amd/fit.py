@@ -75,7 +75,7 @@ def build_work_lists(tile_ts: np.ndarray, n_cand: int, blocks: dict[int, int]) -
 class PlainFitter:
     def __init__(self, tiles: TileSet, n_cand: int, depth: int, seed: int = 0,
                  lam: float | None = None, obj_mode: int = 0, max_steps: int = 64,
-                 blocks: dict[int, int] | None = None, cand_prims: np.ndarray | None = None,
+                 blocks: dict[int, int] | None = None, cand_prims=None,
                  keep_sse: bool = False, device: int = 0, merged: bool = True):
         import torch
 
@@ -109,7 +109,13 @@ class PlainFitter:
         # Candidate source.
         self.cand_prims = None
         self.has_general = 0
-        if cand_prims is not None:
+        if cand_prims is not None and isinstance(cand_prims, torch.Tensor):
+            # device-resident records (for example weft_amd.candidates.generate_pools)
+            if cand_prims.dtype != torch.uint8 or cand_prims.numel() < T * self.n_cand * PRIM_DTYPE.itemsize:
+                raise ValueError("cand_prims must hold n_tiles * n_cand packed records (tile-major)")
+            self.has_general = 1
+            self.cand_prims = cand_prims
+        elif cand_prims is not None:
             cp = np.ascontiguousarray(cand_prims, dtype=PRIM_DTYPE)
             if cp.shape != (T * self.n_cand,):
                 raise ValueError("cand_prims must hold n_tiles * n_cand records (tile-major)")

@@ -34,7 +34,8 @@ Further objects on the same line (skipped with --no-extras):
 - entropy: chunked rANS encode and decode (64 tiles per chunk, the reference default) of that fit's PRIM bytes.
 - candidates: the template candidate pools (encoder._generate_candidates) of every tile
   of the 1920x1080 quadtree on the device, and the NumPy restatement on one CPU core for a
-  sample of the same tiles.
+  sample of the same tiles; under "residual" the same for the residual pools
+  (encoder._generate_residual_candidates) against a seeded noisy prediction.
 - select: the selection helpers (argmin, exclusive scan, compaction, per-segment top-k)
   on 2^22 device elements.
 - template_fit: the stage-3 template fit (fit_tiles_gpu_constant) of the 3840x2160 image at
@@ -412,10 +413,57 @@ def candidates_report(tiles, quality=75, sample=48):
         same += int(cnt[t] == len(ref) and host[t, :cnt[t]].tobytes()
                     == candgen_np.to_records(ref, PRIM_DTYPE).tobytes())
     cpu_per_tile = (time.perf_counter() - t0) / len(pick)
-    return dict(api="generate_pools", tiles=int(tiles.n_tiles), quality=quality, kernel="weft_gen_candidates_kernel",
-                device_s=dev_s, tiles_per_s=tiles.n_tiles / dev_s, mean_pool=float(cnt.mean()),
-                candidates_per_s=float(cnt.sum()) / dev_s, cpu_numpy_s_per_tile=cpu_per_tile,
-                cpu_numpy_tiles=len(pick), speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / dev_s,
+    rep = dict(api="generate_pools", tiles=int(tiles.n_tiles), quality=quality, kernel="weft_gen_candidates_kernel",
+               device_s=dev_s, tiles_per_s=tiles.n_tiles / dev_s, mean_pool=float(cnt.mean()),
+               candidates_per_s=float(cnt.sum()) / dev_s, cpu_numpy_s_per_tile=cpu_per_tile,
+               cpu_numpy_tiles=len(pick), speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / dev_s,
+               sample_equal_restatement=f"{same}/{len(pick)}")
+    rep["residual"] = residual_candidates_report(tiles, d_px, d_off, pick, quality)
+    return rep
+
+
+def residual_candidates_report(tiles, d_px, d_off, pick, quality=75):
+    """Residual pools (encoder._generate_residual_candidates) of every tile against a prediction.
+
+    The prediction is the target with a seeded smooth error added (an unclipped canvas,
+    clipped by the kernel), the shape of a fit's canvas after a few stages.
+    """
+    import torch
+    from oracle import candgen_np
+    from weft_amd.candidates import RES_MAX_CANDS, generate_residual_pools
+    from weft_amd.records import PRIM_DTYPE
+
+    g = torch.Generator(device="cuda").manual_seed(5)
+    d_pred = d_px + 0.15 * (torch.rand(d_px.shape, generator=g, device="cuda") - 0.5)
+    generate_residual_pools(tiles, d_pred, d_pixels=d_px, d_tile_pix=d_off)  # warm
+    stream = torch.cuda.current_stream()
+    times = []
+    for _ in range(3):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        recs, counts, _ = generate_residual_pools(tiles, d_pred, d_pixels=d_px, d_tile_pix=d_off)
+        b.record(stream)
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) * 1e-3)
+    dev_s = float(np.median(times))
+    cnt = counts.cpu().numpy()
+    pred = np.clip(d_pred.cpu().numpy(), 0, 1)
+    host = recs.cpu().numpy().view(PRIM_DTYPE).reshape(-1, RES_MAX_CANDS)
+    t0 = time.perf_counter()
+    same = 0
+    for t in pick:
+        s = int(tiles.ts[t])
+        o = 3 * int(tiles.pix_off[t])
+        ref = candgen_np.generate_residual(tiles.target[o:o + 3 * s * s].reshape(s, s, 3),
+                                           pred[o:o + 3 * s * s].reshape(s, s, 3), quality)
+        same += int(cnt[t] == len(ref) and host[t, :cnt[t]].tobytes()
+                    == candgen_np.to_records(ref, PRIM_DTYPE).tobytes())
+    cpu_per_tile = (time.perf_counter() - t0) / len(pick)
+    return dict(api="generate_residual_pools", kernel="weft_gen_residual_candidates_kernel", device_s=dev_s,
+                tiles_per_s=tiles.n_tiles / dev_s, mean_pool=float(cnt.mean()),
+                cpu_numpy_s_per_tile=cpu_per_tile, cpu_numpy_tiles=len(pick),
+                speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / dev_s,
                 sample_equal_restatement=f"{same}/{len(pick)}")
 
 

@@ -31,6 +31,8 @@
  *                                 411-562), the stage-3 baseline fit
  *   weft_gen_candidates           encoder._generate_candidates (src/weft/encoder.py:535-745)
  *                                 for every tile of one size
+ *   weft_gen_residual_candidates  encoder._generate_residual_candidates
+ *                                 (src/weft/encoder.py:1255-1366) for every tile of one size
  *   weft_argmin_f64, weft_exclusive_scan_i64, weft_compact_flags, weft_segmented_topk
  *                                 cccl_argmin, cccl_exclusive_scan, cccl_compact,
  *                                 cccl_segmented_topk (src/weft/cuda_backend.py:80-181)
@@ -434,6 +436,17 @@ int weft_segmented_topk(const double* d_scores, const int64_t* d_segments, int64
 int weft_gen_candidates(const float* d_pixels, const int64_t* d_tile_pix, const int32_t* d_tiles, int n_tiles,
                         int tile_size, int quality, WeftPrim* d_out, double* d_out_f64, int32_t* d_out_count,
                         int max_cands, void* stream);
+
+/* Residual candidate pools (src/weft/encoder.py:1255-1366, _generate_residual_candidates,
+ * with _split_by_line 325-340): candidates aimed at the error between each tile and its
+ * prediction d_pred (same layout as d_pixels; clip_pred != 0 clips it to [0, 1] first, so
+ * an unclipped canvas can be passed).  Output as for weft_gen_candidates, max_cands >=
+ * WEFT_GEN_RES_MAX_CANDS.  The error pixels are ranked by x with a stable order (the
+ * reference's np.argsort does not promise one for equal x), which picks the curve points. */
+#define WEFT_GEN_RES_MAX_CANDS 128
+int weft_gen_residual_candidates(const float* d_pixels, const float* d_pred, int clip_pred, const int64_t* d_tile_pix,
+                                 const int32_t* d_tiles, int n_tiles, int tile_size, WeftPrim* d_out,
+                                 double* d_out_f64, int32_t* d_out_count, int max_cands, void* stream);
 
 #ifdef __cplusplus
 }

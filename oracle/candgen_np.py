@@ -220,3 +220,100 @@ def to_records(cands: list[tuple], dtype) -> np.ndarray:
         rec[i]["color1"] = c1 if c1 is not None else (0.0, 0.0, 0.0)
         rec[i]["alpha"] = a
     return rec
+
+
+def split_by_line(tile: np.ndarray, x0: float, y0: float, x1: float, y1: float):
+    """_split_by_line (encoder.py:325-340): side of the line by float32 arithmetic, where the
+    line's Python-float coordinates and differences enter as float32 scalars."""
+    h, w, _ = tile.shape
+    ys, xs = np.meshgrid(np.arange(h, dtype=np.float32), np.arange(w, dtype=np.float32), indexing="ij")
+    f = np.float32
+    side = (xs - f(x0)) * f(y1 - y0) - (ys - f(y0)) * f(x1 - x0)
+    a = side >= 0
+    ca = tile[a].mean(axis=0) if a.any() else tile.mean(axis=(0, 1))
+    cb = tile[~a].mean(axis=0) if (~a).any() else tile.mean(axis=(0, 1))
+    return tuple(float(v) for v in ca), tuple(float(v) for v in cb)
+
+
+def generate_residual(tile: np.ndarray, pred: np.ndarray, quality: int = 75) -> list[tuple]:
+    """_generate_residual_candidates (encoder.py:1255-1366): candidates aimed at tile - pred.
+
+    The error pixels are sorted by x with a stable sort (the reference's ``np.argsort``
+    does not promise an order for equal x), which picks the curves' end and middle points.
+    """
+    tile = np.asarray(tile, np.float32)
+    pred = np.asarray(pred, np.float32)
+    h, w, _ = tile.shape
+    hf, wf = float(h - 1), float(w - 1)
+    err = tile - pred
+    el = np.abs(err).mean(axis=2)
+    hi = el >= float(np.percentile(el, 60))
+    if not hi.any():
+        return []
+    clip = lambda a: ct(np.clip(a, 0, 1))  # noqa: E731
+    hi_col = clip(tile[hi].mean(axis=0))
+    corr = clip(pred[hi].mean(axis=0) + err[hi].mean(axis=0))
+    tgt = clip(tile[hi].mean(axis=0))
+    out: list[tuple] = []
+    for c in (hi_col, corr, tgt):
+        for a in (0.6, 0.4, 0.25, 0.15, 0.08):
+            out.append((0, (), c, None, a))
+    qh, qw = h // 2, w // 2
+    for r in range(2):
+        for cc in range(2):
+            qm = hi[r * qh:(r + 1) * qh, cc * qw:(cc + 1) * qw]
+            if qm.any():
+                c = clip(tile[r * qh:(r + 1) * qh, cc * qw:(cc + 1) * qw][qm].mean(axis=0))
+                for a in (0.35, 0.2, 0.1):
+                    out.append((0, (), c, None, a))
+    ely, elx = np.gradient(el)
+    emx, emy = float(np.mean(elx)), float(np.mean(ely))
+    en = math.sqrt(emx * emx + emy * emy) + 1e-8
+    vx, vy = emx / en, emy / en
+    for off in (0.5, 0.3, 0.7):
+        cx, cy = wf * off, hf * off
+        g = (cx - vx * wf * 0.6, cy - vy * hf * 0.6, cx + vx * wf * 0.6, cy + vy * hf * 0.6)
+        c0, c1 = split_by_line(tile, *g)
+        for a in (0.5, 0.3, 0.15):
+            out.append((1, g, c0, c1, a))
+    pvx, pvy = -vy, vx
+    cx, cy = wf * 0.5, hf * 0.5
+    g = (cx - pvx * wf * 0.6, cy - pvy * hf * 0.6, cx + pvx * wf * 0.6, cy + pvy * hf * 0.6)
+    c0, c1 = split_by_line(tile, *g)
+    for a in (0.4, 0.2):
+        out.append((1, g, c0, c1, a))
+    ey, ex = np.where(hi)
+    ecx, ecy = float(np.mean(ex)), float(np.mean(ey))
+    for t in (0.3, 0.6, 1.0, 1.5, 2.5):
+        for c in (hi_col, tgt):
+            for a in (0.5, 0.3, 0.15):
+                out.append((2, (ecx, 0.0, ecx, hf, t), c, None, a))
+                out.append((2, (0.0, ecy, wf, ecy, t), c, None, a))
+    for t in (0.5, 1.0):
+        out.append((2, (0.0, 0.0, wf, hf, t), hi_col, None, 0.25))
+        out.append((2, (0.0, hf, wf, 0.0, t), hi_col, None, 0.25))
+    if len(ex) >= 6:
+        order = np.argsort(ex, kind="stable")
+        p0 = (float(ex[order[0]]), float(ey[order[0]]))
+        pm = (float(ex[order[len(order) // 2]]), float(ey[order[len(order) // 2]]))
+        p1 = (float(ex[order[-1]]), float(ey[order[-1]]))
+        ctrl = (2 * pm[0] - (p0[0] + p1[0]) / 2, 2 * pm[1] - (p0[1] + p1[1]) / 2)
+        ctrl = (max(0.0, min(wf, ctrl[0])), max(0.0, min(hf, ctrl[1])))
+        for t in (0.5, 1.0, 2.0):
+            out.append((3, (p0[0], p0[1], ctrl[0], ctrl[1], p1[0], p1[1], t), hi_col, None, 0.3))
+    if len(ex) >= 3:
+        i0, i1 = int(np.argmin(ex)), int(np.argmax(ex))
+        j0, j1 = int(np.argmin(ey)), int(np.argmax(ey))
+        g1 = (float(ex[i0]), float(ey[i0]), float(ex[i1]), float(ey[i1]), ecx, ecy)
+        g2 = (float(ex[i0]), float(ey[j0]), float(ex[i1]), float(ey[j1]), ecx, ecy)
+        for g in (g1, g2):
+            for a in (0.4, 0.2, 0.1):
+                out.append((4, g, hi_col, None, a))
+                out.append((4, g, tgt, None, a))
+    seen, kept = set(), []
+    for c in out:
+        key = _dedup_key(c[0], c[1], c[2], c[3], c[4])
+        if key not in seen:
+            seen.add(key)
+            kept.append(c)
+    return kept

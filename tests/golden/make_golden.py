@@ -23,6 +23,8 @@ pinned against.  Functions captured (reference file:line):
 - ``encoder._generate_candidates``  src/weft/encoder.py:535-745 (template candidate pools),
   with its helpers ``_kmeans_colors`` (encoder.py:414-441) and ``_split_by_lines_batched``
   (encoder.py:343-404)
+- ``encoder._generate_residual_candidates`` src/weft/encoder.py:1255-1366 (pools aimed at a
+  prediction's error)
 
 ``python tests/golden/make_golden.py entropy`` writes only ``ref_entropy.npz``;
 ``python tests/golden/make_golden.py candidates`` writes only ``ref_candidates.npz``.
@@ -394,6 +396,58 @@ def candidate_cases():
     d["km_out"] = np.concatenate(km_out)
     d["sl_lines"] = np.stack(sl_lines)
     d["sl_out"] = np.stack(sl_out)
+    d.update(residual_cases())
+    return d
+
+
+def residual_cases():
+    """_generate_residual_candidates on tiles and predictions of every kind of content."""
+    from weft.encoder import _generate_residual_candidates
+    from weft.render import render_tile
+
+    rng = np.random.default_rng(1255)
+    d = {k: [] for k in ("res_ts", "res_quality", "res_tile_off", "res_cand_off")}
+    tiles, preds, kinds, geoms, ngeom, c0, c1, has1, alpha = [], [], [], [], [], [], [], [], []
+    toff = coff = 0
+    for ts in (8, 16, 32, 64):
+        for mode in range(5):
+            tile = _cand_tile(rng, ts, mode)
+            for pm in range(2):
+                if pm == 0:  # the prediction of a few random primitives
+                    stack = [_rand_prim(rng, ts, int(rng.choice([0, 1, 4]))) for _ in range(int(rng.integers(1, 4)))]
+                    pred = np.asarray(render_tile(stack, ts), np.float32)
+                else:  # a blurred, shifted copy of the tile
+                    pred = np.clip(0.5 * tile + 0.5 * np.roll(tile, 1, axis=0) + 0.05, 0, 1).astype(np.float32)
+                quality = 75 if pm == 0 else 85
+                cands = _generate_residual_candidates(tile, pred, quality)
+                d["res_ts"].append(ts)
+                d["res_quality"].append(quality)
+                d["res_tile_off"].append(toff)
+                d["res_cand_off"].append(coff)
+                tiles.append(tile.ravel())
+                preds.append(pred.ravel())
+                toff += tile.size
+                coff += len(cands)
+                for c in cands:
+                    kinds.append(c.kind)
+                    geoms.append(list(c.geom) + [0.0] * (7 - len(c.geom)))
+                    ngeom.append(len(c.geom))
+                    c0.append(list(c.color0))
+                    c1.append(list(c.color1) if c.color1 is not None else [0.0, 0.0, 0.0])
+                    has1.append(c.color1 is not None)
+                    alpha.append(c.alpha)
+    d = {k: np.asarray(v, np.int64) for k, v in d.items()}
+    d["res_tile_off"] = np.append(d["res_tile_off"], toff)
+    d["res_cand_off"] = np.append(d["res_cand_off"], coff)
+    d["res_tiles"] = np.concatenate(tiles).astype(np.float32)
+    d["res_preds"] = np.concatenate(preds).astype(np.float32)
+    d["res_kind"] = np.asarray(kinds, np.int32)
+    d["res_geom"] = np.asarray(geoms, np.float64)
+    d["res_ngeom"] = np.asarray(ngeom, np.int32)
+    d["res_color0"] = np.asarray(c0, np.float64)
+    d["res_color1"] = np.asarray(c1, np.float64)
+    d["res_has_color1"] = np.asarray(has1, bool)
+    d["res_alpha"] = np.asarray(alpha, np.float64)
     return d
 
 

@@ -85,3 +85,61 @@ def test_closed_forms():
     assert len(got) == 7 + 72 + 60 + 45 + 24
     assert all(g[2] == col and g[3] in (None, col) for g in got)
     assert [g[0] for g in got] == [0] * 7 + [1] * 72 + [2] * 60 + [3] * 45 + [4] * 24
+
+
+def _ref_res_list(d, c):
+    out = []
+    for j in range(d["res_cand_off"][c], d["res_cand_off"][c + 1]):
+        ng = int(d["res_ngeom"][j])
+        out.append((int(d["res_kind"][j]), tuple(d["res_geom"][j][:ng].tolist()), tuple(d["res_color0"][j].tolist()),
+                    tuple(d["res_color1"][j].tolist()) if d["res_has_color1"][j] else None,
+                    float(d["res_alpha"][j])))
+    return out
+
+
+def test_residual_candidates_match_reference():
+    """_generate_residual_candidates (encoder.py:1255-1366) on 40 reference-written cases.
+
+    Equal lists except the three curves, whose end and middle points come from the
+    reference's np.argsort of the error pixels' x (no promised order for equal x): there
+    both lists must pick a leftmost pixel, a rightmost one and a middle one of the same x.
+    """
+    d = dict(load_golden("ref_candidates.npz"))
+    n_diff = 0
+    for c in range(len(d["res_ts"])):
+        ts = int(d["res_ts"][c])
+        o0, o1 = d["res_tile_off"][c], d["res_tile_off"][c + 1]
+        tile = d["res_tiles"][o0:o1].reshape(ts, ts, 3)
+        pred = d["res_preds"][o0:o1].reshape(ts, ts, 3)
+        got, ref = C.generate_residual(tile, pred, int(d["res_quality"][c])), _ref_res_list(d, c)
+        assert len(got) == len(ref), c
+        err = np.abs(tile - pred).mean(axis=2)
+        ey, ex = np.where(err >= float(np.percentile(err, 60)))
+        xs = np.sort(ex)
+        for g, r in zip(got, ref):
+            if g == r:
+                continue
+            n_diff += 1
+            for x in (g, r):
+                assert x[0] == 3, (c, g, r)
+                p0, p1 = (x[1][0], x[1][1]), (x[1][4], x[1][5])
+                assert p0[0] == xs[0] and p1[0] == xs[-1], (c, x)
+                assert (p0[0], p0[1]) in set(zip(ex.tolist(), ey.tolist()))
+            assert g[1][6] == r[1][6] and g[2:] == r[2:]
+    assert n_diff < 200
+
+
+def test_residual_closed_forms():
+    """A prediction wrong in the top half: the error pixels are that half, the lines cross at
+    its centroid (7.5, 3.5), and every colour is the tile's (black)."""
+    tile = np.zeros((16, 16, 3), np.float32)
+    pred = tile.copy()
+    pred[:8] = 0.5
+    got = C.generate_residual(tile, pred, 75)
+    vert = [g for g in got if g[0] == 2 and g[1][0] == g[1][2] and g[1][1] == 0.0]
+    horz = [g for g in got if g[0] == 2 and g[1][1] == g[1][3] and g[1][0] == 0.0]
+    assert vert and all(g[1][0] == 7.5 for g in vert)
+    assert horz and all(g[1][1] == 3.5 for g in horz)
+    assert all(g[2] == (0.0, 0.0, 0.0) for g in got)
+    tri = [g for g in got if g[0] == 4]
+    assert {g[1] for g in tri} == {(0.0, 0.0, 15.0, 0.0, 7.5, 3.5), (0.0, 0.0, 15.0, 7.0, 7.5, 3.5)}

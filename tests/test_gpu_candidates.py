@@ -119,3 +119,36 @@ def test_fit_on_generated_pools(gpu, oracle_lib):
         np.testing.assert_allclose(sse[t][gauss], osse[gauss], rtol=1e-5, atol=1e-9)
         if rec["win_acc"][0, t]:
             assert 0 <= rec["win_idx"][0, t] < cnt[t]
+
+
+def test_residual_pools_match_restatement(gpu):
+    """Residual pools of the fixture tiles and predictions, and a fit canvas as the prediction."""
+    import torch
+    from weft_amd.candidates import RES_MAX_CANDS, generate_residual_candidates, generate_residual_pools
+    from weft_amd.records import PRIM_DTYPE
+
+    d = dict(load_golden("ref_candidates.npz"))
+    to = d["res_tile_off"]
+    tiles = [d["res_tiles"][to[c]:to[c + 1]].reshape(int(d["res_ts"][c]), int(d["res_ts"][c]), 3)
+             for c in range(len(d["res_ts"]))]
+    preds = [d["res_preds"][to[c]:to[c + 1]].reshape(t.shape) for c, t in enumerate(tiles)]
+    _say(f"residual pools: {len(tiles)} fixture tiles")
+    ts = _Tiles(tiles)
+    d_pred = torch.from_numpy(np.concatenate([p.ravel() for p in preds])).cuda()
+    # an unclipped prediction: the kernel clips it when asked
+    d_raw = d_pred * 1.5 - 0.25
+    for pr, clip in ((d_pred, False), (d_raw, True)):
+        recs, counts, f64 = generate_residual_pools(ts, pr, clip_pred=clip, want_f64=True)
+        recs = recs.cpu().numpy().view(PRIM_DTYPE).reshape(-1, RES_MAX_CANDS)
+        counts = counts.cpu().numpy()
+        host_pred = pr.cpu().numpy()
+        if clip:
+            host_pred = np.clip(host_pred, 0, 1)
+        for i, tile in enumerate(tiles):
+            o = int(ts.pix_off[i]) * 3
+            ref = C.generate_residual(tile, host_pred[o:o + tile.size].reshape(tile.shape), 75)
+            assert counts[i] == len(ref), i
+            assert recs[i, :counts[i]].tobytes() == C.to_records(ref, PRIM_DTYPE).tobytes(), (i, clip)
+            assert (recs[i, counts[i]:]["kind"] == -1).all()
+    got = generate_residual_candidates(tiles[5], preds[5], 75)
+    assert [(p.kind, p.geom, p.color0, p.color1, p.alpha) for p in got] == C.generate_residual(tiles[5], preds[5])

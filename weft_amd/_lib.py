@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 13
+ABI_VERSION = 14
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -109,6 +109,8 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_template_masks.argtypes = [c_int, c_void_p, c_void_p]
     L.weft_gen_candidates.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                       c_int, c_void_p]
+    L.weft_gen_residual_candidates.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p,
+                                               c_void_p, c_void_p, c_int, c_void_p]
     L.weft_select_work_bytes.argtypes = [c_int, ctypes.c_int64]
     L.weft_select_work_bytes.restype = ctypes.c_int64
     L.weft_argmin_f64.argtypes = [c_void_p, ctypes.c_int64, c_void_p, c_void_p, ctypes.c_int64, c_void_p]
@@ -153,7 +155,7 @@ def _bind(L: ctypes.CDLL) -> None:
                  "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks",
                  "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks", "weft_template_fit",
                  "weft_argmin_f64", "weft_exclusive_scan_i64", "weft_compact_flags", "weft_segmented_topk",
-                 "weft_gen_candidates"):
+                 "weft_gen_candidates", "weft_gen_residual_candidates"):
         getattr(L, name).restype = c_int
 
 
@@ -236,5 +238,5 @@ EXPORTED_SYMBOLS = (
     "weft_refine", "weft_fit_score_split", "weft_rans_stage_bytes", "weft_rans_encode_chunks",
     "weft_rans_decode_chunks", "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks",
     "weft_template_fit", "weft_select_work_bytes", "weft_argmin_f64", "weft_exclusive_scan_i64", "weft_compact_flags",
-    "weft_segmented_topk", "weft_gen_candidates",
+    "weft_segmented_topk", "weft_gen_candidates", "weft_gen_residual_candidates",
 )

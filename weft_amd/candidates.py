@@ -1,10 +1,13 @@
-"""Template candidate pools on the GPU: the reference's ``encoder._generate_candidates``.
+"""Candidate pools on the GPU: the reference's ``encoder._generate_candidates`` and
+``encoder._generate_residual_candidates``.
 
-``generate_candidates(tile, quality)`` keeps the reference function's name, arguments and
-return type (src/weft/encoder.py:535-745): the de-duplicated list of ``Primitive`` objects
-in the reference's order, computed by ``weft_gen_candidates``.  ``generate_pools``
-builds the pools of every tile of a ``TileSet`` on the device, packed and padded to
-``MAX_CANDS`` records per tile, ready for ``PlainFitter(..., cand_prims=pools)``.
+``generate_candidates(tile, quality)`` and ``generate_residual_candidates(tile, pred,
+quality)`` keep the reference functions' names, arguments and return types
+(src/weft/encoder.py:535-745, 1255-1366): the de-duplicated list of ``Primitive`` objects in
+the reference's order, computed by ``weft_gen_candidates`` / ``weft_gen_residual_candidates``.
+``generate_pools`` and ``generate_residual_pools`` build the pools of every tile of a
+``TileSet`` on the device, packed and padded with records of no kind, ready for
+``PlainFitter(..., cand_prims=pools)``.
 
 Pools equal the reference's except where the reference depends on unspecified NumPy
 behaviour (see weft_hip.h): equal luminances of different colours in the palette sort,
@@ -20,6 +23,7 @@ from .primitives import Primitive
 from .records import PRIM_DTYPE
 
 MAX_CANDS = 352
+RES_MAX_CANDS = 128
 GEN_TILE_SIZES = (8, 16, 32, 64)
 
 
@@ -68,34 +72,103 @@ def generate_pools(tiles, quality: int = 75, want_f64: bool = False, d_pixels=No
     return out, counts, f64
 
 
-def generate_candidates(tile: np.ndarray, quality: int = 75) -> list[Primitive]:
-    """``encoder._generate_candidates`` for one square tile of 8, 16, 32 or 64 px."""
+def generate_residual_pools(tiles, d_pred, clip_pred: bool = True, want_f64: bool = False, d_pixels=None,
+                            d_tile_pix=None):
+    """Residual pools of every tile against a device prediction laid out like the pixels.
+
+    ``clip_pred`` clips the prediction to [0, 1] first (pass a fit's unclipped canvas).
+    Returns (records uint8 [T * RES_MAX_CANDS * 60], counts int32 [T], f64 or None).
+    """
     import torch
 
-    tile = np.ascontiguousarray(tile, dtype=np.float32)
-    if tile.ndim != 3 or tile.shape[2] != 3 or tile.shape[0] != tile.shape[1]:
-        raise ValueError("tile must be a square (ts, ts, 3) array")
-    ts = int(tile.shape[0])
-    if ts not in GEN_TILE_SIZES:
-        raise ValueError(f"tile size must be one of {GEN_TILE_SIZES}")
+    _lib.ensure_init(0)
+    dev = torch.device("cuda", 0)
+    ts = np.asarray(tiles.ts)
+    bad = set(np.unique(ts).tolist()) - set(GEN_TILE_SIZES)
+    if bad:
+        raise ValueError(f"candidate pools need tiles of {GEN_TILE_SIZES} px, not {sorted(bad)}")
+    T = len(ts)
+    if d_pixels is None:
+        d_pixels = torch.from_numpy(np.ascontiguousarray(tiles.target, dtype=np.float32)).to(dev)
+    if d_tile_pix is None:
+        d_tile_pix = torch.from_numpy(np.asarray(tiles.pix_off, np.int64)).to(dev)
+    if d_pred.numel() < d_pixels.numel() or d_pred.dtype != torch.float32:
+        raise ValueError("the prediction must be float32 and laid out like the pixels")
+    rec = PRIM_DTYPE.itemsize
+    out = torch.empty(max(T, 1) * RES_MAX_CANDS * rec, dtype=torch.uint8, device=dev)
+    f64 = torch.empty(max(T, 1) * RES_MAX_CANDS * 14, dtype=torch.float64, device=dev) if want_f64 else None
+    counts = torch.zeros(max(T, 1), dtype=torch.int32, device=dev)
+    L = _lib.library()
+    for s in GEN_TILE_SIZES:
+        idx = np.nonzero(ts == s)[0].astype(np.int32)
+        if len(idx) == 0:
+            continue
+        d_idx = torch.from_numpy(idx).to(dev)
+        cls_out = torch.empty(len(idx) * RES_MAX_CANDS * rec, dtype=torch.uint8, device=dev)
+        cls_f64 = torch.empty(len(idx) * RES_MAX_CANDS * 14, dtype=torch.float64, device=dev) if want_f64 else None
+        cls_cnt = torch.empty(len(idx), dtype=torch.int32, device=dev)
+        _lib.check(L.weft_gen_residual_candidates(d_pixels.data_ptr(), d_pred.data_ptr(), 1 if clip_pred else 0,
+                                                  d_tile_pix.data_ptr(), d_idx.data_ptr(), len(idx), s,
+                                                  cls_out.data_ptr(),
+                                                  None if cls_f64 is None else cls_f64.data_ptr(),
+                                                  cls_cnt.data_ptr(), RES_MAX_CANDS, _lib.stream_ptr()),
+                   "weft_gen_residual_candidates")
+        di = d_idx.long()
+        out.view(-1, RES_MAX_CANDS * rec)[di] = cls_out.view(-1, RES_MAX_CANDS * rec)
+        counts[di] = cls_cnt
+        if want_f64:
+            f64.view(-1, RES_MAX_CANDS * 14)[di] = cls_f64.view(-1, RES_MAX_CANDS * 14)
+    return out, counts, f64
 
-    class _One:
-        pass
 
-    one = _One()
-    one.ts = np.array([ts], np.int32)
-    one.target = tile.reshape(-1)
-    one.pix_off = np.zeros(1, np.int64)
-    recs, counts, f64 = generate_pools(one, quality, want_f64=True)
-    torch.cuda.synchronize()
+def _to_primitives(recs, counts, f64, width):
     n = int(counts[0].item())
     kinds = recs.view(-1).cpu().numpy().view(PRIM_DTYPE)[:n]["kind"]
     vals = f64.cpu().numpy().reshape(-1, 14)[:n]
     ng = {0: 0, 1: 4, 2: 5, 3: 7, 4: 6}
     out = []
     for k, v in zip(kinds.tolist(), vals):
-        g = tuple(float(x) for x in v[:ng[k]])
-        c0 = tuple(float(x) for x in v[7:10])
-        c1 = tuple(float(x) for x in v[10:13]) if k == 1 else None
-        out.append(Primitive(kind=k, geom=g, color0=c0, color1=c1, alpha=float(v[13])))
+        out.append(Primitive(kind=k, geom=tuple(float(x) for x in v[:ng[k]]),
+                             color0=tuple(float(x) for x in v[7:10]),
+                             color1=tuple(float(x) for x in v[10:13]) if k == 1 else None, alpha=float(v[13])))
     return out
+
+
+class _OneTile:
+    def __init__(self, tile: np.ndarray):
+        self.ts = np.array([tile.shape[0]], np.int32)
+        self.target = tile.reshape(-1)
+        self.pix_off = np.zeros(1, np.int64)
+
+
+def _check_tile(tile: np.ndarray) -> np.ndarray:
+    tile = np.ascontiguousarray(tile, dtype=np.float32)
+    if tile.ndim != 3 or tile.shape[2] != 3 or tile.shape[0] != tile.shape[1]:
+        raise ValueError("tile must be a square (ts, ts, 3) array")
+    if int(tile.shape[0]) not in GEN_TILE_SIZES:
+        raise ValueError(f"tile size must be one of {GEN_TILE_SIZES}")
+    return tile
+
+
+def generate_residual_candidates(tile: np.ndarray, pred: np.ndarray, quality: int = 75) -> list[Primitive]:
+    """``encoder._generate_residual_candidates`` for one square tile and its prediction."""
+    import torch
+
+    tile = _check_tile(tile)
+    pred = np.ascontiguousarray(pred, dtype=np.float32)
+    if pred.shape != tile.shape:
+        raise ValueError("pred must have the tile's shape")
+    d_pred = torch.from_numpy(pred.reshape(-1)).to(torch.device("cuda", 0))
+    recs, counts, f64 = generate_residual_pools(_OneTile(tile), d_pred, clip_pred=False, want_f64=True)
+    torch.cuda.synchronize()
+    return _to_primitives(recs, counts, f64, RES_MAX_CANDS)
+
+
+def generate_candidates(tile: np.ndarray, quality: int = 75) -> list[Primitive]:
+    """``encoder._generate_candidates`` for one square tile of 8, 16, 32 or 64 px."""
+    import torch
+
+    tile = _check_tile(tile)
+    recs, counts, f64 = generate_pools(_OneTile(tile), quality, want_f64=True)
+    torch.cuda.synchronize()
+    return _to_primitives(recs, counts, f64, MAX_CANDS)

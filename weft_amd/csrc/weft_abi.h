@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 13
+#define WEFT_ABI_VERSION 14
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -239,4 +239,18 @@ struct WeftCandGenArgs {
     double sin_ang[16];
     double cos6[6];
     double sin6[6];
+};
+
+struct WeftResGenArgs {
+    const float* pixels;
+    const float* pred;
+    const int64_t* tile_pix;
+    const int32_t* tiles;
+    WeftPrim* out;
+    double* out_f64;
+    int32_t* out_count;
+    int32_t n_tiles;
+    int32_t tile_size;
+    int32_t clip_pred;
+    int32_t max_cands;
 };

@@ -100,7 +100,7 @@ __device__ __forceinline__ int cg_q(double v, double s) {  // int(v * s + 0.5), 
 }
 
 // Candidate j of the pool -> its record and de-duplication key.
-__device__ void cg_emit(CgShared& S, WeftPrim* out, double* f64, int j, int kind, const double* g, int ng,
+__device__ void cg_emit(int (*keys)[CG_KEY], WeftPrim* out, double* f64, int j, int kind, const double* g, int ng,
                         const double* c0, const double* c1, double alpha) {
     WeftPrim P;
     P.kind = kind;
@@ -122,7 +122,7 @@ __device__ void cg_emit(CgShared& S, WeftPrim* out, double* f64, int j, int kind
         }
         d[13] = alpha;
     }
-    int* k = S.keys[j];
+    int* k = keys[j];
     int n = 0;
     k[n++] = kind;
     for (int i = 0; i < ng; ++i) k[n++] = cg_q(g[i], 4.0);
@@ -131,6 +131,66 @@ __device__ void cg_emit(CgShared& S, WeftPrim* out, double* f64, int j, int kind
         for (int c = 0; c < 3; ++c) k[n++] = cg_q(c1[c], 64.0);
     k[n++] = cg_q(alpha, 32.0);
     while (n < CG_KEY) k[n++] = -2147483647 - 1;
+}
+
+// De-duplication (keep the first candidate of every key, _dedup_candidates), compaction in
+// place (every kept record is read before any is written), and records of no kind in the
+// slots past the pool.
+__device__ void cg_finish(int (*keys)[CG_KEY], int* keep, int* n_kept_lds, WeftPrim* out, double* f64, int n_total,
+                          int max_cands, int32_t* out_count) {
+    const int tid = threadIdx.x;
+    for (int j = tid; j < n_total; j += CG_THREADS) {
+        int k = 1;
+        for (int i = 0; i < j && k; ++i) {
+            bool same = true;
+            for (int t = 0; t < CG_KEY && same; ++t) same = keys[i][t] == keys[j][t];
+            if (same) k = 0;
+        }
+        keep[j] = k;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int j = 0; j < n_total; ++j) {
+            const int k = keep[j];
+            keep[j] = k ? run : -1;
+            run += k;
+        }
+        *out_count = run;
+        *n_kept_lds = run;
+    }
+    __syncthreads();
+    WeftPrim mine[2];
+    int dst[2] = {-1, -1};
+    for (int q = 0; q < 2; ++q) {
+        const int j = tid + q * CG_THREADS;
+        if (j < n_total && keep[j] >= 0) {
+            mine[q] = out[j];
+            dst[q] = keep[j];
+        }
+    }
+    __syncthreads();
+    for (int q = 0; q < 2; ++q)
+        if (dst[q] >= 0) out[dst[q]] = mine[q];
+    const int n_kept = *n_kept_lds;
+    for (int j = n_kept + tid; j < max_cands; j += CG_THREADS) {
+        WeftPrim P;
+        P.kind = -1;
+        for (int i = 0; i < 7; ++i) P.geom[i] = 0.0f;
+        for (int c = 0; c < 3; ++c) P.color0[c] = P.color1[c] = 0.0f;
+        P.alpha = 0.0f;
+        out[j] = P;
+    }
+    if (f64) {
+        double v[2][14];
+        for (int q = 0; q < 2; ++q)
+            if (dst[q] >= 0)
+                for (int i = 0; i < 14; ++i) v[q][i] = f64[(int64_t)(tid + q * CG_THREADS) * 14 + i];
+        __syncthreads();
+        for (int q = 0; q < 2; ++q)
+            if (dst[q] >= 0)
+                for (int i = 0; i < 14; ++i) f64[(int64_t)dst[q] * 14 + i] = v[q][i];
+    }
 }
 
 __device__ void cg_body(const WeftCandGenArgs& A) {
@@ -432,19 +492,19 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
         };
         if (r < 4 * S.n_pal) {  // constant patches of the palette
             const double al[4] = {1.0, 0.7, 0.4, 0.2};
-            cg_emit(S, out, f64, j, 0, g, 0, S.pal[r / 4], nullptr, al[r % 4]);
+            cg_emit(S.keys, out, f64, j, 0, g, 0, S.pal[r / 4], nullptr, al[r % 4]);
             continue;
         }
         r -= 4 * S.n_pal;
         if (r < 8) {  // quadrant means
             colct(c0, S.quad[r / 2]);
-            cg_emit(S, out, f64, j, 0, g, 0, c0, nullptr, (r & 1) ? 0.3 : 0.5);
+            cg_emit(S.keys, out, f64, j, 0, g, 0, c0, nullptr, (r & 1) ? 0.3 : 0.5);
             continue;
         }
         r -= 8;
         if (r < 16) {  // 4 x 4 block means
             colct(c0, S.sub[r]);
-            cg_emit(S, out, f64, j, 0, g, 0, c0, nullptr, 0.25);
+            cg_emit(S.keys, out, f64, j, 0, g, 0, c0, nullptr, 0.25);
             continue;
         }
         r -= 16;
@@ -455,7 +515,7 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
             for (int i = 0; i < 4; ++i) g[i] = S.line_geom[l][i];
             col(c0, S.line_col[l][0]);
             col(c1, S.line_col[l][1]);
-            cg_emit(S, out, f64, j, 1, g, 4, c0, c1, a);
+            cg_emit(S.keys, out, f64, j, 1, g, 4, c0, c1, a);
             continue;
         }
         r -= 3 * n_ang + 36;
@@ -471,7 +531,7 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
                 } else {
                     g[0] = 0.0; g[1] = hf * f; g[2] = wf; g[3] = hf * f;
                 }
-                cg_emit(S, out, f64, j, 2, g, 5, ext0, nullptr, S.base_alpha);
+                cg_emit(S.keys, out, f64, j, 2, g, 5, ext0, nullptr, S.base_alpha);
             } else {
                 const int i = q - 10;
                 const int e = i / 2 == 0 ? 2 : (i / 2 == 1 ? 0 : 1);  // edge, bright, dark
@@ -481,7 +541,7 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
                 } else {
                     g[0] = 0.0; g[1] = hf; g[2] = wf; g[3] = 0.0;
                 }
-                cg_emit(S, out, f64, j, 2, g, 5, c0, nullptr, S.base_alpha);
+                cg_emit(S.keys, out, f64, j, 2, g, 5, c0, nullptr, S.base_alpha);
             }
             continue;
         }
@@ -490,7 +550,7 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
             for (int i = 0; i < 4; ++i) g[i] = S.gline[i];
             g[4] = thick[r / 2];
             col(c0, S.ext[(r & 1) ? 0 : 2]);
-            cg_emit(S, out, f64, j, 2, g, 5, c0, nullptr, S.base_alpha);
+            cg_emit(S.keys, out, f64, j, 2, g, 5, c0, nullptr, S.base_alpha);
             continue;
         }
         r -= S.grad_lines;
@@ -505,7 +565,7 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
                 g[0] = wf * 0.5; g[1] = 0.0; g[2] = wf * cf[q / 6]; g[3] = hf * cf[(q / 2) % 3];
                 g[4] = wf * 0.5; g[5] = hf; g[6] = (q & 1) ? 1.0 : 0.5;
             }
-            cg_emit(S, out, f64, j, 3, g, 7, ext0, nullptr, S.base_alpha);
+            cg_emit(S.keys, out, f64, j, 3, g, 7, ext0, nullptr, S.base_alpha);
             continue;
         }
         r -= 45;
@@ -515,7 +575,7 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
             const int i = r / (2 * pc), q = r % (2 * pc);
             g[0] = cx[i]; g[1] = cy[i]; g[2] = cx[(i + 1) & 3]; g[3] = cy[(i + 1) & 3];
             g[4] = wf * 0.5; g[5] = hf * 0.5;
-            cg_emit(S, out, f64, j, 4, g, 6, S.pal[q / 2], nullptr, (q & 1) ? 0.3 : 0.6);
+            cg_emit(S.keys, out, f64, j, 4, g, 6, S.pal[q / 2], nullptr, (q & 1) ? 0.3 : 0.6);
             continue;
         }
         r -= 8 * pc;
@@ -523,7 +583,7 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
             const double hv[4][6] = {{0, 0, wf, 0, wf, hf}, {0, 0, wf, hf, 0, hf}, {0, 0, wf, 0, 0, hf},
                                      {wf, 0, wf, hf, 0, hf}};
             for (int i = 0; i < 6; ++i) g[i] = hv[r / 2][i];
-            cg_emit(S, out, f64, j, 4, g, 6, S.pal[0], nullptr, (r & 1) ? 0.3 : 0.5);
+            cg_emit(S.keys, out, f64, j, 4, g, 6, S.pal[0], nullptr, (r & 1) ? 0.3 : 0.5);
             continue;
         }
         r -= 8;
@@ -541,7 +601,7 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
                 g[2 * v + 1] = (double)(ix[i + v] / ts);
             }
             colct(c0, cs);
-            cg_emit(S, out, f64, j, 4, g, 6, c0, nullptr, 0.5);
+            cg_emit(S.keys, out, f64, j, 4, g, 6, c0, nullptr, 0.5);
             continue;
         }
         r -= S.n_tri_x;
@@ -556,63 +616,329 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
             g[2] = (double)(b % ts); g[3] = (double)(b / ts);
             g[4] = S.km_mx[lab]; g[5] = S.km_my[lab];
             colct(c0, S.km[2 + lab]);
-            cg_emit(S, out, f64, j, 4, g, 6, c0, nullptr, q ? 0.3 : 0.5);
+            cg_emit(S.keys, out, f64, j, 4, g, 6, c0, nullptr, q ? 0.3 : 0.5);
         }
     }
     __syncthreads();
-    // de-duplication: keep the first candidate of every key
-    for (int j = tid; j < n_total; j += CG_THREADS) {
-        int keep = 1;
-        for (int i = 0; i < j && keep; ++i) {
-            bool same = true;
-            for (int t = 0; t < CG_KEY && same; ++t) same = S.keys[i][t] == S.keys[j][t];
-            if (same) keep = 0;
+    cg_finish(S.keys, S.keep, &S.n_kept, out, f64, n_total, A.max_cands, A.out_count + slot);
+}
+
+// ---------------------------------------------------------------------------------------
+// Residual candidate pools: encoder._generate_residual_candidates (src/weft/encoder.py:
+// 1255-1366), candidates aimed at the error between a tile and its current prediction.
+// The error pixels are ranked by x with a stable order (the reference's np.argsort does not
+// promise one for equal x); it picks the curves' end and middle points.
+constexpr int CR_MAXC = 128;  // 117 before de-duplication at most
+
+struct CrShared {
+    int keys[CR_MAXC][CG_KEY];
+    int keep[CR_MAXC];
+    float mt[3], mp[3], me[3], mall[3], quad[4][3];
+    int quad_n[4];
+    float line_col[4][2][3];
+    double line_geom[4][4];
+    float thresh, emx, emy;
+    int n_hi, n_total, n_kept;
+    int64_t sx, sy;
+    int xmin, y_at_xmin, xmax, y_at_xmax, ymin, ymax;  // extent of the error pixels
+    int o_first, o_mid, o_last;                       // pixel indices in the stable x-order
+    double hi_col[3], corr[3];
+};
+
+__device__ __forceinline__ float cr_clip(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+__device__ void cr_body(const WeftResGenArgs& A) {
+    __shared__ CrShared S;
+    extern __shared__ float4 weft_smem[];
+    const int tid = threadIdx.x;
+    const int ts = A.tile_size, n = ts * ts;
+    const int slot = blockIdx.x;
+    const int tile = A.tiles[slot];
+    const double hf = (double)(ts - 1), wf = (double)(ts - 1);
+    float* T = reinterpret_cast<float*>(weft_smem);  // 3 planes; the prediction stays in HBM
+    float* EL = T + 3 * n;
+    float* GX = EL + n;
+    float* GY = GX + n;
+    uint32_t* SK = reinterpret_cast<uint32_t*>(GY + n);
+    int* HI = reinterpret_cast<int*>(GX);  // after the gradient means
+    WeftPrim* out = A.out + (int64_t)slot * A.max_cands;
+    double* f64 = A.out_f64 ? A.out_f64 + (int64_t)slot * A.max_cands * 14 : nullptr;
+
+    const int64_t base = A.tile_pix[tile] * 3;
+    auto pred_at = [&](int p, int c) {
+        const float q = A.pred[base + 3 * p + c];
+        return A.clip_pred ? clip01(q) : q;
+    };
+    for (int p = tid; p < n; p += CG_THREADS) {
+        float e[3];
+        for (int c = 0; c < 3; ++c) {
+            const float t = A.pixels[base + 3 * p + c];
+            T[c * n + p] = t;
+            e[c] = t - pred_at(p, c);
         }
-        S.keep[j] = keep;
+        EL[p] = ((0.0f + fabsf(e[0]) + fabsf(e[1])) + fabsf(e[2])) / 3.0f;
+    }
+    __syncthreads();
+    for (int p = tid; p < n; p += CG_THREADS) {
+        const int y = p / ts, x = p - y * ts;
+        GX[p] = x == 0 ? EL[p + 1] - EL[p] : (x == ts - 1 ? EL[p] - EL[p - 1] : (EL[p + 1] - EL[p - 1]) / 2.0f);
+        GY[p] = y == 0 ? EL[p + ts] - EL[p] : (y == ts - 1 ? EL[p] - EL[p - ts] : (EL[p + ts] - EL[p - ts]) / 2.0f);
+        SK[p] = cg_fkey(EL[p]);
+    }
+    __syncthreads();
+    for (int k = 2; k <= n; k <<= 1)
+        for (int j = k >> 1; j >= 1; j >>= 1) {
+            for (int t = tid; t < n / 2; t += CG_THREADS) {
+                const int lo = (t / j) * 2 * j + (t % j), hi = lo + j;
+                const bool up = (lo & k) == 0;
+                const uint32_t a = SK[lo], b = SK[hi];
+                if ((b < a) == up) {
+                    SK[lo] = b;
+                    SK[hi] = a;
+                }
+            }
+            __syncthreads();
+        }
+    if (tid == 0) {
+        S.thresh = cg_percentile(SK, n, 60.0f / 100.0f);
+        S.emx = cg_pairwise(GX, n) / (float)n;
+    } else if (tid == 1) {
+        S.emy = cg_pairwise(GY, n) / (float)n;
+    }
+    __syncthreads();
+    for (int p = tid; p < n; p += CG_THREADS) HI[p] = EL[p] >= S.thresh ? 1 : 0;
+    __syncthreads();
+    // means over the error pixels (row-major), quadrant means, the tile mean, positions
+    if (tid < 9) {
+        const int w = tid / 3, c = tid % 3;
+        float s = 0.0f;
+        int cnt = 0;
+        for (int p = 0; p < n; ++p)
+            if (HI[p]) {
+                const float v = w == 0 ? T[c * n + p] : (w == 1 ? pred_at(p, c) : T[c * n + p] - pred_at(p, c));
+                s += v;
+                ++cnt;
+            }
+        const float m = cnt ? s / (float)cnt : 0.0f;
+        if (w == 0) S.mt[c] = m;
+        else if (w == 1) S.mp[c] = m;
+        else S.me[c] = m;
+        if (tid == 0) S.n_hi = cnt;
+    } else if (tid < 21) {
+        const int q = (tid - 9) / 3, c = (tid - 9) % 3, qh = ts / 2;
+        const int x0 = (q & 1) * qh, y0 = (q >> 1) * qh;
+        float s = 0.0f;
+        int cnt = 0;
+        for (int y = y0; y < y0 + qh; ++y)
+            for (int x = x0; x < x0 + qh; ++x)
+                if (HI[y * ts + x]) {
+                    s += T[c * n + y * ts + x];
+                    ++cnt;
+                }
+        S.quad[q][c] = cnt ? s / (float)cnt : 0.0f;
+        if (c == 0) S.quad_n[q] = cnt;
+    } else if (tid < 24) {
+        const int c = tid - 21;
+        S.mall[c] = cg_block_sum(T + c * n, ts, 0, ts, 0, ts) / (float)n;
+    } else if (tid == 24) {
+        // error pixels in row-major order: coordinate sums, first min / max x, first max y, and
+        // the stable x-order's first, middle and last element
+        int cnt = 0, xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+        int64_t sx = 0, sy = 0;
+        int colc[64];
+        for (int x = 0; x < 64; ++x) colc[x] = 0;
+        for (int p = 0; p < n; ++p)
+            if (HI[p]) {
+                const int y = p / ts, x = p - y * ts;
+                if (cnt == 0 || x < xmin) {  // np.argmin / argmax: the first occurrence
+                    xmin = x;
+                    S.y_at_xmin = y;
+                }
+                if (cnt == 0 || x > xmax) {
+                    xmax = x;
+                    S.y_at_xmax = y;
+                }
+                if (cnt == 0) ymin = y;
+                if (cnt == 0 || y > ymax) ymax = y;
+                sx += x;
+                sy += y;
+                ++colc[x];
+                ++cnt;
+            }
+        S.xmin = xmin;
+        S.xmax = xmax;
+        S.ymin = ymin;
+        S.ymax = ymax;
+        S.sx = sx;
+        S.sy = sy;
+        // stable sort by x: rank r lies in the first column whose running count passes r, at
+        // the (r - count before) -th error pixel of that column from the top
+        int ranks[3] = {0, cnt / 2, cnt - 1};
+        int res[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) {
+            int r = ranks[k], x = 0;
+            while (x < ts && r >= colc[x]) {
+                r -= colc[x];
+                ++x;
+            }
+            int y = 0;
+            for (; y < ts; ++y)
+                if (HI[y * ts + x]) {
+                    if (r == 0) break;
+                    --r;
+                }
+            res[k] = y * ts + x;
+        }
+        S.o_first = res[0];
+        S.o_mid = res[1];
+        S.o_last = res[2];
     }
     __syncthreads();
     if (tid == 0) {
-        int run = 0;
-        for (int j = 0; j < n_total; ++j) {
-            const int k = S.keep[j];
-            S.keep[j] = k ? run : -1;
-            run += k;
+        for (int c = 0; c < 3; ++c) {
+            S.hi_col[c] = cg_ct((double)cr_clip(S.mt[c]));
+            S.corr[c] = cg_ct((double)cr_clip(S.mp[c] + S.me[c]));
         }
-        A.out_count[slot] = run;
-        S.n_kept = run;
+        const double emx = S.emx, emy = S.emy;
+        const double en = sqrt(emx * emx + emy * emy) + 1e-8;
+        const double vx = emx / en, vy = emy / en;
+        const double offs[3] = {0.5, 0.3, 0.7};
+        for (int l = 0; l < 3; ++l) {
+            const double cx = wf * offs[l], cy = hf * offs[l];
+            S.line_geom[l][0] = cx - vx * wf * 0.6;
+            S.line_geom[l][1] = cy - vy * hf * 0.6;
+            S.line_geom[l][2] = cx + vx * wf * 0.6;
+            S.line_geom[l][3] = cy + vy * hf * 0.6;
+        }
+        const double pvx = -vy, pvy = vx, cx = wf * 0.5, cy = hf * 0.5;
+        S.line_geom[3][0] = cx - pvx * wf * 0.6;
+        S.line_geom[3][1] = cy - pvy * hf * 0.6;
+        S.line_geom[3][2] = cx + pvx * wf * 0.6;
+        S.line_geom[3][3] = cy + pvy * hf * 0.6;
     }
     __syncthreads();
-    // compact in place: every kept record is read before any is written
-    WeftPrim mine[2];
-    int dst[2] = {-1, -1};
-    for (int q = 0; q < 2; ++q) {
-        const int j = tid + q * CG_THREADS;
-        if (j < n_total && S.keep[j] >= 0) {
-            mine[q] = out[j];
-            dst[q] = S.keep[j];
+    // side means of the four lines (_split_by_line: the Python-float line enters as float32)
+    if (tid < 12) {
+        const int l = tid / 3, c = tid % 3;
+        const double* g = S.line_geom[l];
+        const float x0 = (float)g[0], y0 = (float)g[1];
+        const float dy = (float)(g[3] - g[1]), dx = (float)(g[2] - g[0]);
+        float sa = 0.0f, sb = 0.0f;
+        int na = 0;
+        for (int p = 0; p < n; ++p) {
+            const int y = p / ts, x = p - y * ts;
+            const float side = ((float)x - x0) * dy - ((float)y - y0) * dx;
+            if (side >= 0.0f) {
+                sa += T[c * n + p];
+                ++na;
+            } else {
+                sb += T[c * n + p];
+            }
+        }
+        const int nb = n - na;
+        S.line_col[l][0][c] = na ? sa / (float)na : S.mall[c];
+        S.line_col[l][1][c] = nb ? sb / (float)nb : S.mall[c];
+    }
+    if (tid == 0) {
+        int nq = 0;
+        for (int q = 0; q < 4; ++q) nq += S.quad_n[q] ? 3 : 0;
+        S.n_total = S.n_hi == 0 ? 0 : 15 + nq + 11 + 64 + (S.n_hi >= 6 ? 3 : 0) + (S.n_hi >= 3 ? 12 : 0);
+    }
+    __syncthreads();
+
+    const int n_total = S.n_total;
+    const double ecx = S.n_hi ? (double)S.sx / (double)S.n_hi : 0.0;
+    const double ecy = S.n_hi ? (double)S.sy / (double)S.n_hi : 0.0;
+    for (int j = tid; j < n_total; j += CG_THREADS) {
+        double g[7] = {0, 0, 0, 0, 0, 0, 0};
+        double c0[3], c1[3];
+        int r = j;
+        if (r < 15) {  // constant patches: error, corrected and target colour
+            const double al[5] = {0.6, 0.4, 0.25, 0.15, 0.08};
+            cg_emit(S.keys, out, f64, j, 0, g, 0, r / 5 == 1 ? S.corr : S.hi_col, nullptr, al[r % 5]);
+            continue;
+        }
+        r -= 15;
+        int q = 0, rq = r;
+        for (; q < 4; ++q) {
+            if (!S.quad_n[q]) continue;
+            if (rq < 3) break;
+            rq -= 3;
+        }
+        if (q < 4) {  // quadrant means of the error pixels
+            const double al[3] = {0.35, 0.2, 0.1};
+            for (int c = 0; c < 3; ++c) c0[c] = cg_ct((double)cr_clip(S.quad[q][c]));
+            cg_emit(S.keys, out, f64, j, 0, g, 0, c0, nullptr, al[rq]);
+            continue;
+        }
+        r = rq;
+        if (r < 11) {  // linear patches along the error gradient and across it
+            const int l = r < 9 ? r / 3 : 3;
+            const double al9[3] = {0.5, 0.3, 0.15};
+            const double a = r < 9 ? al9[r % 3] : (r == 9 ? 0.4 : 0.2);
+            for (int i = 0; i < 4; ++i) g[i] = S.line_geom[l][i];
+            for (int c = 0; c < 3; ++c) {
+                c0[c] = (double)S.line_col[l][0][c];
+                c1[c] = (double)S.line_col[l][1][c];
+            }
+            cg_emit(S.keys, out, f64, j, 1, g, 4, c0, c1, a);
+            continue;
+        }
+        r -= 11;
+        if (r < 60) {  // lines through the error centroid
+            const double tt[5] = {0.3, 0.6, 1.0, 1.5, 2.5}, al[3] = {0.5, 0.3, 0.15};
+            const int t = r / 12, rem = r % 12;
+            const int a = (rem % 6) / 2;
+            g[4] = tt[t];
+            if ((rem & 1) == 0) {
+                g[0] = ecx; g[1] = 0.0; g[2] = ecx; g[3] = hf;
+            } else {
+                g[0] = 0.0; g[1] = ecy; g[2] = wf; g[3] = ecy;
+            }
+            cg_emit(S.keys, out, f64, j, 2, g, 5, S.hi_col, nullptr, al[a]);  // target colour = error colour
+            continue;
+        }
+        r -= 60;
+        if (r < 4) {  // diagonals
+            g[4] = r < 2 ? 0.5 : 1.0;
+            if ((r & 1) == 0) {
+                g[0] = 0.0; g[1] = 0.0; g[2] = wf; g[3] = hf;
+            } else {
+                g[0] = 0.0; g[1] = hf; g[2] = wf; g[3] = 0.0;
+            }
+            cg_emit(S.keys, out, f64, j, 2, g, 5, S.hi_col, nullptr, 0.25);
+            continue;
+        }
+        r -= 4;
+        if (S.n_hi >= 6) {
+            if (r < 3) {  // a curve through the error pixels' x-order ends and middle
+                const double p0x = S.o_first % ts, p0y = S.o_first / ts;
+                const double pmx = S.o_mid % ts, pmy = S.o_mid / ts;
+                const double p1x = S.o_last % ts, p1y = S.o_last / ts;
+                double cx = 2 * pmx - (p0x + p1x) / 2, cy = 2 * pmy - (p0y + p1y) / 2;
+                cx = cx < wf ? cx : wf;
+                cx = cx > 0.0 ? cx : 0.0;
+                cy = cy < hf ? cy : hf;
+                cy = cy > 0.0 ? cy : 0.0;
+                g[0] = p0x; g[1] = p0y; g[2] = cx; g[3] = cy; g[4] = p1x; g[5] = p1y;
+                g[6] = r == 0 ? 0.5 : (r == 1 ? 1.0 : 2.0);
+                cg_emit(S.keys, out, f64, j, 3, g, 7, S.hi_col, nullptr, 0.3);
+                continue;
+            }
+            r -= 3;
+        }
+        {  // triangles over the error pixels' extent (error colour, then the equal target colour)
+            const double al[3] = {0.4, 0.2, 0.1};
+            const int gi = r / 6, a = (r % 6) / 2;
+            g[0] = (double)S.xmin;
+            g[1] = (double)(gi == 0 ? S.y_at_xmin : S.ymin);
+            g[2] = (double)S.xmax;
+            g[3] = (double)(gi == 0 ? S.y_at_xmax : S.ymax);
+            g[4] = ecx;
+            g[5] = ecy;
+            cg_emit(S.keys, out, f64, j, 4, g, 6, S.hi_col, nullptr, al[a]);
         }
     }
     __syncthreads();
-    for (int q = 0; q < 2; ++q)
-        if (dst[q] >= 0) out[dst[q]] = mine[q];
-    // slots past the pool hold records of no kind (skipped by every renderer)
-    const int n_kept = S.n_kept;
-    for (int j = n_kept + tid; j < A.max_cands; j += CG_THREADS) {
-        WeftPrim P;
-        P.kind = -1;
-        for (int i = 0; i < 7; ++i) P.geom[i] = 0.0f;
-        for (int c = 0; c < 3; ++c) P.color0[c] = P.color1[c] = 0.0f;
-        P.alpha = 0.0f;
-        out[j] = P;
-    }
-    if (f64) {
-        double v[2][14];
-        for (int q = 0; q < 2; ++q)
-            if (dst[q] >= 0)
-                for (int i = 0; i < 14; ++i) v[q][i] = f64[(int64_t)(tid + q * CG_THREADS) * 14 + i];
-        __syncthreads();
-        for (int q = 0; q < 2; ++q)
-            if (dst[q] >= 0)
-                for (int i = 0; i < 14; ++i) f64[(int64_t)dst[q] * 14 + i] = v[q][i];
-    }
+    cg_finish(S.keys, S.keep, &S.n_kept, out, f64, n_total, A.max_cands, A.out_count + slot);
 }

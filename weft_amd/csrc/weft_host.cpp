@@ -56,6 +56,7 @@ struct Module {
     hipFunction_t f_sel_topk_gather = nullptr;
     hipFunction_t f_sel_topk_emit = nullptr;
     hipFunction_t f_gen_cands = nullptr;
+    hipFunction_t f_gen_res = nullptr;
     hipFunction_t f_rans_enc = nullptr;
     hipFunction_t f_rans_scan = nullptr;
     hipFunction_t f_rans_pack = nullptr;
@@ -214,6 +215,7 @@ int weft_init(int device) {
         {&g_mod.f_sel_topk_gather, "weft_sel_topk_gather_kernel"},
         {&g_mod.f_sel_topk_emit, "weft_sel_topk_emit_kernel"},
         {&g_mod.f_gen_cands, "weft_gen_candidates_kernel"},
+        {&g_mod.f_gen_res, "weft_gen_residual_candidates_kernel"},
         {&g_mod.f_rans_enc, "weft_rans_encode_kernel"},
         {&g_mod.f_rans_scan, "weft_rans_scan_kernel"},
         {&g_mod.f_rans_pack, "weft_rans_pack_kernel"},
@@ -829,6 +831,25 @@ int weft_gen_candidates(const float* d_pixels, const int64_t* d_tile_pix, const 
     }
     const unsigned lds = (unsigned)(32 * tile_size * tile_size);
     return launch(g_mod.f_gen_cands, (unsigned)n_tiles, 256, lds, stream, &a);
+}
+
+int weft_gen_residual_candidates(const float* d_pixels, const float* d_pred, int clip_pred, const int64_t* d_tile_pix,
+                                 const int32_t* d_tiles, int n_tiles, int tile_size, WeftPrim* d_out,
+                                 double* d_out_f64, int32_t* d_out_count, int max_cands, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (tile_size != 8 && tile_size != 16 && tile_size != 32 && tile_size != 64)
+        return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_gen_residual_candidates: tile size must be 8, 16, 32 or 64");
+    if (n_tiles < 0) return set_err(WEFT_E_INVALID, "weft_gen_residual_candidates: n_tiles < 0");
+    if (max_cands < WEFT_GEN_RES_MAX_CANDS)
+        return set_err(WEFT_E_INVALID, "weft_gen_residual_candidates: max_cands below WEFT_GEN_RES_MAX_CANDS");
+    if (n_tiles == 0) return WEFT_OK;
+    if (!d_pixels || !d_pred || !d_tile_pix || !d_tiles || !d_out || !d_out_count)
+        return set_err(WEFT_E_INVALID, "weft_gen_residual_candidates: null pointer");
+    WeftResGenArgs a{d_pixels, d_pred, d_tile_pix, d_tiles, d_out, d_out_f64, d_out_count,
+                     n_tiles, tile_size, clip_pred ? 1 : 0, max_cands};
+    const unsigned lds = (unsigned)(28 * tile_size * tile_size);
+    return launch(g_mod.f_gen_res, (unsigned)n_tiles, 256, lds, stream, &a);
 }
 
 }  // extern "C"

@@ -1584,3 +1584,6 @@ extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_emit_kernel(Weft
 
 // Template candidate pools, one workgroup per tile of one size.
 extern "C" __global__ __launch_bounds__(256) void weft_gen_candidates_kernel(WeftCandGenArgs A) { cg_body(A); }
+extern "C" __global__ __launch_bounds__(256) void weft_gen_residual_candidates_kernel(WeftResGenArgs A) {
+    cr_body(A);
+}

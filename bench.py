@@ -61,8 +61,8 @@ PEAK_FP32 = 157.3e12
 # compares 0, per-candidate constants hoisted): coverage + composite (10) + SSE (9).
 FLOP_PER_EVAL = {"triangle": 32, "ellipse": 33, "rect": 28}
 # Soft-edge forward + backward per candidate-pixel (same rules; sigmoid = exp + add + rcp):
-# ellipse 80, rectangle 114, triangle 96, in the drawn family mix.
-SOFT_FLOP_MIX = (80 * 21845 + 114 * 21845 + 96 * 21846) / 65536
+# ellipse 80, rectangle 114, triangle 96, in the drawn family mix (a third each).
+SOFT_FLOP_MIX = (80 + 114 + 96) / 3
 
 
 def log(*a):
@@ -70,10 +70,9 @@ def log(*a):
 
 
 def flop_per_eval_mix() -> float:
-    # family = ((h & 0xFFFF) * 3) >> 16: counts of 16-bit values mapping to 0, 1, 2
-    counts = np.bincount((np.arange(65536, dtype=np.int64) * 3) >> 16, minlength=3) / 65536.0
-    return float(counts[0] * FLOP_PER_EVAL["ellipse"] + counts[1] * FLOP_PER_EVAL["rect"]
-                 + counts[2] * FLOP_PER_EVAL["triangle"])
+    # families are drawn in groups of 64 candidates, (r + group) mod 3 with r uniform per
+    # tile and step: each family has an expected share of exactly one third
+    return (FLOP_PER_EVAL["ellipse"] + FLOP_PER_EVAL["rect"] + FLOP_PER_EVAL["triangle"]) / 3.0
 
 
 def build_workload(args):

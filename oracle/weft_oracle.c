@@ -436,7 +436,7 @@ double orc_tile_objective(const float* src, int ts, const orc_prim* sel, int n_s
 int orc_greedy_add(const float* src, int ts, const orc_prim* pool, int n_pool, int max_prims,
                    double lam, int* sel_idx, double* best_obj_out) {
     long n = (long)ts * ts * 3;
-    orc_prim* sel = (orc_prim*)malloc(sizeof(orc_prim) * (size_t)(max_prims + 1));
+    orc_prim* sel = (orc_prim*)calloc((size_t)(max_prims + 1), sizeof(orc_prim));
     int* alive = (int*)malloc(sizeof(int) * (size_t)(n_pool > 0 ? n_pool : 1));
     int* order = (int*)malloc(sizeof(int) * (size_t)(n_pool > 0 ? n_pool : 1));
     double* objs = (double*)malloc(sizeof(double) * (size_t)(n_pool > 0 ? n_pool : 1));
@@ -479,7 +479,9 @@ int orc_greedy_add(const float* src, int ts, const orc_prim* pool, int n_pool, i
  * A candidate is drawn from two 64-bit hashes of (seed, step, tile, index);
  * fields are integer codes on fixed grids, so float values are exact by
  * construction and identical on host and device.
- *   family  : 0 ellipse (kind 6), 1 rectangle (kind 7), 2 triangle (kind 8)
+ *   family  : 0 ellipse (kind 6), 1 rectangle (kind 7), 2 triangle (kind 8);
+ *             stratified: index group g = idx / 64 of a tile takes family (r + g) mod 3,
+ *             with r = ((base >> 40) & 0xFFFF) * 3 >> 16 from the tile's base hash
  *   cx, cy  : u16, value = code / 1024 (tile-local pixels), confined to the tile
  *   rx, ry  : u16 code from a log-uniform table over [4, 64) px, value = code / 1024
  *   theta   : 12-bit code, angle = code * pi / 4096, cos/sin from a table
@@ -526,7 +528,7 @@ void orc_draw_code(uint64_t seed, uint32_t step, uint32_t tile, uint32_t idx, in
     uint64_t h1 = smix(base ^ (2ull * idx));
     uint64_t h2 = smix(base ^ (2ull * idx + 1ull));
     orc_tables();
-    c->family = (uint16_t)(((h1 & 0xFFFFull) * 3ull) >> 16);
+    c->family = (uint16_t)(((uint32_t)((((base >> 40) & 0xFFFFull) * 3ull) >> 16) + (idx >> 6)) % 3u);
     uint64_t span = (uint64_t)ts * 1024ull;
     c->cx = (uint16_t)((((h1 >> 16) & 0xFFFFull) * span) >> 16);
     c->cy = (uint16_t)((((h1 >> 32) & 0xFFFFull) * span) >> 16);

@@ -224,6 +224,14 @@ def test_codes_are_exact_and_in_range(oracle_lib):
     assert (codes["alpha"] >= ALPHA_CODE_MIN).all()
     fam = np.bincount(codes["family"], minlength=3)
     assert (np.abs(fam - 4000 / 3) < 150).all()
+    # families come in aligned groups of 64 indices, stepping through the three in turn
+    f = codes["family"].astype(np.int64)
+    for g0 in range(0, 4000, 64):
+        assert (f[g0:g0 + 64] == f[g0]).all()
+        assert f[g0] == (f[0] + g0 // 64) % 3
+    # the starting family is spread evenly over tiles (and steps)
+    first = np.array([oracle_lib.draw_codes(0, s, t, 1, 32)["family"][0] for t in range(200) for s in range(3)])
+    assert (np.abs(np.bincount(first, minlength=3) - 200) < 45).all()
     # log-uniform radii: log(r/4)/log(16) ~ U[0, 1)
     u = np.log(codes["rx"] / 4096.0) / np.log(16.0)
     assert abs(u.mean() - 0.5) < 0.03

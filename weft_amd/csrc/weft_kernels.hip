@@ -658,9 +658,13 @@ __device__ __forceinline__ uint64_t draw_base(uint64_t seed, uint32_t step, uint
     return smix(smix(smix(seed) ^ (uint64_t)step) ^ (uint64_t)tile);
 }
 
+// Family of candidate idx, stratified in groups of 64 consecutive indices (one wavefront
+// once the scorers have sorted a tile's candidates by class): group g of a tile takes
+// family (r + g) mod 3, r drawn from the tile's base hash, so every family gets a third of
+// the groups on average and no wavefront holds candidates of two families.
 __device__ __forceinline__ int draw_family(uint64_t base, uint32_t idx) {
-    const uint64_t h1 = smix(base ^ (2ull * idx));
-    return (int)(((h1 & 0xFFFFull) * 3ull) >> 16);
+    const uint32_t r = (uint32_t)((((base >> 40) & 0xFFFFull) * 3ull) >> 16);
+    return (int)((r + (idx >> 6)) % 3u);
 }
 
 __device__ __forceinline__ WeftCode draw_code(uint64_t base, uint32_t idx, int ts) {
@@ -668,7 +672,7 @@ __device__ __forceinline__ WeftCode draw_code(uint64_t base, uint32_t idx, int t
     const uint64_t h2 = smix(base ^ (2ull * idx + 1ull));
     const uint64_t span = (uint64_t)ts * 1024ull;
     WeftCode c;
-    c.family = (uint16_t)(((h1 & 0xFFFFull) * 3ull) >> 16);
+    c.family = (uint16_t)draw_family(base, idx);
     c.cx = (uint16_t)((((h1 >> 16) & 0xFFFFull) * span) >> 16);
     c.cy = (uint16_t)((((h1 >> 32) & 0xFFFFull) * span) >> 16);
     c.rx = weft_tab_logr[(h1 >> 48) & 0xFFFull];
@@ -1004,7 +1008,16 @@ __device__ void fit_order_body(const WeftFitOrderArgs& A) {
         s_run[tid] = 0;
     }
     __syncthreads();
-    for (int k = tid; k < n; k += nthr) atomicAdd(&s_cnt[candidate_class(A.cd, dbase, cand_base, k)], 1);
+    // class counts, one LDS atomic per wave and class (drawn families come in runs of 64)
+    for (int j0 = 0; j0 < n; j0 += nthr) {
+        const int k = j0 + tid;
+        const int cls = k < n ? candidate_class(A.cd, dbase, cand_base, k) : CL_NONE;
+#pragma unroll
+        for (int c = 0; c < WEFT_N_CLASSES; ++c) {
+            const uint64_t bal = __ballot(cls == c);
+            if ((tid & 63) == 0 && bal && c != CL_NONE) atomicAdd(&s_cnt[c], (int)__popcll(bal));
+        }
+    }
     __syncthreads();
     if (tid == 0) {
         int off = 0;
@@ -1076,7 +1089,16 @@ __device__ void batch_sort(const WeftCands& cd, uint64_t dbase, int n, int* perm
         sh.run[tid] = 0;
     }
     __syncthreads();
-    for (int k = tid; k < n; k += nthr) atomicAdd(&sh.cnt[candidate_class(cd, dbase, 0, k)], 1);
+    // class counts, one LDS atomic per wave and class (drawn families come in runs of 64)
+    for (int j0 = 0; j0 < n; j0 += nthr) {
+        const int k = j0 + tid;
+        const int cls = k < n ? candidate_class(cd, dbase, 0, k) : CL_NONE;
+#pragma unroll
+        for (int c = 0; c < WEFT_N_CLASSES; ++c) {
+            const uint64_t bal = __ballot(cls == c);
+            if ((tid & 63) == 0 && bal && c != CL_NONE) atomicAdd(&sh.cnt[c], (int)__popcll(bal));
+        }
+    }
     __syncthreads();
     if (tid == 0) {
         int off = 0;

@@ -233,6 +233,39 @@ def test_fit_c1_bit_exact(gpu, oracle_lib):
     assert acc.sum() > 64  # the fit actually accepts primitives
 
 
+def test_fit_c2_full_size_bit_exact(gpu, oracle_lib):
+    """Config 2 at full size: the 1920x1080 image's 11,256 quadtree tiles (8-64 px), 1024
+    drawn candidates, two greedy rounds (4.3 G evaluations) on the GPU and in the C oracle
+    (OpenMP): every tile's winner, objective bits, acceptance, code and canvas agree."""
+    import os
+
+    from oracle.coracle import PlainFit
+    from weft_amd.fit import PlainFitter
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.rd import tile_lambda
+    from weft_amd.synth import CONFIG_C2, synth_image
+
+    c = CONFIG_C2
+    img = synth_image(c["height"], c["width"], c["n_prims"], seed=c["seed"])
+    qt = decompose_quadtree_variance(img, threshold=c["var_threshold"], min_tile=c["min_tile"],
+                                     max_tile=c["max_tile"])
+    tiles = TileSet.from_images([img], [qt])
+    lam = tile_lambda(75)
+    _say(f"fit C2 full size: {tiles.n_tiles} tiles x {c['n_cand']} candidates x 2 steps, GPU vs oracle")
+    f = PlainFitter(tiles, n_cand=c["n_cand"], depth=c["depth"], seed=0, lam=lam, max_steps=2)
+    f.step(2)
+    rec = f.records()
+    o = PlainFit(tiles.target, tiles.pix_off, tiles.ts, c["n_cand"], c["depth"], 0, lam)
+    ro = o.run(2, n_threads=min(16, len(os.sched_getaffinity(0))))
+    assert np.array_equal(rec["win_idx"], ro["win_idx"])
+    assert np.array_equal(rec["win_acc"], ro["win_acc"])
+    assert bits_equal(rec["win_obj"], ro["win_obj"])
+    acc = ro["win_acc"].astype(bool)
+    assert bits_equal(rec["win_code"][acc], ro["win_code"][acc].view(rec["win_code"].dtype))
+    assert bits_equal(f.canvas(), o.canvas)
+    assert acc[0].mean() > 0.9  # round one accepts on almost every tile
+
+
 def test_fit_mixed_tile_sizes_bit_exact(gpu, oracle_lib):
     """Quadtree tiles of 8-64 px (64-px tiles use 100 KiB of LDS), odd candidate count."""
     from weft_amd.quadtree import TileSet, decompose_quadtree_variance

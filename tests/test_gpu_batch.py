@@ -103,6 +103,38 @@ def test_batch_c3_full_size(gpu, oracle_lib):
     _same_records(sub, ro, "batch vs oracle sample")
 
 
+def test_batch_c5_full_size(gpu, oracle_lib):
+    """Config 5 at full size: the 3840x2160 textured image (44,598 tiles), 2048 candidates,
+    depth 24, two rounds in one persistent launch equal the stepwise fit on every tile and
+    the C oracle on a sample."""
+    from oracle.coracle import PlainFit
+    from weft_amd.batch import BatchFitter
+    from weft_amd.fit import PlainFitter
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.rd import tile_lambda
+    from weft_amd.synth import synth_image
+
+    img = synth_image(2160, 3840, 2500, seed=0, texture=True)
+    tiles = TileSet.from_images([img], [decompose_quadtree_variance(img, threshold=1e-3, min_tile=8, max_tile=64)])
+    n_cand, depth, steps, seed, lam = 2048, 24, 2, 0, tile_lambda(75)
+    _say(f"batch C5 full size: {tiles.n_tiles} tiles, {n_cand} candidates, {steps} rounds")
+    bf = BatchFitter(tiles, n_cand=n_cand, depth=depth, n_steps=steps, seed=seed, lam=lam)
+    bf.run()
+    rb = bf.records()
+    pf = PlainFitter(tiles, n_cand=n_cand, depth=depth, seed=seed, lam=lam, max_steps=steps)
+    pf.step(steps)
+    _same_records(rb, pf.records(), "batch vs stepwise")
+    assert bits_equal(bf.canvas(), pf.canvas())
+    del pf
+
+    ids = np.sort(np.random.default_rng(11).choice(tiles.n_tiles, 256, replace=False)).astype(np.int32)
+    ts = tiles.ts[ids]
+    offs = np.concatenate(([0], np.cumsum(ts.astype(np.int64) ** 2)[:-1]))
+    tgt = np.concatenate([tiles.tile_pixels(int(t)).ravel() for t in ids])
+    ro = PlainFit(tgt, offs, ts, n_cand, depth, seed, lam, tile_ids=ids).run(steps)
+    _same_records({k: v[:, ids] for k, v in rb.items()}, ro, "batch vs oracle sample")
+
+
 @pytest.mark.parametrize("order", ["sjf", "fifo", "lpt"])
 def test_batch_launch_shapes_agree(gpu, order):
     """Every queue order, workgroup size and grid gives the same fit."""

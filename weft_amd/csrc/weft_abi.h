@@ -194,13 +194,6 @@ struct WeftTemplateFitArgs {
     int32_t tile_cols;
 };
 
-struct WeftSelRec {
-    uint64_t a;
-    uint64_t b;
-    uint32_t idx;
-    uint32_t _pad;
-};
-
 struct WeftSelectArgs {
     const double* f64_in;
     const int64_t* i64_in;
@@ -209,19 +202,21 @@ struct WeftSelectArgs {
     int64_t* out_total;
     double* work_f64;
     int64_t* work_i64;
-    WeftSelRec* recs;
-    WeftSelRec* recs2;
     uint8_t* flags;
     const int64_t* positions;
     const int64_t* count_in;
+    const uint64_t* rs_keys_in;   // radix pass input / output
+    const uint32_t* rs_vals_in;
+    uint64_t* rs_keys_out;
+    uint32_t* rs_vals_out;
+    int64_t* rs_table;            // digit-major per-workgroup digit counts
+    const int64_t* rs_prefix;     // their exclusive scan
+    const uint32_t* rs_order;     // element index at each position of the score order
     int64_t n;
-    int64_t n_pad;
     int64_t n_blocks;
     int64_t k;
-    int64_t k_stage;
-    int64_t j_step;
     int32_t largest;
-    int32_t _pad;
+    int32_t rs_shift;
 };
 
 struct WeftCandGenArgs {

@@ -49,12 +49,12 @@ struct Module {
     hipFunction_t f_sel_tile_sums = nullptr;
     hipFunction_t f_sel_scan_tiles = nullptr;
     hipFunction_t f_sel_scan_apply = nullptr;
-    hipFunction_t f_sel_topk_pack = nullptr;
-    hipFunction_t f_sel_bitonic_local = nullptr;
-    hipFunction_t f_sel_bitonic_global = nullptr;
-    hipFunction_t f_sel_topk_keep = nullptr;
-    hipFunction_t f_sel_topk_gather = nullptr;
-    hipFunction_t f_sel_topk_emit = nullptr;
+    hipFunction_t f_sel_rs_score_keys = nullptr;
+    hipFunction_t f_sel_rs_hist = nullptr;
+    hipFunction_t f_sel_rs_scatter = nullptr;
+    hipFunction_t f_sel_rs_segment_keys = nullptr;
+    hipFunction_t f_sel_rs_keep = nullptr;
+    hipFunction_t f_sel_rs_emit = nullptr;
     hipFunction_t f_gen_cands = nullptr;
     hipFunction_t f_gen_res = nullptr;
     hipFunction_t f_rans_enc = nullptr;
@@ -208,12 +208,12 @@ int weft_init(int device) {
         {&g_mod.f_sel_tile_sums, "weft_sel_tile_sums_kernel"},
         {&g_mod.f_sel_scan_tiles, "weft_sel_scan_tiles_kernel"},
         {&g_mod.f_sel_scan_apply, "weft_sel_scan_apply_kernel"},
-        {&g_mod.f_sel_topk_pack, "weft_sel_topk_pack_kernel"},
-        {&g_mod.f_sel_bitonic_local, "weft_sel_bitonic_local_kernel"},
-        {&g_mod.f_sel_bitonic_global, "weft_sel_bitonic_global_kernel"},
-        {&g_mod.f_sel_topk_keep, "weft_sel_topk_keep_kernel"},
-        {&g_mod.f_sel_topk_gather, "weft_sel_topk_gather_kernel"},
-        {&g_mod.f_sel_topk_emit, "weft_sel_topk_emit_kernel"},
+        {&g_mod.f_sel_rs_score_keys, "weft_sel_rs_score_keys_kernel"},
+        {&g_mod.f_sel_rs_hist, "weft_sel_rs_hist_kernel"},
+        {&g_mod.f_sel_rs_scatter, "weft_sel_rs_scatter_kernel"},
+        {&g_mod.f_sel_rs_segment_keys, "weft_sel_rs_segment_keys_kernel"},
+        {&g_mod.f_sel_rs_keep, "weft_sel_rs_keep_kernel"},
+        {&g_mod.f_sel_rs_emit, "weft_sel_rs_emit_kernel"},
         {&g_mod.f_gen_cands, "weft_gen_candidates_kernel"},
         {&g_mod.f_gen_res, "weft_gen_residual_candidates_kernel"},
         {&g_mod.f_rans_enc, "weft_rans_encode_kernel"},
@@ -597,7 +597,7 @@ int weft_template_fit(const float* d_image, int width, int height, int tile_size
 // ---- selection primitives: workspace layout shared by the size query and the launchers
 namespace {
 constexpr int64_t SEL_TILE_ELEMS = 2048;
-constexpr int64_t SEL_SORT_MIN = 512;
+constexpr int64_t SEL_RS_TILE = 2048;  // keys per workgroup of a radix pass
 constexpr int64_t SEL_MAX_SORT = (int64_t)1 << 30;
 
 int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
@@ -606,14 +606,11 @@ int64_t sel_argmin_grid(int64_t n) {
     const int64_t g = sel_blocks(n);
     return g < 1024 ? (g < 1 ? 1 : g) : 1024;
 }
-int64_t sel_pad(int64_t n) {
-    int64_t p = SEL_SORT_MIN;
-    while (p < n) p <<= 1;
-    return p;
-}
+int64_t sel_rs_blocks(int64_t n) { return (n + SEL_RS_TILE - 1) / SEL_RS_TILE; }
 
 struct SelLayout {  // byte offsets into the workspace
-    int64_t f64 = 0, i64 = 0, recs = 0, recs2 = 0, flags = 0, pos = 0, count = 0, total = 0;
+    int64_t f64 = 0, i64 = 0, k0 = 0, k1 = 0, v0 = 0, v1 = 0, vb0 = 0, vb1 = 0, table = 0, prefix = 0, flags = 0,
+            pos = 0, total = 0;
 };
 
 SelLayout sel_layout(int op, int64_t n) {
@@ -629,19 +626,29 @@ SelLayout sel_layout(int op, int64_t n) {
         L.i64 = off;
         off += align256(8 * sel_blocks(n));
     } else {
-        const int64_t np = sel_pad(n);
-        L.recs = off;
-        off += align256((int64_t)sizeof(WeftSelRec) * np);
-        L.recs2 = off;
-        off += align256((int64_t)sizeof(WeftSelRec) * np);
+        const int64_t nt = 256 * sel_rs_blocks(n);  // digit table entries
+        L.k0 = off;
+        off += align256(8 * n);
+        L.k1 = off;
+        off += align256(8 * n);
+        L.v0 = off;
+        off += align256(4 * n);
+        L.v1 = off;
+        off += align256(4 * n);
+        L.vb0 = off;
+        off += align256(4 * n);
+        L.vb1 = off;
+        off += align256(4 * n);
+        L.table = off;
+        off += align256(8 * nt);
+        L.prefix = off;
+        off += align256(8 * nt);
+        L.i64 = off;
+        off += align256(8 * (sel_blocks(nt) > sel_blocks(n) ? sel_blocks(nt) : sel_blocks(n)));
         L.flags = off;
         off += align256(n);
         L.pos = off;
         off += align256(8 * n);
-        L.i64 = off;
-        off += align256(8 * sel_blocks(n));
-        L.count = off;
-        off += 256;
     }
     L.total = off;
     return L;
@@ -657,22 +664,28 @@ int sel_scan_passes(WeftSelectArgs& a, void* stream) {
     return launch(g_mod.f_sel_scan_apply, (unsigned)a.n_blocks, 256, 0, stream, &a);
 }
 
-// Bitonic sort of a.n_pad records at `recs`.
-int sel_sort(WeftSelectArgs& a, WeftSelRec* recs, void* stream) {
-    WeftSelectArgs s = a;
-    s.recs = recs;
-    s.k_stage = 0;
-    const unsigned blocks = (unsigned)(s.n_pad / SEL_SORT_MIN);
-    int rc = launch(g_mod.f_sel_bitonic_local, blocks, 256, 0, stream, &s);
-    for (int64_t k = 2 * SEL_SORT_MIN; k <= s.n_pad && rc == 0; k <<= 1) {
-        s.k_stage = k;
-        for (int64_t j = k / 2; j >= SEL_SORT_MIN && rc == 0; j >>= 1) {
-            s.j_step = j;
-            rc = launch(g_mod.f_sel_bitonic_global, (unsigned)(s.n_pad / 2 / 256), 256, 0, stream, &s);
-        }
-        if (rc == 0) rc = launch(g_mod.f_sel_bitonic_local, blocks, 256, 0, stream, &s);
-    }
-    return rc;
+// One stable radix pass over bits [shift, shift + 8) of the keys: digit counts per
+// workgroup, their digit-major exclusive scan, then the scatter.
+int sel_radix_pass(const WeftSelectArgs& base, const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                   uint32_t* vals_out, int shift, char* w, const SelLayout& L, void* stream) {
+    WeftSelectArgs a = base;
+    a.rs_keys_in = keys_in;
+    a.rs_vals_in = vals_in;
+    a.rs_keys_out = keys_out;
+    a.rs_vals_out = vals_out;
+    a.rs_shift = shift;
+    a.n_blocks = sel_rs_blocks(a.n);
+    a.rs_table = (int64_t*)(w + L.table);
+    a.rs_prefix = (const int64_t*)(w + L.prefix);
+    int rc = launch(g_mod.f_sel_rs_hist, (unsigned)a.n_blocks, 256, 0, stream, &a);
+    if (rc) return rc;
+    WeftSelectArgs c{};
+    c.i64_in = a.rs_table;
+    c.n = 256 * a.n_blocks;
+    c.out_i64 = (int64_t*)(w + L.prefix);
+    c.work_i64 = (int64_t*)(w + L.i64);
+    if ((rc = sel_scan_passes(c, stream))) return rc;
+    return launch(g_mod.f_sel_rs_scatter, (unsigned)a.n_blocks, 256, 0, stream, &a);
 }
 
 int sel_check_work(int op, int64_t n, void* d_work, int64_t work_bytes, const char* who) {
@@ -760,21 +773,45 @@ int weft_segmented_topk(const double* d_scores, const int64_t* d_segments, int64
     if ((rc = sel_check_work(WEFT_SELECT_TOPK, n, d_work, work_bytes, "weft_segmented_topk"))) return rc;
     const SelLayout L = sel_layout(WEFT_SELECT_TOPK, n);
     char* w = (char*)d_work;
+    uint64_t* k0 = (uint64_t*)(w + L.k0);
+    uint64_t* k1 = (uint64_t*)(w + L.k1);
+    uint32_t* v0 = (uint32_t*)(w + L.v0);
+    uint32_t* v1 = (uint32_t*)(w + L.v1);
+    uint32_t* vb0 = (uint32_t*)(w + L.vb0);
+    uint32_t* vb1 = (uint32_t*)(w + L.vb1);
     WeftSelectArgs a{};
     a.f64_in = d_scores;
     a.i64_in = d_segments;
     a.n = n;
-    a.n_pad = sel_pad(n);
     a.k = k;
     a.largest = largest ? 1 : 0;
-    a.recs = (WeftSelRec*)(w + L.recs);
-    a.recs2 = (WeftSelRec*)(w + L.recs2);
+    const unsigned elem_blocks = (unsigned)((n + 255) / 256);
+    // A: score order; 8 passes end back in (k0, v0)
+    a.rs_keys_out = k0;
+    a.rs_vals_out = v0;
+    if ((rc = launch(g_mod.f_sel_rs_score_keys, elem_blocks, 256, 0, stream, &a))) return rc;
+    for (int pass = 0; pass < 8 && rc == 0; ++pass) {
+        const bool even = (pass & 1) == 0;
+        rc = sel_radix_pass(a, even ? k0 : k1, even ? v0 : v1, even ? k1 : k0, even ? v1 : v0, 8 * pass, w, L,
+                            stream);
+    }
+    if (rc) return rc;
+    // B: segments in A order (values = A position); 8 passes end back in (k1, vb0)
+    a.rs_order = v0;
+    a.rs_keys_out = k1;
+    a.rs_vals_out = vb0;
+    if ((rc = launch(g_mod.f_sel_rs_segment_keys, elem_blocks, 256, 0, stream, &a))) return rc;
+    for (int pass = 0; pass < 8 && rc == 0; ++pass) {
+        const bool even = (pass & 1) == 0;
+        rc = sel_radix_pass(a, even ? k1 : k0, even ? vb0 : vb1, even ? k0 : k1, even ? vb1 : vb0, 8 * pass, w, L,
+                            stream);
+    }
+    if (rc) return rc;
+    a.rs_keys_in = k1;
+    a.rs_vals_in = vb0;
     a.flags = (uint8_t*)(w + L.flags);
-    const unsigned pad_blocks = (unsigned)(a.n_pad / 256);
-    if ((rc = launch(g_mod.f_sel_topk_pack, pad_blocks, 256, 0, stream, &a))) return rc;
-    if ((rc = sel_sort(a, a.recs, stream))) return rc;
-    if ((rc = launch(g_mod.f_sel_topk_keep, (unsigned)((n + 255) / 256), 256, 0, stream, &a))) return rc;
-    // kept sorted positions, in order, and their count
+    if ((rc = launch(g_mod.f_sel_rs_keep, elem_blocks, 256, 0, stream, &a))) return rc;
+    // kept A positions, in order, and their count
     WeftSelectArgs c{};
     c.u8_in = a.flags;
     c.n = n;
@@ -784,11 +821,8 @@ int weft_segmented_topk(const double* d_scores, const int64_t* d_segments, int64
     if ((rc = sel_scan_passes(c, stream))) return rc;
     a.positions = c.out_i64;
     a.count_in = d_out_count;
-    if ((rc = launch(g_mod.f_sel_topk_gather, pad_blocks, 256, 0, stream, &a))) return rc;
-    if ((rc = sel_sort(a, a.recs2, stream))) return rc;
-    a.recs = a.recs2;
     a.out_i64 = d_out_indices;
-    return launch(g_mod.f_sel_topk_emit, (unsigned)((n + 255) / 256), 256, 0, stream, &a);
+    return launch(g_mod.f_sel_rs_emit, elem_blocks, 256, 0, stream, &a);
 }
 
 int weft_gen_candidates(const float* d_pixels, const int64_t* d_tile_pix, const int32_t* d_tiles, int n_tiles,

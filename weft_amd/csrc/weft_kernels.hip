@@ -1591,18 +1591,18 @@ extern "C" __global__ __launch_bounds__(256) void weft_sel_scan_tiles_kernel(Wef
 extern "C" __global__ __launch_bounds__(256) void weft_sel_scan_apply_kernel(WeftSelectArgs A) {
     sel_scan_apply_body(A);
 }
-extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_pack_kernel(WeftSelectArgs A) { sel_topk_pack_body(A); }
-extern "C" __global__ __launch_bounds__(256) void weft_sel_bitonic_local_kernel(WeftSelectArgs A) {
-    sel_bitonic_local_body(A);
+extern "C" __global__ __launch_bounds__(256) void weft_sel_rs_score_keys_kernel(WeftSelectArgs A) {
+    sel_rs_score_keys_body(A);
 }
-extern "C" __global__ __launch_bounds__(256) void weft_sel_bitonic_global_kernel(WeftSelectArgs A) {
-    sel_bitonic_global_body(A);
+extern "C" __global__ __launch_bounds__(256) void weft_sel_rs_hist_kernel(WeftSelectArgs A) { sel_rs_hist_body(A); }
+extern "C" __global__ __launch_bounds__(256) void weft_sel_rs_scatter_kernel(WeftSelectArgs A) {
+    sel_rs_scatter_body(A);
 }
-extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_keep_kernel(WeftSelectArgs A) { sel_topk_keep_body(A); }
-extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_gather_kernel(WeftSelectArgs A) {
-    sel_topk_gather_body(A);
+extern "C" __global__ __launch_bounds__(256) void weft_sel_rs_segment_keys_kernel(WeftSelectArgs A) {
+    sel_rs_segment_keys_body(A);
 }
-extern "C" __global__ __launch_bounds__(256) void weft_sel_topk_emit_kernel(WeftSelectArgs A) { sel_topk_emit_body(A); }
+extern "C" __global__ __launch_bounds__(256) void weft_sel_rs_keep_kernel(WeftSelectArgs A) { sel_rs_keep_body(A); }
+extern "C" __global__ __launch_bounds__(256) void weft_sel_rs_emit_kernel(WeftSelectArgs A) { sel_rs_emit_body(A); }
 
 // Template candidate pools, one workgroup per tile of one size.
 extern "C" __global__ __launch_bounds__(256) void weft_gen_candidates_kernel(WeftCandGenArgs A) { cg_body(A); }

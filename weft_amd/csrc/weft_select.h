@@ -9,7 +9,6 @@
 
 constexpr int SEL_BLOCK = 256;
 constexpr int SEL_TILE = 2048;       // elements per workgroup in scan / count passes (8 per thread)
-constexpr int SEL_SORT_BLOCK = 512;  // elements sorted in LDS by one 256-thread workgroup
 
 // ---------------------------------------------------------------- argmin (first index)
 // (value, index) pairs ordered as np.argmin sees them: NaN first, then by value, then index.
@@ -156,14 +155,20 @@ __device__ void sel_scan_apply_body(const WeftSelectArgs& A) {
 }
 
 // ---------------------------------------------------------------- segmented top-k
-// Sort records (WeftSelRec: a, b, idx), compared on (a, b, idx) as unsigned integers.
-using SelRec = WeftSelRec;
+// Two stable LSD radix sorts (8-bit digits, 8 passes each) of 64-bit keys with 32-bit
+// values:
+//   A: score keys in NumPy's ascending order (`largest` sorts the complemented keys of the
+//      elements taken in reverse index order, which is NumPy's stable ascending order read
+//      backwards), values = element index p; position q in this order is the output order.
+//   B: segment keys in A order, values = q; stability keeps every segment's run in A order.
+// An element of B is kept when fewer than k earlier elements of its run exist (the element
+// k places back belongs to another segment); its flag is set at q, and compacting the flags
+// lists the kept q in output order.
 
-__device__ __forceinline__ bool sel_rec_less(const SelRec& x, const SelRec& y) {
-    if (x.a != y.a) return x.a < y.a;
-    if (x.b != y.b) return x.b < y.b;
-    return x.idx < y.idx;
-}
+constexpr int RS_ITEMS = 8;                      // keys per thread per pass
+constexpr int RS_TILE = SEL_BLOCK * RS_ITEMS;    // keys per workgroup
+constexpr int RS_WAVES = SEL_BLOCK / 64;
+static_assert(SEL_BLOCK == 256, "one thread per radix digit");
 
 // float64 -> uint64 whose unsigned order is NumPy's ascending order (-0.0 = +0.0, NaN last).
 __device__ __forceinline__ uint64_t sel_f64_key(double v) {
@@ -173,102 +178,112 @@ __device__ __forceinline__ uint64_t sel_f64_key(double v) {
     return (u >> 63) ? ~u : (u | (1ull << 63));
 }
 
-// Records for the first sort: (segment, score key, index); `largest` reverses both the
-// score and the index order, so that ascending order equals NumPy's stable ascending order
-// read backwards.  Slots [n, n_pad) get the largest record.
-__device__ void sel_topk_pack_body(const WeftSelectArgs& A) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= A.n_pad) return;
-    SelRec r;
-    r._pad = 0;
-    if (p < A.n) {
-        r.a = (uint64_t)A.i64_in[p] ^ (1ull << 63);
-        const uint64_t key = sel_f64_key(A.f64_in[p]);
-        r.b = A.largest ? ~key : key;
-        r.idx = A.largest ? ~(uint32_t)p : (uint32_t)p;
-    } else {
-        r.a = ~0ull;
-        r.b = ~0ull;
-        r.idx = ~0u;
-    }
-    A.recs[p] = r;
+__device__ __forceinline__ uint64_t sel_lanemask_lt() {
+    const unsigned lane = __lane_id();
+    return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
 }
 
-// Bitonic sort of A.n_pad records (a power of two).  Local: every block of SEL_SORT_BLOCK
-// records in LDS, either sorted from scratch (k_stage == 0) or merged for stage k_stage
-// with every distance j < SEL_SORT_BLOCK.
-__device__ void sel_bitonic_local_body(const WeftSelectArgs& A) {
-    __shared__ SelRec s[SEL_SORT_BLOCK];
-    const int64_t base = (int64_t)blockIdx.x * SEL_SORT_BLOCK;
-    const int t = threadIdx.x;
-    s[t] = A.recs[base + t];
-    s[t + SEL_BLOCK] = A.recs[base + t + SEL_BLOCK];
+__device__ __forceinline__ uint32_t rs_digit(uint64_t key, int shift) { return (uint32_t)(key >> shift) & 255u; }
+
+// A's input: keys of the scores, values = index.
+__device__ void sel_rs_score_keys_body(const WeftSelectArgs& A) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.n) return;
+    const int64_t p = A.largest ? A.n - 1 - j : j;
+    const uint64_t key = sel_f64_key(A.f64_in[p]);
+    A.rs_keys_out[j] = A.largest ? ~key : key;
+    A.rs_vals_out[j] = (uint32_t)p;
+}
+
+// Per workgroup, the count of each digit among its RS_TILE keys, stored digit-major:
+// table[digit * n_blocks + block].
+__device__ void sel_rs_hist_body(const WeftSelectArgs& A) {
+    __shared__ int s_h[RS_WAVES][256];
+    const int t = threadIdx.x, w = t >> 6;
+    for (int i = t; i < RS_WAVES * 256; i += SEL_BLOCK) (&s_h[0][0])[i] = 0;
     __syncthreads();
-    const int64_t k0 = A.k_stage == 0 ? 2 : A.k_stage;
-    const int64_t k1 = A.k_stage == 0 ? SEL_SORT_BLOCK : A.k_stage;
-    for (int64_t k = k0; k <= k1; k <<= 1) {
-        for (int j = (int)((k >> 1) < SEL_SORT_BLOCK / 2 ? (k >> 1) : SEL_SORT_BLOCK / 2); j >= 1; j >>= 1) {
-            const int lo = ((t / j) * 2 * j) + (t % j);  // pair (lo, lo + j)
-            const int64_t gi = base + lo;
-            const bool up = (gi & k) == 0;
-            SelRec x = s[lo], y = s[lo + j];
-            if (sel_rec_less(y, x) == up) {
-                s[lo] = y;
-                s[lo + j] = x;
-            }
-            __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const int64_t j = base + (int64_t)r * SEL_BLOCK + t;
+        if (j < A.n) atomicAdd(&s_h[w][rs_digit(A.rs_keys_in[j], A.rs_shift)], 1);
+    }
+    __syncthreads();
+    int c = 0;
+#pragma unroll
+    for (int ww = 0; ww < RS_WAVES; ++ww) c += s_h[ww][t];
+    A.rs_table[(int64_t)t * A.n_blocks + blockIdx.x] = c;
+}
+
+// Stable scatter of one pass.  The workgroup walks its keys in index order, SEL_BLOCK at a
+// time; a key's place is the digit's global start for this workgroup (exclusive scan of the
+// digit-major table) + the keys of that digit earlier in the workgroup: earlier rounds,
+// earlier waves of this round, and lower lanes of its wave (found by matching the 8 digit
+// bits across the wave with ballots).
+__device__ void sel_rs_scatter_body(const WeftSelectArgs& A) {
+    __shared__ int64_t s_base[256];
+    __shared__ int s_run[256];
+    __shared__ int s_cnt[RS_WAVES][256];
+    __shared__ int s_off[RS_WAVES][256];
+    const int t = threadIdx.x, w = t >> 6;
+    s_base[t] = A.rs_prefix[(int64_t)t * A.n_blocks + blockIdx.x];
+    s_run[t] = 0;
+    const uint64_t lt = sel_lanemask_lt();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll 1
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        for (int i = t; i < RS_WAVES * 256; i += SEL_BLOCK) (&s_cnt[0][0])[i] = 0;
+        __syncthreads();
+        const int64_t j = base + (int64_t)r * SEL_BLOCK + t;
+        const bool valid = j < A.n;
+        const uint64_t key = valid ? A.rs_keys_in[j] : 0ull;
+        const uint32_t val = valid ? A.rs_vals_in[j] : 0u;
+        const uint32_t d = rs_digit(key, A.rs_shift);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) s_cnt[w][d] = __popcll(peers);
+        __syncthreads();
+        int run = s_run[t];  // thread t: digit t
+#pragma unroll
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
+            s_off[ww][t] = run;
+            run += s_cnt[ww][t];
+        }
+        s_run[t] = run;
+        __syncthreads();
+        if (valid) {
+            const int64_t pos = s_base[d] + s_off[w][d] + rank;
+            A.rs_keys_out[pos] = key;
+            A.rs_vals_out[pos] = val;
         }
     }
-    A.recs[base + t] = s[t];
-    A.recs[base + t + SEL_BLOCK] = s[t + SEL_BLOCK];
 }
 
-// One global compare-exchange step of stage k at distance j (j >= SEL_SORT_BLOCK).
-__device__ void sel_bitonic_global_body(const WeftSelectArgs& A) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.n_pad / 2) return;
-    const int64_t j = A.j_step, k = A.k_stage;
-    const int64_t lo = (t / j) * 2 * j + (t % j);
-    const bool up = (lo & k) == 0;
-    const SelRec x = A.recs[lo], y = A.recs[lo + j];
-    if (sel_rec_less(y, x) == up) {
-        A.recs[lo] = y;
-        A.recs[lo + j] = x;
-    }
+// B's input: the segment key of each element in A order, values = position in A order.
+__device__ void sel_rs_segment_keys_body(const WeftSelectArgs& A) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= A.n) return;
+    A.rs_keys_out[q] = (uint64_t)A.i64_in[A.rs_order[q]] ^ (1ull << 63);
+    A.rs_vals_out[q] = (uint32_t)q;
 }
 
-// After the first sort: keep sorted position p if fewer than k earlier positions share its
-// segment (the record k places back belongs to another segment), as a 0/1 flag.
-__device__ void sel_topk_keep_body(const WeftSelectArgs& A) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= A.n) return;
-    const bool keep = p < A.k || A.recs[p - A.k].a != A.recs[p].a;
-    A.flags[p] = keep ? 1 : 0;
+// Flag (at its A position) every element of B among the first k of its segment's run.
+__device__ void sel_rs_keep_body(const WeftSelectArgs& A) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= A.n) return;
+    const bool keep = j < A.k || A.rs_keys_in[j - A.k] != A.rs_keys_in[j];
+    A.flags[A.rs_vals_in[j]] = keep ? 1 : 0;
 }
 
-// Gather the *count_in kept records (sorted positions from the compaction) as
-// second-sort records keyed by (score key, index); pad to n_pad.
-__device__ void sel_topk_gather_body(const WeftSelectArgs& A) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= A.n_pad) return;
-    SelRec r;
-    r._pad = 0;
-    if (p < *A.count_in) {
-        const SelRec& s = A.recs[A.positions[p]];
-        r.a = s.b;
-        r.b = 0;
-        r.idx = s.idx;
-    } else {
-        r.a = ~0ull;
-        r.b = ~0ull;
-        r.idx = ~0u;
-    }
-    A.recs2[p] = r;
-}
-
-__device__ void sel_topk_emit_body(const WeftSelectArgs& A) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= *A.count_in) return;
-    const uint32_t i = A.recs[p].idx;
-    A.out_i64[p] = (int64_t)(A.largest ? ~i : i);
+// out[i] = index of the i-th kept element in A order.
+__device__ void sel_rs_emit_body(const WeftSelectArgs& A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *A.count_in) return;
+    A.out_i64[i] = (int64_t)A.rs_order[A.positions[i]];
 }

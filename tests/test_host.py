@@ -1,7 +1,6 @@
 """Host-side logic and the C ABI library, without a GPU."""
 from __future__ import annotations
 
-import ctypes
 import os
 import re
 import struct

@@ -121,7 +121,7 @@ def generate_residual_pools(tiles, d_pred, clip_pred: bool = True, want_f64: boo
     return out, counts, f64
 
 
-def _to_primitives(recs, counts, f64, width):
+def _to_primitives(recs, counts, f64):
     n = int(counts[0].item())
     kinds = recs.view(-1).cpu().numpy().view(PRIM_DTYPE)[:n]["kind"]
     vals = f64.cpu().numpy().reshape(-1, 14)[:n]
@@ -161,7 +161,7 @@ def generate_residual_candidates(tile: np.ndarray, pred: np.ndarray, quality: in
     d_pred = torch.from_numpy(pred.reshape(-1)).to(torch.device("cuda", 0))
     recs, counts, f64 = generate_residual_pools(_OneTile(tile), d_pred, clip_pred=False, want_f64=True)
     torch.cuda.synchronize()
-    return _to_primitives(recs, counts, f64, RES_MAX_CANDS)
+    return _to_primitives(recs, counts, f64)
 
 
 def generate_candidates(tile: np.ndarray, quality: int = 75) -> list[Primitive]:
@@ -171,4 +171,4 @@ def generate_candidates(tile: np.ndarray, quality: int = 75) -> list[Primitive]:
     tile = _check_tile(tile)
     recs, counts, f64 = generate_pools(_OneTile(tile), quality, want_f64=True)
     torch.cuda.synchronize()
-    return _to_primitives(recs, counts, f64, MAX_CANDS)
+    return _to_primitives(recs, counts, f64)

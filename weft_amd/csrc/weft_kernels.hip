@@ -1330,17 +1330,56 @@ extern "C" __global__ __launch_bounds__(256) void weft_score_evals_kernel(WeftSc
     if (A.out_sse) A.out_sse[i] = sse;
 }
 
-// One 64-lane workgroup per tile: objective of the current canvas.  The tile's SSE is
-// the pairwise tree split over lanes (weft_split.h) with no candidate composited, so it
-// equals the sequential sum bit for bit.
+// One 64-lane workgroup per tile: objective of the current canvas, the pairwise tree of
+// the clipped canvas's squared errors with every lane busy.  Lane = 8 * leaf lane + a:
+// accumulator a of each of its leaf lane's leaves adds the leaf's elements a, a + 8, ...
+// in order (reading them directly, as no candidate is composited); the 8 accumulators of a
+// leaf meet in the ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)) tree through xor
+// shuffles 1, 2, 4 (a + b == b + a), and the leaves in the pairwise tree as in
+// weft_split.h.  Equal to the sequential sum bit for bit.
+template <int LS>
+__device__ __forceinline__ float init_chunk(const float* tg, const float* cv, int c0, int nleaves) {
+    const int lane = threadIdx.x & 63;
+    const int a = lane & 7, ll = lane >> 3;
+    const int L = nleaves < 8 ? nleaves : 8;  // leaf lanes in use (a power of two)
+    const int m = nleaves / L;                 // leaves per leaf lane
+    SplitTree T;
+    T.n = 0;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) T.lvl[q] = 0.0f;
+#pragma unroll 1
+    for (int li = 0; li < m; ++li) {
+        float r = 0.0f;
+        if (ll < L) {
+            const int e0 = c0 + (ll * m + li) * LS + a;
+#pragma unroll
+            for (int k = 0; k < LS / 8; ++k) {
+                const float d = clip01(cv[e0 + 8 * k]) - tg[e0 + 8 * k];
+                r = r + d * d;
+            }
+        }
+        r += __shfl_xor(r, 1, 64);
+        r += __shfl_xor(r, 2, 64);
+        r += __shfl_xor(r, 4, 64);
+        split_tree_push(T, r);
+    }
+    float v = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 7; ++q)
+        if ((1 << q) == m) v = T.lvl[q];
+    for (int off = 8; off < 8 * L; off <<= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 template <int TS>
 __device__ __forceinline__ float init_tile_sse(const float* tg, const float* cv) {
-    constexpr int L = TS == 8 ? 2 : (TS == 16 ? 8 : 32);
-    WeftPrim none;
-    none.kind = -1;  // skipped: the canvas passes through
-    none.alpha = 0.0f;
-    none.color0[0] = none.color0[1] = none.color0[2] = 0.0f;
-    return split_sse<TS, true>(none, cv, tg, L, (int)(threadIdx.x & (L - 1)));
+    if constexpr (TS == 64) {
+        const float a = init_chunk<128>(tg, cv, 0, 64);
+        const float b = init_chunk<128>(tg, cv, 8192, 32);
+        return (0.0f + a) + b;
+    } else {
+        return init_chunk<96>(tg, cv, 0, (3 * TS * TS) / 96);
+    }
 }
 extern "C" __global__ __launch_bounds__(256) void weft_fit_order_kernel(WeftFitOrderArgs A) { fit_order_body(A); }
 

@@ -307,16 +307,22 @@ int weft_batch_refine_fit(const WeftFitState* st, const WeftCands* cands, const 
 /* Exact scoring (the same values as weft_fit_score) of a few candidates per tile, for
  * refined candidates (no reference counterpart; the candidates' SSE follows NumPy's
  * pairwise float32 order exactly, see weft_score_evals).  Work item i (one 64-lane
- * workgroup) scores candidates d_work_c0[i] .. + cpw - 1 of tile d_work_tile[i] (those
- * below cands->n_cand), cpw = cpw_by_size[log2(ts) - 3], each candidate's pixels split
- * over 64 / cpw lanes along the pairwise tree: 64 / cpw must be a power of two at most
- * 2, 8, 32, 32 for tiles of 8, 16, 32, 64 px.  Outputs as weft_fit_score: per-item first
- * minimum of the objective (for weft_fit_select) and optional SSE / MSE tables. */
+ * workgroup) holds tpw tiles, d_work_tile[i * WEFT_SPLIT_MAX_TILES + g] for g < tpw (-1
+ * for an unused slot; slot 0 is always a tile), and scores candidates d_work_c0[i] ..
+ * + cpw - 1 of each (those below cands->n_cand), with cpw = cpw_by_size[log2(ts) - 3] and
+ * tpw = tpw_by_size[log2(ts) - 3]: each candidate's pixels are split over
+ * 64 / (cpw * tpw) lanes along the pairwise tree, which must be a power of two at most
+ * 2, 8, 32, 32 for tiles of 8, 16, 32, 64 px (several small tiles share a workgroup).
+ * Outputs as weft_fit_score: the first minimum of the objective of tile slot g of item i
+ * at d_part_obj / d_part_idx[i * WEFT_SPLIT_MAX_TILES + g] (for weft_fit_select), and
+ * optional SSE / MSE tables. */
+#define WEFT_SPLIT_MAX_TILES 32
 typedef struct WeftSplitWork {
-    const int32_t* d_work_tile;
-    const int32_t* d_work_c0;
+    const int32_t* d_work_tile;  /* [n_work * WEFT_SPLIT_MAX_TILES] */
+    const int32_t* d_work_c0;    /* [n_work] */
     int32_t n_work;
     int32_t cpw_by_size[4];
+    int32_t tpw_by_size[4];
     int32_t _pad;
 } WeftSplitWork;
 

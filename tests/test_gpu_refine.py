@@ -330,9 +330,11 @@ def test_refined_fit_is_exact_and_no_worse(gpu, oracle_lib):
     assert bits_equal(rf.canvas(), canvas)
 
 
-@pytest.mark.parametrize("cpw", [None, {8: 64, 16: 64, 32: 64, 64: 64}, {8: 32, 16: 32, 32: 4, 64: 8}])
-def test_split_scorer_matches_oracle(gpu, oracle_lib, cpw):
-    """Lane-split exact scores equal the oracle's and the sequential kernel's bit for bit."""
+@pytest.mark.parametrize("cpw,n", [(None, 9), (None, 3), (None, 1), ({8: 64, 16: 64, 32: 64, 64: 64}, 9),
+                                   ({8: 32, 16: 32, 32: 4, 64: 8}, 9)])
+def test_split_scorer_matches_oracle(gpu, oracle_lib, cpw, n):
+    """Lane-split exact scores equal the oracle's and the sequential kernel's bit for bit,
+    with one tile per workgroup and with several small tiles sharing one."""
     import ctypes
 
     import torch
@@ -341,8 +343,8 @@ def test_split_scorer_matches_oracle(gpu, oracle_lib, cpw):
     from weft_amd.records import PRIM_DTYPE
     from weft_amd.refine import split_work
 
-    tiles, canvas = _random_tiles([8, 16, 32, 64, 8, 16, 32, 64, 64, 32], seed=12)
-    T, n = tiles.n_tiles, 9
+    tiles, canvas = _random_tiles([8, 16, 32, 64, 8, 16, 32, 64, 64, 32] + [8] * 19 + [16] * 5, seed=12)
+    T = tiles.n_tiles
     prims = np.zeros(T * n, PRIM_DTYPE)
     for t in range(T):
         prims[t * n:(t + 1) * n] = oracle_lib.codes_to_prims(oracle_lib.draw_codes(3, 1, t, n, int(tiles.ts[t])))
@@ -353,13 +355,14 @@ def test_split_scorer_matches_oracle(gpu, oracle_lib, cpw):
     f.set_canvas(canvas)
     f.score(0)
     seq = f.sse_table()
-    wt, wc, w0, nw, st, cp = split_work(tiles.ts, n, cpw)
+    wt, wc, w0, nw, st, cp, tp = split_work(tiles.ts, n, cpw)
     d = [torch.from_numpy(a).cuda() for a in (wt, wc)]
-    w = _lib.SplitWork(d[0].data_ptr(), d[1].data_ptr(), len(wt), (ctypes.c_int32 * 4)(*cp), 0)
+    w = _lib.SplitWork(d[0].data_ptr(), d[1].data_ptr(), len(wc), (ctypes.c_int32 * 4)(*cp),
+                       (ctypes.c_int32 * 4)(*tp), 0)
     sse = torch.zeros(T * n, dtype=torch.float32, device="cuda")
     po = torch.zeros(len(wt), dtype=torch.float64, device="cuda")
     pi = torch.zeros(len(wt), dtype=torch.int32, device="cuda")
-    _say(f"split scorer cpw {cp}, {len(wt)} work items")
+    _say(f"split scorer cpw {cp}, tiles per item {tp}, {len(wc)} work items")
     _lib.check(f.L.weft_fit_score_split(ctypes.byref(f._state), ctypes.byref(f._cands(0)), ctypes.byref(w),
                                         sse.data_ptr(), None, po.data_ptr(), pi.data_ptr(), _lib.stream_ptr()),
                "weft_fit_score_split")

@@ -280,14 +280,23 @@ def test_split_work_tables():
 
     rng = np.random.default_rng(1)
     ts = rng.choice([8, 16, 32, 64], 101)
-    for n in (9, 5, 17, 1, 65):
-        wt, wc, w0, nw, st, cp = split_work(ts, n)
-        cpd = dict(zip((8, 16, 32, 64), cp))
-        assert len(wt) == int(nw.sum())
+    M = 32
+    for n in (9, 5, 17, 1, 65, 3):
+        wt, wc, w0, nw, st, cp, tp = split_work(ts, n)
+        cpd, tpd = dict(zip((8, 16, 32, 64), cp)), dict(zip((8, 16, 32, 64), tp))
+        assert len(wt) == M * len(wc)
+        for s in (8, 16, 32, 64):
+            assert 64 // (cpd[s] * tpd[s]) <= {8: 2, 16: 8, 32: 32, 64: 32}[s]
+            assert tpd[s] == 1 or cpd[s] >= n  # a shared workgroup holds all of a tile's candidates
+        seen = np.zeros(len(ts), np.int64)
         for t in range(len(ts)):
-            items = [int(w0[t]) + j * int(st[t]) for j in range(int(nw[t]))]
-            assert all(wt[i] == t for i in items)
-            assert sorted(int(wc[i]) for i in items) == list(range(0, n, cpd[int(ts[t])]))
+            parts = [int(w0[t]) + j * int(st[t]) for j in range(int(nw[t]))]
+            assert all(wt[p] == t for p in parts)  # partial slot = the tile's slot in its item
+            assert sorted(int(wc[p // M]) for p in parts) == list(range(0, n, cpd[int(ts[t])]))
+            for p in parts:
+                seen[t] += 1
+                assert all(ts[x] == ts[t] for x in wt[(p // M) * M:(p // M + 1) * M] if x >= 0)
+        assert (seen == nw).all() and (nw >= 1).all()
 
 
 def test_quantize_primitive_equals_decode_of_encode():

@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 14
+ABI_VERSION = 15
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -82,14 +82,17 @@ class RefineOpts(ctypes.Structure):
     ]
 
 
+SPLIT_MAX_TILES = 32
+
+
 class SplitWork(ctypes.Structure):
     _fields_ = [
         ("d_work_tile", c_void_p), ("d_work_c0", c_void_p), ("n_work", c_int32), ("cpw_by_size", c_int32 * 4),
-        ("_pad", c_int32),
+        ("tpw_by_size", c_int32 * 4), ("_pad", c_int32),
     ]
 
 
-assert ctypes.sizeof(SplitWork) == 40
+assert ctypes.sizeof(SplitWork) == 56
 assert ctypes.sizeof(SoftBatch) == 56 and ctypes.sizeof(RefineOpts) == 112
 assert ctypes.sizeof(Batch) == 112
 assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 32 and ctypes.sizeof(Cands) == 40

@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 14
+#define WEFT_ABI_VERSION 15
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -127,6 +127,7 @@ struct WeftSplitScoreArgs {
     const int32_t* work_tile;
     const int32_t* work_c0;
     int32_t cpw[4];
+    int32_t tpw[4];
     float* out_sse;
     float* out_mse;
     double* part_obj;

@@ -159,7 +159,7 @@ static_assert(sizeof(WeftBatch) == 112, "WeftBatch layout");
 static_assert(sizeof(WeftSoftBatch) == 56, "WeftSoftBatch layout");
 static_assert(sizeof(WeftRefineOpts) == 112, "WeftRefineOpts layout");
 static_assert(sizeof(WeftCode) == 20, "WeftCode layout");
-static_assert(sizeof(WeftSplitWork) == 40, "WeftSplitWork layout");
+static_assert(sizeof(WeftSplitWork) == 56, "WeftSplitWork layout");
 static_assert(sizeof(WeftChunkDesc) == 24, "WeftChunkDesc layout");
 static_assert(sizeof(WeftPrim) == 60, "WeftPrim layout");
 
@@ -443,16 +443,18 @@ int weft_fit_score_split(const WeftFitState* st, const WeftCands* cands, const W
     if (cands->n_cand < 1) return set_err(WEFT_E_INVALID, "weft_fit_score_split: n_cand < 1");
     const int max_lanes[4] = {2, 8, 32, 32};
     for (int c = 0; c < 4; ++c) {
-        const int cpw = w->cpw_by_size[c];
-        if (!pow2_le64(cpw) || 64 / cpw > max_lanes[c])
-            return set_err(WEFT_E_INVALID, "weft_fit_score_split: lanes per candidate must be a power of two "
-                                           "<= 2, 8, 32, 32 for 8, 16, 32, 64-px tiles");
+        const int cpw = w->cpw_by_size[c], tpw = w->tpw_by_size[c];
+        if (!pow2_le64(cpw) || !pow2_le64(tpw) || tpw > WEFT_SPLIT_MAX_TILES || cpw * tpw > 64 ||
+            64 / (cpw * tpw) > max_lanes[c])
+            return set_err(WEFT_E_INVALID, "weft_fit_score_split: lanes per candidate, 64 / (cpw * tpw), must be "
+                                           "a power of two <= 2, 8, 32, 32 for 8, 16, 32, 64-px tiles");
     }
     if (w->n_work < 0 || (w->n_work > 0 && (!w->d_work_tile || !w->d_work_c0)))
         return set_err(WEFT_E_INVALID, "weft_fit_score_split: bad work list");
     if (w->n_work == 0) return WEFT_OK;
     WeftSplitScoreArgs a{*st, *cands, w->d_work_tile, w->d_work_c0,
                          {w->cpw_by_size[0], w->cpw_by_size[1], w->cpw_by_size[2], w->cpw_by_size[3]},
+                         {w->tpw_by_size[0], w->tpw_by_size[1], w->tpw_by_size[2], w->tpw_by_size[3]},
                          d_out_sse, d_out_mse, d_part_obj, d_part_idx};
     return launch(g_mod.f_split, (unsigned)w->n_work, 64, 0, stream, &a);
 }

@@ -178,14 +178,20 @@ __device__ void score_evals_split_item(const WeftScoreEvalsSplitArgs& A) {
 }
 
 template <int TS>
-__device__ void split_score_item(const WeftSplitScoreArgs& A, int item, int t, int cpw) {
+__device__ void split_score_item(const WeftSplitScoreArgs& A, int item, int cpw, int tpw) {
     const WeftFitState& st = A.st;
-    const int L = 64 / cpw;
+    const int G = 64 / tpw;  // lanes per tile slot
+    const int L = G / cpw;   // lanes per candidate
     const int lane = threadIdx.x;
-    const int jl = lane / L, sl = lane - jl * L;
+    const int g = lane / G, gl = lane - g * G;
+    const int jl = gl / L, sl = gl - jl * L;
+    const int64_t slot = (int64_t)item * WEFT_SPLIT_MAX_TILES + g;
+    const int t = A.work_tile[slot];
+    const int tt = t >= 0 ? t : A.work_tile[(int64_t)item * WEFT_SPLIT_MAX_TILES];  // pixels read by an unused slot
+    const bool tile_live = t >= 0 && st.d_tile_nsel[t] < st.depth;
     const int n = A.cd.n_cand;
     const int c = A.work_c0[item] + jl;
-    const bool live = c < n;
+    const bool live = tile_live && c < n;
     WeftPrim P;
     P.kind = -1;
     P.alpha = 0.0f;
@@ -198,7 +204,7 @@ __device__ void split_score_item(const WeftSplitScoreArgs& A, int item, int t, i
             P = load_prim(A.cd.d_prims + cb + c);
         }
     }
-    const int64_t base = st.d_tile_pix[t] * 3;
+    const int64_t base = st.d_tile_pix[tt] * 3;
     const float sse = split_sse<TS>(P, st.d_canvas + base, st.d_target + base, L, sl);
     double obj = __builtin_huge_val();
     int idx = 0x7fffffff;
@@ -212,33 +218,26 @@ __device__ void split_score_item(const WeftSplitScoreArgs& A, int item, int t, i
         obj = objective(mse, st.d_tile_bits[t] + prim_bytes(P.kind) * 8, st.lam, st.obj_mode);
         idx = c;
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
+    // first minimum over the lanes of this tile slot (xor offsets below G stay in it)
+    for (int off = G >> 1; off >= 1; off >>= 1) {
         const double o2 = __shfl_xor(obj, off, 64);
         const int i2 = __shfl_xor(idx, off, 64);
         argmin_pair(obj, idx, o2, i2);
     }
-    if (lane == 0) {
-        A.part_obj[item] = obj;
-        A.part_idx[item] = idx == 0x7fffffff ? -1 : idx;
+    if (gl == 0 && t >= 0) {
+        A.part_obj[slot] = obj;
+        A.part_idx[slot] = idx == 0x7fffffff ? -1 : idx;
     }
 }
 
 __device__ void split_score_body(const WeftSplitScoreArgs& A) {
     const int item = blockIdx.x;
-    const int t = A.work_tile[item];
-    if (A.st.d_tile_nsel[t] >= A.st.depth) {
-        if (threadIdx.x == 0) {
-            A.part_obj[item] = __builtin_huge_val();
-            A.part_idx[item] = -1;
-        }
-        return;
-    }
-    switch (A.st.d_tile_ts[t]) {
-    case 8: split_score_item<8>(A, item, t, A.cpw[0]); break;
-    case 16: split_score_item<16>(A, item, t, A.cpw[1]); break;
-    case 32: split_score_item<32>(A, item, t, A.cpw[2]); break;
-    case 64: split_score_item<64>(A, item, t, A.cpw[3]); break;
+    const int t0 = A.work_tile[(int64_t)item * WEFT_SPLIT_MAX_TILES];
+    switch (A.st.d_tile_ts[t0]) {
+    case 8: split_score_item<8>(A, item, A.cpw[0], A.tpw[0]); break;
+    case 16: split_score_item<16>(A, item, A.cpw[1], A.tpw[1]); break;
+    case 32: split_score_item<32>(A, item, A.cpw[2], A.tpw[2]); break;
+    case 64: split_score_item<64>(A, item, A.cpw[3], A.tpw[3]); break;
     default: break;
     }
 }

@@ -178,43 +178,76 @@ def refine_work(tile_ts, k: int, cpw: dict[int, int] | None = None, tiles=None):
 # Candidates per work item of the lane-split exact scorer, by tile size: 64 / cpw lanes
 # per candidate, at most the pairwise tree's leaf counts 2, 8, 32, 32.
 SPLIT_CPW = {8: 32, 16: 8, 32: 2, 64: 2}
+SPLIT_MAX_LANES = {8: 2, 16: 8, 32: 32, 64: 32}
+
+
+def _pow2_at_least(v: int) -> int:
+    p = 1
+    while p < v:
+        p <<= 1
+    return p
 
 
 def split_work(tile_ts, n: int, cpw: dict[int, int] | None = None):
-    """Work items of weft_fit_score_split for n candidates per tile, with the per-tile
-    (first item, item count, stride) table weft_fit_select reads; a group of XCDS
-    same-size tiles has its items interleaved (a tile's items are XCDS apart)."""
-    cpw = {**SPLIT_CPW, **(cpw or {})}
+    """Work items of weft_fit_score_split for n candidates per tile.
+
+    Returns (work_tile [n_work * SPLIT_MAX_TILES], work_c0 [n_work], work0, nwork, stride,
+    cpw list, tpw list): item i scores candidates work_c0[i] .. + cpw - 1 of the tiles in
+    its slots; tile t's first minima are partials work0[t] + j * stride[t], j < nwork[t]
+    (partial arrays hold n_work * SPLIT_MAX_TILES entries).  By default, when a tile's n
+    candidates fit in 64 / tpw lanes at the size's lane limit, several tiles (tpw) share a
+    workgroup; otherwise a tile's candidates are spread over items of cpw (SPLIT_CPW),
+    and a group of XCDS same-size tiles has its items interleaved (a tile's items are XCDS
+    apart).  An explicit cpw keeps one tile per item."""
+    M = _lib.SPLIT_MAX_TILES
     tile_ts = np.asarray(tile_ts)
     T = len(tile_ts)
+    cpw_d, tpw_d = dict(SPLIT_CPW), {8: 1, 16: 1, 32: 1, 64: 1}
+    for ts, lmax in SPLIT_MAX_LANES.items():
+        c = _pow2_at_least(n)
+        if cpw is None and c * lmax <= 64:
+            cpw_d[ts], tpw_d[ts] = c, min(M, 64 // (c * lmax))
+    if cpw:
+        cpw_d.update(cpw)
     order = np.argsort(-tile_ts, kind="stable")
-    per = np.array([-(-n // cpw[int(v)]) for v in tile_ts[order]], np.int64)
     wt, wc = [], []
     work0 = np.zeros(T, np.int32)
     nwork = np.zeros(T, np.int32)
     stride = np.ones(T, np.int32)
+
+    def item(tiles, c0):
+        row = np.full(M, -1, np.int32)
+        row[:len(tiles)] = tiles
+        wt.append(row)
+        wc.append(c0)
+
     pos = 0
-    for g0 in range(0, T, XCDS):
-        grp = order[g0:g0 + XCDS]
-        pg = per[g0:g0 + XCDS]
-        same = bool((pg == pg[0]).all())
-        if same:
+    sizes = tile_ts[order]
+    for ts in (64, 32, 16, 8):
+        ids = order[sizes == ts]
+        if len(ids) == 0:
+            continue
+        g, c = tpw_d[ts], cpw_d[ts]
+        if g > 1:
+            for i0 in range(0, len(ids), g):
+                grp = ids[i0:i0 + g]
+                for s, t in enumerate(grp):
+                    work0[t], nwork[t], stride[t] = pos * M + s, 1, 1
+                item(grp, 0)
+                pos += 1
+            continue
+        per = -(-n // c)
+        for g0 in range(0, len(ids), XCDS):
+            grp = ids[g0:g0 + XCDS]
             for j, t in enumerate(grp):
-                work0[t], nwork[t], stride[t] = pos + j, pg[0], len(grp)
-            for r in range(int(pg[0])):
+                work0[t], nwork[t], stride[t] = (pos + j) * M, per, len(grp) * M
+            for r in range(per):
                 for t in grp:
-                    wt.append(int(t))
-                    wc.append(r * cpw[int(tile_ts[t])])
-            pos += len(grp) * int(pg[0])
-        else:
-            for t, q in zip(grp, pg):
-                work0[t], nwork[t], stride[t] = pos, q, 1
-                for r in range(int(q)):
-                    wt.append(int(t))
-                    wc.append(r * cpw[int(tile_ts[t])])
-                pos += int(q)
-    return (np.array(wt, np.int32), np.array(wc, np.int32), work0, nwork, stride,
-            [cpw[8], cpw[16], cpw[32], cpw[64]])
+                    item([t], r * c)
+            pos += len(grp) * per
+    work_tile = np.concatenate(wt).astype(np.int32) if wt else np.zeros(0, np.int32)
+    return (work_tile, np.array(wc, np.int32), work0, nwork, stride,
+            [cpw_d[8], cpw_d[16], cpw_d[32], cpw_d[64]], [tpw_d[8], tpw_d[16], tpw_d[32], tpw_d[64]])
 
 
 def autotune_cpw(fitter, cands: _lib.Cands, topk_idx, k: int, cfg: AdamConfig,
@@ -311,11 +344,12 @@ class RefinedFitter(PlainFitter):
         self.d_rprims = torch.zeros(T * K1 * PRIM_DTYPE.itemsize, dtype=torch.uint8, device=dev)
         sw = split_work(tiles.ts, K1)
         self._split_arrays = [torch.from_numpy(a).to(dev) for a in sw[:5]]
-        self.n_rwork = len(sw[0])
+        self.n_rwork = len(sw[1])
         self._split = _lib.SplitWork(self._split_arrays[0].data_ptr(), self._split_arrays[1].data_ptr(),
-                                     self.n_rwork, (ctypes.c_int32 * 4)(*sw[5]), 0)
-        self.d_rpart_obj = torch.empty(max(1, self.n_rwork), dtype=torch.float64, device=dev)
-        self.d_rpart_idx = torch.empty(max(1, self.n_rwork), dtype=torch.int32, device=dev)
+                                     self.n_rwork, (ctypes.c_int32 * 4)(*sw[5]), (ctypes.c_int32 * 4)(*sw[6]), 0)
+        n_part = max(1, self.n_rwork * _lib.SPLIT_MAX_TILES)
+        self.d_rpart_obj = torch.empty(n_part, dtype=torch.float64, device=dev)
+        self.d_rpart_idx = torch.empty(n_part, dtype=torch.int32, device=dev)
         S = self.max_steps
         self.d_rwin_idx = torch.full((S, T), -1, dtype=torch.int32, device=dev)
         self.d_rwin_code = torch.zeros((S, T * CODE_DTYPE.itemsize), dtype=torch.uint8, device=dev)

@@ -168,7 +168,7 @@ def test_topk_matches_stable_sort(gpu):
             tab[0, 7] = np.nan
             tab[1, 20] = 0.0
             tab[1, 30] = -0.0
-        for k in sorted({1, min(n, 16), min(n, 64), min(n, 256)}):
+        for k in sorted({1, min(n, 2), min(n, 4), min(n, 8), min(n, 16), min(n, 64), min(n, 256)}):
             _say(f"topk n {n} k {k}")
             out = topk(f, torch.from_numpy(tab).cuda(), n, k).cpu().numpy()
             ref = np.stack([np.argsort(np.where(np.isnan(r), np.inf, r), kind="stable")[:k] for r in tab])

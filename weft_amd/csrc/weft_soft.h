@@ -376,7 +376,16 @@ __device__ void soft_eval_body(const WeftSoftArgs& A) {
     if (slice == 0) A.b.d_out_sse[ci] = S.sse;
 }
 
-// Segmented top-k by bitonic sort of (value bits, index) keys in LDS.
+// (value, index) key of a score: order-preserving map of the float bits (-0 as +0, NaN
+// last) above the index, so keys are distinct and ordered as a stable ascending sort.
+__device__ __forceinline__ unsigned long long topk_key(float v, int i) {
+    const unsigned int b = v == 0.0f ? 0u : __float_as_uint(v);
+    const unsigned int hi = v != v ? 0xFFFFFFFEu : ((b & 0x80000000u) ? ~b : (b | 0x80000000u));
+    return ((unsigned long long)hi << 32) | (unsigned int)i;
+}
+
+// Segmented top-k: for small k, k rounds of a workgroup-wide minimum over the keys above
+// the last one taken; otherwise a bitonic sort of the keys in LDS.
 __device__ void topk_body(const WeftTopkArgs& A) {
     extern __shared__ float4 weft_smem[];
     unsigned long long* key = reinterpret_cast<unsigned long long*>(weft_smem);
@@ -385,16 +394,35 @@ __device__ void topk_body(const WeftTopkArgs& A) {
     if (A.st.d_tile_nsel[t] >= A.st.depth) return;
     const int n = A.n, N = A.n_pow2;
     const float* row = A.table + (int64_t)t * n;
-    for (int i = tid; i < N; i += nthr) {
-        unsigned int hi = 0xFFFFFFFFu;
-        if (i < n) {
-            const float v = row[i];
-            const unsigned int b = v == 0.0f ? 0u : __float_as_uint(v);  // -0 sorts as +0
-            // order-preserving map of float bits to unsigned (NaN last)
-            hi = v != v ? 0xFFFFFFFEu : ((b & 0x80000000u) ? ~b : (b | 0x80000000u));
+    if (A.k <= 8) {
+        __shared__ unsigned long long s_wmin[WEFT_MAX_BLOCK / 64];
+        __shared__ unsigned long long s_last;
+        unsigned long long last = 0ull;
+        for (int r = 0; r < A.k; ++r) {
+            unsigned long long m = ~0ull;
+            for (int i = tid; i < n; i += nthr) {
+                const unsigned long long kk = topk_key(row[i], i);
+                if ((r == 0 || kk > last) && kk < m) m = kk;
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                const unsigned long long o = __shfl_xor(m, off, 64);
+                m = o < m ? o : m;
+            }
+            if ((tid & 63) == 0) s_wmin[tid >> 6] = m;
+            __syncthreads();
+            if (tid == 0) {
+                unsigned long long mm = s_wmin[0];
+                for (int w = 1; w < (nthr >> 6); ++w) mm = s_wmin[w] < mm ? s_wmin[w] : mm;
+                A.out_idx[(int64_t)t * A.k + r] = (int32_t)(mm & 0xFFFFFFFFull);
+                s_last = mm;
+            }
+            __syncthreads();
+            last = s_last;
         }
-        key[i] = ((unsigned long long)hi << 32) | (unsigned int)i;
+        return;
     }
+    for (int i = tid; i < N; i += nthr) key[i] = i < n ? topk_key(row[i], i) : ~0ull;
     __syncthreads();
     for (int size = 2; size <= N; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {

@@ -272,16 +272,23 @@ int weft_topk(const WeftFitState* st, const float* d_table, int n, int k, int32_
  * onto the code ranges.  Outputs per tile k + 1 slots: slot 0 the best drawn
  * candidate unchanged, slot j + 1 candidate j refined and quantized to a code, as
  * codes and as records (exactly as the scoring kernels decode codes). */
+#define WEFT_WORK_MAX_TILES 32  /* tile slots per work item (weft_refine, weft_fit_score_split) */
 typedef struct WeftRefineOpts {
     float lr[9];
     float beta1, one_minus_beta1, beta2, one_minus_beta2, eps;
     int32_t k;       /* 1, 2, 4, ..., 64 */
     int32_t n_iter;  /* >= 0 */
     const float* d_sched;  /* [n_iter][4] */
-    /* Work items, one 64-lane workgroup each: candidates d_work_j0[i] ..
-     * d_work_j0[i] + cpw - 1 of tile d_work_tile[i], cpw = cpw_by_size[log2(ts) - 3]
-     * (a power of two dividing k); the 64 / cpw lanes of a candidate split its pixels.
-     * Every (tile, candidate) must be covered once; tiles whose stack is full are skipped. */
+    /* Work items, one 64-lane workgroup each, holding cpw = cpw_by_size[log2(ts) - 3]
+     * candidates (a power of two that divides k or is a multiple of it); the 64 / cpw
+     * lanes of a candidate split its pixels, which fixes the order of the soft sums, so
+     * cpw is part of the refinement's arithmetic.  cpw <= k: candidates d_work_j0[i] ..
+     * + cpw - 1 of tile d_work_tile[i * WEFT_WORK_MAX_TILES]; cpw > k: all k candidates of
+     * each of the cpw / k (<= WEFT_WORK_MAX_TILES) tiles d_work_tile[i *
+     * WEFT_WORK_MAX_TILES + g] (-1 for an unused slot; slot 0 is always a tile).  Every
+     * (tile, candidate) must be covered once; tiles whose stack is full are skipped.
+     * weft_batch_refine_fit reads only cpw_by_size (one tile at a time, 64 / cpw lanes per
+     * candidate, idle lanes when cpw > k). */
     const int32_t* d_work_tile;
     const int32_t* d_work_j0;
     int32_t n_work;
@@ -307,18 +314,17 @@ int weft_batch_refine_fit(const WeftFitState* st, const WeftCands* cands, const 
 /* Exact scoring (the same values as weft_fit_score) of a few candidates per tile, for
  * refined candidates (no reference counterpart; the candidates' SSE follows NumPy's
  * pairwise float32 order exactly, see weft_score_evals).  Work item i (one 64-lane
- * workgroup) holds tpw tiles, d_work_tile[i * WEFT_SPLIT_MAX_TILES + g] for g < tpw (-1
+ * workgroup) holds tpw tiles, d_work_tile[i * WEFT_WORK_MAX_TILES + g] for g < tpw (-1
  * for an unused slot; slot 0 is always a tile), and scores candidates d_work_c0[i] ..
  * + cpw - 1 of each (those below cands->n_cand), with cpw = cpw_by_size[log2(ts) - 3] and
  * tpw = tpw_by_size[log2(ts) - 3]: each candidate's pixels are split over
  * 64 / (cpw * tpw) lanes along the pairwise tree, which must be a power of two at most
  * 2, 8, 32, 32 for tiles of 8, 16, 32, 64 px (several small tiles share a workgroup).
  * Outputs as weft_fit_score: the first minimum of the objective of tile slot g of item i
- * at d_part_obj / d_part_idx[i * WEFT_SPLIT_MAX_TILES + g] (for weft_fit_select), and
+ * at d_part_obj / d_part_idx[i * WEFT_WORK_MAX_TILES + g] (for weft_fit_select), and
  * optional SSE / MSE tables. */
-#define WEFT_SPLIT_MAX_TILES 32
 typedef struct WeftSplitWork {
-    const int32_t* d_work_tile;  /* [n_work * WEFT_SPLIT_MAX_TILES] */
+    const int32_t* d_work_tile;  /* [n_work * WEFT_WORK_MAX_TILES] */
     const int32_t* d_work_c0;    /* [n_work] */
     int32_t n_work;
     int32_t cpw_by_size[4];

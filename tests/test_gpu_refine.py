@@ -176,24 +176,33 @@ def test_topk_matches_stable_sort(gpu):
 
 
 def _mirror_grads(f, tiles, kinds, prm, k, cpw, w):
-    """Kernel gradients with the refinement's lane split: per tile size, groups of
-    cpw[ts] candidates of one tile (64 / cpw[ts] lanes each)."""
+    """Kernel gradients with the refinement's lane split: per tile size, 64 / cpw[ts] lanes
+    per candidate (groups of cpw[ts] candidates of one tile; when cpw[ts] > k, a tile's k
+    candidates padded with copies of themselves)."""
     import torch
     from weft_amd.refine import soft_eval
 
     g = np.zeros_like(prm)
     for ts in sorted(set(tiles.ts.tolist())):
-        c = min(cpw[ts], k)
+        c = cpw[ts]
         ids = np.nonzero(tiles.ts == ts)[0]
-        rows = (ids[:, None] * k + np.arange(k)[None, :]).ravel()
-        gt = torch.from_numpy(np.repeat(ids, k // c).astype(np.int32)).cuda()
-        _, gg = soft_eval(f, gt, torch.from_numpy(kinds[rows]).cuda(), torch.from_numpy(prm[rows]).cuda(), c, w)
-        g[rows] = gg.cpu().numpy()
+        rows = ids[:, None] * k + np.arange(k)[None, :]
+        if c <= k:
+            gt = torch.from_numpy(np.repeat(ids, k // c).astype(np.int32)).cuda()
+            r = rows.ravel()
+            _, gg = soft_eval(f, gt, torch.from_numpy(kinds[r]).cuda(), torch.from_numpy(prm[r]).cuda(), c, w)
+            g[r] = gg.cpu().numpy()
+        else:
+            r = np.tile(rows, (1, c // k)).ravel()
+            gt = torch.from_numpy(ids.astype(np.int32)).cuda()
+            _, gg = soft_eval(f, gt, torch.from_numpy(kinds[r]).cuda(), torch.from_numpy(prm[r]).cuda(), c, w)
+            g[rows.ravel()] = gg.cpu().numpy().reshape(len(ids), c, 9)[:, :k].reshape(-1, 9)
     return g
 
 
 @pytest.mark.parametrize("k,n_iter,cpw", [(8, 0, None), (8, 3, None), (64, 2, None), (2, 5, None),
-                                          (16, 3, {8: 16, 16: 2, 32: 1, 64: 4}), (64, 2, {8: 64, 16: 64, 32: 64, 64: 64})])
+                                          (16, 3, {8: 16, 16: 2, 32: 1, 64: 4}), (64, 2, {8: 64, 16: 64, 32: 64, 64: 64}),
+                                          (4, 3, {8: 64, 16: 16, 32: 8, 64: 2}), (1, 2, {8: 32, 16: 4, 32: 1, 64: 2})])
 def test_refine_matches_kernel_gradient_and_cpu_adam(gpu, oracle_lib, k, n_iter, cpw):
     import torch
     from oracle import soft_np
@@ -211,7 +220,7 @@ def test_refine_matches_kernel_gradient_and_cpu_adam(gpu, oracle_lib, k, n_iter,
     cpw = cpw or DEFAULT_CPW
     wt, wj, cpwl = refine_work(tiles.ts, k, cpw)
     work = (torch.from_numpy(wt).cuda(), torch.from_numpy(wj).cuda(), cpwl)
-    _say(f"refine k {k} n_iter {n_iter} cpw {cpwl}, {len(wt)} work items")
+    _say(f"refine k {k} n_iter {n_iter} cpw {cpwl}, {len(wj)} work items")
     codes, prims = refine(f, _lib.Cands(None, None, seed, step, n_cand, 0, 0), torch.from_numpy(tk).cuda(), k, cfg,
                           out_params=(pout := torch.zeros((T * k, 9), dtype=torch.float32, device="cuda")),
                           work=work)

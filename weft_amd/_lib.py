@@ -82,7 +82,7 @@ class RefineOpts(ctypes.Structure):
     ]
 
 
-SPLIT_MAX_TILES = 32
+WORK_MAX_TILES = 32
 
 
 class SplitWork(ctypes.Structure):

@@ -152,7 +152,7 @@ class BatchFitter:
         """Refined candidates per wave by tile size (as RefinedFitter.set_cpw)."""
         from .refine import DEFAULT_CPW
 
-        c = {ts: min(v, self.k) for ts, v in {**DEFAULT_CPW, **cpw}.items()}
+        c = {**DEFAULT_CPW, **cpw}
         o = self.adam.opts(self.k, self.d_sched)
         for i, ts in enumerate((8, 16, 32, 64)):
             o.cpw_by_size[i] = int(c[ts])

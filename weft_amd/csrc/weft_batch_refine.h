@@ -21,8 +21,10 @@ struct RefineShared {
     WeftCode codes[65];
 };
 
-// Adam on the tile's top k (cpw candidates per wave, 64 / cpw lanes each); writes
-// rs.codes[1 + j] and the unrefined best as rs.codes[0].
+// Adam on the tile's top k (64 / cpw lanes per candidate, min(cpw, k) candidates per
+// wave; with cpw > k the other lanes idle, so the soft sums run in the order of the
+// stepwise refinement's shared workgroups); writes rs.codes[1 + j] and the unrefined best
+// as rs.codes[0].
 template <int TS>
 __device__ __forceinline__ void refine_phase(const WeftBatchRefineArgs& R, uint64_t dbase, const float* T,
                                           const float* cvp, RefineShared& rs) {
@@ -30,11 +32,14 @@ __device__ __forceinline__ void refine_phase(const WeftBatchRefineArgs& R, uint6
     const int tid = threadIdx.x, wave = tid >> 6, nwaves = blockDim.x >> 6, lane = tid & 63;
     const int k = R.o.k;
     const int cpw = R.o.cpw_by_size[CLS];
+    const int cpt = cpw < k ? cpw : k;
     const int Ln = 64 / cpw;
-    const int groups = k / cpw;
+    const int groups = k / cpt;
 #pragma unroll 1
     for (int g = wave; g < groups; g += nwaves) {
-        const int j = g * cpw + lane / Ln, slice = lane - (lane / Ln) * Ln;
+        const int q = lane / Ln, slice = lane - q * Ln;
+        if (q >= cpt) continue;  // idle lanes (whole candidate groups)
+        const int j = g * cpt + q;
         SoftCand C = soft_from_code(draw_code(dbase, (uint32_t)rs.topk[j], TS));
         refine_candidate<WEFT_BATCH_REFINE_UNROLL>(R.o, T, cvp, TS, slice, Ln, C);
         if (slice == 0) rs.codes[1 + j] = soft_to_code(C, TS);

@@ -388,6 +388,12 @@ int weft_batch_fit(const WeftFitState* st, const WeftCands* cands, const WeftBat
 
 static bool pow2_le64(int k) { return k >= 1 && k <= 64 && (k & (k - 1)) == 0; }
 
+// Candidates per refinement workgroup: a power of two <= 64 dividing k (one tile), or a
+// multiple of k by at most WEFT_WORK_MAX_TILES (that many tiles).
+static bool refine_cpw_ok(int cpw, int k) {
+    return pow2_le64(cpw) && k >= 1 && (k % cpw == 0 || (cpw % k == 0 && cpw / k <= WEFT_WORK_MAX_TILES));
+}
+
 int weft_soft_eval(const WeftFitState* st, const WeftSoftBatch* b, void* stream) {
     int rc = require_init();
     if (rc) return rc;
@@ -424,8 +430,9 @@ int weft_refine(const WeftFitState* st, const WeftCands* cands, const int32_t* d
         return set_err(WEFT_E_INVALID, "weft_refine: k must be a power of two <= min(64, n_cand)");
     if (o->n_iter < 0 || (o->n_iter > 0 && !o->d_sched)) return set_err(WEFT_E_INVALID, "weft_refine: bad schedule");
     for (int c = 0; c < 4; ++c)
-        if (!pow2_le64(o->cpw_by_size[c]) || o->k % o->cpw_by_size[c])
-            return set_err(WEFT_E_INVALID, "weft_refine: cpw must be powers of two dividing k");
+        if (!refine_cpw_ok(o->cpw_by_size[c], o->k))
+            return set_err(WEFT_E_INVALID, "weft_refine: cpw must be powers of two that divide k or are "
+                                           "multiples of it by at most WEFT_WORK_MAX_TILES");
     if (o->n_work < 0 || (o->n_work > 0 && (!o->d_work_tile || !o->d_work_j0)))
         return set_err(WEFT_E_INVALID, "weft_refine: bad work list");
     if (o->n_work == 0) return WEFT_OK;
@@ -444,7 +451,7 @@ int weft_fit_score_split(const WeftFitState* st, const WeftCands* cands, const W
     const int max_lanes[4] = {2, 8, 32, 32};
     for (int c = 0; c < 4; ++c) {
         const int cpw = w->cpw_by_size[c], tpw = w->tpw_by_size[c];
-        if (!pow2_le64(cpw) || !pow2_le64(tpw) || tpw > WEFT_SPLIT_MAX_TILES || cpw * tpw > 64 ||
+        if (!pow2_le64(cpw) || !pow2_le64(tpw) || tpw > WEFT_WORK_MAX_TILES || cpw * tpw > 64 ||
             64 / (cpw * tpw) > max_lanes[c])
             return set_err(WEFT_E_INVALID, "weft_fit_score_split: lanes per candidate, 64 / (cpw * tpw), must be "
                                            "a power of two <= 2, 8, 32, 32 for 8, 16, 32, 64-px tiles");
@@ -516,8 +523,9 @@ int weft_batch_refine_fit(const WeftFitState* st, const WeftCands* cands, const 
     if (o->n_iter < 0 || (o->n_iter > 0 && !o->d_sched))
         return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: bad schedule");
     for (int c = 0; c < 4; ++c)
-        if (!pow2_le64(o->cpw_by_size[c]) || o->k % o->cpw_by_size[c])
-            return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: cpw must be powers of two dividing k");
+        if (!refine_cpw_ok(o->cpw_by_size[c], o->k))
+            return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: cpw must be powers of two that divide k or are "
+                                           "multiples of it by at most WEFT_WORK_MAX_TILES");
     WeftBatchRefineArgs a{};
     a.b = WeftBatchArgs{*st, *cands, b->d_tile_order, b->d_tile_image, b->d_image_tiles, b->d_image_left,
                         reinterpret_cast<unsigned long long*>(b->d_image_done),

@@ -473,22 +473,25 @@ __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const 
 }
 
 // Adam on the soft SSE of each tile's top-k drawn candidates; one 64-lane workgroup
-// per work item (cpw candidates of one tile, 64 / cpw lanes each).  Every lane of a
-// candidate holds the same parameters and moments (the butterfly sums give all of them
-// the same totals).
+// per work item: cpw candidates, 64 / cpw lanes each, of one tile (cpw <= k) or of
+// cpw / k tiles (cpw > k, all k candidates of each).  Every lane of a candidate holds the
+// same parameters and moments (the butterfly sums give all of them the same totals).
 __device__ void refine_body(const WeftRefineArgs& A) {
     const int item = blockIdx.x;
     const WeftFitState& st = A.st;
-    const int t = A.o.d_work_tile[item];
-    if (st.d_tile_nsel[t] >= st.depth) return;
     const int k = A.o.k;
-    const int ts = st.d_tile_ts[t];
+    const int ts = st.d_tile_ts[A.o.d_work_tile[(int64_t)item * WEFT_WORK_MAX_TILES]];
     const int cls = ts == 8 ? 0 : (ts == 16 ? 1 : (ts == 32 ? 2 : 3));
     const int cpw = A.o.cpw_by_size[cls];
+    const int cpt = cpw < k ? cpw : k;  // candidates per tile in this item
     const int L = 64 / cpw;
     const int lane = threadIdx.x;
+    const int q = lane / L, slice = lane - q * L;
+    const int g = q / cpt;  // tile slot: a candidate's lanes share it, so they leave together
+    const int t = A.o.d_work_tile[(int64_t)item * WEFT_WORK_MAX_TILES + g];
+    if (t < 0 || st.d_tile_nsel[t] >= st.depth) return;
     const int j0 = A.o.d_work_j0[item];
-    const int j = j0 + lane / L, slice = lane - (lane / L) * L;
+    const int j = j0 + (q - g * cpt);
     const int64_t base = st.d_tile_pix[t] * 3;
     const float* tg = st.d_target + base;
     const float* cv = st.d_canvas + base;
@@ -496,7 +499,7 @@ __device__ void refine_body(const WeftRefineArgs& A) {
     const int64_t slot0 = (int64_t)t * (k + 1);
     const int idx = A.topk[(int64_t)t * k + j];
     const WeftCode code0 = draw_code(dbase, (uint32_t)idx, ts);
-    if (j0 == 0 && lane == 0) {
+    if (j == 0 && slice == 0) {
         // slot 0: the best drawn candidate (top-1) as drawn
         const WeftCode best = draw_code(dbase, (uint32_t)A.topk[(int64_t)t * k], ts);
         if (A.out_codes) A.out_codes[slot0] = best;
@@ -505,9 +508,9 @@ __device__ void refine_body(const WeftRefineArgs& A) {
     SoftCand C = soft_from_code(code0);
     refine_candidate(A.o, tg, cv, ts, slice, L, C);
     if (slice == 0) {
-        const WeftCode q = soft_to_code(C, ts);
-        if (A.out_codes) A.out_codes[slot0 + 1 + j] = q;
-        A.out_prims[slot0 + 1 + j] = code_to_prim(q);
+        const WeftCode qc = soft_to_code(C, ts);
+        if (A.out_codes) A.out_codes[slot0 + 1 + j] = qc;
+        A.out_prims[slot0 + 1 + j] = code_to_prim(qc);
         if (A.out_params) {
 #pragma unroll
             for (int i = 0; i < SOFT_NP; ++i) A.out_params[((int64_t)t * k + j) * SOFT_NP + i] = C.p[i];

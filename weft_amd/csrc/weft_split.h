@@ -185,9 +185,9 @@ __device__ void split_score_item(const WeftSplitScoreArgs& A, int item, int cpw,
     const int lane = threadIdx.x;
     const int g = lane / G, gl = lane - g * G;
     const int jl = gl / L, sl = gl - jl * L;
-    const int64_t slot = (int64_t)item * WEFT_SPLIT_MAX_TILES + g;
+    const int64_t slot = (int64_t)item * WEFT_WORK_MAX_TILES + g;
     const int t = A.work_tile[slot];
-    const int tt = t >= 0 ? t : A.work_tile[(int64_t)item * WEFT_SPLIT_MAX_TILES];  // pixels read by an unused slot
+    const int tt = t >= 0 ? t : A.work_tile[(int64_t)item * WEFT_WORK_MAX_TILES];  // pixels read by an unused slot
     const bool tile_live = t >= 0 && st.d_tile_nsel[t] < st.depth;
     const int n = A.cd.n_cand;
     const int c = A.work_c0[item] + jl;
@@ -232,7 +232,7 @@ __device__ void split_score_item(const WeftSplitScoreArgs& A, int item, int cpw,
 
 __device__ void split_score_body(const WeftSplitScoreArgs& A) {
     const int item = blockIdx.x;
-    const int t0 = A.work_tile[(int64_t)item * WEFT_SPLIT_MAX_TILES];
+    const int t0 = A.work_tile[(int64_t)item * WEFT_WORK_MAX_TILES];
     switch (A.st.d_tile_ts[t0]) {
     case 8: split_score_item<8>(A, item, A.cpw[0], A.tpw[0]); break;
     case 16: split_score_item<16>(A, item, A.cpw[1], A.tpw[1]); break;

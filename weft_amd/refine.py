@@ -146,33 +146,54 @@ class AdamConfig:
 
 
 # Candidates per 64-lane work item by tile size (8, 16, 32, 64 px): small tiles put
-# several candidates in one wave, large tiles spread one candidate's pixels over it.
+# several candidates (of one tile, or of several tiles when this exceeds k) in one wave,
+# large tiles spread one candidate's pixels over it.  The 64 / cpw lanes of a candidate
+# fix the order of its soft sums, so this is part of the refinement's arithmetic.
 DEFAULT_CPW = {8: 8, 16: 4, 32: 2, 64: 1}
 XCDS = 8
 
 
 def refine_work(tile_ts, k: int, cpw: dict[int, int] | None = None, tiles=None):
-    """Work items (tile, first candidate) of weft_refine for the tiles `tiles` (default
-    all), large tiles first; the items of a group of XCDS tiles are interleaved so that
-    a tile's items are XCDS apart, run on one XCD (workgroup i goes to XCD i mod 8) and
-    share its L2."""
-    cpw = {ts: min(v, k) for ts, v in {**DEFAULT_CPW, **(cpw or {})}.items()}
+    """Work items of weft_refine for the tiles `tiles` (default all), large tiles first:
+    (work_tile [n_work * WORK_MAX_TILES], work_j0 [n_work], cpw list).  With cpw <= k an
+    item holds candidates j0 .. j0 + cpw - 1 of one tile, and the items of a group of XCDS
+    tiles are interleaved so that a tile's items are XCDS apart, run on one XCD (workgroup
+    i goes to XCD i mod 8) and share its L2; with cpw > k an item holds all k candidates
+    of cpw / k tiles."""
+    M = _lib.WORK_MAX_TILES
+    cpw = {**DEFAULT_CPW, **(cpw or {})}
+    for ts, c in cpw.items():
+        if not (c & (c - 1) == 0 and 1 <= c <= 64 and (k % c == 0 or (c % k == 0 and c // k <= M))):
+            raise ValueError(f"cpw[{ts}] = {c}: a power of two <= 64 dividing k = {k} or a multiple of it")
     tile_ts = np.asarray(tile_ts)
     ids = np.arange(len(tile_ts)) if tiles is None else np.asarray(tiles)
     ts_ids = tile_ts[ids]
     order = ids[np.argsort(-ts_ids, kind="stable")]
-    per = np.array([k // cpw[int(v)] for v in tile_ts[order]], np.int64)
-    cp = np.array([cpw[int(v)] for v in tile_ts[order]], np.int64)
     wt, wj = [], []
-    for g0 in range(0, len(order), XCDS):
-        grp, pg, cg = order[g0:g0 + XCDS], per[g0:g0 + XCDS], cp[g0:g0 + XCDS]
-        n = int(pg.max())
-        r = np.arange(n)[:, None]
-        live = r < pg[None, :]
-        wt.append(np.broadcast_to(grp[None, :], live.shape)[live])
-        wj.append((r * cg[None, :])[live])
-    cat = (lambda a: np.concatenate(a).astype(np.int32)) if wt else (lambda a: np.zeros(0, np.int32))
-    return cat(wt), cat(wj), [cpw[8], cpw[16], cpw[32], cpw[64]]
+
+    def item(tl, j0):
+        row = np.full(M, -1, np.int32)
+        row[:len(tl)] = tl
+        wt.append(row)
+        wj.append(j0)
+
+    sizes = tile_ts[order]
+    for ts in (64, 32, 16, 8):
+        sel = order[sizes == ts]
+        c = cpw[ts]
+        if c > k:
+            g = c // k
+            for i0 in range(0, len(sel), g):
+                item(sel[i0:i0 + g], 0)
+            continue
+        per = k // c
+        for g0 in range(0, len(sel), XCDS):
+            grp = sel[g0:g0 + XCDS]
+            for r in range(per):
+                for t in grp:
+                    item([t], r * c)
+    cat = np.concatenate(wt).astype(np.int32) if wt else np.zeros(0, np.int32)
+    return cat, np.array(wj, np.int32), [cpw[8], cpw[16], cpw[32], cpw[64]]
 
 
 # Candidates per work item of the lane-split exact scorer, by tile size: 64 / cpw lanes
@@ -191,15 +212,15 @@ def _pow2_at_least(v: int) -> int:
 def split_work(tile_ts, n: int, cpw: dict[int, int] | None = None):
     """Work items of weft_fit_score_split for n candidates per tile.
 
-    Returns (work_tile [n_work * SPLIT_MAX_TILES], work_c0 [n_work], work0, nwork, stride,
+    Returns (work_tile [n_work * WORK_MAX_TILES], work_c0 [n_work], work0, nwork, stride,
     cpw list, tpw list): item i scores candidates work_c0[i] .. + cpw - 1 of the tiles in
     its slots; tile t's first minima are partials work0[t] + j * stride[t], j < nwork[t]
-    (partial arrays hold n_work * SPLIT_MAX_TILES entries).  By default, when a tile's n
+    (partial arrays hold n_work * WORK_MAX_TILES entries).  By default, when a tile's n
     candidates fit in 64 / tpw lanes at the size's lane limit, several tiles (tpw) share a
     workgroup; otherwise a tile's candidates are spread over items of cpw (SPLIT_CPW),
     and a group of XCDS same-size tiles has its items interleaved (a tile's items are XCDS
     apart).  An explicit cpw keeps one tile per item."""
-    M = _lib.SPLIT_MAX_TILES
+    M = _lib.WORK_MAX_TILES
     tile_ts = np.asarray(tile_ts)
     T = len(tile_ts)
     cpw_d, tpw_d = dict(SPLIT_CPW), {8: 1, 16: 1, 32: 1, 64: 1}
@@ -255,7 +276,9 @@ def autotune_cpw(fitter, cands: _lib.Cands, topk_idx, k: int, cfg: AdamConfig,
     """Candidates per work item for each tile size, the fastest on this workload.
 
     Each size class is timed alone (HIP events around weft_refine over that class's
-    tiles) for every option that divides k; results do not depend on the choice."""
+    tiles) for every valid option.  The choice sets the lanes per candidate and with them
+    the order of the soft sums, so a fit's refined candidates follow it; the choice itself
+    depends on timing, so pass a fixed cpw where runs must repeat bit for bit."""
     torch = fitter.torch
     stream = torch.cuda.current_stream()
     best = {}
@@ -263,7 +286,7 @@ def autotune_cpw(fitter, cands: _lib.Cands, topk_idx, k: int, cfg: AdamConfig,
         ids = np.nonzero(np.asarray(fitter.tiles.ts) == ts)[0]
         times = {}
         for c in options:
-            if c > k or k % c:
+            if not (k % c == 0 or (c % k == 0 and c // k <= _lib.WORK_MAX_TILES)):
                 continue
             wt, wj, cl = refine_work(fitter.tiles.ts, k, {ts: c}, tiles=ids)
             work = (torch.from_numpy(wt).to(fitter.dev), torch.from_numpy(wj).to(fitter.dev), cl)
@@ -303,7 +326,7 @@ def refine(fitter, cands: _lib.Cands, topk_idx, k: int, cfg: AdamConfig, out_cod
     o = cfg.opts(k, d_sched)
     o.d_work_tile = work[0].data_ptr()
     o.d_work_j0 = work[1].data_ptr()
-    o.n_work = int(work[0].numel())
+    o.n_work = int(work[1].numel())
     for i, v in enumerate(work[2]):
         o.cpw_by_size[i] = int(v)
     sp = _lib.stream_ptr() if stream is None else stream
@@ -347,7 +370,7 @@ class RefinedFitter(PlainFitter):
         self.n_rwork = len(sw[1])
         self._split = _lib.SplitWork(self._split_arrays[0].data_ptr(), self._split_arrays[1].data_ptr(),
                                      self.n_rwork, (ctypes.c_int32 * 4)(*sw[5]), (ctypes.c_int32 * 4)(*sw[6]), 0)
-        n_part = max(1, self.n_rwork * _lib.SPLIT_MAX_TILES)
+        n_part = max(1, self.n_rwork * _lib.WORK_MAX_TILES)
         self.d_rpart_obj = torch.empty(n_part, dtype=torch.float64, device=dev)
         self.d_rpart_idx = torch.empty(n_part, dtype=torch.int32, device=dev)
         S = self.max_steps
@@ -363,7 +386,8 @@ class RefinedFitter(PlainFitter):
 
     def autotune(self) -> dict[int, int]:
         """Time the refinement's work split per tile size on this round's candidates and
-        keep the fastest (the fit's results do not depend on it)."""
+        keep the fastest.  The split is part of the refinement's arithmetic (lanes per
+        candidate order the soft sums), so the fit's refined candidates follow the choice."""
         s = self.step_idx
         self.score(s)
         topk(self, self.d_mse, self.n_cand, self.k, self.d_topk)

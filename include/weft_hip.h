@@ -167,11 +167,14 @@ int weft_fit_init(const WeftFitState* st, void* stream);
 
 /* Plain fit, part 2a: per tile, the candidate indices sorted (stably) by coverage class
  * (ellipse, rectangle, triangle, constant, generic), so that each wavefront of the
- * scoring kernel runs one class.  d_order: n_tiles * n_cand int32. */
+ * scoring kernel runs one class.  d_order: n_tiles * n_cand int32.  Needed for explicit
+ * candidates only: drawn candidates come in class order by construction (families in
+ * aligned groups of 64), and weft_fit_score forms their order itself. */
 int weft_fit_order(const WeftFitState* st, const WeftCands* cands, int32_t* d_order, void* stream);
 
 /* Plain fit, part 2b: score candidates of the tiles in `work` against their cached
- * canvases, taking slots [cand0, cand0 + block) of each tile's d_order.  Per candidate k
+ * canvases, taking slots [cand0, cand0 + block) of each tile's class order (d_order for
+ * explicit candidates; for drawn ones d_order is not read).  Per candidate k
  * of tile t (index t * n_cand + k) writes SSE / MSE when the tables are given, and per
  * work item the first minimum of the objective into d_part_obj / d_part_idx.  Tiles
  * whose stack is full are skipped. */

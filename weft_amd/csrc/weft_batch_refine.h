@@ -78,7 +78,7 @@ __device__ __forceinline__ void rescore_phase(const WeftBatchRefineArgs& R, int 
 // Inlined into the kernel on purpose: a call would pass the kernel arguments by address,
 // which copies the whole argument block into scratch memory.
 template <int TS>
-__device__ __forceinline__ void batch_refine_tile(const WeftBatchRefineArgs& R, int t, float4* LA, float* LT, int* perm,
+__device__ __forceinline__ void batch_refine_tile(const WeftBatchRefineArgs& R, int t, float4* LA, float* LT,
                                   unsigned long long* keys, BatchShared& sh, RefineShared& rs) {
     const WeftBatchArgs& A = R.b;
     constexpr int ROWS = Band<TS>::ROWS;
@@ -110,13 +110,12 @@ __device__ __forceinline__ void batch_refine_tile(const WeftBatchRefineArgs& R, 
         }
         const uint32_t step = A.step0 + (uint32_t)s;
         const uint64_t dbase = draw_base(A.cd.seed, step, (uint32_t)t);
-        batch_sort(A.cd, dbase, n, perm, sh);
         // ---- exact scores of the drawn candidates -> keys ----
         bool staged = false;
 #pragma unroll 1
         for (int c0 = 0; c0 < n; c0 += nthr) {
             const int slot = c0 + tid;
-            const int ks = slot < n ? perm[slot] : -1;
+            const int ks = slot < n ? drawn_sorted_index(dbase, n, slot) : -1;
             WeftPrim P;
             P.kind = -1;
             P.alpha = 0.0f;
@@ -239,7 +238,6 @@ __device__ __forceinline__ void batch_refine_body(const WeftBatchRefineArgs& R) 
     float4* LA = weft_smem;                                             // [2 * 512]
     float* LT = reinterpret_cast<float*>(LA + 2 * 512);                 // [512]
     unsigned long long* keys = reinterpret_cast<unsigned long long*>(LT + 512);  // [n_pow2]
-    int* perm = reinterpret_cast<int*>(keys + R.n_pow2);                // [n_cand]
     const int tid = threadIdx.x;
     if (tid == 0 && A.t_start) atomicMin(A.t_start, (unsigned long long)__builtin_amdgcn_s_memrealtime());
 #pragma unroll 1
@@ -253,10 +251,10 @@ __device__ __forceinline__ void batch_refine_body(const WeftBatchRefineArgs& R) 
         __syncthreads();
         if (t < 0) break;
         switch (A.st.d_tile_ts[t]) {
-        case 8: batch_refine_tile<8>(R, t, LA, LT, perm, keys, sh, rs); break;
-        case 16: batch_refine_tile<16>(R, t, LA, LT, perm, keys, sh, rs); break;
-        case 32: batch_refine_tile<32>(R, t, LA, LT, perm, keys, sh, rs); break;
-        case 64: batch_refine_tile<64>(R, t, LA, LT, perm, keys, sh, rs); break;
+        case 8: batch_refine_tile<8>(R, t, LA, LT, keys, sh, rs); break;
+        case 16: batch_refine_tile<16>(R, t, LA, LT, keys, sh, rs); break;
+        case 32: batch_refine_tile<32>(R, t, LA, LT, keys, sh, rs); break;
+        case 64: batch_refine_tile<64>(R, t, LA, LT, keys, sh, rs); break;
         default: break;
         }
         if (tid == 0 && A.image_left) {

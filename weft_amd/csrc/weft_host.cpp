@@ -382,7 +382,7 @@ int weft_batch_fit(const WeftFitState* st, const WeftCands* cands, const WeftBat
     const int reset_n = b->n_images > 1 ? b->n_images : 1;
     rc = launch(g_mod.f_batch_reset, (unsigned)((reset_n + 255) / 256), 256, 0, stream, &a);
     if (rc) return rc;
-    const unsigned lds = (unsigned)(36 * 512 + 4 * cands->n_cand);
+    const unsigned lds = (unsigned)(36 * 512);
     return launch(g_mod.f_batch, (unsigned)b->grid, (unsigned)b->block, lds, stream, &a);
 }
 
@@ -549,7 +549,7 @@ int weft_batch_refine_fit(const WeftFitState* st, const WeftCands* cands, const 
     const int reset_n = b->n_images > 1 ? b->n_images : 1;
     rc = launch(g_mod.f_batch_reset, (unsigned)((reset_n + 255) / 256), 256, 0, stream, &a.b);
     if (rc) return rc;
-    const unsigned lds = (unsigned)(36 * 512 + 8 * np2 + 4 * cands->n_cand);
+    const unsigned lds = (unsigned)(36 * 512 + 8 * np2);
     return launch(g_mod.f_batch_refine, (unsigned)b->grid, (unsigned)b->block, lds, stream, &a);
 }
 

@@ -662,9 +662,36 @@ __device__ __forceinline__ uint64_t draw_base(uint64_t seed, uint32_t step, uint
 // once the scorers have sorted a tile's candidates by class): group g of a tile takes
 // family (r + g) mod 3, r drawn from the tile's base hash, so every family gets a third of
 // the groups on average and no wavefront holds candidates of two families.
+__device__ __forceinline__ uint32_t draw_rot(uint64_t base) {
+    return (uint32_t)((((base >> 40) & 0xFFFFull) * 3ull) >> 16);
+}
+
 __device__ __forceinline__ int draw_family(uint64_t base, uint32_t idx) {
-    const uint32_t r = (uint32_t)((((base >> 40) & 0xFFFFull) * 3ull) >> 16);
-    return (int)((r + (idx >> 6)) % 3u);
+    return (int)((draw_rot(base) + (idx >> 6)) % 3u);
+}
+
+// Index at position `slot` (< n) of a tile's n drawn candidates stably sorted by class.
+// A drawn candidate's class is its family's (ellipse < rectangle < triangle; drawn radii
+// are at least 4 px, so none folds to the constant class), so the order lists family 0's
+// groups of 64, then family 1's, then 2's, each in index order; only the last group
+// (G - 1) can be partial, and it is the last of its family.
+__device__ __forceinline__ int drawn_sorted_index(uint64_t base, int n, int slot) {
+    const int r = (int)draw_rot(base);
+    const int G = (n + 63) >> 6;
+    int start = 0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const int g0 = (f - r + 3) % 3;
+        const int ng = g0 < G ? (G - 1 - g0) / 3 + 1 : 0;
+        int cnt = ng * 64;
+        if (ng > 0 && (G - 1 - g0) % 3 == 0) cnt -= 64 * G - n;
+        if (slot < start + cnt) {
+            const int local = slot - start;
+            return ((g0 + 3 * (local >> 6)) << 6) + (local & 63);
+        }
+        start += cnt;
+    }
+    return -1;
 }
 
 __device__ __forceinline__ WeftCode draw_code(uint64_t base, uint32_t idx, int ts) {
@@ -913,7 +940,9 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
                               : (A.cd.d_tile_cand ? A.cd.d_tile_cand[tile]
                                                   : (int64_t)tile * A.cd.n_cand);
     const int slot = c0 + tid;
-    const int ks = slot < A.cd.n_cand ? A.order[(int64_t)tile * A.cd.n_cand + slot] : -1;
+    const int ks = slot >= A.cd.n_cand ? -1
+                   : (drawn ? drawn_sorted_index(dbase, A.cd.n_cand, slot)
+                            : A.order[(int64_t)tile * A.cd.n_cand + slot]);
     WeftPrim P;
     P.kind = -1;
     P.alpha = 0.0f;
@@ -1063,16 +1092,12 @@ __device__ float stack_sse(const float* tgt, const WeftPrim* prims, int n) {
 
 // ---------------------------------------------------------------------------
 // Persistent batched fit: workgroups take tiles from a queue and run each tile's whole
-// greedy fit (n_steps rounds of: class-sort the tile's candidates, score them all,
-// first minimum, accept, composite) before taking the next tile.  Candidates are drawn
+// greedy fit (n_steps rounds of: score the tile's candidates in class order, first
+// minimum, accept, composite) before taking the next tile.  Candidates are drawn
 // from (seed, step0 + s, tile, index) exactly as the stepwise kernels draw them, so the
 // result equals that of weft_fit_order/score/select run n_steps times.
 // ---------------------------------------------------------------------------
 struct BatchShared {
-    int cnt[8];
-    int run[8];
-    int wcnt[16 * 8];
-    int woff[16 * 8];
     double red_o[16];
     int red_i[16];
     int tile;
@@ -1081,61 +1106,8 @@ struct BatchShared {
     double best_obj;
 };
 
-// Stable counting sort of the tile's n candidates by class into perm[0, n).
-__device__ void batch_sort(const WeftCands& cd, uint64_t dbase, int n, int* perm, BatchShared& sh) {
-    const int tid = threadIdx.x, nthr = blockDim.x, wave = tid >> 6, nwaves = nthr >> 6;
-    if (tid < 8) {
-        sh.cnt[tid] = 0;
-        sh.run[tid] = 0;
-    }
-    __syncthreads();
-    // class counts, one LDS atomic per wave and class (drawn families come in runs of 64)
-    for (int j0 = 0; j0 < n; j0 += nthr) {
-        const int k = j0 + tid;
-        const int cls = k < n ? candidate_class(cd, dbase, 0, k) : CL_NONE;
-#pragma unroll
-        for (int c = 0; c < WEFT_N_CLASSES; ++c) {
-            const uint64_t bal = __ballot(cls == c);
-            if ((tid & 63) == 0 && bal && c != CL_NONE) atomicAdd(&sh.cnt[c], (int)__popcll(bal));
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int off = 0;
-        for (int c = 0; c < WEFT_N_CLASSES; ++c) {
-            const int v = sh.cnt[c];
-            sh.cnt[c] = off;
-            off += v;
-        }
-    }
-    __syncthreads();
-    for (int j0 = 0; j0 < n; j0 += nthr) {
-        const int k = j0 + tid;
-        const int cls = k < n ? candidate_class(cd, dbase, 0, k) : CL_NONE;
-        uint64_t mine = 0;
-#pragma unroll
-        for (int c = 0; c < WEFT_N_CLASSES; ++c) {
-            const uint64_t bal = __ballot(cls == c);
-            if (cls == c) mine = bal;
-            if ((tid & 63) == 0) sh.wcnt[wave * 8 + c] = __popcll(bal);
-        }
-        __syncthreads();
-        if (tid < WEFT_N_CLASSES) {
-            int off = sh.cnt[tid] + sh.run[tid];
-            for (int ww = 0; ww < nwaves; ++ww) {
-                sh.woff[ww * 8 + tid] = off;
-                off += sh.wcnt[ww * 8 + tid];
-            }
-            sh.run[tid] = off - sh.cnt[tid];
-        }
-        __syncthreads();
-        if (k < n) perm[sh.woff[wave * 8 + cls] + __popcll(mine & lanemask_lt())] = k;
-        __syncthreads();
-    }
-}
-
 template <int TS>
-__device__ void batch_fit_tile(const WeftBatchArgs& A, int t, float4* LA, float* LT, int* perm,
+__device__ void batch_fit_tile(const WeftBatchArgs& A, int t, float4* LA, float* LT,
                                BatchShared& sh) {
     constexpr int ROWS = Band<TS>::ROWS;
     constexpr int NB = TS / ROWS;
@@ -1162,14 +1134,13 @@ __device__ void batch_fit_tile(const WeftBatchArgs& A, int t, float4* LA, float*
         }
         const uint32_t step = A.step0 + (uint32_t)s;
         const uint64_t dbase = draw_base(A.cd.seed, step, (uint32_t)t);
-        batch_sort(A.cd, dbase, n, perm, sh);
         double bo = __builtin_huge_val();
         int bi = 0x7fffffff;
         bool staged = false;
 #pragma unroll 1
         for (int c0 = 0; c0 < n; c0 += nthr) {
             const int slot = c0 + tid;
-            const int ks = slot < n ? perm[slot] : -1;
+            const int ks = slot < n ? drawn_sorted_index(dbase, n, slot) : -1;
             WeftPrim P;
             P.kind = -1;
             P.alpha = 0.0f;
@@ -1266,7 +1237,6 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
     __shared__ BatchShared sh;
     float4* LA = weft_smem;                                   // [2 * 512]
     float* LT = reinterpret_cast<float*>(LA + 2 * 512);       // [512]
-    int* perm = reinterpret_cast<int*>(LT + 512);             // [n_cand]
     const int tid = threadIdx.x;
     if (tid == 0 && A.t_start) {
         atomicMin(A.t_start, (unsigned long long)__builtin_amdgcn_s_memrealtime());
@@ -1282,10 +1252,10 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
         __syncthreads();  // every thread has read sh.tile before thread 0 may overwrite it
         if (t < 0) break;
         switch (A.st.d_tile_ts[t]) {
-        case 8: batch_fit_tile<8>(A, t, LA, LT, perm, sh); break;
-        case 16: batch_fit_tile<16>(A, t, LA, LT, perm, sh); break;
-        case 32: batch_fit_tile<32>(A, t, LA, LT, perm, sh); break;
-        case 64: batch_fit_tile<64>(A, t, LA, LT, perm, sh); break;
+        case 8: batch_fit_tile<8>(A, t, LA, LT, sh); break;
+        case 16: batch_fit_tile<16>(A, t, LA, LT, sh); break;
+        case 32: batch_fit_tile<32>(A, t, LA, LT, sh); break;
+        case 64: batch_fit_tile<64>(A, t, LA, LT, sh); break;
         default: break;
         }
         if (tid == 0 && A.image_left) {

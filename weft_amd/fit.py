@@ -136,7 +136,9 @@ class PlainFitter:
         self.d_tile_work0 = torch.from_numpy(tile_work0).to(dev)
         self.d_tile_nwork = torch.from_numpy(tile_nwork).to(dev)
         self.d_tile_wstride = torch.from_numpy(tile_wstride).to(dev)
-        self.d_order = torch.empty(T * self.n_cand, dtype=torch.int32, device=dev)
+        # class order of explicit candidates (drawn ones need none)
+        self.d_order = torch.empty(T * self.n_cand if self.cand_prims is not None else 1, dtype=torch.int32,
+                                   device=dev)
         self.d_part_obj = torch.empty(max(1, self.n_work), dtype=torch.float64, device=dev)
         self.d_part_idx = torch.empty(max(1, self.n_work), dtype=torch.int32, device=dev)
 
@@ -204,6 +206,9 @@ class PlainFitter:
             self.score_class(w, start, cands, sse, mse, sp)
 
     def order(self, step: int, stream: int) -> None:
+        """Class order of explicit candidates (drawn ones are in class order by construction)."""
+        if self.cand_prims is None:
+            return
         _lib.check(self.L.weft_fit_order(ctypes.byref(self._state), ctypes.byref(self._cands(step)),
                                          self.d_order.data_ptr(), stream), "weft_fit_order")
 

@@ -16,12 +16,14 @@ cpu_baseline = the CPU oracle (C, float32, OpenMP) on a bounded sample of the sa
 
 Further objects on the same line (skipped with --no-extras):
 - time_to_psnr: on the same image, the plain fit (1024 candidates, 16 rounds, one
-  persistent launch) against the refined fit (384 drawn candidates, top 2 refined by 10
+  persistent launch) against the refined fit (256 drawn candidates, top 2 refined by 12
   Adam steps, exact re-score), also one persistent launch: wall time until the refined
   fit reaches the plain fit's final PSNR, and both fits' PSNR at the end and at equal
   primitive bytes (the stepwise refined fit gives the PSNR after every round).
 - time_to_psnr_4k: the same at 3840x2160 (2500 primitives + value-noise texture),
-  depth 24, plain with 2048 candidates, refined with 512 drawn and the top 2 refined.
+  depth 24, plain with 2048 candidates, refined with 128 drawn and the best one refined
+  by 20 Adam steps.  (Both refined settings are the fastest to the plain PSNR in a sweep
+  of drawn count, top k and Adam steps.)
 - batched: 64 synthetic images (~88.8 Mpx) in one persistent hipGraph-captured launch,
   512 candidates, 16 rounds, autotuned queue order and workgroup size; throughput and
   per-image p50 / p99 latency.
@@ -745,7 +747,7 @@ def main():
     extras = {}
     if not args.no_extras:
         mask = valid_mask(tiles, [c["height"]], [c["width"]])
-        extras["time_to_psnr"], _ = time_to_psnr(tiles, mask, c["n_cand"], c["depth"], seed, lam, (384, 2, 10))
+        extras["time_to_psnr"], _ = time_to_psnr(tiles, mask, c["n_cand"], c["depth"], seed, lam, (256, 2, 12))
         t = extras["time_to_psnr"]
         log(f"[bench] time-to-PSNR 1080p: plain {t['plain']['psnr_db']:.2f} dB in {t['plain']['wall_s']:.4f} s; "
             f"speedup {t['speedup_to_plain_psnr']:.1f} ({t['fastest_refined_form']}; persistent "
@@ -757,7 +759,7 @@ def main():
         img4 = synth_image(2160, 3840, 2500, seed=0, texture=True)
         t4 = TileSet.from_images([img4], [decompose_quadtree_variance(img4, threshold=1e-3, min_tile=8, max_tile=64)])
         m4 = valid_mask(t4, [2160], [3840])
-        extras["time_to_psnr_4k"], rec4 = time_to_psnr(t4, m4, 2048, 24, seed, lam, (512, 2, 10))
+        extras["time_to_psnr_4k"], rec4 = time_to_psnr(t4, m4, 2048, 24, seed, lam, (128, 1, 20))
         extras["time_to_psnr_4k"]["tiles"] = int(t4.n_tiles)
         t = extras["time_to_psnr_4k"]
         log(f"[bench] time-to-PSNR 4k: plain {t['plain']['psnr_db']:.2f} dB in {t['plain']['wall_s']:.4f} s; "

@@ -41,7 +41,7 @@ __device__ __forceinline__ void refine_phase(const WeftBatchRefineArgs& R, uint6
         if (q >= cpt) continue;  // idle lanes (whole candidate groups)
         const int j = g * cpt + q;
         SoftCand C = soft_from_code(draw_code(dbase, (uint32_t)rs.topk[j], TS));
-        refine_candidate<WEFT_BATCH_REFINE_UNROLL>(R.o, T, cvp, TS, slice, Ln, C);
+        refine_candidate<WEFT_BATCH_REFINE_UNROLL, false>(R.o, T, cvp, TS, slice, Ln, C);
         if (slice == 0) rs.codes[1 + j] = soft_to_code(C, TS);
     }
     if (tid == 0) rs.codes[0] = draw_code(dbase, (uint32_t)rs.topk[0], TS);

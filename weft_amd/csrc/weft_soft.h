@@ -241,39 +241,97 @@ __device__ __forceinline__ void soft_finish(const SoftCand& C, const SoftPre& P,
     g[4] = gth;
 }
 
-// Sum of the lanes of one candidate (the L lanes i with i / L equal, L a power of two);
-// every lane ends with the same total.
-__device__ __forceinline__ float soft_lane_sum(float v, int L) {
-    for (int off = 1; off < L; off <<= 1) v += __shfl_xor(v, off, 64);
+// One level of the butterfly sum: v + (v of lane i ^ OFF), the levels taken in the order
+// OFF = 1, 2, 4, ...  After the levels below OFF, every lane of an aligned group of OFF
+// lanes holds the same value, so any partner in the other half of the aligned 2·OFF group
+// gives the same sum, bit for bit.  The quad permutations (OFF 1, 2), the row's half
+// mirror (i <-> 7 - i, OFF 4) and its mirror (i <-> 15 - i, OFF 8) are such partners and
+// run as DPP operand modifiers; OFF 16 and 32 go through the LDS crossbar.
+template <int OFF>
+__device__ __forceinline__ float bfly_add(float v) {
+    if constexpr (OFF <= 8) {
+        constexpr int ctrl = OFF == 1 ? 0xB1 : (OFF == 2 ? 0x4E : (OFF == 4 ? 0x141 : 0x140));
+        return v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, false));
+    } else {
+        return v + __shfl_xor(v, OFF, 64);
+    }
+}
+
+// v summed over its aligned group of L lanes (L a power of two, uniform over the wave) by
+// the butterfly of bfly_add; every lane of the group ends with the same total.
+__device__ __forceinline__ float lane_group_sum(float v, int L) {
+    if (L > 1) v = bfly_add<1>(v);
+    if (L > 2) v = bfly_add<2>(v);
+    if (L > 4) v = bfly_add<4>(v);
+    if (L > 8) v = bfly_add<8>(v);
+    if (L > 16) v = bfly_add<16>(v);
+    if (L > 32) v = bfly_add<32>(v);
     return v;
 }
 
-__device__ __forceinline__ void soft_acc_reduce(SoftAcc& A, int L) {
-    A.sse = soft_lane_sum(A.sse, L);
-    A.gc0 = soft_lane_sum(A.gc0, L);
-    A.gc1 = soft_lane_sum(A.gc1, L);
-    A.gc2 = soft_lane_sum(A.gc2, L);
-    A.ga = soft_lane_sum(A.ga, L);
+template <int OFF>
+__device__ __forceinline__ void soft_acc_level(SoftAcc& A) {
+    A.sse = bfly_add<OFF>(A.sse);
+    A.gc0 = bfly_add<OFF>(A.gc0);
+    A.gc1 = bfly_add<OFF>(A.gc1);
+    A.gc2 = bfly_add<OFF>(A.gc2);
+    A.ga = bfly_add<OFF>(A.ga);
 #pragma unroll
-    for (int i = 0; i < 12; ++i) A.q[i] = soft_lane_sum(A.q[i], L);
+    for (int i = 0; i < 12; ++i) A.q[i] = bfly_add<OFF>(A.q[i]);
+}
+
+// Sums over the lanes of one candidate (the L lanes i with i / L equal, L a power of two,
+// uniform over the wave); every lane ends with the same totals.  Level by level over all
+// seventeen sums, so the crossbar latencies of a level overlap.
+__device__ __forceinline__ void soft_acc_reduce(SoftAcc& A, int L) {
+    if (L > 1) soft_acc_level<1>(A);
+    if (L > 2) soft_acc_level<2>(A);
+    if (L > 4) soft_acc_level<4>(A);
+    if (L > 8) soft_acc_level<8>(A);
+    if (L > 16) soft_acc_level<16>(A);
+    if (L > 32) soft_acc_level<32>(A);
 }
 
 // SSE (and sums for the gradient) of candidate C over tile pixels slice, slice + L, ...
-// UNROLL pixels in flight per lane: memory-level parallelism against registers; the
-// sums are accumulated in the same order whatever its value.
-template <bool GRAD, int UNROLL = SOFT_UNROLL>
+// (ts a power of two, L a power of two <= ts * ts, so every lane takes ts * ts / L pixels).
+// With PREFETCH the next pixel's canvas and target are loaded before the current pixel is
+// computed, so a lane keeps one pixel's loads in flight behind its arithmetic (six more
+// registers; the persistent kernel, short of registers, goes without).  UNROLL pixels per
+// loop trip.  The sums are accumulated in the same order in every variant.
+template <bool GRAD, int UNROLL = SOFT_UNROLL, bool PREFETCH = true>
 __device__ __forceinline__ void soft_tile(const SoftCand& C, const SoftPre& P, float invw,
                                           const float* tg, const float* cv, int ts, int slice, int L,
                                           SoftAcc& A) {
     soft_acc_zero(A);
     const int npx = ts * ts;
+    const int lg = __builtin_ctz((unsigned)ts);
+    if constexpr (PREFETCH) {
+        float n0 = cv[3 * slice], n1 = cv[3 * slice + 1], n2 = cv[3 * slice + 2];
+        float m0 = tg[3 * slice], m1 = tg[3 * slice + 1], m2 = tg[3 * slice + 2];
 #pragma unroll UNROLL
-    for (int px = slice; px < npx; px += L) {
-        const int y = px / ts, x = px - y * ts;
-        const float dx = ((float)x + 0.5f) - C.p[0];
-        const float dy = ((float)y + 0.5f) - C.p[1];
-        soft_px<GRAD>(C, P, invw, dx, dy, cv[3 * px], cv[3 * px + 1], cv[3 * px + 2], tg[3 * px],
-                      tg[3 * px + 1], tg[3 * px + 2], A);
+        for (int px = slice; px < npx; px += L) {
+            const float c0 = n0, c1 = n1, c2 = n2, t0 = m0, t1 = m1, t2 = m2;
+            const int pn = px + L < npx ? px + L : px;  // the last trip reloads its own pixel
+            n0 = cv[3 * pn];
+            n1 = cv[3 * pn + 1];
+            n2 = cv[3 * pn + 2];
+            m0 = tg[3 * pn];
+            m1 = tg[3 * pn + 1];
+            m2 = tg[3 * pn + 2];
+            const int y = px >> lg, x = px & (ts - 1);
+            const float dx = ((float)x + 0.5f) - C.p[0];
+            const float dy = ((float)y + 0.5f) - C.p[1];
+            soft_px<GRAD>(C, P, invw, dx, dy, c0, c1, c2, t0, t1, t2, A);
+        }
+    } else {
+#pragma unroll UNROLL
+        for (int px = slice; px < npx; px += L) {
+            const int y = px >> lg, x = px & (ts - 1);
+            const float dx = ((float)x + 0.5f) - C.p[0];
+            const float dy = ((float)y + 0.5f) - C.p[1];
+            soft_px<GRAD>(C, P, invw, dx, dy, cv[3 * px], cv[3 * px + 1], cv[3 * px + 2], tg[3 * px],
+                          tg[3 * px + 1], tg[3 * px + 2], A);
+        }
     }
     soft_acc_reduce(A, L);
 }
@@ -445,7 +503,7 @@ __device__ void topk_body(const WeftTopkArgs& A) {
 // n_iter Adam steps on candidate C's soft SSE over its tile, L lanes per candidate
 // (this lane takes pixels slice, slice + L, ...), projected onto the code ranges after
 // every step.  Every lane of the candidate ends with the same parameters.
-template <int UNROLL = SOFT_UNROLL>
+template <int UNROLL = SOFT_UNROLL, bool PREFETCH = true>
 __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const float* tg, const float* cv,
                                                  int ts, int slice, int L, SoftCand& C) {
     float m1[SOFT_NP], m2[SOFT_NP];
@@ -457,7 +515,7 @@ __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const 
         SoftPre P;
         soft_pre(C, P);
         SoftAcc S;
-        soft_tile<true, UNROLL>(C, P, 1.0f / w, tg, cv, ts, slice, L, S);
+        soft_tile<true, UNROLL, PREFETCH>(C, P, 1.0f / w, tg, cv, ts, slice, L, S);
         float gr[SOFT_NP];
         soft_finish(C, P, S, gr);
 #pragma unroll

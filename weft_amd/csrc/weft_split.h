@@ -137,8 +137,7 @@ __device__ __forceinline__ float split_chunk(const F& f, int c0, int nleaves, in
 #pragma unroll
     for (int q = 0; q < 7; ++q)
         if ((1 << q) == m) v = T.lvl[q];
-    for (int off = 1; off < L; off <<= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    return lane_group_sum(v, L);
 }
 
 template <int TS, class F>

@@ -160,6 +160,28 @@ def psnr_db(fitter, mask) -> float:
     return float(10 * np.log10(1 / mse))
 
 
+def refined_round_report(c, img, tiles, reps: int = 4):
+    """Round one of the 1080p refined fit at its time-to-PSNR setting (256 drawn, top 2, 12
+    Adam steps, fixed work split), `reps` times from a reset: wall time and PSNR."""
+    import torch
+    from weft_amd.rd import tile_lambda
+    from weft_amd.refine import AdamConfig, RefinedFitter
+
+    rf = RefinedFitter(tiles, n_cand=256, depth=c["depth"], k=2, adam=AdamConfig(n_iter=12), seed=0,
+                       lam=tile_lambda(75), max_steps=c["depth"], cpw={8: 8, 16: 2, 32: 1, 64: 1})
+    walls = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rf.reset()
+        rf.step(1)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    mask = valid_mask(tiles, [img.shape[0]], [img.shape[1]])
+    return dict(n_cand=256, top_k=2, adam_steps=12, cpw={8: 8, 16: 2, 32: 1, 64: 1}, reps=reps,
+                wall_ms=[round(w * 1e3, 3) for w in walls], psnr_db=psnr_db(rf, mask))
+
+
 def time_to_psnr(tiles, mask, n_cand_plain, depth, seed, lam, refined_cfg):
     """Plain fit (persistent launch) vs refined fit round by round."""
     import torch
@@ -625,7 +647,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the time-to-PSNR, batched and refine reports")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES of device memory once (HBM counter calibration) and exit")
-    ap.add_argument("--only", choices=["template_fit", "select", "candidates"], default=None,
+    ap.add_argument("--only", choices=["template_fit", "select", "candidates", "refined_round"], default=None,
                     help="run only that report (for profiling) and print it as JSON")
     args = ap.parse_args()
     if args.only:
@@ -635,6 +657,8 @@ def main():
             rep = template_report(synth_image(2160, 3840, 2500, seed=0, texture=True))
         elif args.only == "candidates":
             rep = candidates_report(build_workload(args)[2])
+        elif args.only == "refined_round":
+            rep = refined_round_report(*build_workload(args)[:3])
         else:
             rep = select_report()
         print(json.dumps({args.only: rep}), flush=True)

@@ -270,8 +270,8 @@ int weft_topk(const WeftFitState* st, const float* d_table, int n, int k, int32_
 /* Adam refinement of each tile's top-k drawn candidates (d_topk[t * k + j], drawn
  * from cands' seed and step), n_iter steps on the soft SSE, candidates spread over the
  * work items of WeftRefineOpts.  Step i uses edge width
- * d_sched[4 i], bias corrections d_sched[4 i + 1] = 1 - beta1^(i+1) and
- * d_sched[4 i + 2] = 1 - beta2^(i+1).  After each step the parameters are projected
+ * d_sched[4 i], reciprocal bias corrections d_sched[4 i + 1] = 1 / (1 - beta1^(i+1)) and
+ * d_sched[4 i + 2] = 1 / (1 - beta2^(i+1)) (m_hat = m1 * d_sched[4 i + 1]).  After each step the parameters are projected
  * onto the code ranges.  Outputs per tile k + 1 slots: slot 0 the best drawn
  * candidate unchanged, slot j + 1 candidate j refined and quantized to a code, as
  * codes and as records (exactly as the scoring kernels decode codes). */

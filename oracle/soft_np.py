@@ -156,12 +156,13 @@ def to_code(kind: int, p: np.ndarray, ts: int, dtype) -> np.ndarray:
     return c
 
 
-def adam_step(p, m1, m2, g, lr, beta1, omb1, beta2, omb2, eps, bc1, bc2):
-    """One Adam step in float32 with the device's operation order (no fused ops)."""
+def adam_step(p, m1, m2, g, lr, beta1, omb1, beta2, omb2, eps, ibc1, ibc2):
+    """One Adam step in float32 with the device's operation order (no fused ops); ibc1 and
+    ibc2 are the reciprocal bias corrections 1 / (1 - beta^t)."""
     p, m1, m2, g = (np.asarray(v, np.float32) for v in (p, m1, m2, g))
     m1 = F(beta1) * m1 + F(omb1) * g
     m2 = F(beta2) * m2 + F(omb2) * (g * g)
-    mh = m1 / F(bc1)
-    vh = m2 / F(bc2)
+    mh = m1 * F(ibc1)
+    vh = m2 * F(ibc2)
     p = p - np.asarray(lr, np.float32) * (mh / (np.sqrt(vh) + F(eps)))
     return p.astype(np.float32), m1.astype(np.float32), m2.astype(np.float32)

@@ -86,6 +86,6 @@ def test_adam_first_step_moves_by_lr():
     p0 = np.zeros(9, np.float32)
     lr = np.array([0.5] * 4 + [0.02] * 5, np.float32)
     p, m1, m2 = soft_np.adam_step(p0, np.zeros(9), np.zeros(9), g, lr, 0.9, np.float32(1) - np.float32(0.9),
-                                  0.999, np.float32(1) - np.float32(0.999), 1e-8, np.float32(0.1),
-                                  np.float32(1 - 0.999))
+                                  0.999, np.float32(1) - np.float32(0.999), 1e-8, np.float32(1 / 0.1),
+                                  np.float32(1 / (1 - 0.999)))
     np.testing.assert_allclose(p, -lr * np.sign(g), rtol=1e-4)

@@ -511,7 +511,8 @@ __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const 
     for (int i = 0; i < SOFT_NP; ++i) m1[i] = m2[i] = 0.0f;
 #pragma unroll 1
     for (int it = 0; it < o.n_iter; ++it) {
-        const float w = o.d_sched[4 * it], bc1 = o.d_sched[4 * it + 1], bc2 = o.d_sched[4 * it + 2];
+        // edge width and the reciprocal bias corrections of this step
+        const float w = o.d_sched[4 * it], ibc1 = o.d_sched[4 * it + 1], ibc2 = o.d_sched[4 * it + 2];
         SoftPre P;
         soft_pre(C, P);
         SoftAcc S;
@@ -522,8 +523,8 @@ __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const 
         for (int i = 0; i < SOFT_NP; ++i) {
             m1[i] = o.beta1 * m1[i] + o.one_minus_beta1 * gr[i];
             m2[i] = o.beta2 * m2[i] + o.one_minus_beta2 * (gr[i] * gr[i]);
-            const float mh = m1[i] / bc1;
-            const float vh = m2[i] / bc2;
+            const float mh = m1[i] * ibc1;
+            const float vh = m2[i] * ibc2;
             C.p[i] = C.p[i] - o.lr[i] * (mh / (sqrtf(vh) + o.eps));
         }
         soft_project(C, ts);

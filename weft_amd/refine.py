@@ -125,13 +125,15 @@ class AdamConfig:
         return [g, g, g, g, t, c, c, c, c]
 
     def schedule(self) -> np.ndarray:
-        """[n_iter, 4] float32: edge width, 1 - beta1^(i+1), 1 - beta2^(i+1), 0."""
+        """[n_iter, 4] float32: edge width, 1 / (1 - beta1^(i+1)), 1 / (1 - beta2^(i+1)), 0
+        (the bias corrections as reciprocals, each rounded once from double, so a step
+        multiplies instead of dividing)."""
         n = self.n_iter
         s = np.zeros((max(1, n), 4), np.float32)
         for i in range(n):
             s[i, 0] = self.w0 + (self.w1 - self.w0) * (i / max(1, n - 1))
-            s[i, 1] = np.float32(1.0 - self.beta1 ** (i + 1))
-            s[i, 2] = np.float32(1.0 - self.beta2 ** (i + 1))
+            s[i, 1] = np.float32(1.0 / (1.0 - self.beta1 ** (i + 1)))
+            s[i, 2] = np.float32(1.0 / (1.0 - self.beta2 ** (i + 1)))
         return s
 
     def opts(self, k: int, d_sched) -> _lib.RefineOpts:

@@ -365,7 +365,11 @@ class RefinedFitter(PlainFitter):
         dev = self.dev
         self.set_cpw(cpw)
         self.d_topk = torch.zeros((T, self.k), dtype=torch.int32, device=dev)
-        self.d_rcodes = torch.zeros(T * K1 * CODE_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+        # refined codes of every round slot (k + 1 per tile); the winners' codes are gathered
+        # from them when records are read, not in the round
+        self.d_rcodes_all = torch.zeros((self.max_steps, T * K1 * CODE_DTYPE.itemsize), dtype=torch.uint8,
+                                        device=dev)
+        self.d_rcodes = self.d_rcodes_all[0]  # the latest round's codes
         self.d_rprims = torch.zeros(T * K1 * PRIM_DTYPE.itemsize, dtype=torch.uint8, device=dev)
         sw = split_work(tiles.ts, K1)
         self._split_arrays = [torch.from_numpy(a).to(dev) for a in sw[:5]]
@@ -408,6 +412,7 @@ class RefinedFitter(PlainFitter):
             sp = _lib.stream_ptr()
             self.score(s, sp)                                    # exact scores of the drawn set
             topk(self, self.d_mse, self.n_cand, self.k, self.d_topk, sp)
+            self.d_rcodes = self.d_rcodes_all[slot]
             refine(self, self._cands(s), self.d_topk, self.k, self.adam, self.d_rcodes, self.d_rprims,
                    stream=sp, work=self.refine_work)
             rc = self._rcands(s)
@@ -421,12 +426,20 @@ class RefinedFitter(PlainFitter):
                 self.d_rpart_obj.data_ptr(), self.d_rpart_idx.data_ptr(),
                 self.d_win_idx.data_ptr() + 4 * T * slot, self.d_win_obj.data_ptr() + 8 * T * slot,
                 self.d_win_acc.data_ptr() + 4 * T * slot, None, sp), "weft_fit_select")
-            # winner codes: slot win_idx of the tile's k + 1 refined codes
-            widx = self.d_win_idx[slot].long()
-            codes = self.d_rcodes.view(T, K1 * CODE_DTYPE.itemsize).view(T, K1, CODE_DTYPE.itemsize)
-            pick = codes[torch_arange(self, T), widx.clamp(min=0)]
-            self.d_win_code[slot].view(T, CODE_DTYPE.itemsize).copy_(pick)
             self.step_idx += 1
+
+    def records(self, first: int = 0, count: int | None = None) -> dict:
+        """As PlainFitter.records; each round's winner code is slot win_idx of that round's
+        k + 1 refined codes, gathered here."""
+        count = self.step_idx - first if count is None else count
+        if count > 0:
+            T, K1, C = self.n_tiles, self.k + 1, CODE_DTYPE.itemsize
+            sl = self.torch.tensor([(first + i) % self.max_steps for i in range(count)], device=self.dev)
+            widx = self.d_win_idx[sl].long().clamp(min=0)                     # [count, T]
+            codes = self.d_rcodes_all[sl].view(count, T, K1, C)
+            pick = codes[self.torch.arange(count, device=self.dev)[:, None], torch_arange(self, T)[None, :], widx]
+            self.d_win_code[sl] = pick.reshape(count, T * C)
+        return super().records(first, count)
 
 
 def torch_arange(fitter, n: int):

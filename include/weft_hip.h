@@ -165,6 +165,12 @@ int weft_score_evals_split(const WeftEval* d_evals, const int32_t* d_order, int 
  * canvas is set).  d_tile_obj[t] = MSE(clip(canvas_t), target_t) + lam * bits_t. */
 int weft_fit_init(const WeftFitState* st, void* stream);
 
+/* Plain fit, part 1 from scratch: every tile back to an empty canvas and stack (canvas 0,
+ * nsel 0, bits = bits0) and d_tile_obj[t] = MSE(0, target_t) + lam * bits0, in one launch.
+ * The state an encode starts from in `_fit_adaptive_state` (`encoder.py:2745-2811`): the
+ * same result as zeroing the canvas and nsel, setting bits and calling weft_fit_init. */
+int weft_fit_reset(const WeftFitState* st, int32_t bits0, void* stream);
+
 /* Plain fit, part 2a: per tile, the candidate indices sorted (stably) by coverage class
  * (ellipse, rectangle, triangle, constant, generic), so that each wavefront of the
  * scoring kernel runs one class.  d_order: n_tiles * n_cand int32.  Needed for explicit

@@ -390,3 +390,29 @@ def test_render_tiles_matches_oracle(gpu, oracle_lib):
     assert img is not None
     for (x, y, out, ref) in expect:
         assert bits_equal(img[y:y + out, x:x + out], ref), (x, y, out)
+
+
+def test_fit_reset_equals_fill_and_init(gpu):
+    """weft_fit_reset (one launch) = zeroing canvas and stacks, bits 8, then weft_fit_init."""
+    import ctypes
+
+    from weft_amd import _lib
+    from weft_amd.fit import PlainFitter
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.synth import synth_image
+
+    img = synth_image(136, 200, 30, seed=3)
+    tiles = TileSet.from_images([img], [decompose_quadtree_variance(img, threshold=1e-3, min_tile=8, max_tile=64)])
+    f = PlainFitter(tiles, n_cand=100, depth=4, seed=2, lam=1e-5, max_steps=3)
+    f.step(3)  # a canvas and stacks to reset
+    f.d_canvas.zero_()
+    f.d_nsel.zero_()
+    f.d_bits.fill_(8)
+    _lib.check(f.L.weft_fit_init(ctypes.byref(f._state), _lib.stream_ptr()), "weft_fit_init")
+    ref = f.state()
+    f.step(2)
+    f.reset()
+    got = f.state()
+    assert (f.canvas() == 0).all()
+    assert np.array_equal(got["nsel"], ref["nsel"]) and np.array_equal(got["bits"], ref["bits"])
+    assert bits_equal(got["obj"], ref["obj"])

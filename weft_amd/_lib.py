@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 15
+ABI_VERSION = 16
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -109,6 +109,7 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_shutdown.argtypes = []
     L.weft_score_evals.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_fit_init.argtypes = [ctypes.POINTER(FitState), c_void_p]
+    L.weft_fit_reset.argtypes = [ctypes.POINTER(FitState), c_int32, c_void_p]
     L.weft_template_masks.argtypes = [c_int, c_void_p, c_void_p]
     L.weft_gen_candidates.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                       c_int, c_void_p]
@@ -151,7 +152,7 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_refine.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, ctypes.POINTER(RefineOpts),
                               c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_render_tiles.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
-    for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_order",
+    for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_reset", "weft_fit_order",
                  "weft_fit_score",
                  "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles",
                  "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk", "weft_refine",
@@ -236,7 +237,8 @@ def tables() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
 
 EXPORTED_SYMBOLS = (
     "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_score_evals",
-    "weft_fit_init", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes", "weft_get_tables",
+    "weft_fit_init", "weft_fit_reset", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes",
+    "weft_get_tables",
     "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk",
     "weft_refine", "weft_fit_score_split", "weft_rans_stage_bytes", "weft_rans_encode_chunks",
     "weft_rans_decode_chunks", "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks",

@@ -191,12 +191,10 @@ class BatchFitter:
             self.grid, self.block)
 
     def _launch_fit(self) -> None:
-        """Reset the tile state, initial objectives, then the persistent fit (one stream)."""
-        self.d_canvas.zero_()
-        self.d_nsel.zero_()
-        self.d_bits.fill_(8)
+        """Reset the tile state with the initial objectives (one launch), then the persistent
+        fit (one stream)."""
         sp = _lib.stream_ptr()
-        _lib.check(self.L.weft_fit_init(ctypes.byref(self._state), sp), "weft_fit_init")
+        _lib.check(self.L.weft_fit_reset(ctypes.byref(self._state), 8, sp), "weft_fit_reset")
         b = self._batch()
         if self.k:
             _lib.check(self.L.weft_batch_refine_fit(ctypes.byref(self._state), ctypes.byref(self._cands),

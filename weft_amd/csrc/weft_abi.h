@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 15
+#define WEFT_ABI_VERSION 16
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -21,6 +21,8 @@ struct WeftScoreEvalsArgs {
 
 struct WeftFitInitArgs {
     WeftFitState st;
+    int32_t reset;  // go back to an empty canvas and stack first
+    int32_t bits0;  // the bits of an empty stack (with reset)
 };
 
 struct WeftFitOrderArgs {

@@ -294,7 +294,16 @@ int weft_fit_init(const WeftFitState* st, void* stream) {
     if (rc) return rc;
     if (!st || st->n_tiles < 0) return set_err(WEFT_E_INVALID, "weft_fit_init: bad state");
     if (st->n_tiles == 0) return WEFT_OK;
-    WeftFitInitArgs a{*st};
+    WeftFitInitArgs a{*st, 0, 0};
+    return launch(g_mod.f_init, (unsigned)st->n_tiles, 64, 0, stream, &a);
+}
+
+int weft_fit_reset(const WeftFitState* st, int32_t bits0, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || st->n_tiles < 0) return set_err(WEFT_E_INVALID, "weft_fit_reset: bad state");
+    if (st->n_tiles == 0) return WEFT_OK;
+    WeftFitInitArgs a{*st, 1, bits0};
     return launch(g_mod.f_init, (unsigned)st->n_tiles, 64, 0, stream, &a);
 }
 

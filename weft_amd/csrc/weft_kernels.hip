@@ -1307,7 +1307,8 @@ extern "C" __global__ __launch_bounds__(256) void weft_score_evals_kernel(WeftSc
 // leaf meet in the ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7)) tree through xor
 // shuffles 1, 2, 4 (a + b == b + a), and the leaves in the pairwise tree as in
 // weft_split.h.  Equal to the sequential sum bit for bit.
-template <int LS>
+// EMPTY: the canvas is zero (clip(0) - t = -t exactly), so it is not read.
+template <int LS, bool EMPTY = false>
 __device__ __forceinline__ float init_chunk(const float* tg, const float* cv, int c0, int nleaves) {
     const int lane = threadIdx.x & 63;
     const int a = lane & 7, ll = lane >> 3;
@@ -1324,7 +1325,7 @@ __device__ __forceinline__ float init_chunk(const float* tg, const float* cv, in
             const int e0 = c0 + (ll * m + li) * LS + a;
 #pragma unroll
             for (int k = 0; k < LS / 8; ++k) {
-                const float d = clip01(cv[e0 + 8 * k]) - tg[e0 + 8 * k];
+                const float d = (EMPTY ? 0.0f : clip01(cv[e0 + 8 * k])) - tg[e0 + 8 * k];
                 r = r + d * d;
             }
         }
@@ -1341,14 +1342,14 @@ __device__ __forceinline__ float init_chunk(const float* tg, const float* cv, in
     return v;
 }
 
-template <int TS>
+template <int TS, bool EMPTY = false>
 __device__ __forceinline__ float init_tile_sse(const float* tg, const float* cv) {
     if constexpr (TS == 64) {
-        const float a = init_chunk<128>(tg, cv, 0, 64);
-        const float b = init_chunk<128>(tg, cv, 8192, 32);
+        const float a = init_chunk<128, EMPTY>(tg, cv, 0, 64);
+        const float b = init_chunk<128, EMPTY>(tg, cv, 8192, 32);
         return (0.0f + a) + b;
     } else {
-        return init_chunk<96>(tg, cv, 0, (3 * TS * TS) / 96);
+        return init_chunk<96, EMPTY>(tg, cv, 0, (3 * TS * TS) / 96);
     }
 }
 extern "C" __global__ __launch_bounds__(256) void weft_fit_order_kernel(WeftFitOrderArgs A) { fit_order_body(A); }
@@ -1548,25 +1549,41 @@ extern "C" __global__ __launch_bounds__(256, WEFT_BATCH_REFINE_MIN_WAVES) void w
     batch_refine_body(A);
 }
 
-extern "C" __global__ __launch_bounds__(64) void weft_fit_init_kernel(WeftFitInitArgs A) {
+// One wave per tile: the objective of its canvas.  With A.reset the tile first goes back to
+// an empty canvas and stack (canvas zero, nsel 0, bits A.bits0) and the objective is that of
+// the empty canvas, which is not read back.
+template <bool RESET>
+__device__ __forceinline__ void fit_init_body(const WeftFitInitArgs& A) {
     const int t = blockIdx.x;
     const WeftFitState& st = A.st;
     const int64_t base = st.d_tile_pix[t] * 3;
     const float* tg = st.d_target + base;
-    const float* cv = st.d_canvas + base;
+    float* cv = st.d_canvas + base;
     const int ts = st.d_tile_ts[t];
+    if (RESET)
+        for (int i = threadIdx.x; i < 3 * ts * ts; i += 64) cv[i] = 0.0f;
     float sse;
     switch (ts) {
-    case 8: sse = init_tile_sse<8>(tg, cv); break;
-    case 16: sse = init_tile_sse<16>(tg, cv); break;
-    case 32: sse = init_tile_sse<32>(tg, cv); break;
-    case 64: sse = init_tile_sse<64>(tg, cv); break;
+    case 8: sse = init_tile_sse<8, RESET>(tg, cv); break;
+    case 16: sse = init_tile_sse<16, RESET>(tg, cv); break;
+    case 32: sse = init_tile_sse<32, RESET>(tg, cv); break;
+    case 64: sse = init_tile_sse<64, RESET>(tg, cv); break;
     default: sse = __builtin_nanf(""); break;
     }
     if (threadIdx.x == 0) {
+        const int bits = RESET ? A.bits0 : st.d_tile_bits[t];
+        if (RESET) {
+            st.d_tile_nsel[t] = 0;
+            st.d_tile_bits[t] = bits;
+        }
         const float mse = sse / (float)(3 * ts * ts);
-        st.d_tile_obj[t] = objective(mse, st.d_tile_bits[t], st.lam, st.obj_mode);
+        st.d_tile_obj[t] = objective(mse, bits, st.lam, st.obj_mode);
     }
+}
+
+extern "C" __global__ __launch_bounds__(64) void weft_fit_init_kernel(WeftFitInitArgs A) {
+    if (A.reset) fit_init_body<true>(A);
+    else fit_init_body<false>(A);
 }
 
 // Drop-in stack scoring, each eval's pixels split over lanes (one tile size per launch).

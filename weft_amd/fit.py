@@ -176,12 +176,9 @@ class PlainFitter:
         return _lib.Cands(prims, None, self.seed, step, self.n_cand, self.has_general, 0)
 
     def reset(self) -> None:
-        """Back to an empty canvas and empty stacks, on the device (fill kernels + the
-        initial objectives); the step counter restarts at 0."""
-        self.d_canvas.zero_()
-        self.d_nsel.zero_()
-        self.d_bits.fill_(8)
-        _lib.check(self.L.weft_fit_init(ctypes.byref(self._state), _lib.stream_ptr()), "weft_fit_init")
+        """Back to an empty canvas and empty stacks with their initial objectives, in one
+        launch on the device; the step counter restarts at 0."""
+        _lib.check(self.L.weft_fit_reset(ctypes.byref(self._state), 8, _lib.stream_ptr()), "weft_fit_reset")
         self.step_idx = 0
 
     def set_canvas(self, canvas_flat: np.ndarray, nsel: np.ndarray | None = None,

@@ -4,8 +4,8 @@
 // its anonymous namespace, after the batched fit, the soft surrogate and the split scorer.
 //
 // Per tile and round (one 256-thread workgroup): class-sort and score the n_cand drawn
-// candidates (as weft_batch_fit), keep each candidate's (MSE bits, index) key in LDS, sort
-// the keys (bitonic) for the top k, refine them (cpw candidates per wave, the
+// candidates (as weft_batch_fit), keep each candidate's (MSE bits, index) key in LDS, take
+// the top k (k workgroup-wide minima for k <= 8, else a bitonic sort), refine them (cpw candidates per wave, the
 // arithmetic of weft_refine_kernel), score the k refined codes and the unrefined best
 // exactly (each candidate's pixels split over lanes along the pairwise tree), take the
 // first minimum and accept it if it beats the tile's objective.  The result equals
@@ -19,6 +19,8 @@
 struct RefineShared {
     int topk[64];
     WeftCode codes[65];
+    unsigned long long wmin[16];  // per-wave minima of the top-k rounds (up to 1024 threads)
+    unsigned long long last;
 };
 
 // Adam on the tile's top k (64 / cpw lanes per candidate, min(cpw, k) candidates per
@@ -155,24 +157,51 @@ __device__ __forceinline__ void batch_refine_tile(const WeftBatchRefineArgs& R, 
         }
         for (int i = n + tid; i < np2; i += nthr) keys[i] = ~0ull;
         __syncthreads();
-        // ---- top k by (MSE, index): bitonic sort of the keys ----
-        for (int size = 2; size <= np2; size <<= 1) {
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int i = tid; i < (np2 >> 1); i += nthr) {
-                    const int lo = 2 * i - (i & (stride - 1));
-                    const int hi = lo + stride;
-                    const bool asc = (lo & size) == 0;
-                    const unsigned long long a = keys[lo], b = keys[hi];
-                    if ((a > b) == asc) {
-                        keys[lo] = b;
-                        keys[hi] = a;
-                    }
+        // ---- top k by (MSE, index) (keys are distinct): for k <= 8, k workgroup-wide
+        // minima, each above the last; above that, a bitonic sort of the keys ----
+        if (k <= 8) {
+            unsigned long long last = 0ull;
+            for (int r = 0; r < k; ++r) {
+                unsigned long long m = ~0ull;
+                for (int i = tid; i < n; i += nthr) {
+                    const unsigned long long kk = keys[i];
+                    if ((r == 0 || kk > last) && kk < m) m = kk;
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    const unsigned long long o = __shfl_xor(m, off, 64);
+                    m = o < m ? o : m;
+                }
+                if (lane == 0) rs.wmin[wave] = m;
+                __syncthreads();
+                if (tid == 0) {
+                    unsigned long long mm = rs.wmin[0];
+                    for (int ww = 1; ww < nwaves; ++ww) mm = rs.wmin[ww] < mm ? rs.wmin[ww] : mm;
+                    rs.topk[r] = (int)(mm & 0xFFFFFFFFull);
+                    rs.last = mm;
                 }
                 __syncthreads();
+                last = rs.last;
             }
+        } else {
+            for (int size = 2; size <= np2; size <<= 1) {
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int i = tid; i < (np2 >> 1); i += nthr) {
+                        const int lo = 2 * i - (i & (stride - 1));
+                        const int hi = lo + stride;
+                        const bool asc = (lo & size) == 0;
+                        const unsigned long long a = keys[lo], b = keys[hi];
+                        if ((a > b) == asc) {
+                            keys[lo] = b;
+                            keys[hi] = a;
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            if (tid < k) rs.topk[tid] = (int)(keys[tid] & 0xFFFFFFFFull);
+            __syncthreads();
         }
-        if (tid < k) rs.topk[tid] = (int)(keys[tid] & 0xFFFFFFFFull);
-        __syncthreads();
         // ---- Adam on the top k; exact scores of the best drawn and the refined ----
         refine_phase<TS>(R, dbase, T, cvp, rs);
         __syncthreads();

@@ -683,7 +683,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the tile-fit path "
-              "shards by tiles with no collective, see DESIGN.md", file=sys.stderr)
+              "shards by tiles (each GPU fits its own tiles, no collective), and one GPU is measured here", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -144,6 +144,13 @@ int weft_shutdown(void);
 const char* weft_last_error(void);
 /* Library and kernel ABI version (for the Python loader's sanity check). */
 int weft_abi_version(void);
+/* 0 if one workgroup may hold `bytes` of LDS (static + dynamic): at most
+ * hipDeviceAttributeMaxSharedMemoryPerBlock of the initialised device, or gfx950's 160 KiB
+ * before weft_init; otherwise WEFT_E_INVALID.  Every launch of the library is checked this
+ * way (its kernel's static LDS plus the dynamic request) before it is issued.  No
+ * reference counterpart: the reference launched through the CUDA driver unchecked
+ * (src/weft/gpu_render.py:549-550, 670-675). */
+int weft_check_lds(int64_t bytes);
 
 /* Reference-semantics scoring of explicit primitive stacks (one lane per eval):
  * out_mse[i] = float32(SSE_i / (3 * ts^2)), SSE in NumPy's pairwise float32 order

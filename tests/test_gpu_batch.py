@@ -187,9 +187,11 @@ def test_batch_rejects_bad_launch(gpu):
 
 
 @pytest.mark.parametrize("k,cpw,order,block", [(4, None, "sjf", 256), (8, {8: 8, 16: 2, 32: 1, 64: 1}, "lpt", 128),
-                                               (2, {8: 1, 16: 2, 32: 2, 64: 1}, "fifo", 64)])
+                                               (2, {8: 1, 16: 2, 32: 2, 64: 1}, "fifo", 64),
+                                               (16, None, "lpt", 256), (64, {8: 64, 16: 16, 32: 4, 64: 1}, "sjf", 128)])
 def test_batch_refine_equals_stepwise_refined(gpu, k, cpw, order, block):
-    """The persistent refined fit = RefinedFitter round by round, bit for bit."""
+    """The persistent refined fit = RefinedFitter round by round, bit for bit.  k <= 8 takes
+    the top k by k workgroup-wide minima, k = 16 and 64 by the bitonic sort of the keys."""
     from weft_amd.batch import BatchFitter
     from weft_amd.refine import AdamConfig, RefinedFitter
 

@@ -19,6 +19,20 @@ def _say(*a):
     print(*a, flush=True)
 
 
+def test_lds_limit_is_the_device_attribute(gpu):
+    """After weft_init the LDS check uses the device's own limit (160 KiB on MI355X), and
+    the largest LDS request of the library (128 KiB, candidate pools of 64-px tiles) is
+    within it."""
+    import torch
+
+    lim = torch.cuda.get_device_properties(0).shared_memory_per_block
+    _say(f"device LDS limit per workgroup: {lim} bytes")
+    assert gpu.weft_check_lds(128 * 1024) == 0
+    assert gpu.weft_check_lds(lim) == 0
+    assert gpu.weft_check_lds(lim + 1) == -1001
+    assert str(lim).encode() in gpu.weft_last_error()
+
+
 def test_draw_codes_match_oracle(gpu, oracle_lib):
     from weft_amd.fit import draw_codes
 
@@ -267,7 +281,7 @@ def test_fit_c2_full_size_bit_exact(gpu, oracle_lib):
 
 
 def test_fit_mixed_tile_sizes_bit_exact(gpu, oracle_lib):
-    """Quadtree tiles of 8-64 px (64-px tiles use 100 KiB of LDS), odd candidate count."""
+    """Quadtree tiles of 8-64 px (64-px tiles staged through LDS in 8-row bands), odd candidate count."""
     from weft_amd.quadtree import TileSet, decompose_quadtree_variance
     from weft_amd.synth import synth_image
 

@@ -49,6 +49,22 @@ def test_entry_points_fail_cleanly_without_init():
     assert b"weft_init" in L.weft_last_error()
 
 
+def test_lds_request_checked_against_the_limit():
+    """Every launch's LDS (the kernel's static bytes plus the dynamic request) is checked
+    against the device limit before it is issued; before weft_init the limit is gfx950's
+    160 KiB."""
+    from weft_amd import _lib
+
+    L = _lib.library()
+    if _lib.gpu_available():
+        pytest.skip("a GPU is visible; the device's own limit is checked in the GPU tests")
+    assert L.weft_check_lds(0) == 0
+    assert L.weft_check_lds(160 * 1024) == 0
+    assert L.weft_check_lds(160 * 1024 + 1) == -1001
+    assert b"exceed the device limit of 163840" in L.weft_last_error()
+    assert L.weft_check_lds(-1) == -1001
+
+
 def test_tables_match_oracle(oracle_lib):
     from weft_amd import _lib
 
@@ -166,6 +182,20 @@ def test_tileset_packing_and_assembly():
     assert bits_equal(back, img)
 
 
+def test_tileset_rejects_tiles_outside_the_padded_image():
+    """A tile reaching past the padded image would give a short slice and shift every
+    later tile's pixel offset (kernels would then read past the target buffer)."""
+    from weft_amd.quadtree import QuadTile, TileSet, uniform_tiles
+
+    img = np.zeros((1056, 96, 3), np.float32)
+    with pytest.raises(ValueError, match="does not fit"):
+        TileSet.from_images([img], [uniform_tiles(1056, 96, 64)], pad_multiple=32)
+    with pytest.raises(ValueError, match="does not fit"):
+        TileSet.from_images([img], [[QuadTile(-8, 0, 8, 0)]], pad_multiple=32)
+    ok = TileSet.from_images([img], [uniform_tiles(1056, 96, 32)], pad_multiple=32)
+    assert len(ok.target) == 3 * ok.n_pixels
+
+
 def test_drop_in_returns_none_without_gpu():
     """The reference contract: no GPU -> None, so the caller falls back to its CPU path."""
     from weft_amd import _lib
@@ -179,6 +209,42 @@ def test_drop_in_returns_none_without_gpu():
     assert gpu_batch_objectives(tile, []).shape == (0,)
     assert gpu_batch_score_prepacked(tile.ravel(), {0: 0}, [0], [16], [], [np.zeros(0, np.int32)]) == [
         pytest.approx(np.zeros(0))]
+
+
+def test_decode_descs_are_checked_before_any_launch(monkeypatch):
+    """Descriptors that would read past the primitive array, or describe an empty tile
+    or image, are refused on the host: None (the reference's contract), or ValueError
+    under WEFT_HIP_STRICT, and the device is never touched."""
+    from weft_amd import gpu_render
+    from weft_amd.records import PRIM_DTYPE, TILE_DECODE_DTYPE
+
+    def no_launch():
+        raise AssertionError("a launch was attempted")
+
+    monkeypatch.setattr(gpu_render, "_ready", no_launch)
+    prims = np.zeros(5, PRIM_DTYPE)
+    good = np.zeros(2, TILE_DECODE_DTYPE)
+    good["prim_offset"] = [0, 3]
+    good["prim_count"] = [3, 2]
+    good["target_w"] = good["target_h"] = good["source_size"] = 8
+    bad = []
+    for field, value in (("prim_offset", 4), ("prim_count", 3), ("prim_offset", -1), ("prim_count", -1),
+                         ("source_size", 0), ("target_w", 0), ("target_h", -2)):
+        d = good.copy()
+        d[field][1] = value
+        bad.append((d, 16, 16))
+    bad.append((good, 0, 16))
+    bad.append((good, 16, 0))
+    for d, h, w in bad:
+        monkeypatch.delenv("WEFT_HIP_STRICT", raising=False)
+        with pytest.warns(RuntimeWarning):
+            assert gpu_render.gpu_render_tiles_to_image(d, prims, h, w) is None
+        monkeypatch.setenv("WEFT_HIP_STRICT", "1")
+        with pytest.raises(ValueError):
+            gpu_render.gpu_render_tiles_to_image(d, prims, h, w)
+    # valid descriptors pass the check and reach the device path
+    with pytest.raises(AssertionError, match="launch was attempted"):
+        gpu_render.gpu_render_tiles_to_image(good, prims, 16, 16)
 
 
 def test_fitter_rejects_unsupported_tile_sizes():

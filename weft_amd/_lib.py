@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 16
+ABI_VERSION = 17
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -104,6 +104,7 @@ _device: int | None = None
 
 def _bind(L: ctypes.CDLL) -> None:
     L.weft_abi_version.restype = c_int
+    L.weft_check_lds.argtypes = [ctypes.c_int64]
     L.weft_last_error.restype = ctypes.c_char_p
     L.weft_init.argtypes = [c_int]
     L.weft_shutdown.argtypes = []
@@ -154,8 +155,9 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_render_tiles.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
     for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_reset", "weft_fit_order",
                  "weft_fit_score",
-                 "weft_fit_select", "weft_draw_codes", "weft_get_tables", "weft_render_tiles",
-                 "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk", "weft_refine",
+                 "weft_fit_select", "weft_draw_codes", "weft_get_tables",
+                 "weft_render_tiles", "weft_check_lds", "weft_batch_fit", "weft_wallclock_rate_khz",
+                 "weft_soft_eval", "weft_topk", "weft_refine",
                  "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks",
                  "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks", "weft_template_fit",
                  "weft_argmin_f64", "weft_exclusive_scan_i64", "weft_compact_flags", "weft_segmented_topk",
@@ -236,8 +238,9 @@ def tables() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
 
 
 EXPORTED_SYMBOLS = (
-    "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_score_evals",
-    "weft_fit_init", "weft_fit_reset", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_draw_codes",
+    "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_check_lds", "weft_score_evals",
+    "weft_fit_init", "weft_fit_reset", "weft_fit_order", "weft_fit_score", "weft_fit_select",
+    "weft_draw_codes",
     "weft_get_tables",
     "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk",
     "weft_refine", "weft_fit_score_split", "weft_rans_stage_bytes", "weft_rans_encode_chunks",

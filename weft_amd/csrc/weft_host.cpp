@@ -14,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "weft_abi.h"
 #include "weft_hip.h"
@@ -63,7 +64,12 @@ struct Module {
     hipFunction_t f_rans_dec = nullptr;
     hipFunction_t f_batch_reset = nullptr;
     hipFunction_t f_render = nullptr;
+    int64_t max_lds = 0;                            // hipDeviceAttributeMaxSharedMemoryPerBlock
+    std::unordered_map<hipFunction_t, int> static_lds;  // __shared__ bytes of each kernel
 };
+
+// LDS per workgroup on gfx950 (the limit checked before a device is initialised).
+constexpr int64_t kGfx950LdsBytes = 160 * 1024;
 
 Module g_mod;
 std::mutex g_mu;
@@ -116,8 +122,21 @@ int require_init() {
     return WEFT_OK;
 }
 
+// A workgroup's LDS (static + dynamic) against the device limit.
+int lds_check(int64_t bytes, const char* what) {
+    const int64_t limit = g_mod.mod ? g_mod.max_lds : kGfx950LdsBytes;
+    if (bytes < 0 || bytes > limit)
+        return set_err(WEFT_E_INVALID, std::string(what) + ": " + std::to_string(bytes) +
+                                           " bytes of LDS per workgroup exceed the device limit of " +
+                                           std::to_string(limit));
+    return WEFT_OK;
+}
+
 int launch(hipFunction_t f, unsigned grid, unsigned block, unsigned shmem, void* stream, void* args) {
     if (grid == 0) return WEFT_OK;
+    const auto it = g_mod.static_lds.find(f);
+    const int64_t static_bytes = it == g_mod.static_lds.end() ? 0 : it->second;
+    if (int rc = lds_check(static_bytes + (int64_t)shmem, "kernel launch")) return rc;
     void* params[] = {args};
     WEFT_HIP(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, shmem, (hipStream_t)stream, params,
                                    nullptr),
@@ -150,6 +169,8 @@ extern "C" {
 int weft_abi_version(void) { return WEFT_ABI_VERSION; }
 
 const char* weft_last_error(void) { return g_err.c_str(); }
+
+int weft_check_lds(int64_t bytes) { return lds_check(bytes, "weft_check_lds"); }
 
 // The Python binding (weft_amd/_lib.py) mirrors these layouts.
 static_assert(sizeof(WeftFitState) == 80, "WeftFitState layout");
@@ -225,12 +246,23 @@ int weft_init(int device) {
     };
     for (auto& fn : fns) {
         hipError_t e = hipModuleGetFunction(fn.f, g_mod.mod, fn.name);
+        int lds = 0;
+        if (e == hipSuccess) e = hipFuncGetAttribute(&lds, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, *fn.f);
         if (e != hipSuccess) {
             (void)hipModuleUnload(g_mod.mod);
             g_mod = Module();
             return hip_fail(e, fn.name);
         }
+        g_mod.static_lds[*fn.f] = lds;
     }
+    int max_lds = 0;
+    hipError_t e_lds = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    if (e_lds != hipSuccess) {
+        (void)hipModuleUnload(g_mod.mod);
+        g_mod = Module();
+        return hip_fail(e_lds, "hipDeviceGetAttribute(MaxSharedMemoryPerBlock)");
+    }
+    g_mod.max_lds = max_lds;
     build_tables();
     int rc = upload_global("weft_tab_logr", g_logr, sizeof(g_logr));
     if (rc == WEFT_OK) rc = upload_global("weft_tab_cos", g_cos, sizeof(g_cos));

@@ -73,6 +73,21 @@ def _validate_evals(evals: np.ndarray, n_prims: int, n_floats: int) -> None:
         raise ValueError(f"tile sizes must be in {KERNEL_TILE_SIZES}")
 
 
+def _validate_decode_descs(descs: np.ndarray, n_prims: int, width: int, height: int) -> None:
+    """Every decode descriptor must read inside the primitive array and describe a
+    non-empty tile (checked on the host before anything is uploaded)."""
+    if width <= 0 or height <= 0:
+        raise ValueError("output dimensions must be > 0")
+    po = descs["prim_offset"].astype(np.int64)
+    pc = descs["prim_count"].astype(np.int64)
+    if (pc < 0).any() or (po < 0).any() or (po + pc > n_prims).any():
+        raise ValueError("tile primitive ranges exceed the packed primitive array")
+    if (descs["source_size"] <= 0).any():
+        raise ValueError("source_size must be > 0")
+    if (descs["target_w"] <= 0).any() or (descs["target_h"] <= 0).any():
+        raise ValueError("target_w and target_h must be > 0")
+
+
 def split_order(tile_sizes: np.ndarray) -> tuple[np.ndarray, list[tuple[int, int]]]:
     """Evals grouped by tile size for weft_score_evals_split: (order, [(size, count), ...])."""
     ts = np.asarray(tile_sizes)
@@ -222,13 +237,15 @@ def gpu_render_tiles_to_image(
     if n == 0:
         return np.zeros((output_height, output_width, 3), dtype=np.float32)
     try:
+        descs = np.ascontiguousarray(tile_descs, dtype=TILE_DECODE_DTYPE)
+        prims = np.ascontiguousarray(all_prims, dtype=PRIM_DTYPE)
+        _validate_decode_descs(descs, len(prims), int(output_width), int(output_height))
         if not _ready():
             return None
         import torch
 
         dev = torch.device("cuda", 0)
-        d = torch.from_numpy(np.ascontiguousarray(tile_descs, dtype=TILE_DECODE_DTYPE).view(np.uint8)).to(dev)
-        prims = np.ascontiguousarray(all_prims, dtype=PRIM_DTYPE)
+        d = torch.from_numpy(descs.view(np.uint8)).to(dev)
         if len(prims) == 0:
             prims = np.zeros(1, PRIM_DTYPE)
         p = torch.from_numpy(prims.view(np.uint8)).to(dev)

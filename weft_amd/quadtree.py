@@ -138,7 +138,12 @@ class TileSet:
         run = 0
         for k, (img, tiles) in enumerate(zip(images, tiles_per_image)):
             padded = pad_to_multiple(np.asarray(img, np.float32), pad_multiple)
+            ph, pw = padded.shape[:2]
             for t in tiles:
+                if t.size <= 0 or t.x < 0 or t.y < 0 or t.y + t.size > ph or t.x + t.size > pw:
+                    # a short slice would shift every later tile's pixel offset
+                    raise ValueError(f"tile at ({t.x}, {t.y}) of size {t.size} does not fit image {k} "
+                                     f"padded to {ph}x{pw} (pad_multiple={pad_multiple})")
                 xs.append(t.x)
                 ys.append(t.y)
                 ss.append(t.size)

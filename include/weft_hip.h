@@ -206,6 +206,27 @@ int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_
                     const int32_t* d_part_idx, int32_t* d_win_idx, double* d_win_obj,
                     int32_t* d_win_acc, WeftCode* d_win_code, void* stream);
 
+/* Allocation-free plain fit step.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  One call = weft_fit_order (explicit candidates only)
+ * + weft_fit_score + weft_fit_select with the same arguments, the same results bit for
+ * bit, but the class order and the partial minima live in a static device array of
+ * WEFT_STATIC_SCRATCH_BYTES that the library holds on the device from weft_init until
+ * the module is unloaded, instead of caller buffers.  `work` is one work list covering
+ * every tile (as in one merged weft_fit_score launch), with d_tile_work0 / d_tile_nwork /
+ * d_tile_wstride indexing into it from 0.  The scratch needs 4 * n_tiles * n_cand bytes
+ * for explicit candidates (rounded up to 256) plus 12 * n_work bytes; a larger request
+ * returns WEFT_E_TOO_LARGE and launches nothing.  The call allocates and frees nothing.
+ * Every call on a device shares that device's one array: calls whose launches overlap
+ * on one device (other streams, other threads) race and must be serialised by the
+ * caller.  Nothing in the array has to start at zero (every entry a call reads, it wrote
+ * first in the same call). */
+#define WEFT_STATIC_SCRATCH_BYTES (16u << 20)
+int weft_fit_step_static(const WeftFitState* st, const WeftWork* work, const WeftCands* cands,
+                         const int32_t* d_tile_work0, const int32_t* d_tile_nwork,
+                         const int32_t* d_tile_wstride, int32_t* d_win_idx, double* d_win_obj,
+                         int32_t* d_win_acc, WeftCode* d_win_code, void* stream);
+
 /* Draw candidate codes / records for the given tiles exactly as weft_fit_score does on
  * the device (for checking and serialization).  d_codes / d_prims: n_tiles * n_cand.
  *   family : 0 ellipse (kind 6), 1 rectangle (kind 7), 2 triangle (kind 8)

@@ -134,6 +134,9 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_fit_select.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p]
+    L.weft_fit_step_static.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Work), ctypes.POINTER(Cands),
+                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_void_p]
     L.weft_draw_codes.argtypes = [c_uint64, c_uint32, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                   c_void_p, c_void_p]
     L.weft_get_tables.argtypes = [c_void_p, c_void_p, c_void_p]
@@ -155,7 +158,7 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_render_tiles.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
     for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_reset", "weft_fit_order",
                  "weft_fit_score",
-                 "weft_fit_select", "weft_draw_codes", "weft_get_tables",
+                 "weft_fit_select", "weft_fit_step_static", "weft_draw_codes", "weft_get_tables",
                  "weft_render_tiles", "weft_check_lds", "weft_batch_fit", "weft_wallclock_rate_khz",
                  "weft_soft_eval", "weft_topk", "weft_refine",
                  "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks",
@@ -239,7 +242,7 @@ def tables() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
 
 EXPORTED_SYMBOLS = (
     "weft_init", "weft_shutdown", "weft_last_error", "weft_abi_version", "weft_check_lds", "weft_score_evals",
-    "weft_fit_init", "weft_fit_reset", "weft_fit_order", "weft_fit_score", "weft_fit_select",
+    "weft_fit_init", "weft_fit_reset", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_fit_step_static",
     "weft_draw_codes",
     "weft_get_tables",
     "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk",

@@ -65,6 +65,7 @@ struct Module {
     hipFunction_t f_batch_reset = nullptr;
     hipFunction_t f_render = nullptr;
     int64_t max_lds = 0;                            // hipDeviceAttributeMaxSharedMemoryPerBlock
+    unsigned char* scratch = nullptr;               // device address of weft_scratch on `device`
     std::unordered_map<hipFunction_t, int> static_lds;  // __shared__ bytes of each kernel
 };
 
@@ -263,6 +264,18 @@ int weft_init(int device) {
         return hip_fail(e_lds, "hipDeviceGetAttribute(MaxSharedMemoryPerBlock)");
     }
     g_mod.max_lds = max_lds;
+    // The allocation-free entry point's scratch: the code object's static array on this
+    // device (the device is current), looked up once and kept with the module.
+    hipDeviceptr_t scratch = nullptr;
+    size_t scratch_bytes = 0;
+    hipError_t e_scr = hipModuleGetGlobal(&scratch, &scratch_bytes, g_mod.mod, "weft_scratch");
+    if (e_scr != hipSuccess || scratch_bytes != WEFT_STATIC_SCRATCH_BYTES) {
+        (void)hipModuleUnload(g_mod.mod);
+        g_mod = Module();
+        if (e_scr != hipSuccess) return hip_fail(e_scr, "hipModuleGetGlobal(weft_scratch)");
+        return set_err(WEFT_E_MODULE, "weft_scratch has an unexpected size");
+    }
+    g_mod.scratch = static_cast<unsigned char*>(scratch);
     build_tables();
     int rc = upload_global("weft_tab_logr", g_logr, sizeof(g_logr));
     if (rc == WEFT_OK) rc = upload_global("weft_tab_cos", g_cos, sizeof(g_cos));
@@ -381,6 +394,35 @@ int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_
     WeftFitSelectArgs a{*st, *cands, d_tile_work0, d_tile_nwork, d_tile_wstride, d_part_obj, d_part_idx,
                         d_win_idx, d_win_obj, d_win_acc, d_win_code};
     return launch(g_mod.f_select, (unsigned)st->n_tiles, 256, 0, stream, &a);
+}
+
+int weft_fit_step_static(const WeftFitState* st, const WeftWork* work, const WeftCands* cands,
+                         const int32_t* d_tile_work0, const int32_t* d_tile_nwork,
+                         const int32_t* d_tile_wstride, int32_t* d_win_idx, double* d_win_obj,
+                         int32_t* d_win_acc, WeftCode* d_win_code, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !work || !cands) return set_err(WEFT_E_INVALID, "weft_fit_step_static: null argument");
+    if (st->n_tiles < 0 || work->n_work < 0 || cands->n_cand < 1)
+        return set_err(WEFT_E_INVALID, "weft_fit_step_static: bad sizes");
+    // Scratch layout: [class order, explicit candidates only][partial objectives][partial indices].
+    const bool explicit_cands = cands->d_prims != nullptr;
+    const uint64_t order_bytes =
+        explicit_cands ? ((4ull * (uint64_t)st->n_tiles * (uint64_t)cands->n_cand + 255ull) & ~255ull) : 0ull;
+    const uint64_t obj_bytes = (8ull * (uint64_t)work->n_work + 255ull) & ~255ull;
+    const uint64_t need = order_bytes + obj_bytes + 4ull * (uint64_t)work->n_work;
+    if (need > WEFT_STATIC_SCRATCH_BYTES)
+        return set_err(WEFT_E_TOO_LARGE, "weft_fit_step_static: " + std::to_string(need) +
+                                             " bytes of scratch needed, the static array holds " +
+                                             std::to_string(WEFT_STATIC_SCRATCH_BYTES));
+    unsigned char* base = g_mod.scratch;
+    int32_t* order = reinterpret_cast<int32_t*>(base);
+    double* part_obj = reinterpret_cast<double*>(base + order_bytes);
+    int32_t* part_idx = reinterpret_cast<int32_t*>(base + order_bytes + obj_bytes);
+    if (explicit_cands && (rc = weft_fit_order(st, cands, order, stream))) return rc;
+    if ((rc = weft_fit_score(st, work, cands, order, nullptr, nullptr, part_obj, part_idx, stream))) return rc;
+    return weft_fit_select(st, cands, d_tile_work0, d_tile_nwork, d_tile_wstride, part_obj, part_idx, d_win_idx,
+                           d_win_obj, d_win_acc, d_win_code, stream);
 }
 
 int weft_draw_codes(uint64_t seed, uint32_t step, const int32_t* d_tile_ids,

@@ -27,6 +27,12 @@ __device__ uint16_t weft_tab_logr[WEFT_TABLE_N];
 __device__ float weft_tab_cos[WEFT_TABLE_N];
 __device__ float weft_tab_sin[WEFT_TABLE_N];
 
+// Scratch of the allocation-free fit step (weft_fit_step_static): class orders and
+// partial minima.  It lives as long as the loaded code object (one copy per device);
+// weft_init takes its address with hipModuleGetGlobal and the host passes that address
+// to the kernels.  External linkage so that the module lookup by name finds it.
+__device__ __align__(256) unsigned char weft_scratch[WEFT_STATIC_SCRATCH_BYTES];
+
 namespace {
 
 enum : int { CL_CONST = 0, CL_ELL = 1, CL_RECT = 2, CL_TRI = 3, CL_GEN = 4, CL_NONE = 5 };

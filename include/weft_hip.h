@@ -26,6 +26,9 @@
  *   weft_render_tiles             kernel decode_tile_pixels (gpu_render.py:194-237) as
  *                                 launched by gpu_render_tiles_to_image
  *                                 (gpu_render.py:594-690), with render.py's alpha clamp
+ *   weft_render_scene             render.render_scene_adaptive (src/weft/render.py:270-358)
+ *                                 as decoder._render_primitive_stack_gpu
+ *                                 (src/weft/decoder.py:231-390) places its tiles, RES1 included
  *   weft_template_masks /         kernel weft_tile_fit_candidates and its launcher
  *   weft_template_fit             fit_tiles_gpu_constant (src/weft/gpu_encoder.py:42-251,
  *                                 411-562), the stage-3 baseline fit
@@ -410,6 +413,21 @@ int weft_get_tables(uint16_t* logr, float* cosv, float* sinv);
  * to [0, 1]); pixels no tile covers are left untouched. */
 int weft_render_tiles(const WeftTileDecodeDesc* d_descs, int n_tiles, const WeftPrim* d_prims,
                       float* d_out, int width, int height, void* stream);
+
+/* Scale-independent scene render: render.render_scene_adaptive (src/weft/render.py:270-358)
+ * for the descriptors its placement gives, as decoder._render_primitive_stack_gpu
+ * (src/weft/decoder.py:231-390) does with the reference decode kernel, plus its RES1 step.
+ * Tile i is rendered as a square of d_descs[i]._pad pixels sampling [0, source_size) with
+ * step source_size / _pad on both axes (the pixel-centre grid when they are equal), cut to
+ * its top-left target_w x target_h, given the RES0 bias and clipped (render_tile); tiles
+ * i < n_res1 then add the bilinear upsample of their res1_grid x res1_grid x 3 float32 grid
+ * d_res1[i] (upsample_residual_map, render.py:474-515, over target_h pixels: the caller
+ * passes RES1 only for square rects) and are clipped again.  Each tile's rect
+ * (target_x0, target_y0, target_w, target_h) is written where it falls inside the
+ * width x height image; rects must not overlap (their writes are unordered). */
+int weft_render_scene(const WeftTileDecodeDesc* d_descs, int n_tiles, const WeftPrim* d_prims,
+                      const float* d_res1, int n_res1, int res1_grid, float* d_out, int width, int height,
+                      void* stream);
 
 /* Stage-3 template fit (src/weft/gpu_encoder.py:42-251, weft_model_bucket and
  * weft_tile_fit_candidates; launched by fit_tiles_gpu_constant, gpu_encoder.py:411-562).

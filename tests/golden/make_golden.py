@@ -25,9 +25,13 @@ pinned against.  Functions captured (reference file:line):
   (encoder.py:343-404)
 - ``encoder._generate_residual_candidates`` src/weft/encoder.py:1255-1366 (pools aimed at a
   prediction's error)
+- ``render.render_tile`` with ``source_size`` (the scaled grid, render.py:52-78) and
+  ``render.render_scene_adaptive`` (render.py:270-358, with ``upsample_residual_map``
+  474-515) -> ``ref_scene.npz``
 
 ``python tests/golden/make_golden.py entropy`` writes only ``ref_entropy.npz``;
-``python tests/golden/make_golden.py candidates`` writes only ``ref_candidates.npz``.
+``python tests/golden/make_golden.py candidates`` writes only ``ref_candidates.npz``;
+``python tests/golden/make_golden.py scene`` writes only ``ref_scene.npz``.
 
 Every primitive parameter is a float32-representable value, so the float32 records the
 HIP path receives carry exactly what the reference's Python floats carry.
@@ -451,8 +455,103 @@ def residual_cases():
     return d
 
 
+def scene_cases():
+    """render_tile on the scaled grid (source_size != tile_size) and render_scene_adaptive
+    (per-tile rects placed by round(), square render then crop, RES0, RES1 upsample, final
+    clip) at integer, non-integer and anisotropic target scales."""
+    from weft.primitives import TileRecord
+    from weft.quadtree import QuadTile, decompose_quadtree
+    from weft.render import render_scene_adaptive, render_tile
+
+    rng = np.random.default_rng(270358)
+    d = {}
+    # --- render_tile(..., tile_size=out, source_size=src)
+    stacks, srcs, outs_sz, resid, hres, outs = [], [], [], [], [], []
+    plan = [(src, r) for src in (8, 16, 32) for r in (0.5, 0.75, 1.5, 2.0, 2.5)] + [(64, 0.5), (64, 1.5)]
+    for ci, (src, ratio) in enumerate(plan):
+        for j in range(2):
+            out_sz = int(round(src * ratio))
+            depth = int(rng.integers(1, 7))
+            mode = (2 * ci + j) % 5
+            if mode == 0:
+                kinds = [0] * depth
+            elif mode == 1:
+                kinds = [4] * depth
+            elif mode == 2:
+                kinds = list(rng.choice([0, 1, 4], depth))
+            else:
+                kinds = list(rng.choice([0, 1, 2, 3, 4], depth))
+            prims = [_rand_prim(rng, src, int(k)) for k in kinds]
+            rr = tuple(int(v) for v in rng.integers(-40, 41, 3)) if j == 1 else None
+            img = render_tile(prims, tile_size=out_sz, residual_rgb=rr, source_size=src)
+            stacks.append(prims)
+            srcs.append(src)
+            outs_sz.append(out_sz)
+            resid.append(rr if rr is not None else (0, 0, 0))
+            hres.append(0 if rr is None else 1)
+            outs.append(np.asarray(img, np.float32).ravel())
+    t = _stack_arrays(stacks)
+    d.update({"st_" + k: v for k, v in t.items()})
+    d["st_source_size"] = np.array(srcs, np.int32)
+    d["st_tile_size"] = np.array(outs_sz, np.int32)
+    d["st_residual"] = np.array(resid, np.int32)
+    d["st_has_residual"] = np.array(hres, np.int32)
+    d["st_out_off"] = np.cumsum([0] + [s * s * 3 for s in outs_sz[:-1]]).astype(np.int64)
+    d["st_out"] = np.concatenate(outs)
+
+    # --- render_scene_adaptive
+    # Scaled scenes use sizes that are multiples of the 32-px macro tile: the reference
+    # writes each scaled rect into a target padded to a multiple of 32 and raises when a
+    # rect of the (padded) quadtree reaches past it.
+    scenes = [  # (width, height, target_w, target_h, with RES1 maps)
+        (100, 72, None, None, True),      # unscaled
+        (96, 64, 192, 128, True),         # 2x (every rect square)
+        (96, 64, 48, 32, True),           # 0.5x (every rect square)
+        (96, 64, 137, 91, False),         # non-integer, rects of unequal sides (square render, crop)
+        (64, 64, 150, 50, False),         # anisotropic
+        (96, 64, 61, 41, False),          # non-integer below 1x
+    ]
+    tiles_all, recs_all, maps_all, out_all = [], [], [], []
+    meta = []
+    for si, (w, h, tw, th, with_res1) in enumerate(scenes):
+        yy, xx = np.mgrid[0:h, 0:w]
+        img = np.zeros((h, w, 3), np.float32)
+        img[..., 0] = (xx / w).astype(np.float32)
+        img[..., 1] = ((xx - w / 2) ** 2 + (yy - h / 3) ** 2 < (h / 4) ** 2).astype(np.float32)
+        img[..., 2] = (rng.random((h, w)) * 0.4).astype(np.float32) * (xx > w // 2)
+        qts = decompose_quadtree(img, split_threshold=0.08, min_tile=8, max_tile=32)
+        recs, maps = [], []
+        for qt in qts:
+            depth = int(rng.integers(0, 6))
+            prims = [_rand_prim(rng, qt.size, int(k)) for k in rng.choice([0, 1, 4], depth)]
+            rr = tuple(int(v) for v in rng.integers(-30, 31, 3)) if rng.random() < 0.5 else (0, 0, 0)
+            recs.append(TileRecord(primitives=prims, residual_rgb=rr))
+            m = rng.integers(-40, 41, (4, 4, 3)).astype(np.int8).astype(np.float32) / 255.0
+            maps.append(m)
+        n_maps = len(qts) - 3 if with_res1 else 0  # render_scene_adaptive maps tiles i < len(maps)
+        out = render_scene_adaptive(recs, qts, w, h, residual_maps=maps[:n_maps] if with_res1 else None,
+                                    res1_grid_size=4, target_width=tw, target_height=th)
+        meta.append([w, h, tw or 0, th or 0, n_maps, len(qts)])
+        tiles_all.append(np.array([[q.x, q.y, q.size] for q in qts], np.int32))
+        recs_all.extend(recs)
+        maps_all.append(np.stack(maps[:n_maps]) if n_maps else np.zeros((0, 4, 4, 3), np.float32))
+        out_all.append(np.asarray(out, np.float32).ravel())
+    t = _stack_arrays([r.primitives for r in recs_all])
+    d.update({"sc_" + k: v for k, v in t.items()})
+    d["sc_meta"] = np.array(meta, np.int32)
+    d["sc_tiles"] = np.concatenate(tiles_all)
+    d["sc_residual"] = np.array([r.residual_rgb for r in recs_all], np.int32)
+    d["sc_maps"] = np.concatenate(maps_all).astype(np.float32)
+    d["sc_out"] = np.concatenate(out_all)
+    return d
+
+
 def main() -> None:
     sys.path.insert(0, REF_SRC)
+    if len(sys.argv) > 1 and sys.argv[1] == "scene":
+        np.savez_compressed(os.path.join(OUT_DIR, "ref_scene.npz"), **scene_cases())
+        print("wrote ref_scene.npz to", OUT_DIR)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "candidates":
         np.savez_compressed(os.path.join(OUT_DIR, "ref_candidates.npz"), **candidate_cases())
         print("wrote ref_candidates.npz to", OUT_DIR)
@@ -468,6 +567,7 @@ def main() -> None:
     np.savez_compressed(os.path.join(OUT_DIR, "ref_misc.npz"), **misc_cases(rng))
     np.savez_compressed(os.path.join(OUT_DIR, "ref_entropy.npz"), **entropy_cases())
     np.savez_compressed(os.path.join(OUT_DIR, "ref_candidates.npz"), **candidate_cases())
+    np.savez_compressed(os.path.join(OUT_DIR, "ref_scene.npz"), **scene_cases())
     print("wrote fixtures to", OUT_DIR)
 
 

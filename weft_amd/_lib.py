@@ -156,10 +156,13 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_refine.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, ctypes.POINTER(RefineOpts),
                               c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_render_tiles.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
+    L.weft_render_scene.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
+                                    c_void_p]
     for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_reset", "weft_fit_order",
                  "weft_fit_score",
                  "weft_fit_select", "weft_fit_step_static", "weft_draw_codes", "weft_get_tables",
-                 "weft_render_tiles", "weft_check_lds", "weft_batch_fit", "weft_wallclock_rate_khz",
+                 "weft_render_tiles", "weft_render_scene", "weft_check_lds", "weft_batch_fit",
+                 "weft_wallclock_rate_khz",
                  "weft_soft_eval", "weft_topk", "weft_refine",
                  "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks",
                  "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks", "weft_template_fit",
@@ -245,7 +248,8 @@ EXPORTED_SYMBOLS = (
     "weft_fit_init", "weft_fit_reset", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_fit_step_static",
     "weft_draw_codes",
     "weft_get_tables",
-    "weft_render_tiles", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval", "weft_topk",
+    "weft_render_tiles", "weft_render_scene", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval",
+    "weft_topk",
     "weft_refine", "weft_fit_score_split", "weft_rans_stage_bytes", "weft_rans_encode_chunks",
     "weft_rans_decode_chunks", "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks",
     "weft_template_fit", "weft_select_work_bytes", "weft_argmin_f64", "weft_exclusive_scan_i64", "weft_compact_flags",

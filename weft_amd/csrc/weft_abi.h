@@ -72,10 +72,13 @@ struct WeftRenderArgs {
     const WeftTileDecodeDesc* descs;
     const WeftPrim* prims;
     float* out;
+    const float* res1;   // RES1 grids of tiles 0 .. n_res1-1 (g x g x 3 float32 each), or NULL
     int32_t n_tiles;
     int32_t width;
     int32_t height;
-    int32_t _pad;
+    int32_t scene;       // 1: desc._pad is the tile's square render size (render_scene_adaptive)
+    int32_t n_res1;
+    int32_t res1_grid;
 };
 
 struct WeftBatchArgs {

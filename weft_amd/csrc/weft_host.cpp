@@ -651,7 +651,23 @@ int weft_render_tiles(const WeftTileDecodeDesc* d_descs, int n_tiles, const Weft
     if (n_tiles < 0 || width < 0 || height < 0) return set_err(WEFT_E_INVALID, "weft_render_tiles: bad sizes");
     if (n_tiles == 0) return WEFT_OK;
     if (!d_descs || !d_prims || !d_out) return set_err(WEFT_E_INVALID, "weft_render_tiles: null pointer");
-    WeftRenderArgs a{d_descs, d_prims, d_out, n_tiles, width, height, 0};
+    WeftRenderArgs a{d_descs, d_prims, d_out, nullptr, n_tiles, width, height, 0, 0, 0};
+    return launch(g_mod.f_render, (unsigned)n_tiles, 256, 0, stream, &a);
+}
+
+int weft_render_scene(const WeftTileDecodeDesc* d_descs, int n_tiles, const WeftPrim* d_prims,
+                      const float* d_res1, int n_res1, int res1_grid, float* d_out, int width, int height,
+                      void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n_tiles < 0 || width < 0 || height < 0 || n_res1 < 0 || n_res1 > n_tiles)
+        return set_err(WEFT_E_INVALID, "weft_render_scene: bad sizes");
+    if (n_res1 > 0 && (!d_res1 || res1_grid < 1 || res1_grid > 64))
+        return set_err(WEFT_E_INVALID, "weft_render_scene: RES1 grids need a pointer and a grid size in 1..64");
+    if (n_tiles == 0) return WEFT_OK;
+    if (!d_descs || !d_prims || !d_out) return set_err(WEFT_E_INVALID, "weft_render_scene: null pointer");
+    WeftRenderArgs a{d_descs, d_prims, d_out, n_res1 > 0 ? d_res1 : nullptr, n_tiles, width, height, 1,
+                     n_res1, res1_grid};
     return launch(g_mod.f_render, (unsigned)n_tiles, 256, 0, stream, &a);
 }
 

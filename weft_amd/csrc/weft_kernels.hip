@@ -1488,32 +1488,68 @@ extern "C" __global__ __launch_bounds__(256) void weft_draw_kernel(WeftDrawArgs 
     }
 }
 
-// Decode: one workgroup per tile, threads over the tile's output pixels.
+// RES1: bilinear weights of the g x g residual grid along one axis of a tile rendered at
+// `up` pixels (upsample_residual_map, src/weft/render.py:474-515).  The sample position
+// is float32 (pixel centre times the float32 grid step, minus 0.5, clamped to the grid);
+// its fraction is float64, as NumPy promotes float32 minus int32.
+struct Res1Axis {
+    int i0, i1;
+    double f;
+};
+
+__device__ __forceinline__ Res1Axis res1_axis(int i, float step, int g) {
+    const float raw = ((float)i + 0.5f) * step - 0.5f;
+    const float cl = fminf(fmaxf(raw, 0.0f), (float)(g - 1));
+    const int i0 = (int)floorf(cl);
+    return Res1Axis{i0, min(i0 + 1, g - 1), (double)cl - (double)i0};
+}
+
+// Decode: one workgroup per tile, threads over the tile's output pixels.  Plain mode
+// (weft_render_tiles) samples the source square with per-axis steps source / target_w and
+// source / target_h, as the reference decode kernel does; scene mode (weft_render_scene)
+// renders a square of desc._pad pixels with step source / _pad on both axes and keeps its
+// top-left target_w x target_h, as render_scene_adaptive renders then crops
+// (src/weft/render.py:322-345), then adds the tile's RES1 grid when it has one.
 extern "C" __global__ __launch_bounds__(256) void weft_render_tiles_kernel(WeftRenderArgs A) {
     const int t = blockIdx.x;
     if (t >= A.n_tiles) return;
     const WeftTileDecodeDesc d = A.descs[t];
     const WeftPrim* pr = A.prims + d.prim_offset;
-    const float stepx = d.source_size == d.target_w ? 1.0f
-                        : (float)((double)d.source_size / (double)d.target_w);
-    const float stepy = d.source_size == d.target_h ? 1.0f
-                        : (float)((double)d.source_size / (double)d.target_h);
+    const int sx = A.scene ? d._pad : d.target_w;
+    const int sy = A.scene ? d._pad : d.target_h;
+    const float stepx = d.source_size == sx ? 1.0f : (float)((double)d.source_size / (double)sx);
+    const float stepy = d.source_size == sy ? 1.0f : (float)((double)d.source_size / (double)sy);
+    const bool res1 = A.res1 != nullptr && t < A.n_res1;
+    const int g = A.res1_grid;
+    const float* map = res1 ? A.res1 + (int64_t)t * g * g * 3 : nullptr;
+    const float ustep = res1 ? (float)((double)g / (double)d.target_h) : 0.0f;
     const int npx = d.target_w * d.target_h;
     for (int p = threadIdx.x; p < npx; p += blockDim.x) {
         const int ly = p / d.target_w, lx = p - ly * d.target_w;
         const float fx = ((float)lx + 0.5f) * stepx;
         const float fy = ((float)ly + 0.5f) * stepy;
-        float r = 0.0f, g = 0.0f, b = 0.0f;
-        for (int i = 0; i < d.prim_count; ++i) composite_px(pr[i], fx, fy, r, g, b);
-        r = clip01(r + d.res0_r);
-        g = clip01(g + d.res0_g);
-        b = clip01(b + d.res0_b);
+        float r = 0.0f, g3 = 0.0f, b = 0.0f;
+        for (int i = 0; i < d.prim_count; ++i) composite_px(pr[i], fx, fy, r, g3, b);
+        float v[3] = {clip01(r + d.res0_r), clip01(g3 + d.res0_g), clip01(b + d.res0_b)};
+        if (res1) {
+            const Res1Axis ax = res1_axis(lx, ustep, g), ay = res1_axis(ly, ustep, g);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double c00 = map[(ay.i0 * g + ax.i0) * 3 + c], c01 = map[(ay.i0 * g + ax.i1) * 3 + c];
+                const double c10 = map[(ay.i1 * g + ax.i0) * 3 + c], c11 = map[(ay.i1 * g + ax.i1) * 3 + c];
+                const double top = c00 * (1.0 - ax.f) + c01 * ax.f;
+                const double bot = c10 * (1.0 - ax.f) + c11 * ax.f;
+                const double up = top * (1.0 - ay.f) + bot * ay.f;
+                const double s = (double)v[c] + up;
+                v[c] = (float)(s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s));  // np.clip keeps -0.0
+            }
+        }
         const int gx = d.target_x0 + lx, gy = d.target_y0 + ly;
         if (gx >= 0 && gx < A.width && gy >= 0 && gy < A.height) {
             const int64_t o = ((int64_t)gy * A.width + gx) * 3;
-            A.out[o] = r;
-            A.out[o + 1] = g;
-            A.out[o + 2] = b;
+            A.out[o] = v[0];
+            A.out[o + 1] = v[1];
+            A.out[o + 2] = v[2];
         }
     }
 }

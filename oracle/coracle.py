@@ -69,6 +69,10 @@ def lib() -> ctypes.CDLL:
     L.orc_fit_plain.restype = ctypes.c_int
     L.orc_fit_plain_ids.argtypes = L.orc_fit_plain.argtypes + [P]
     L.orc_fit_plain_ids.restype = ctypes.c_int
+    L.orc_fit_plain_mode.argtypes = L.orc_fit_plain_ids.argtypes + [ctypes.c_int]
+    L.orc_fit_plain_mode.restype = ctypes.c_int
+    L.orc_objective.argtypes = [ctypes.c_float, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    L.orc_objective.restype = ctypes.c_double
     L.orc_score_candidates.argtypes = [P, P, ctypes.c_int, P, ctypes.c_int, P, P]
     L.orc_max_threads.restype = ctypes.c_int
     L.orc_rans_model.argtypes = [P, ctypes.c_int64, P]
@@ -161,6 +165,11 @@ def codes_to_prims(codes: np.ndarray) -> np.ndarray:
     return out
 
 
+def objective(mse, bits: int, lam: float, mode: int) -> float:
+    """The C oracle's tile objective (mode 0 float64, mode 1 the batched round's float32)."""
+    return lib().orc_objective(float(mse), int(bits), float(lam), int(mode))
+
+
 def score_candidates(src: np.ndarray, canvas: np.ndarray, cands: np.ndarray):
     s = np.ascontiguousarray(src, dtype=np.float32)
     c = np.ascontiguousarray(canvas, dtype=np.float32)
@@ -175,7 +184,7 @@ class PlainFit:
     """Plain tile fit on the CPU (the oracle of weft_amd.fit.PlainFitter)."""
 
     def __init__(self, target_flat: np.ndarray, pix_off: np.ndarray, tsz: np.ndarray, n_cand: int,
-                 depth: int, seed: int, lam: float, tile_ids: np.ndarray | None = None):
+                 depth: int, seed: int, lam: float, tile_ids: np.ndarray | None = None, obj_mode: int = 0):
         self.target = np.ascontiguousarray(target_flat, dtype=np.float32)
         self.canvas = np.zeros_like(self.target)
         self.pix_off = np.ascontiguousarray(pix_off, dtype=np.int64)
@@ -187,6 +196,7 @@ class PlainFit:
         self.cur_obj = np.full(self.n_tiles, -1.0, np.float64)
         self.step = 0
         self.tile_ids = None if tile_ids is None else np.ascontiguousarray(tile_ids, dtype=np.int32)
+        self.obj_mode = int(obj_mode)
 
     def run(self, n_steps: int, want_sse: bool = False, n_threads: int = 0):
         T, P = self.n_tiles, self.n_cand
@@ -195,11 +205,11 @@ class PlainFit:
         wobj = np.zeros((n_steps, T), np.float64)
         wacc = np.zeros((n_steps, T), np.int32)
         wcode = np.zeros((n_steps, T), CODE_DTYPE)
-        lib().orc_fit_plain_ids(_p(self.target), _p(self.canvas), _p(self.pix_off), _p(self.tsz), T, P,
-                                self.depth, self.seed, self.step, n_steps, self.lam, _p(self.n_sel),
-                                _p(self.sel_bits), _p(self.cur_obj), None if sse is None else _p(sse),
-                                _p(widx), _p(wobj), _p(wacc), _p(wcode), n_threads,
-                                None if self.tile_ids is None else _p(self.tile_ids))
+        lib().orc_fit_plain_mode(_p(self.target), _p(self.canvas), _p(self.pix_off), _p(self.tsz), T, P,
+                                 self.depth, self.seed, self.step, n_steps, self.lam, _p(self.n_sel),
+                                 _p(self.sel_bits), _p(self.cur_obj), None if sse is None else _p(sse),
+                                 _p(widx), _p(wobj), _p(wacc), _p(wcode), n_threads,
+                                 None if self.tile_ids is None else _p(self.tile_ids), self.obj_mode)
         self.step += n_steps
         return dict(sse=sse, win_idx=widx, win_obj=wobj, win_acc=wacc, win_code=wcode)
 

@@ -591,12 +591,41 @@ void orc_code_to_prim(const orc_code* c, orc_prim* P) {
  * n_sel, cur_obj: per-tile state, updated in place (initialise n_sel = 0 and
  * cur_obj < 0 to have it computed).
  * ------------------------------------------------------------------------- */
+/* Objective of a tile: mode 0 is _greedy_add's float64 ``float(mse) + lam * bits``
+ * (encoder.py:794-797); mode 1 the batched round's float32 ``mses + lam * rates`` with
+ * float32 arrays (encoder.py:2793-2800: lam rounded to float32, product and sum rounded to
+ * float32), widened to float64 for the comparison with the tile's objective (2802). */
+double orc_objective(float mse, int bits, double lam, int mode) {
+    if (mode == 0) return (double)mse + lam * (double)bits;
+    float prod = (float)lam * (float)bits;
+    float o = mse + prod;
+    return (double)o;
+}
+
+int orc_fit_plain_mode(const float* target, float* canvas, const int64_t* pix_off,
+                       const int32_t* tsz, int n_tiles, int n_cand, int depth, uint64_t seed,
+                       int step0, int n_steps, double lam, int* n_sel, int* sel_bits,
+                       double* cur_obj, float* out_sse, int32_t* win_idx, double* win_obj,
+                       int32_t* win_acc, orc_code* win_code, int n_threads,
+                       const int32_t* tile_ids, int obj_mode);
+
 int orc_fit_plain_ids(const float* target, float* canvas, const int64_t* pix_off,
                       const int32_t* tsz, int n_tiles, int n_cand, int depth, uint64_t seed,
                       int step0, int n_steps, double lam, int* n_sel, int* sel_bits,
                       double* cur_obj, float* out_sse, int32_t* win_idx, double* win_obj,
                       int32_t* win_acc, orc_code* win_code, int n_threads,
                       const int32_t* tile_ids) {
+    return orc_fit_plain_mode(target, canvas, pix_off, tsz, n_tiles, n_cand, depth, seed, step0, n_steps,
+                              lam, n_sel, sel_bits, cur_obj, out_sse, win_idx, win_obj, win_acc, win_code,
+                              n_threads, tile_ids, 0);
+}
+
+int orc_fit_plain_mode(const float* target, float* canvas, const int64_t* pix_off,
+                       const int32_t* tsz, int n_tiles, int n_cand, int depth, uint64_t seed,
+                       int step0, int n_steps, double lam, int* n_sel, int* sel_bits,
+                       double* cur_obj, float* out_sse, int32_t* win_idx, double* win_obj,
+                       int32_t* win_acc, orc_code* win_code, int n_threads,
+                       const int32_t* tile_ids, int obj_mode) {
     orc_tables();
 #ifdef _OPENMP
     if (n_threads > 0) omp_set_num_threads(n_threads);
@@ -621,7 +650,7 @@ int orc_fit_plain_ids(const float* target, float* canvas, const int64_t* pix_off
                 }
                 float sse = orc_np_sum_f32(e, n);
                 float mse = sse / (float)n;
-                cur_obj[t] = (double)mse + lam * (double)sel_bits[t];
+                cur_obj[t] = orc_objective(mse, sel_bits[t], lam, obj_mode);
             }
             int32_t best = -1;
             double best_obj = 0.0;
@@ -640,7 +669,7 @@ int orc_fit_plain_ids(const float* target, float* canvas, const int64_t* pix_off
                     float sse = orc_np_sum_f32(e, n);
                     float mse = sse / (float)n;
                     if (out_sse) out_sse[((size_t)s * n_tiles + t) * n_cand + k] = sse;
-                    double obj = (double)mse + lam * (double)(sel_bits[t] + orc_prim_bytes(P.kind) * 8);
+                    double obj = orc_objective(mse, sel_bits[t] + orc_prim_bytes(P.kind) * 8, lam, obj_mode);
                     if (best < 0 || obj < best_obj) {
                         best = k;
                         best_obj = obj;

@@ -546,8 +546,161 @@ def scene_cases():
     return d
 
 
+def _f32_prim(p):
+    """The primitive with every parameter rounded to float32 (what its packed record holds)."""
+    from weft.primitives import Primitive
+
+    c1 = None if p.color1 is None else tuple(_f(v) for v in p.color1)
+    return Primitive(kind=p.kind, geom=tuple(_f(v) for v in p.geom), color0=tuple(_f(v) for v in p.color0),
+                     color1=c1, alpha=_f(p.alpha))
+
+
+def adaptive_cases():
+    """The batched greedy core of encoder._fit_adaptive_state (encoder.py:2650-2831) as the
+    reference runs it, with gpu_batch_score_prepacked served by its own CPU renderer
+    (render_tile + np.mean over the packed records, the scorer's values) and candidate
+    parameters rounded to float32 where they are generated (the packed records the scorer
+    sees carry exactly those values).  Recorded per scorer call: the tiles, sizes and MSEs;
+    per greedy argmin: the float32 objective array the reference built (captured from the
+    encoder module's own np.argmin calls); at the end every tile's selected stack.  The
+    encoder module's np.argsort calls take the stable order during the run (see _NpProxy)."""
+    import types
+
+    import weft.encoder as enc
+    import weft.gpu_render as gr
+    from weft.render import render_tile
+    from weft.types import EncodeConfig
+
+    real_np = enc.np
+    calls, argmins = [], []
+
+    class _NpProxy(types.ModuleType):
+        def __getattr__(self, name):
+            return getattr(real_np, name)
+
+        @staticmethod
+        def argsort(a, axis=-1, kind=None, order=None, **kw):
+            # NumPy leaves the order of equal keys of the default sort unspecified (it depends
+            # on the CPU's SIMD level); the run takes the stable order, the one this build
+            # implements (the residual curves' end points are picked from such a sort).
+            return real_np.argsort(a, axis=axis, kind="stable" if kind is None else kind, order=order, **kw)
+
+        @staticmethod
+        def argmin(a, *args, **kw):
+            if calls and not args and not kw:
+                argmins.append((len(calls) - 1, np.array(a, copy=True)))
+            return real_np.argmin(a, *args, **kw)
+
+    def _records_to_prims(arr):
+        from weft.primitives import Primitive
+
+        out = []
+        for r in arr:
+            k = int(r["kind"])
+            ng = {0: 0, 1: 4, 2: 5, 3: 7, 4: 6}.get(k, 7)
+            out.append(Primitive(kind=k, geom=tuple(float(v) for v in r["geom"][:ng]),
+                                 color0=tuple(float(v) for v in r["color0"]),
+                                 color1=tuple(float(v) for v in r["color1"]) if k == 1 else None,
+                                 alpha=float(r["alpha"])))
+        return out
+
+    def cpu_scorer(pixels, offsets, tile_indices, tile_sizes, eval_arrays, eval_counts):
+        res = []
+        for ti, ts, arr, cnt in zip(tile_indices, tile_sizes, eval_arrays, eval_counts):
+            src = pixels[offsets[ti]:offsets[ti] + ts * ts * 3].reshape(ts, ts, 3)
+            prims = _records_to_prims(arr)
+            mses, o = [], 0
+            for c in cnt:
+                pred = render_tile(prims[o:o + int(c)], tile_size=ts)
+                mses.append(np.mean((src - pred) ** 2))
+                o += int(c)
+            res.append(np.array(mses, np.float32))
+        calls.append(dict(tiles=list(tile_indices), sizes=list(tile_sizes), mses=[r.copy() for r in res],
+                          counts=[np.asarray(c).copy() for c in eval_counts],
+                          # the first call scores each tile's whole pool as 1-primitive stacks
+                          pools=[np.array(a, copy=True) for a in eval_arrays] if not calls else None))
+        return res
+
+    real_worker, real_resid = enc._worker_gen_cands, enc._generate_residual_candidates
+
+    def worker(args):
+        cands, _ = real_worker(args)
+        cands = [_f32_prim(c) for c in cands]
+        packed = gr._pack_prims(cands) if cands else np.zeros(1, dtype=gr._PRIM_DTYPE)
+        return cands, packed
+
+    def resid(patch, pred, quality):
+        return [_f32_prim(c) for c in real_resid(patch, pred, quality)]
+
+    enc.np = _NpProxy("numpy")
+    enc._worker_gen_cands = worker
+    enc._generate_residual_candidates = resid
+    enc._get_persistent_worker_pool = lambda: None
+    gr.gpu_batch_score_prepacked = cpu_scorer
+    rng = np.random.default_rng(2650)
+    d = {}
+    runs = []
+    try:
+        for run, (w, h, q, n_prims) in enumerate([(64, 64, 75, 14), (48, 40, 92, 10), (64, 32, 60, 8)]):
+            from weft.render import render_tile as _rt  # noqa: F401
+            yy, xx = np.mgrid[0:h, 0:w]
+            img = np.zeros((h, w, 3), np.float32)
+            c = rng.random((4, 3))
+            u, v = xx / (w - 1), yy / (h - 1)
+            img[:] = (((1 - u) * (1 - v))[..., None] * c[0] + (u * (1 - v))[..., None] * c[1]
+                      + ((1 - u) * v)[..., None] * c[2] + (u * v)[..., None] * c[3]).astype(np.float32)
+            for _ in range(n_prims):
+                cx, cy, r = rng.uniform(0, w), rng.uniform(0, h), rng.uniform(3, 14)
+                m = (xx - cx) ** 2 + (yy - cy) ** 2 <= r * r if rng.random() < 0.5 else \
+                    (np.abs(xx - cx) <= r) & (np.abs(yy - cy) <= r / 2)
+                img[m] = rng.random(3).astype(np.float32)
+            img += (rng.normal(0, 2 / 255, img.shape)).astype(np.float32)
+            img = np.clip(img, 0, 1).astype(np.float32)
+            enc.load_image_linear = lambda _p, _img=img: (_img, _img.shape[1], _img.shape[0])
+            calls.clear()
+            argmins.clear()
+            state = enc._fit_adaptive_state("synthetic", EncodeConfig(quality=q))
+            qts = state.quad_tiles
+            runs.append(dict(img=img, q=q, lam=state.lam, tiles=np.array([[t.x, t.y, t.size] for t in qts], np.int32),
+                             calls=list(calls), argmins=list(argmins),
+                             selected=[gr._pack_prims(s) if s else np.zeros(0, gr._PRIM_DTYPE)
+                                       for s in state.selected_per_tile],
+                             nsel=[len(s) for s in state.selected_per_tile]))
+    finally:
+        enc.np = real_np
+        enc._worker_gen_cands, enc._generate_residual_candidates = real_worker, real_resid
+    for k, r in enumerate(runs):
+        p = f"r{k}_"
+        d[p + "img"] = r["img"]
+        d[p + "quality"] = np.array(r["q"], np.int32)
+        d[p + "lam"] = np.array(r["lam"], np.float64)
+        d[p + "tiles"] = r["tiles"]
+        d[p + "n_calls"] = np.array(len(r["calls"]), np.int32)
+        for ci, c in enumerate(r["calls"]):
+            d[p + f"c{ci}_tiles"] = np.array(c["tiles"], np.int32)
+            d[p + f"c{ci}_mse_off"] = np.cumsum([0] + [len(m) for m in c["mses"]]).astype(np.int64)
+            d[p + f"c{ci}_mse"] = np.concatenate(c["mses"]) if c["mses"] else np.zeros(0, np.float32)
+            objs = [a for (cc, a) in r["argmins"] if cc == ci][:len(c["tiles"])] if ci != 1 else []
+            if objs:
+                d[p + f"c{ci}_obj"] = np.concatenate(objs).astype(np.float32)
+                d[p + f"c{ci}_obj_off"] = np.cumsum([0] + [len(a) for a in objs]).astype(np.int64)
+        pools = r["calls"][0]["pools"]
+        d[p + "pool_tiles"] = np.array(r["calls"][0]["tiles"], np.int32)
+        d[p + "pool_off"] = np.cumsum([0] + [len(a) for a in pools]).astype(np.int64)
+        d[p + "pool"] = np.concatenate(pools).view(np.uint8)
+        sel = r["selected"]
+        d[p + "sel"] = np.concatenate(sel) if sel else np.zeros(0, gr._PRIM_DTYPE)
+        d[p + "sel"] = d[p + "sel"].view(np.uint8)
+        d[p + "nsel"] = np.array(r["nsel"], np.int32)
+    return d
+
+
 def main() -> None:
     sys.path.insert(0, REF_SRC)
+    if len(sys.argv) > 1 and sys.argv[1] == "adaptive":
+        np.savez_compressed(os.path.join(OUT_DIR, "ref_adaptive.npz"), **adaptive_cases())
+        print("wrote ref_adaptive.npz to", OUT_DIR)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "scene":
         np.savez_compressed(os.path.join(OUT_DIR, "ref_scene.npz"), **scene_cases())
         print("wrote ref_scene.npz to", OUT_DIR)

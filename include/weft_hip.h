@@ -119,15 +119,30 @@ typedef struct WeftWork {
     int32_t _pad;
 } WeftWork;
 
-/* Candidate source: explicit records or drawn on the device from (seed, step). */
+/* Candidate source: explicit records or drawn on the device from (seed, step).
+ * Explicit pools may be ragged and shrink: with d_tile_ncand, tile t has d_tile_ncand[t]
+ * live candidates (at most n_cand, which is then each pool's capacity and the row stride of
+ * the class order and the score tables); with d_tile_active, only tiles whose entry is
+ * nonzero take part in the step (the others are handled like tiles whose stack is full);
+ * with WEFT_CANDS_REMOVE_WINNER in flags, weft_fit_select removes an accepted winner from its
+ * pool as the reference does (src/weft/encoder.py:2805-2808: the records behind it move
+ * down one place in order and d_tile_ncand[t] decreases) and stores it at slot
+ * n_cand - 1 - nsel of the tile's region (the accepted stack grows down from the top), so
+ * d_prims is written; with
+ * WEFT_CANDS_FORCE_ACCEPT it accepts the winner whatever its objective (the reference's
+ * initial pick, encoder.py:2700-2715).  The extra fields are for explicit candidates. */
+#define WEFT_CANDS_REMOVE_WINNER 1
+#define WEFT_CANDS_FORCE_ACCEPT 2
 typedef struct WeftCands {
     const WeftPrim* d_prims;     /* NULL: draw codes on the device */
     const int64_t* d_tile_cand;  /* per tile, offset of its first candidate in d_prims */
     uint64_t seed;
     uint32_t step;
-    int32_t n_cand;              /* candidates per tile */
+    int32_t n_cand;              /* candidates per tile (with d_tile_ncand: pool capacity) */
     int32_t has_general;         /* nonzero if any explicit candidate has kind 1, 2 or 3 */
-    int32_t _pad;
+    int32_t flags;               /* WEFT_CANDS_* */
+    const int32_t* d_tile_active;  /* NULL: every tile takes part */
+    int32_t* d_tile_ncand;       /* NULL: n_cand candidates in every pool */
 } WeftCands;
 
 enum {
@@ -208,6 +223,50 @@ int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_
                     const int32_t* d_tile_nwork, const int32_t* d_tile_wstride, const double* d_part_obj,
                     const int32_t* d_part_idx, int32_t* d_win_idx, double* d_win_obj,
                     int32_t* d_win_acc, WeftCode* d_win_code, void* stream);
+
+/* ---- The reference's batched greedy rounds (src/weft/encoder.py:2682-2831) ----------
+ * Round control of encoder._fit_adaptive_state on the device, for explicit ragged pools
+ * (WeftCands with d_tile_ncand / d_tile_active pointing at this struct's arrays): a round is
+ * weft_fit_order + weft_fit_score + weft_fit_select (float32 objective, obj_mode 1,
+ * WEFT_CANDS_REMOVE_WINNER), then weft_adaptive_gate decides what the reference's loop does
+ * next.  One workgroup walks all tiles, so every mode is one small launch:
+ *   WEFT_ADAPT_START  complex[t] = obj[t] > threshold && ncand[t] > 0 (encoder.py:2737-2739)
+ *   WEFT_ADAPT_COUNT  counts[1] = tiles whose winner was accepted (d_win_acc)
+ *   WEFT_ADAPT_KEEP   complex[t] = win_acc[t]: only improved tiles stay (encoder.py:2831)
+ *   WEFT_ADAPT_RESEED active[t] = complex[t] && nsel[t] < max_prims[t]: the tiles whose pools
+ *                     a stall re-seeds (encoder.py:2813-2828)
+ *   WEFT_ADAPT_NEXT   (also after START and KEEP) active[t] = complex[t] && ncand[t] > 0 &&
+ *                     nsel[t] < max_prims[t] (encoder.py:2753-2757)
+ * counts[0] receives the number of active tiles. */
+#define WEFT_ADAPT_START 0
+#define WEFT_ADAPT_COUNT 1
+#define WEFT_ADAPT_KEEP 2
+#define WEFT_ADAPT_RESEED 3
+#define WEFT_ADAPT_NEXT 4
+typedef struct WeftAdaptive {
+    int32_t* d_complex;          /* per tile: still searched (complex_tiles) */
+    int32_t* d_active;           /* per tile: takes part in the next round / re-seed */
+    int32_t* d_tile_ncand;       /* per tile: live pool size */
+    const int32_t* d_max_prims;  /* per tile: QuadTile.max_primitives */
+    const int32_t* d_win_acc;    /* per tile: the last select's accepted flags */
+    int32_t* d_counts;           /* [2]: active tiles, improved tiles (or records dropped) */
+    double threshold;            /* complexity threshold of the initial objective */
+    int32_t n_tiles;
+    int32_t _pad;
+} WeftAdaptive;
+
+int weft_adaptive_gate(const WeftFitState* st, const WeftAdaptive* a, int mode, void* stream);
+
+/* Re-seed: append the residual candidates of list entry i (d_new[i * max_new ..], d_new_count[i]
+ * records, from weft_gen_residual_candidates over the tiles d_tiles[0 .. n)) to the pool of
+ * tile d_tiles[i] when a->d_active marks it (after WEFT_ADAPT_RESEED), in order behind its live
+ * candidates.  Pools hold `pool_cap` records each (the WeftCands n_cand of the rounds), tile t's
+ * at d_pools + t * pool_cap, with the tile's accepted stack at the top of the region (see
+ * WEFT_CANDS_REMOVE_WINNER), so live + accepted records never exceed pool_cap: records that
+ * would not fit are dropped and counted in a->d_counts[1] (which WEFT_ADAPT_RESEED sets to 0). */
+int weft_adaptive_append(const WeftFitState* st, const WeftAdaptive* a, WeftPrim* d_pools, int pool_cap,
+                         const WeftPrim* d_new, const int32_t* d_new_count, int max_new, const int32_t* d_tiles,
+                         int n, void* stream);
 
 /* Allocation-free plain fit step.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops,

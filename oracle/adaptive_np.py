@@ -74,7 +74,10 @@ def fit(tiles: list[np.ndarray], pools: list[np.ndarray], quality: int, lam: flo
         init["objs"].append(o)
         init.setdefault("kinds", []).append(pool[t]["kind"].copy())
         init.setdefault("stack_gauss", []).append(False)
-        accept(t, int(np.argmin(o)))
+        k = int(np.argmin(o))
+        init.setdefault("best", []).append(k)
+        init.setdefault("acc", []).append(True)
+        accept(t, k)
     calls.append(init)
     cur = np.zeros(T, np.float64)
     for t in range(T):

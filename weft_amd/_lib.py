@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 17
+ABI_VERSION = 18
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -52,8 +52,24 @@ class Work(ctypes.Structure):
 class Cands(ctypes.Structure):
     _fields_ = [
         ("d_prims", c_void_p), ("d_tile_cand", c_void_p), ("seed", c_uint64), ("step", c_uint32),
-        ("n_cand", c_int32), ("has_general", c_int32), ("_pad", c_int32),
+        ("n_cand", c_int32), ("has_general", c_int32), ("flags", c_int32), ("d_tile_active", c_void_p),
+        ("d_tile_ncand", c_void_p),
     ]
+
+
+CANDS_REMOVE_WINNER = 1
+CANDS_FORCE_ACCEPT = 2
+
+
+class Adaptive(ctypes.Structure):
+    _fields_ = [
+        ("d_complex", c_void_p), ("d_active", c_void_p), ("d_tile_ncand", c_void_p), ("d_max_prims", c_void_p),
+        ("d_win_acc", c_void_p), ("d_counts", c_void_p), ("threshold", c_double), ("n_tiles", c_int32),
+        ("_pad", c_int32),
+    ]
+
+
+ADAPT_START, ADAPT_COUNT, ADAPT_KEEP, ADAPT_RESEED, ADAPT_NEXT = range(5)
 
 
 class Batch(ctypes.Structure):
@@ -95,7 +111,8 @@ class SplitWork(ctypes.Structure):
 assert ctypes.sizeof(SplitWork) == 56
 assert ctypes.sizeof(SoftBatch) == 56 and ctypes.sizeof(RefineOpts) == 112
 assert ctypes.sizeof(Batch) == 112
-assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 32 and ctypes.sizeof(Cands) == 40
+assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 32 and ctypes.sizeof(Cands) == 56
+assert ctypes.sizeof(Adaptive) == 64
 
 _lock = threading.Lock()
 _lib: ctypes.CDLL | None = None
@@ -156,12 +173,16 @@ def _bind(L: ctypes.CDLL) -> None:
     L.weft_refine.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Cands), c_void_p, ctypes.POINTER(RefineOpts),
                               c_void_p, c_void_p, c_void_p, c_void_p]
     L.weft_render_tiles.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]
+    L.weft_adaptive_gate.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Adaptive), c_int, c_void_p]
+    L.weft_adaptive_append.argtypes = [ctypes.POINTER(FitState), ctypes.POINTER(Adaptive), c_void_p, c_int, c_void_p,
+                                       c_void_p, c_int, c_void_p, c_int, c_void_p]
     L.weft_render_scene.argtypes = [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_int,
                                     c_void_p]
     for name in ("weft_init", "weft_shutdown", "weft_score_evals", "weft_fit_init", "weft_fit_reset", "weft_fit_order",
                  "weft_fit_score",
                  "weft_fit_select", "weft_fit_step_static", "weft_draw_codes", "weft_get_tables",
                  "weft_render_tiles", "weft_render_scene", "weft_check_lds", "weft_batch_fit",
+                 "weft_adaptive_gate", "weft_adaptive_append",
                  "weft_wallclock_rate_khz",
                  "weft_soft_eval", "weft_topk", "weft_refine",
                  "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks",
@@ -248,7 +269,7 @@ EXPORTED_SYMBOLS = (
     "weft_fit_init", "weft_fit_reset", "weft_fit_order", "weft_fit_score", "weft_fit_select", "weft_fit_step_static",
     "weft_draw_codes",
     "weft_get_tables",
-    "weft_render_tiles", "weft_render_scene", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval",
+    "weft_render_tiles", "weft_render_scene", "weft_adaptive_gate", "weft_adaptive_append", "weft_batch_fit", "weft_wallclock_rate_khz", "weft_soft_eval",
     "weft_topk",
     "weft_refine", "weft_fit_score_split", "weft_rans_stage_bytes", "weft_rans_encode_chunks",
     "weft_rans_decode_chunks", "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks",

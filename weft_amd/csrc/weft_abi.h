@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 17
+#define WEFT_ABI_VERSION 18
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -254,4 +254,24 @@ struct WeftResGenArgs {
     int32_t tile_size;
     int32_t clip_pred;
     int32_t max_cands;
+};
+
+struct WeftAdaptGateArgs {
+    WeftFitState st;
+    WeftAdaptive ad;
+    int32_t mode;
+    int32_t _pad;
+};
+
+struct WeftAdaptAppendArgs {
+    WeftFitState st;
+    WeftAdaptive ad;
+    WeftPrim* pools;
+    const WeftPrim* fresh;
+    const int32_t* fresh_count;
+    const int32_t* tiles;
+    int32_t pool_cap;
+    int32_t max_new;
+    int32_t n;
+    int32_t _pad;
 };

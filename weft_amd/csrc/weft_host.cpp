@@ -64,6 +64,8 @@ struct Module {
     hipFunction_t f_rans_dec = nullptr;
     hipFunction_t f_batch_reset = nullptr;
     hipFunction_t f_render = nullptr;
+    hipFunction_t f_adapt_gate = nullptr;
+    hipFunction_t f_adapt_append = nullptr;
     int64_t max_lds = 0;                            // hipDeviceAttributeMaxSharedMemoryPerBlock
     unsigned char* scratch = nullptr;               // device address of weft_scratch on `device`
     std::unordered_map<hipFunction_t, int> static_lds;  // __shared__ bytes of each kernel
@@ -176,7 +178,8 @@ int weft_check_lds(int64_t bytes) { return lds_check(bytes, "weft_check_lds"); }
 // The Python binding (weft_amd/_lib.py) mirrors these layouts.
 static_assert(sizeof(WeftFitState) == 80, "WeftFitState layout");
 static_assert(sizeof(WeftWork) == 32, "WeftWork layout");
-static_assert(sizeof(WeftCands) == 40, "WeftCands layout");
+static_assert(sizeof(WeftCands) == 56, "WeftCands layout");
+static_assert(sizeof(WeftAdaptive) == 64, "WeftAdaptive layout");
 static_assert(sizeof(WeftBatch) == 112, "WeftBatch layout");
 static_assert(sizeof(WeftSoftBatch) == 56, "WeftSoftBatch layout");
 static_assert(sizeof(WeftRefineOpts) == 112, "WeftRefineOpts layout");
@@ -244,6 +247,8 @@ int weft_init(int device) {
         {&g_mod.f_rans_dec, "weft_rans_decode_kernel"},
         {&g_mod.f_batch_reset, "weft_batch_reset_kernel"},
         {&g_mod.f_render, "weft_render_tiles_kernel"},
+        {&g_mod.f_adapt_gate, "weft_adaptive_gate_kernel"},
+        {&g_mod.f_adapt_append, "weft_adaptive_append_kernel"},
     };
     for (auto& fn : fns) {
         hipError_t e = hipModuleGetFunction(fn.f, g_mod.mod, fn.name);
@@ -391,9 +396,37 @@ int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_
     if (!st || !cands || !d_tile_work0 || !d_tile_nwork || !d_part_obj || !d_part_idx ||
         !d_win_idx || !d_win_obj || !d_win_acc)
         return set_err(WEFT_E_INVALID, "weft_fit_select: null argument");
+    if ((cands->flags & WEFT_CANDS_REMOVE_WINNER) && (!cands->d_prims || !cands->d_tile_ncand))
+        return set_err(WEFT_E_INVALID, "weft_fit_select: removing winners needs explicit pools with live counts");
     WeftFitSelectArgs a{*st, *cands, d_tile_work0, d_tile_nwork, d_tile_wstride, d_part_obj, d_part_idx,
                         d_win_idx, d_win_obj, d_win_acc, d_win_code};
     return launch(g_mod.f_select, (unsigned)st->n_tiles, 256, 0, stream, &a);
+}
+
+int weft_adaptive_gate(const WeftFitState* st, const WeftAdaptive* a, int mode, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !a || !a->d_complex || !a->d_active || !a->d_tile_ncand || !a->d_max_prims || !a->d_counts)
+        return set_err(WEFT_E_INVALID, "weft_adaptive_gate: null argument");
+    if (mode < WEFT_ADAPT_START || mode > WEFT_ADAPT_NEXT) return set_err(WEFT_E_INVALID, "weft_adaptive_gate: bad mode");
+    if ((mode == WEFT_ADAPT_COUNT || mode == WEFT_ADAPT_KEEP) && !a->d_win_acc)
+        return set_err(WEFT_E_INVALID, "weft_adaptive_gate: accepted flags needed");
+    if (a->n_tiles != st->n_tiles || a->n_tiles < 0) return set_err(WEFT_E_INVALID, "weft_adaptive_gate: tile counts differ");
+    WeftAdaptGateArgs g{*st, *a, mode, 0};
+    return launch(g_mod.f_adapt_gate, 1, 1024, 0, stream, &g);
+}
+
+int weft_adaptive_append(const WeftFitState* st, const WeftAdaptive* a, WeftPrim* d_pools, int pool_cap,
+                         const WeftPrim* d_new, const int32_t* d_new_count, int max_new, const int32_t* d_tiles,
+                         int n, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (!st || !a || !a->d_active || !a->d_tile_ncand || !a->d_counts || n < 0 || pool_cap < 1 || max_new < 1)
+        return set_err(WEFT_E_INVALID, "weft_adaptive_append: bad arguments");
+    if (n == 0) return WEFT_OK;
+    if (!d_pools || !d_new || !d_new_count || !d_tiles) return set_err(WEFT_E_INVALID, "weft_adaptive_append: null pointer");
+    WeftAdaptAppendArgs g{*st, *a, d_pools, d_new, d_new_count, d_tiles, pool_cap, max_new, n, 0};
+    return launch(g_mod.f_adapt_append, (unsigned)n, 256, 0, stream, &g);
 }
 
 int weft_fit_step_static(const WeftFitState* st, const WeftWork* work, const WeftCands* cands,

@@ -907,6 +907,14 @@ __device__ __forceinline__ int candidate_class(const WeftCands& cd, uint64_t dba
     return cls;
 }
 
+// Live candidates of a tile's pool, and whether the tile takes part in this step.
+__device__ __forceinline__ int tile_ncand(const WeftCands& cd, int t) {
+    return cd.d_tile_ncand ? cd.d_tile_ncand[t] : cd.n_cand;
+}
+__device__ __forceinline__ bool tile_skipped(const WeftFitState& st, const WeftCands& cd, int t) {
+    return st.d_tile_nsel[t] >= st.depth || (cd.d_tile_active != nullptr && cd.d_tile_active[t] == 0);
+}
+
 // ---------------------------------------------------------------------------
 // Plain fit: score one work item (one tile, `blockDim.x` consecutive slots of the
 // tile's class-sorted candidate order, so waves are class-uniform except at the at
@@ -931,7 +939,8 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
     const int c0 = A.wk.d_work_cand0[w];
     const WeftFitState& st = A.st;
 
-    if (st.d_tile_nsel[tile] >= st.depth) {
+    const int ncand = tile_ncand(A.cd, tile);
+    if (tile_skipped(st, A.cd, tile) || c0 >= ncand) {
         if (tid == 0) {
             A.part_obj[w] = __builtin_huge_val();
             A.part_idx[w] = -1;
@@ -946,7 +955,7 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
                               : (A.cd.d_tile_cand ? A.cd.d_tile_cand[tile]
                                                   : (int64_t)tile * A.cd.n_cand);
     const int slot = c0 + tid;
-    const int ks = slot >= A.cd.n_cand ? -1
+    const int ks = slot >= ncand ? -1
                    : (drawn ? drawn_sorted_index(dbase, A.cd.n_cand, slot)
                             : A.order[(int64_t)tile * A.cd.n_cand + slot]);
     WeftPrim P;
@@ -1031,8 +1040,8 @@ __device__ void fit_order_body(const WeftFitOrderArgs& A) {
     const int wave = tid >> 6;
     const int nwaves = nthr >> 6;
     const WeftFitState& st = A.st;
-    if (st.d_tile_nsel[tile] >= st.depth) return;
-    const int n = A.cd.n_cand;
+    if (tile_skipped(st, A.cd, tile)) return;
+    const int n = tile_ncand(A.cd, tile);
     const bool drawn = A.cd.d_prims == nullptr;
     const uint64_t dbase = draw_base(A.cd.seed, A.cd.step, (uint32_t)tile);
     const int64_t cand_base = drawn ? 0
@@ -1063,7 +1072,7 @@ __device__ void fit_order_body(const WeftFitOrderArgs& A) {
         }
     }
     __syncthreads();
-    int32_t* out = A.order + (int64_t)tile * n;
+    int32_t* out = A.order + (int64_t)tile * A.cd.n_cand;
     for (int j0 = 0; j0 < n; j0 += nthr) {
         const int k = j0 + tid;
         const int cls = k < n ? candidate_class(A.cd, dbase, cand_base, k) : CL_NONE;
@@ -1393,7 +1402,7 @@ extern "C" __global__ __launch_bounds__(256) void weft_fit_select_kernel(WeftFit
     const int t = blockIdx.x;
     const int tid = threadIdx.x;
     const WeftFitState& st = A.st;
-    if (st.d_tile_nsel[t] >= st.depth) {
+    if (tile_skipped(st, A.cd, t)) {
         if (tid == 0) {
             A.win_idx[t] = -1;
             A.win_obj[t] = __builtin_huge_val();
@@ -1413,7 +1422,7 @@ extern "C" __global__ __launch_bounds__(256) void weft_fit_select_kernel(WeftFit
         const bool found = bi != 0x7fffffff;
         s_idx = found ? bi : -1;
         s_obj = bo;
-        s_acc = found && (bo < st.d_tile_obj[t]);
+        s_acc = found && ((A.cd.flags & WEFT_CANDS_FORCE_ACCEPT) != 0 || bo < st.d_tile_obj[t]);
     }
     __syncthreads();
     const int bi = s_idx;
@@ -1429,6 +1438,28 @@ extern "C" __global__ __launch_bounds__(256) void weft_fit_select_kernel(WeftFit
         } else {
             const int64_t cb = A.cd.d_tile_cand ? A.cd.d_tile_cand[t] : (int64_t)t * A.cd.n_cand;
             P = load_prim(A.cd.d_prims + cb + bi);
+            if (A.cd.flags & WEFT_CANDS_REMOVE_WINNER) {
+                // the winner leaves the pool: records bi+1 .. n-1 move down one place, in
+                // chunks of one 32-bit word per thread (every thread has read the winner and
+                // each chunk is read in full before any of it is written)
+                __syncthreads();
+                const int n = A.cd.d_tile_ncand[t];
+                uint32_t* pool = reinterpret_cast<uint32_t*>(const_cast<WeftPrim*>(A.cd.d_prims) + cb);
+                constexpr int RW = (int)(sizeof(WeftPrim) / 4);
+                const int w0 = bi * RW, w1 = (n - 1) * RW;
+                for (int c = w0; c < w1; c += blockDim.x) {
+                    const int i = c + tid;
+                    const uint32_t v = i < w1 ? pool[i + RW] : 0u;
+                    __syncthreads();
+                    if (i < w1) pool[i] = v;
+                    __syncthreads();
+                }
+                if (tid == 0) {
+                    A.cd.d_tile_ncand[t] = n - 1;
+                    // the accepted stack grows down from the top of the tile's pool region
+                    const_cast<WeftPrim*>(A.cd.d_prims)[cb + A.cd.n_cand - 1 - st.d_tile_nsel[t]] = P;
+                }
+            }
         }
         float* cv = st.d_canvas + st.d_tile_pix[t] * 3;
         for (int p = tid; p < ts * ts; p += blockDim.x) {
@@ -1449,6 +1480,69 @@ extern "C" __global__ __launch_bounds__(256) void weft_fit_select_kernel(WeftFit
             st.d_tile_bits[t] += prim_bytes(P.kind) * 8;
             st.d_tile_obj[t] = s_obj;
         }
+    }
+}
+
+// Round control of the reference's batched greedy loop: one workgroup over all tiles.
+extern "C" __global__ __launch_bounds__(1024) void weft_adaptive_gate_kernel(WeftAdaptGateArgs A) {
+    __shared__ int s_sum[2][16];
+    const WeftAdaptive& ad = A.ad;
+    const WeftFitState& st = A.st;
+    int n_act = 0, n_acc = 0;
+    for (int t = threadIdx.x; t < ad.n_tiles; t += blockDim.x) {
+        int cx = ad.d_complex[t];
+        const int ncand = ad.d_tile_ncand[t];
+        const bool room = st.d_tile_nsel[t] < ad.d_max_prims[t];
+        if (A.mode == WEFT_ADAPT_START) cx = st.d_tile_obj[t] > ad.threshold && ncand > 0;
+        if (A.mode == WEFT_ADAPT_KEEP) cx = ad.d_win_acc[t] != 0;
+        if (A.mode == WEFT_ADAPT_COUNT) {
+            n_acc += ad.d_win_acc[t] != 0;
+            continue;
+        }
+        ad.d_complex[t] = cx;
+        const int act = A.mode == WEFT_ADAPT_RESEED ? (cx && room) : (cx && ncand > 0 && room);
+        ad.d_active[t] = act;
+        n_act += act;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        n_act += __shfl_xor(n_act, off, 64);
+        n_acc += __shfl_xor(n_acc, off, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s_sum[0][wave] = n_act;
+        s_sum[1][wave] = n_acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int a = 0, b = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            a += s_sum[0][w];
+            b += s_sum[1][w];
+        }
+        if (A.mode == WEFT_ADAPT_COUNT) ad.d_counts[1] = b;
+        else ad.d_counts[0] = a;
+        if (A.mode == WEFT_ADAPT_RESEED) ad.d_counts[1] = 0;  // records the append drops
+    }
+}
+
+// Re-seed append: one workgroup per list entry, its new records behind the tile's live ones.
+extern "C" __global__ __launch_bounds__(256) void weft_adaptive_append_kernel(WeftAdaptAppendArgs A) {
+    const int i = blockIdx.x;
+    if (i >= A.n) return;
+    const int t = A.tiles[i];
+    if (!A.ad.d_active[t]) return;
+    const int n0 = A.ad.d_tile_ncand[t];
+    const int want = A.fresh_count[i];
+    const int room = A.pool_cap - n0 - A.st.d_tile_nsel[t];  // the accepted stack holds the top
+    const int k = max(0, want < room ? want : room);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(A.fresh + (int64_t)i * A.max_new);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(A.pools + (int64_t)t * A.pool_cap + n0);
+    constexpr int RW = (int)(sizeof(WeftPrim) / 4);
+    for (int w = threadIdx.x; w < k * RW; w += blockDim.x) dst[w] = src[w];
+    if (threadIdx.x == 0) {
+        A.ad.d_tile_ncand[t] = n0 + k;
+        if (want > k) atomicAdd(&A.ad.d_counts[1], want - k);
     }
 }
 

@@ -70,6 +70,32 @@ def test_batch_equals_stepwise_and_oracle(gpu, oracle_lib):
     assert (bf.d_image_left.cpu().numpy() == 0).all()
 
 
+def test_float32_objective_mode_equals_oracle(gpu, oracle_lib):
+    """obj_mode 1, the reference batched round's float32 objective (encoder.py:2793-2800),
+    in the stepwise and the persistent fit against the C oracle's mode 1, bit for bit; with
+    a lambda large enough that the float32 rounding of lam * bits decides acceptances."""
+    from oracle.coracle import PlainFit
+    from weft_amd.batch import BatchFitter
+    from weft_amd.fit import PlainFitter
+
+    tiles = _batch_tiles()
+    n_cand, depth, steps, seed, lam = 200, 5, 6, 31, 0.00049
+    _say(f"float32 objective: {tiles.n_tiles} tiles, lam {lam}")
+    pf = PlainFitter(tiles, n_cand=n_cand, depth=depth, seed=seed, lam=lam, obj_mode=1, max_steps=steps)
+    pf.step(steps)
+    bf = BatchFitter(tiles, n_cand=n_cand, depth=depth, n_steps=steps, seed=seed, lam=lam, obj_mode=1)
+    bf.run(use_graph=False)
+    o = PlainFit(tiles.target, tiles.pix_off, tiles.ts, n_cand, depth, seed, lam, obj_mode=1)
+    ro = o.run(steps)
+    _same_records(pf.records(), ro, "stepwise vs oracle")
+    _same_records(bf.records(), ro, "persistent vs oracle")
+    assert bits_equal(pf.canvas(), o.canvas) and bits_equal(bf.canvas(), o.canvas)
+    assert bits_equal(pf.state()["obj"], o.cur_obj)
+    o64 = PlainFit(tiles.target, tiles.pix_off, tiles.ts, n_cand, depth, seed, lam, obj_mode=0)
+    o64.run(steps)
+    assert not bits_equal(o64.cur_obj, o.cur_obj), "the two objective modes differ"
+
+
 def test_batch_c3_full_size(gpu, oracle_lib):
     """Config 3 at full size: 64 images (482,021 tiles), 512 candidates, two rounds in one
     persistent launch equal the stepwise fit on every tile, and the C oracle on a sample."""

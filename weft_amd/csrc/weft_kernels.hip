@@ -423,6 +423,7 @@ __device__ __forceinline__ void pairwise_rows(Px& px, PwState& st, int y0, int y
         px.row(y);
         if constexpr (Px::kUnrollRow) {
             // whole row unrolled: column index x is a compile-time constant
+            if constexpr (Px::kGroup) px.group(0, 0.5f);
 #pragma unroll
             for (int x = 0; x < TS; ++x) {
                 float e0, e1, e2;
@@ -436,6 +437,7 @@ __device__ __forceinline__ void pairwise_rows(Px& px, PwState& st, int y0, int y
             float gxf = 0.5f;  // sample x of the group's first pixel (exact)
 #pragma unroll 1
             for (int gx = 0; gx < TS; gx += 8) {
+                if constexpr (Px::kGroup) px.group(gx, gxf);
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     float e0, e1, e2;
@@ -473,6 +475,7 @@ __device__ __forceinline__ float pairwise_tile(Px& px) {
 // Reference-semantics stack walk over a zero canvas; target read from global memory.
 struct PxStack {
     static constexpr bool kUnrollRow = false;
+    static constexpr bool kGroup = false;
     const float* tgt;
     const WeftPrim* prims;
     int n;
@@ -526,6 +529,7 @@ struct PxLdsBase {
 template <int TS>
 struct PxConst : PxLdsBase<TS> {
     static constexpr bool kUnrollRow = false;
+    static constexpr bool kGroup = false;
     __device__ __forceinline__ void row(int) {}
     __device__ __forceinline__ void pixel(int y, int x, float, float& e0, float& e1, float& e2) {
         this->finish(y, x, true, e0, e1, e2);
@@ -538,6 +542,7 @@ struct PxConst : PxLdsBase<TS> {
 template <int TS, bool ELL>
 struct PxEllRect : PxLdsBase<TS> {
     static constexpr bool kUnrollRow = TS <= WEFT_COLPRE_MAX;
+    static constexpr bool kGroup = false;
     static constexpr int NCOL = kUnrollRow ? TS : 1;
     float cx, cy, ca, sa, cb, sb;
     float p2, q1;
@@ -581,10 +586,12 @@ struct PxEllRect : PxLdsBase<TS> {
 // same float -- the IEEE division's result, at the cost of one float64 multiply.
 template <int TS>
 struct PxTri : PxLdsBase<TS> {
+    static constexpr bool kGroup = true;
     static constexpr bool kUnrollRow = TS <= WEFT_COLPRE_MAX;
     static constexpr int NCOL = kUnrollRow ? TS : 1;
     float X2, Y2, A, B, C, E;
-    double RD;
+    double RD;  // RN64(1 / D)
+    float RF;   // RN32(1 / D)
     float bdy, edy;
     float AD[NCOL], CD[NCOL];
     __device__ __forceinline__ void init() {
@@ -602,8 +609,8 @@ struct PxTri : PxLdsBase<TS> {
         bdy = B * dy;
         edy = E * dy;
     }
-    __device__ __forceinline__ void pixel(int y, int x, float pxf, float& e0, float& e1, float& e2) {
-        float na, nb;
+    // Numerators of pixel x (column group start gx, sample x pxf) of the current row.
+    __device__ __forceinline__ void numer(int x, float pxf, float& na, float& nb) const {
         if constexpr (kUnrollRow) {
             na = AD[x] + bdy;
             nb = CD[x] + edy;
@@ -612,11 +619,42 @@ struct PxTri : PxLdsBase<TS> {
             na = A * dx + bdy;
             nb = C * dx + edy;
         }
-        const float a = (float)((double)na * RD);
-        const float b = (float)((double)nb * RD);
-        const float c = (1.0f - a) - b;
-        const bool in = (a >= 0.0f) && (b >= 0.0f) && (c >= 0.0f);
-        this->finish(y, x, in, e0, e1, e2);
+    }
+    // Coverage of the 8 pixels from column gx (sample x gxf) on: the inside test on the
+    // float32 quotients a = RN(na / D), b = RN(nb / D) and c = RN(RN(1 - a) - b)
+    // (render.py:118-123).  The approximations a1 = RN(na * RF), b1, c1 (RF = RN(1 / D))
+    // differ from them by less than 2^-20 wherever the decision is close, so
+    // min(|a1|, |b1|, |c1|) >= 2^-19 makes a1 >= 0, b1 >= 0, c1 >= 0 the same decision;
+    // lanes with a pixel closer to an edge take the exact quotients RN(na * RN64(1 / D))
+    // for the group, and so does every pixel of a candidate with RF = NaN (coefficients that
+    // are not finite or beyond 2^60, see cand_consts): a1, b1, c1 and `near` are NaN.
+    bool inb[8];
+    __device__ __forceinline__ void group(int gx, float gxf) {
+        bool anynear = false;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            float na, nb;
+            numer(gx + i, gxf + (float)i, na, nb);
+            const float a1 = na * RF;
+            const float b1 = nb * RF;
+            const float c1 = (1.0f - a1) - b1;
+            inb[i] = fminf(fminf(a1, b1), c1) >= 0.0f;
+            anynear |= !(fminf(fminf(fabsf(a1), fabsf(b1)), fabsf(c1)) >= 0x1p-19f);
+        }
+        if (anynear) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                float na, nb;
+                numer(gx + i, gxf + (float)i, na, nb);
+                const float a = (float)((double)na * RD);
+                const float b = (float)((double)nb * RD);
+                const float c = (1.0f - a) - b;
+                inb[i] = (a >= 0.0f) && (b >= 0.0f) && (c >= 0.0f);
+            }
+        }
+    }
+    __device__ __forceinline__ void pixel(int y, int x, float pxf, float& e0, float& e1, float& e2) {
+        this->finish(y, x, inb[x & 7], e0, e1, e2);
     }
 };
 
@@ -624,6 +662,7 @@ struct PxTri : PxLdsBase<TS> {
 template <int TS>
 struct PxGeneral {
     static constexpr bool kUnrollRow = false;
+    static constexpr bool kGroup = false;
     LdsBand L;
     int y0;
     WeftPrim P;
@@ -811,6 +850,7 @@ struct CandConsts {
     float ks0, ks1, ks2, om;               // s * a per channel, 1 - a
     float g0, g1, k11, k12, k21, k22;      // class-specific coverage constants
     double rd;                             // triangles: RN64(1 / D)
+    float rf;                              // triangles: RN32(RN64(1 / D)), or NaN: exact quotients only
 };
 
 // Constants of candidate P (class `cls`) for the hard-edge scorers; returns the class
@@ -824,6 +864,7 @@ __device__ __forceinline__ int cand_consts(const WeftPrim& P, int cls, CandConst
     cc.om = 1.0f - a;
     cc.g0 = cc.g1 = cc.k11 = cc.k12 = cc.k21 = cc.k22 = 0.0f;
     cc.rd = 0.0;
+    cc.rf = 0.0f;
     if (cls == CL_TRI) {
         const float* g = P.geom;
         double x0 = g[0], y0 = g[1], x1 = g[2], y1 = g[3], x2 = g[4], y2 = g[5];
@@ -835,6 +876,14 @@ __device__ __forceinline__ int cand_consts(const WeftPrim& P, int cls, CandConst
             cc.k21 = (float)(y2 - y0); cc.k22 = (float)(x0 - x2);
             cc.rd = 1.0 / (double)(float)den;
             cc.g0 = (float)x2; cc.g1 = (float)y2;
+            // the filtered inside test needs products that cannot overflow and a normal
+            // reciprocal; otherwise every pixel takes the exact quotients
+            const float lim = 0x1p60f;
+            const float rf = (float)cc.rd;
+            const bool fast = fabsf(cc.k11) <= lim && fabsf(cc.k12) <= lim && fabsf(cc.k21) <= lim &&
+                              fabsf(cc.k22) <= lim && fabsf(cc.g0) <= lim && fabsf(cc.g1) <= lim &&
+                              fabsf(rf) >= 0x1p-120f && fabsf(rf) <= lim;
+            cc.rf = fast ? rf : __builtin_nanf("");
         }
     }
     if (cls == CL_ELL || cls == CL_RECT) {
@@ -872,6 +921,7 @@ __device__ __forceinline__ void score_band(int mycls, const CandConsts& c, const
         PxTri<TS> px;
         px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
         px.X2 = c.g0; px.Y2 = c.g1; px.A = c.k11; px.B = c.k12; px.C = c.k21; px.E = c.k22; px.RD = c.rd;
+        px.RF = c.rf;
         px.init();
         pairwise_rows<TS>(px, pw, y0, y1);
     } else if (mycls == CL_CONST) {
@@ -1385,7 +1435,7 @@ __device__ __forceinline__ void fit_score_mixed(const WeftFitScoreArgs& A) {
     default: break;
     }
 }
-extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_mixed(WeftFitScoreArgs A) { fit_score_mixed<false>(A); }
+extern "C" __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void weft_fit_score_mixed(WeftFitScoreArgs A) { fit_score_mixed<false>(A); }
 extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_mixed(WeftFitScoreArgs A) { fit_score_mixed<true>(A); }
 
 // Variants that also accept kinds 1-3 (linear gradient, Gaussian line, Gaussian curve).

@@ -268,7 +268,8 @@ def time_to_psnr(tiles, mask, n_cand_plain, depth, seed, lam, refined_cfg):
 
 def drop_in_report(tiles, seed):
     """The reference API (gpu_batch_score_prepacked): 32 stacks of 4 drawn primitives per
-    tile of the 1920x1080 quadtree, host arrays in and out."""
+    tile of the 1920x1080 quadtree (one separate record array per tile), host arrays in
+    and out."""
     import torch
     from weft_amd import _lib
     from weft_amd.fit import draw_codes
@@ -277,7 +278,8 @@ def drop_in_report(tiles, seed):
 
     T, P, D = tiles.n_tiles, 32, 4
     _, prims = draw_codes(np.arange(T, dtype=np.int32), tiles.ts, P * D, seed=seed, step=1000)
-    arrays = [prims[t * P * D:(t + 1) * P * D] for t in range(T)]
+    # one separate array per tile, as the reference encoder builds its trial stacks
+    arrays = [prims[t * P * D:(t + 1) * P * D].copy() for t in range(T)]
     counts = [np.full(P, D, np.int32)] * T
     offsets = {t: int(tiles.pix_off[t]) * 3 for t in range(T)}
     args = (tiles.target, offsets, list(range(T)), [int(v) for v in tiles.ts], arrays, counts)

@@ -436,4 +436,33 @@ def test_prepacked_eval_packing_matches_loop():
         want.append((e0, len(c)))
         e0 += len(c)
         p0 += len(arrays[j])
-    assert ev.tobytes() == ref.tobytes() and ranges == want
+    assert ev.tobytes() == ref.tobytes() and list(zip(ranges[0].tolist(), ranges[1].tolist())) == want
+
+
+def test_prepacked_size_order_and_validation_per_tile():
+    """The per-tile size order of the prepacked drop-in equals the stable per-eval order, and
+    the per-tile validation refuses stacks that read past their tile's records."""
+    from weft_amd.gpu_render import _pack_evals, _tile_split_order, _validate_tiles, split_order
+    from weft_amd.records import PRIM_DTYPE
+
+    rng = np.random.default_rng(5)
+    T = 300
+    ts = rng.choice([8, 16, 32, 64], T)
+    nper = rng.integers(0, 7, T)
+    e0 = np.concatenate([[0], np.cumsum(nper)[:-1]])
+    a, ga = _tile_split_order(ts, e0, nper)
+    b, gb = split_order(np.repeat(ts, nper))
+    assert np.array_equal(a, b) and ga == gb
+    counts = [rng.integers(1, 5, int(n)).astype(np.int32) for n in nper]
+    arrays = [np.zeros(int(c.sum()), PRIM_DTYPE) for c in counts]
+    offs = {t: 3 * 4096 * t for t in range(T)}
+    ev, (s0, n_per), arr_len = _pack_evals(offs, list(range(T)), ts.tolist(), arrays, counts)
+    _validate_tiles(ev, s0, n_per, arr_len, ts.tolist(), 3 * 4096 * T)
+    j = int(np.flatnonzero(nper > 0)[0])
+    short = list(arrays)
+    short[j] = arrays[j][:-1]
+    ev2, (s2, n2), len2 = _pack_evals(offs, list(range(T)), ts.tolist(), short, counts)
+    with pytest.raises(ValueError, match="primitive"):
+        _validate_tiles(ev2, s2, n2, len2, ts.tolist(), 3 * 4096 * T)
+    with pytest.raises(ValueError, match="pixel"):
+        _validate_tiles(ev, s0, n_per, arr_len, ts.tolist(), 3 * 4096 * (T - 1))

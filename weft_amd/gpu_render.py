@@ -73,6 +73,26 @@ def _validate_evals(evals: np.ndarray, n_prims: int, n_floats: int) -> None:
         raise ValueError(f"tile sizes must be in {KERNEL_TILE_SIZES}")
 
 
+def _validate_tiles(evals: np.ndarray, e_start: np.ndarray, n_per: np.ndarray, arr_len: np.ndarray,
+                    tile_sizes, n_floats: int) -> None:
+    """_validate_evals for evals packed tile by tile (checked per tile): every stack reads
+    inside its tile's record array and every tile inside the pixel buffer."""
+    cnt = evals["prim_count"].astype(np.int64)
+    if (cnt < 0).any():
+        raise ValueError("eval primitive ranges exceed the packed primitive array")
+    csum = np.concatenate([[0], np.cumsum(cnt)])
+    used = csum[e_start + n_per] - csum[e_start]
+    if (used > arr_len).any():
+        raise ValueError("eval primitive ranges exceed the packed primitive array")
+    ts = np.asarray(tile_sizes, np.int64)
+    if not np.isin(ts, KERNEL_TILE_SIZES).all():
+        raise ValueError(f"tile sizes must be in {KERNEL_TILE_SIZES}")
+    has = n_per > 0
+    to = evals["tile_offset"][e_start[has]].astype(np.int64)
+    if (to < 0).any() or (to + 3 * ts[has] * ts[has] > n_floats).any():
+        raise ValueError("eval tile ranges exceed the packed pixel buffer")
+
+
 def _validate_decode_descs(descs: np.ndarray, n_prims: int, width: int, height: int) -> None:
     """Every decode descriptor must read inside the primitive array and describe a
     non-empty tile (checked on the host before anything is uploaded)."""
@@ -112,18 +132,100 @@ def score_evals_split_device(d_evals, d_order, groups, d_prims, d_tiles, d_out_m
         pos += n
 
 
-def _score_evals(tiles_flat: np.ndarray, evals: np.ndarray, prims: np.ndarray) -> np.ndarray:
+class _Staging:
+    """Reused pinned host buffers and the device copy of the last pixel buffer.
+
+    The pixel buffer the reference encoder passes (all tiles' pixels, built once per fit,
+    encoder.py:2657-2666) is uploaded again only when another array, another buffer or other
+    contents arrive: the contents are compared through a 64-bit xxh3 digest (without xxhash
+    the buffer is uploaded on every call)."""
+
+    def __init__(self):
+        self.pix_key = None
+        self.d_pix = None
+        self.host = {}
+
+    def pinned(self, name: str, nbytes: int):
+        import torch
+
+        buf = self.host.get(name)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(nbytes, 1 << 16) * 5 // 4, dtype=torch.uint8, pin_memory=True)
+            self.host[name] = buf
+        return buf
+
+    def pixels(self, tiles_flat: np.ndarray, dev):
+        import torch
+
+        flat = np.ascontiguousarray(tiles_flat, dtype=np.float32)
+        try:
+            import xxhash
+
+            digest = xxhash.xxh3_64_intdigest(flat)
+        except ImportError:
+            digest = None
+        key = (flat.ctypes.data, flat.nbytes, digest)
+        if digest is None or key != self.pix_key or self.d_pix is None:
+            self.d_pix = torch.from_numpy(flat).to(dev)
+            self.pix_key = key
+        return self.d_pix
+
+
+_STAGING = _Staging()
+
+
+def _gather_records(arrays, arr_len: np.ndarray) -> tuple[np.ndarray, int]:
+    """The stacks' records back to back in a reused pinned buffer, copied as bytes (a
+    structured-dtype concatenate copies field by field, several times slower); returns
+    (uint8 view of the pinned buffer, bytes)."""
+    rec = PRIM_DTYPE.itemsize
+    n = int(arr_len.sum()) * rec
+    buf = _STAGING.pinned("prims", max(n, rec)).numpy()
+    if n == 0:
+        buf[:rec] = 0
+        return buf, rec
+    try:
+        views = [a.view(np.uint8) for a in arrays]
+    except ValueError:  # a non-contiguous record array
+        views = [np.ascontiguousarray(a).view(np.uint8) for a in arrays]
+    np.concatenate(views, out=buf[:n])
+    return buf, n
+
+
+def _tile_split_order(tile_sizes: np.ndarray, e_start: np.ndarray, n_per: np.ndarray):
+    """split_order for evals that come tile by tile: the tiles' eval ranges taken by size."""
+    ts = np.asarray(tile_sizes, np.int64)
+    parts, groups = [], []
+    for s in KERNEL_TILE_SIZES:
+        sel = np.flatnonzero(ts == s)
+        if len(sel) == 0:
+            continue
+        cnt = n_per[sel]
+        tot = int(cnt.sum())
+        if tot == 0:
+            continue
+        # the eval indices e_start[t] .. e_start[t] + n_per[t] - 1 of each selected tile
+        first = np.repeat(e_start[sel] - (np.cumsum(cnt) - cnt), cnt)
+        parts.append(first + np.arange(tot))
+        groups.append((int(s), tot))
+    order = np.concatenate(parts).astype(np.int32) if parts else np.zeros(0, np.int32)
+    return order, groups
+
+
+def _score_evals(tiles_flat: np.ndarray, evals: np.ndarray, prim_bytes: np.ndarray, n_prim_bytes: int,
+                 order: np.ndarray, groups) -> np.ndarray:
     import torch
 
     dev = torch.device("cuda", 0)
-    t_tiles = torch.from_numpy(np.ascontiguousarray(tiles_flat, dtype=np.float32)).to(dev)
-    t_evals = torch.from_numpy(np.ascontiguousarray(evals).view(np.uint8)).to(dev)
-    t_prims = torch.from_numpy(np.ascontiguousarray(prims, dtype=PRIM_DTYPE).view(np.uint8)).to(dev)
+    t_tiles = _STAGING.pixels(tiles_flat, dev)
+    t_evals = torch.from_numpy(np.ascontiguousarray(evals).view(np.uint8)).to(dev, non_blocking=False)
+    t_prims = torch.from_numpy(prim_bytes[:n_prim_bytes]).to(dev, non_blocking=True)
     out = torch.empty(len(evals), dtype=torch.float32, device=dev)
-    order, groups = split_order(evals["tile_size"])
     t_order = torch.from_numpy(order).to(dev)
     score_evals_split_device(t_evals, t_order, groups, t_prims, t_tiles, out)
-    return out.cpu().numpy()
+    host = _STAGING.pinned("mse", 4 * len(evals))[:4 * len(evals)].view(torch.float32)
+    host.copy_(out)
+    return host.numpy().copy()
 
 
 def _pack_evals(tile_pixel_offsets, tile_indices, tile_sizes, eval_prim_arrays, eval_prim_counts):
@@ -132,10 +234,10 @@ def _pack_evals(tile_pixel_offsets, tile_indices, tile_sizes, eval_prim_arrays, 
     Tile j's stacks lie back to back in eval_prim_arrays[j], which sit back to back in the
     concatenated primitive array; stack i of tile j starts after the stacks before it.
     """
-    counts = [np.asarray(c, dtype=np.int64).ravel() for c in eval_prim_counts]
-    n_per = np.array([len(c) for c in counts], np.int64)
-    cnt = np.concatenate(counts) if len(counts) else np.zeros(0, np.int64)
-    arr_len = np.array([len(a) for a in eval_prim_arrays], np.int64)
+    n_per = np.fromiter(map(len, eval_prim_counts), np.int64, len(eval_prim_counts))
+    cnt = (np.concatenate(eval_prim_counts).astype(np.int64, copy=False) if len(eval_prim_counts)
+           else np.zeros(0, np.int64))
+    arr_len = np.fromiter(map(len, eval_prim_arrays), np.int64, len(eval_prim_arrays))
     arr_start = np.cumsum(arr_len) - arr_len
     e_start = np.cumsum(n_per) - n_per
     ex = np.cumsum(cnt) - cnt  # exclusive prefix over every stack
@@ -143,9 +245,10 @@ def _pack_evals(tile_pixel_offsets, tile_indices, tile_sizes, eval_prim_arrays, 
     evals = np.zeros(len(cnt), dtype=EVAL_DTYPE)
     evals["prim_offset"] = np.repeat(arr_start - tile_base, n_per) + ex
     evals["prim_count"] = cnt
-    evals["tile_offset"] = np.repeat(np.array([int(tile_pixel_offsets[ti]) for ti in tile_indices], np.int64), n_per)
-    evals["tile_size"] = np.repeat(np.array([int(v) for v in tile_sizes], np.int64), n_per)
-    return evals, list(zip(e_start.tolist(), n_per.tolist())), arr_len
+    toff = np.fromiter(map(tile_pixel_offsets.__getitem__, tile_indices), np.int64, len(tile_indices))
+    evals["tile_offset"] = np.repeat(toff, n_per)
+    evals["tile_size"] = np.repeat(np.asarray(tile_sizes, np.int64), n_per)
+    return evals, (e_start, n_per), arr_len
 
 
 def gpu_batch_score_prepacked(
@@ -170,18 +273,20 @@ def gpu_batch_score_prepacked(
             return None
         if any(int(s) not in KERNEL_TILE_SIZES for s in tile_sizes):
             return _fail(ValueError(f"tile sizes must be in {KERNEL_TILE_SIZES}"))
-        evals, ranges, arr_len = _pack_evals(tile_pixel_offsets, tile_indices, tile_sizes, eval_prim_arrays,
-                                             eval_prim_counts)
+        evals, (e_start, n_per), arr_len = _pack_evals(tile_pixel_offsets, tile_indices, tile_sizes,
+                                                       eval_prim_arrays, eval_prim_counts)
         p0 = int(arr_len.sum())
         if p0 >= 2 ** 31 or len(tile_pixels_flat) >= 2 ** 31:
             return _fail(ValueError("batch too large for 32-bit offsets"))
-        prims = (np.concatenate([np.asarray(a, dtype=PRIM_DTYPE) for a in eval_prim_arrays])
-                 if eval_prim_arrays else np.zeros(1, PRIM_DTYPE))
-        _validate_evals(evals, len(prims), len(tile_pixels_flat))
-        if len(prims) == 0:
-            prims = np.zeros(1, PRIM_DTYPE)
-        mse = _score_evals(tile_pixels_flat, evals, prims)
-        return [mse[s:s + n].copy() for s, n in ranges]
+        if any(np.asarray(a).dtype != PRIM_DTYPE for a in eval_prim_arrays):
+            eval_prim_arrays = [np.asarray(a, dtype=PRIM_DTYPE) for a in eval_prim_arrays]
+        _validate_tiles(evals, e_start, n_per, arr_len, tile_sizes, len(tile_pixels_flat))
+        prim_bytes, nb = _gather_records(eval_prim_arrays, arr_len)
+        order, groups = _tile_split_order(tile_sizes, e_start, n_per)
+        mse = _score_evals(tile_pixels_flat, evals, prim_bytes, nb, order, groups)
+        # one result array, split into per-tile views
+        ends = (e_start + n_per).tolist()
+        return [mse[a:b] for a, b in zip(e_start.tolist(), ends)]
     except _lib.WeftHipUnavailable:
         raise
     except Exception as exc:  # reference contract: any failure -> None

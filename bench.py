@@ -108,33 +108,47 @@ def cpu_baseline(tiles, c, lam, seed, budget_s, gpu_rec0):
     from oracle import coracle
 
     coracle.build()
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = len(os.sched_getaffinity(0))  # every host CPU this process may run on
     order = np.random.default_rng(1).permutation(tiles.n_tiles)
 
-    def run(n):
+    def run(n, threads):
         ids = np.sort(order[:n]).astype(np.int32)
         ts = tiles.ts[ids]
         offs = np.concatenate(([0], np.cumsum(ts.astype(np.int64) ** 2)[:-1]))
         tgt = np.concatenate([tiles.tile_pixels(int(t)).ravel() for t in ids])
         fit = coracle.PlainFit(tgt, offs, ts, c["n_cand"], c["depth"], seed, lam, tile_ids=ids)
         t0 = time.perf_counter()
-        res = fit.run(1, n_threads=cores)
+        res = fit.run(1, n_threads=threads)
         return ids, ts, res, time.perf_counter() - t0
 
-    n = max(4, tiles.n_tiles // 500)
-    while True:
-        ids, ts, res, dt = run(n)
-        if dt > 0.3 * budget_s or n >= tiles.n_tiles:
-            break
-        n = min(tiles.n_tiles, int(n * min(20.0, max(2.0, 0.6 * budget_s / max(dt, 1e-3)))))
+    def sample(threads, budget):
+        n = max(4, tiles.n_tiles // 500)
+        while True:
+            ids, ts, res, dt = run(n, threads)
+            if dt > 0.3 * budget or n >= tiles.n_tiles:
+                return ids, ts, res, dt
+            n = min(tiles.n_tiles, int(n * min(20.0, max(2.0, 0.6 * budget / max(dt, 1e-3)))))
+
+    ids, ts, res, dt = sample(cores, budget_s)
     evals = int((ts.astype(np.int64) ** 2).sum()) * c["n_cand"]
     agree = bool(np.array_equal(res["win_idx"][0], gpu_rec0["win_idx"][ids])
                  and np.array_equal(res["win_acc"][0], gpu_rec0["win_acc"][ids])
                  and res["win_obj"][0].tobytes() == np.ascontiguousarray(gpu_rec0["win_obj"][ids]).tobytes())
+    ids1, ts1, _, dt1 = sample(1, budget_s / 3)
+    evals1 = int((ts1.astype(np.int64) ** 2).sum()) * c["n_cand"]
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": evals / dt, "unit": "candidate·pixel evals/s", "cores": cores, "kind": "port",
         "sample": f"step 0 of the same fit on {len(ids)} of {tiles.n_tiles} tiles "
                   f"({evals / 1e9:.2f} G evals in {dt:.1f} s; C float32 oracle, OpenMP {cores} threads)",
+        "cpu_model": model, "affinity_cpus": cores,
+        "single_core_value": evals1 / dt1,
+        "single_core_sample": f"{len(ids1)} tiles, {evals1 / 1e9:.3f} G evals in {dt1:.1f} s, one thread",
         "winners_match_gpu": agree,
     }
 

@@ -662,6 +662,66 @@ def refine_report(seed, lam):
                 round1_objective_ratio_mean=float((o_r / o_p).mean()))
 
 
+def class_probe_report(tiles, c, lam, reps: int = 3):
+    """The 1080p scoring launch split by tile size and by primitive family.
+
+    Each row is `reps` launches of weft_fit_score_mixed over one subset of the step's work
+    items (one tile-size class, or all of them), timed with HIP events on the launch stream;
+    the rows are launched in the order listed, so the dispatches of a counter run map onto
+    them one to one.  Sources: the drawn candidates of step 0 (families stratified in groups
+    of 64), and three explicit pools of 1024 candidates per tile holding one family each (the
+    16 groups of that family among 3072 drawn candidates; explicit pools add a class-order
+    launch, weft_fit_order_kernel, before their rows)."""
+    import torch
+    from weft_amd import _lib
+    from weft_amd.fit import PlainFitter, draw_codes
+
+    n_cand, depth = c["n_cand"], c["depth"]
+    stream = torch.cuda.current_stream()
+    sp = _lib.stream_ptr()
+    px = tiles.ts.astype(np.int64) ** 2
+    rows = []
+
+    def probe(source, fitter):
+        cands = fitter._cands(0)
+        fitter.order(0, sp)
+        subsets = [(str(cl["ts"]), cl["start"], cl["n"], tiles.ts == cl["ts"]) for cl in fitter.classes]
+        subsets.append(("all", 0, fitter.n_work, np.ones(tiles.n_tiles, bool)))
+        blk = fitter.classes[0]["block"]
+        for label, start, n, ids in subsets:
+            w = _lib.Work(fitter.d_work_tile.data_ptr() + 4 * start, fitter.d_work_c0.data_ptr() + 4 * start,
+                          n, 0, blk, 0)
+            times = []
+            for _ in range(reps):
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                fitter.score_class(w, start, cands, None, None, sp)
+                b.record(stream)
+                torch.cuda.synchronize()
+                times.append(a.elapsed_time(b) * 1e-3)
+            evals = int(px[ids].sum()) * n_cand
+            t = float(np.mean(times))
+            rows.append(dict(source=source, tile_size=label, work_items=int(n), launches=reps, mean_ms=t * 1e3,
+                             evals=evals, t_evals_per_s=evals / t / 1e12))
+            log(f"[probe] {source:9s} ts {label:>3s}: {n:6d} items, {t * 1e3:7.3f} ms, {evals / t / 1e12:.3f} T evals/s")
+
+    probe("drawn", PlainFitter(tiles, n_cand=n_cand, depth=depth, seed=0, lam=lam, max_steps=1))
+    T = tiles.n_tiles
+    codes, prims = draw_codes(np.arange(T, dtype=np.int32), tiles.ts, 3 * n_cand, seed=0, step=0)
+    fam = codes["family"].reshape(T, 3 * n_cand)
+    prims = prims.reshape(T, 3 * n_cand)
+    for f, name in enumerate(("ellipse", "rect", "triangle")):
+        sel = fam == f
+        if not (sel.sum(axis=1) == n_cand).all():
+            raise RuntimeError("a family does not hold exactly a third of 3 * n_cand drawn candidates")
+        pool = np.ascontiguousarray(prims[sel].reshape(T * n_cand))
+        probe(name, PlainFitter(tiles, n_cand=n_cand, depth=depth, seed=0, lam=lam, max_steps=1, cand_prims=pool))
+        del pool
+    return dict(workload="1920x1080 plain-fit step 0, 1024 candidates per tile", kernel="weft_fit_score_mixed",
+                reps=reps, rows=rows)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -672,7 +732,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the time-to-PSNR, batched and refine reports")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES of device memory once (HBM counter calibration) and exit")
-    ap.add_argument("--only", choices=["template_fit", "select", "candidates", "refined_round"], default=None,
+    ap.add_argument("--only", choices=["template_fit", "select", "candidates", "refined_round", "class_probe"],
+                    default=None,
                     help="run only that report (for profiling) and print it as JSON")
     args = ap.parse_args()
     if args.only:
@@ -684,6 +745,11 @@ def main():
             rep = candidates_report(build_workload(args)[2])
         elif args.only == "refined_round":
             rep = refined_round_report(*build_workload(args)[:3])
+        elif args.only == "class_probe":
+            from weft_amd.rd import tile_lambda
+
+            c, _, tiles, _ = build_workload(args)
+            rep = class_probe_report(tiles, c, tile_lambda(75))
         else:
             rep = select_report()
         print(json.dumps({args.only: rep}), flush=True)

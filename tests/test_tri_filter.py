@@ -1,10 +1,11 @@
 """The scoring kernels' triangle inside test, restated in NumPy float32 on the CPU.
 
 The kernels decide coverage from a1 = RN(na * RF), b1 = RN(nb * RF), c1 = RN(RN(1 - a1) - b1)
-(RF = RN(1 / D)) and take the exact float32 quotients a = RN(na / D), b = RN(nb / D),
-c = RN(RN(1 - a) - b) of render.py:118-123 only where min(|a1|, |b1|, |c1|) < 2^-19.  Here both
-decisions are computed for random triangles over their pixel grids and for pixels placed
-on and within a few ulps of the edges, and must agree wherever the fast one is taken.
+(RF = RN(1 / D)) through m = min(a1, b1, c1) (inside iff m >= 0) and take the exact float32
+quotients a = RN(na / D), b = RN(nb / D), c = RN(RN(1 - a) - b) of render.py:118-123 only
+where |m| < 2^-19.  Here both decisions are computed for random triangles over their pixel
+grids, for pixels placed on and within a few ulps of the edges, and for triangles far larger
+than the tile (barycentrics in the hundreds), and must agree wherever the fast one is taken.
 """
 from __future__ import annotations
 
@@ -23,16 +24,17 @@ def _decisions(A, B, C, E, X2, Y2, D, px, py):
     rf = F(1.0 / float(D))
     a1, b1 = na * rf, nb * rf
     c1 = (F(1.0) - a1) - b1
-    fast = np.minimum(np.minimum(a1, b1), c1) >= 0
-    near = ~(np.minimum(np.minimum(np.abs(a1), np.abs(b1)), np.abs(c1)) >= T)
+    m = np.minimum(np.minimum(a1, b1), c1)
+    fast = m >= 0
+    near = ~(np.abs(m) >= T)
     a, b = na / D, nb / D  # IEEE float32 division: the correctly rounded quotient
     c = (F(1.0) - a) - b
     exact = (a >= 0) & (b >= 0) & (c >= 0)
     return fast, near, exact
 
 
-def _triangle(rng, ts):
-    v = rng.uniform(-0.25 * ts, 1.25 * ts, 6).astype(np.float32)
+def _triangle(rng, ts, spread=(-0.25, 1.25)):
+    v = rng.uniform(spread[0] * ts, spread[1] * ts, 6).astype(np.float32)
     x0, y0, x1, y1, x2, y2 = (float(t) for t in v)
     den = (y1 - y2) * (x0 - x2) + (x2 - x1) * (y0 - y2)
     if abs(den) < 1e-6:
@@ -92,6 +94,28 @@ def test_filtered_inside_test_equals_exact_at_the_edges():
             assert np.array_equal(fast[~near], exact[~near]), (it, s)
             n_fast += int((~near).sum())
     assert n_fast > 10000
+
+
+def test_filtered_inside_test_equals_exact_for_large_triangles():
+    """Vertices up to 40 tiles away: pixels see barycentrics in the hundreds, where the
+    absolute error of c1 exceeds 2^-19 and only the sign argument for m keeps the decision."""
+    rng = np.random.default_rng(3)
+    n_fast = n_big = 0
+    for it in range(1500):
+        ts = int(rng.choice([8, 16, 32, 64]))
+        tri = _triangle(rng, ts, spread=(-40.0, 41.0))
+        if tri is None:
+            continue
+        A, B, C, E, X2, Y2, D = tri[0]
+        ys, xs = np.meshgrid(np.arange(ts, dtype=np.float32) + F(0.5), np.arange(ts, dtype=np.float32) + F(0.5),
+                             indexing="ij")
+        fast, near, exact = _decisions(A, B, C, E, X2, Y2, D, xs, ys)
+        assert np.array_equal(fast[~near], exact[~near]), it
+        n_fast += int((~near).sum())
+        rf = F(1.0 / float(D))
+        a1 = ((A * (xs - X2) + B * (ys - Y2)).astype(np.float32) * rf)
+        n_big += int((np.abs(a1) > 16).sum())
+    assert n_big > 10000 and n_fast > 100000
 
 
 def test_a_looser_threshold_is_caught():

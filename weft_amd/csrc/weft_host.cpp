@@ -79,6 +79,7 @@ std::mutex g_mu;
 uint16_t g_logr[WEFT_TABLE_N];
 float g_cos[WEFT_TABLE_N];
 float g_sin[WEFT_TABLE_N];
+float g_u8[256];
 bool g_tables = false;
 thread_local std::string g_err;
 
@@ -106,6 +107,7 @@ void build_tables() {
         g_logr[k] = (uint16_t)std::llround(r * 1024.0);
         double th = (double)k * 3.14159265358979323846 / (double)WEFT_TABLE_N;
         g_cos[k] = (float)std::cos(th);
+        if (k < 256) g_u8[k] = (float)((double)k / 255.0);
         g_sin[k] = (float)std::sin(th);
     }
     g_tables = true;
@@ -285,6 +287,7 @@ int weft_init(int device) {
     int rc = upload_global("weft_tab_logr", g_logr, sizeof(g_logr));
     if (rc == WEFT_OK) rc = upload_global("weft_tab_cos", g_cos, sizeof(g_cos));
     if (rc == WEFT_OK) rc = upload_global("weft_tab_sin", g_sin, sizeof(g_sin));
+    if (rc == WEFT_OK) rc = upload_global("weft_tab_u8", g_u8, sizeof(g_u8));
     if (rc != WEFT_OK) {
         (void)hipModuleUnload(g_mod.mod);
         g_mod = Module();

@@ -26,6 +26,7 @@
 __device__ uint16_t weft_tab_logr[WEFT_TABLE_N];
 __device__ float weft_tab_cos[WEFT_TABLE_N];
 __device__ float weft_tab_sin[WEFT_TABLE_N];
+__device__ float weft_tab_u8[256];  // RN32(c / 255)
 
 // Scratch of the allocation-free fit step (weft_fit_step_static): class orders and
 // partial minima.  It lives as long as the loaded code object (one copy per device);
@@ -410,42 +411,44 @@ struct PwState {
     }
 };
 
+// One group of 8 pixels (24 elements from element `base`) of row y, columns gx..gx+7.
+template <int TS, class Px>
+__device__ __forceinline__ void pw_group(Px& px, PwState& st, int y, int gx, float gxf, int base) {
+    constexpr int BLOCK = PwCfg<TS>::BLOCK;
+    if constexpr (Px::kGroup) px.group(gx, gxf);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        float e0, e1, e2;
+        px.pixel(y, gx + i, i, gxf + (float)i, e0, e1, e2);
+        st.template feed<BLOCK>(3 * i + 0, e0, base);
+        st.template feed<BLOCK>(3 * i + 1, e1, base);
+        st.template feed<BLOCK>(3 * i + 2, e2, base);
+    }
+}
+
 // Pairwise SSE over rows [y0, y1) of a tile, pixels in row-major order, 8 pixels
 // (24 elements) per unrolled group; the state carries over between calls so a tile can
-// be processed in row bands.  Px provides row(y) and pixel(y, x, e0, e1, e2) (the three
-// squared errors of pixel (y, x)).
+// be processed in row bands.  Px provides row(y) and pixel(y, x, i, pxf, e0, e1, e2) (the
+// three squared errors of pixel (y, x); i = x mod 8, a compile-time constant, and pxf =
+// x + 0.5).
 template <int TS, class Px>
 __device__ __forceinline__ void pairwise_rows(Px& px, PwState& st, int y0, int y1) {
-    constexpr int BLOCK = PwCfg<TS>::BLOCK;
     int base = 3 * TS * y0;
 #pragma unroll 1
     for (int y = y0; y < y1; ++y) {
         px.row(y);
         if constexpr (Px::kUnrollRow) {
             // whole row unrolled: column index x is a compile-time constant
-            if constexpr (Px::kGroup) px.group(0, 0.5f);
 #pragma unroll
-            for (int x = 0; x < TS; ++x) {
-                float e0, e1, e2;
-                px.pixel(y, x, (float)x + 0.5f, e0, e1, e2);
-                st.template feed<BLOCK>(3 * x + 0, e0, base);
-                st.template feed<BLOCK>(3 * x + 1, e1, base);
-                st.template feed<BLOCK>(3 * x + 2, e2, base);
+            for (int g = 0; g < TS / 8; ++g) {
+                pw_group<TS>(px, st, y, 8 * g, 8.0f * (float)g + 0.5f, base);
+                base += 24;
             }
-            base += 3 * TS;
         } else {
             float gxf = 0.5f;  // sample x of the group's first pixel (exact)
 #pragma unroll 1
             for (int gx = 0; gx < TS; gx += 8) {
-                if constexpr (Px::kGroup) px.group(gx, gxf);
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    float e0, e1, e2;
-                    px.pixel(y, gx + i, gxf + (float)i, e0, e1, e2);
-                    st.template feed<BLOCK>(3 * i + 0, e0, base);
-                    st.template feed<BLOCK>(3 * i + 1, e1, base);
-                    st.template feed<BLOCK>(3 * i + 2, e2, base);
-                }
+                pw_group<TS>(px, st, y, gx, gxf, base);
                 base += 24;
                 gxf += 8.0f;
             }
@@ -482,7 +485,7 @@ struct PxStack {
     int ts;
     float py;
     __device__ __forceinline__ void row(int y) { py = (float)y + 0.5f; }
-    __device__ __forceinline__ void pixel(int y, int x, float px, float& e0, float& e1, float& e2) {
+    __device__ __forceinline__ void pixel(int y, int x, int, float px, float& e0, float& e1, float& e2) {
         float r = 0.0f, g = 0.0f, b = 0.0f;
         for (int i = 0; i < n; ++i) composite_px_call(prims[i], px, py, r, g, b);
         const float* t = tgt + 3 * (y * ts + x);
@@ -531,7 +534,7 @@ struct PxConst : PxLdsBase<TS> {
     static constexpr bool kUnrollRow = false;
     static constexpr bool kGroup = false;
     __device__ __forceinline__ void row(int) {}
-    __device__ __forceinline__ void pixel(int y, int x, float, float& e0, float& e1, float& e2) {
+    __device__ __forceinline__ void pixel(int y, int x, int, float, float& e0, float& e1, float& e2) {
         this->finish(y, x, true, e0, e1, e2);
     }
 };
@@ -562,7 +565,7 @@ struct PxEllRect : PxLdsBase<TS> {
         p2 = sa * dy;
         q1 = cb * dy;
     }
-    __device__ __forceinline__ void pixel(int y, int x, float pxf, float& e0, float& e1, float& e2) {
+    __device__ __forceinline__ void pixel(int y, int x, int i, float pxf, float& e0, float& e1, float& e2) {
         float p, q;
         if constexpr (kUnrollRow) {
             p = P1[x] + p2;
@@ -610,7 +613,7 @@ struct PxTri : PxLdsBase<TS> {
         edy = E * dy;
     }
     // Numerators of pixel x (column group start gx, sample x pxf) of the current row.
-    __device__ __forceinline__ void numer(int x, float pxf, float& na, float& nb) const {
+    __device__ __forceinline__ void numer(int x, int i, float pxf, float& na, float& nb) const {
         if constexpr (kUnrollRow) {
             na = AD[x] + bdy;
             nb = CD[x] + edy;
@@ -622,30 +625,34 @@ struct PxTri : PxLdsBase<TS> {
     }
     // Coverage of the 8 pixels from column gx (sample x gxf) on: the inside test on the
     // float32 quotients a = RN(na / D), b = RN(nb / D) and c = RN(RN(1 - a) - b)
-    // (render.py:118-123).  The approximations a1 = RN(na * RF), b1, c1 (RF = RN(1 / D))
-    // differ from them by less than 2^-20 wherever the decision is close, so
-    // min(|a1|, |b1|, |c1|) >= 2^-19 makes a1 >= 0, b1 >= 0, c1 >= 0 the same decision;
-    // lanes with a pixel closer to an edge take the exact quotients RN(na * RN64(1 / D))
-    // for the group, and so does every pixel of a candidate with RF = NaN (coefficients that
-    // are not finite or beyond 2^60, see cand_consts): a1, b1, c1 and `near` are NaN.
+    // (render.py:118-123), from the approximations a1 = RN(na * RF), b1, c1 (RF = RN(1 / D))
+    // and m = min(a1, b1, c1).  a1 and b1 are within 2^-22 (relative) of a and b, so
+    // m >= 2^-19 puts all three quotients clearly inside (then a, b <= 1 and c1 is within
+    // 2^-20 of c); m <= -2^-19 is outside for the quotients too: a1 or b1 that negative fixes
+    // the sign of a or b, and a c1 that negative with a1, b1 > -2^-19 means a + b exceeds 1
+    // by more than c's error, which grows only like 2^-22 (|a| + |b|).  Lanes with |m| below
+    // 2^-19 take the exact quotients RN(na * RN64(1 / D)) for the group, and so does every
+    // pixel of a candidate with RF = NaN (coefficients that are not finite or beyond 2^60,
+    // see cand_consts): then m is NaN.
     bool inb[8];
     __device__ __forceinline__ void group(int gx, float gxf) {
         bool anynear = false;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             float na, nb;
-            numer(gx + i, gxf + (float)i, na, nb);
+            numer(gx + i, i, gxf + (float)i, na, nb);
             const float a1 = na * RF;
             const float b1 = nb * RF;
             const float c1 = (1.0f - a1) - b1;
-            inb[i] = fminf(fminf(a1, b1), c1) >= 0.0f;
-            anynear |= !(fminf(fminf(fabsf(a1), fabsf(b1)), fabsf(c1)) >= 0x1p-19f);
+            const float m = fminf(fminf(a1, b1), c1);
+            inb[i] = m >= 0.0f;
+            anynear |= !(fabsf(m) >= 0x1p-19f);
         }
         if (anynear) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 float na, nb;
-                numer(gx + i, gxf + (float)i, na, nb);
+                numer(gx + i, i, gxf + (float)i, na, nb);
                 const float a = (float)((double)na * RD);
                 const float b = (float)((double)nb * RD);
                 const float c = (1.0f - a) - b;
@@ -653,8 +660,8 @@ struct PxTri : PxLdsBase<TS> {
             }
         }
     }
-    __device__ __forceinline__ void pixel(int y, int x, float pxf, float& e0, float& e1, float& e2) {
-        this->finish(y, x, inb[x & 7], e0, e1, e2);
+    __device__ __forceinline__ void pixel(int y, int x, int i, float, float& e0, float& e1, float& e2) {
+        this->finish(y, x, inb[i], e0, e1, e2);
     }
 };
 
@@ -668,7 +675,7 @@ struct PxGeneral {
     WeftPrim P;
     float py;
     __device__ __forceinline__ void row(int y) { py = (float)y + 0.5f; }
-    __device__ __forceinline__ void pixel(int y, int x, float pxf, float& e0, float& e1, float& e2) {
+    __device__ __forceinline__ void pixel(int y, int x, int, float pxf, float& e0, float& e1, float& e2) {
         const int p = (y - y0) * TS + x;
         const float4 a = L.A[2 * p], b = L.A[2 * p + 1];
         float r = a.x, g = a.y, bb = a.z;
@@ -764,11 +771,14 @@ __device__ __forceinline__ WeftPrim code_to_prim(const WeftCode& c) {
     const float rx = (float)c.rx * (1.0f / 1024.0f);
     const float ry = (float)c.ry * (1.0f / 1024.0f);
     const float ct = weft_tab_cos[c.theta & 0xFFF], st = weft_tab_sin[c.theta & 0xFFF];
-    P.color0[0] = (float)((double)c.r / 255.0);
-    P.color0[1] = (float)((double)c.g / 255.0);
-    P.color0[2] = (float)((double)c.b / 255.0);
+    // RN32(c / 255) from a table, and RN32(a / 65535) as a float32 division: each equals the
+    // float64 quotient rounded to float32 (no quotient of these codes is within 2^-41
+    // relative of a float32 rounding midpoint, so the double rounding never differs)
+    P.color0[0] = weft_tab_u8[c.r & 0xFF];
+    P.color0[1] = weft_tab_u8[c.g & 0xFF];
+    P.color0[2] = weft_tab_u8[c.b & 0xFF];
     P.color1[0] = P.color1[1] = P.color1[2] = 0.0f;
-    P.alpha = (float)((double)c.alpha / 65535.0);
+    P.alpha = (float)c.alpha / 65535.0f;
     P.geom[6] = 0.0f;
     if (c.family == 2) {
         const float ux[3] = {0.0f, 0.8660254037844386f, -0.8660254037844386f};
@@ -902,28 +912,34 @@ __device__ __forceinline__ int cand_consts(const WeftPrim& P, int cls, CandConst
     return cls;
 }
 
+template <int TS, bool ELL>
+__device__ __forceinline__ void score_ellrect(const CandConsts& c, LdsBand L, PwState& pw, int y0, int y1) {
+    PxEllRect<TS, ELL> px;
+    px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
+    px.cx = c.g0; px.cy = c.g1; px.ca = c.k11; px.sa = c.k12; px.cb = c.k21; px.sb = c.k22;
+    px.init();
+    pairwise_rows<TS>(px, pw, y0, y1);
+}
+
+template <int TS>
+__device__ __forceinline__ void score_tri(const CandConsts& c, LdsBand L, PwState& pw, int y0, int y1) {
+    PxTri<TS> px;
+    px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
+    px.X2 = c.g0; px.Y2 = c.g1; px.A = c.k11; px.B = c.k12; px.C = c.k21; px.E = c.k22; px.RD = c.rd;
+    px.RF = c.rf;
+    px.init();
+    pairwise_rows<TS>(px, pw, y0, y1);
+}
+
 template <int TS, bool GEN>
 __device__ __forceinline__ void score_band(int mycls, const CandConsts& c, const WeftPrim& P,
                                            LdsBand L, PwState& pw, int y0, int y1) {
     if (mycls == CL_ELL) {
-        PxEllRect<TS, true> px;
-        px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
-        px.cx = c.g0; px.cy = c.g1; px.ca = c.k11; px.sa = c.k12; px.cb = c.k21; px.sb = c.k22;
-        px.init();
-        pairwise_rows<TS>(px, pw, y0, y1);
+        score_ellrect<TS, true>(c, L, pw, y0, y1);
     } else if (mycls == CL_RECT) {
-        PxEllRect<TS, false> px;
-        px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
-        px.cx = c.g0; px.cy = c.g1; px.ca = c.k11; px.sa = c.k12; px.cb = c.k21; px.sb = c.k22;
-        px.init();
-        pairwise_rows<TS>(px, pw, y0, y1);
+        score_ellrect<TS, false>(c, L, pw, y0, y1);
     } else if (mycls == CL_TRI) {
-        PxTri<TS> px;
-        px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;
-        px.X2 = c.g0; px.Y2 = c.g1; px.A = c.k11; px.B = c.k12; px.C = c.k21; px.E = c.k22; px.RD = c.rd;
-        px.RF = c.rf;
-        px.init();
-        pairwise_rows<TS>(px, pw, y0, y1);
+        score_tri<TS>(c, L, pw, y0, y1);
     } else if (mycls == CL_CONST) {
         PxConst<TS> px;
         px.L = L; px.y0 = y0; px.ks0 = c.ks0; px.ks1 = c.ks1; px.ks2 = c.ks2; px.om = c.om;

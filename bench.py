@@ -299,7 +299,7 @@ def drop_in_report(tiles, seed):
     args = (tiles.target, offsets, list(range(T)), [int(v) for v in tiles.ts], arrays, counts)
     gpu_batch_score_prepacked(*args)  # warm
     host = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         out = gpu_batch_score_prepacked(*args)
         host.append(time.perf_counter() - t0)
@@ -341,7 +341,8 @@ def drop_in_report(tiles, seed):
     return dict(api="gpu_batch_score_prepacked", tiles=T, evals=T * P, stack_depth=D,
                 kernel="weft_score_evals_split_kernel", device_s=dev_s, evals_per_s=T * P / dev_s,
                 eval_px_per_s=px / dev_s, eval_px_prim_per_s=px * D / dev_s, host_inclusive_s=min(host),
-                host_inclusive_evals_per_s=T * P / min(host), device_equals_api=same,
+                host_inclusive_evals_per_s=T * P / min(host), host_inclusive_all_s=[round(h, 6) for h in host],
+                device_equals_api=same,
                 one_lane_kernel_s=one_s, one_lane_evals_per_s=T * P / one_s, split_equals_one_lane=same_one,
                 split_speedup=one_s / dev_s)
 
@@ -732,7 +733,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the time-to-PSNR, batched and refine reports")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES of device memory once (HBM counter calibration) and exit")
-    ap.add_argument("--only", choices=["template_fit", "select", "candidates", "refined_round", "class_probe"],
+    ap.add_argument("--only", choices=["template_fit", "select", "candidates", "refined_round", "class_probe",
+                                       "drop_in"],
                     default=None,
                     help="run only that report (for profiling) and print it as JSON")
     args = ap.parse_args()
@@ -743,6 +745,9 @@ def main():
             rep = template_report(synth_image(2160, 3840, 2500, seed=0, texture=True))
         elif args.only == "candidates":
             rep = candidates_report(build_workload(args)[2])
+        elif args.only == "drop_in":
+            c, _, tiles, _ = build_workload(args)
+            rep = drop_in_report(tiles, c["seed"])
         elif args.only == "refined_round":
             rep = refined_round_report(*build_workload(args)[:3])
         elif args.only == "class_probe":

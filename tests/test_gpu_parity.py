@@ -86,6 +86,46 @@ def test_score_evals_matches_reference_render(gpu, oracle_lib):
             assert o[0].tobytes() == np.float32(r).tobytes(), (j, float(o[0]), float(r))
 
 
+def test_prepacked_native_packing_equals_numpy_packing(gpu, oracle_lib):
+    """gpu_batch_score_prepacked through the packing extension equals its NumPy path (taken
+    for strided record arrays) on ragged stacks of every tile size, and a pixel buffer
+    changed in place is uploaded again (the cached device copy is not reused)."""
+    from weft_amd.gpu_render import gpu_batch_score_prepacked
+    from weft_amd.records import PRIM_DTYPE
+
+    d = load_golden("ref_render_tile.npz")
+    pool = np.concatenate([golden_prims(d, i) for i in range(len(d["tile_size"])) if not d["has_residual"][i]])
+    rng = np.random.default_rng(31)
+    T = 300
+    sizes = rng.choice([8, 16, 32, 64], T).tolist()
+    offs, run = {}, 0
+    idx = rng.permutation(T).tolist()
+    for ti in idx:
+        offs[ti] = run
+        run += 3 * sizes[idx.index(ti)] ** 2
+    flat = rng.random(run, dtype=np.float32)
+    counts = [rng.integers(0, 6, int(rng.integers(0, 9))).astype(np.int32) for _ in range(T)]
+    arrays = [pool[rng.integers(0, len(pool), int(c.sum()))] for c in counts]
+    strided = []
+    for a in arrays:
+        s2 = np.zeros(2 * len(a), PRIM_DTYPE)[::2]
+        s2[:] = a
+        strided.append(s2)
+    _say(f"prepacked native vs numpy packing: {T} tiles, {sum(len(c) for c in counts)} evals")
+    a1 = gpu_batch_score_prepacked(flat, offs, idx, sizes, arrays, counts)
+    a2 = gpu_batch_score_prepacked(flat, offs, idx, sizes, strided, counts)
+    assert a1 is not None and a2 is not None and len(a1) == T
+    for j in range(T):
+        assert len(a1[j]) == len(counts[j]) and a1[j].tobytes() == a2[j].tobytes(), j
+    flat[:] = rng.random(run, dtype=np.float32)
+    b1 = gpu_batch_score_prepacked(flat, offs, idx, sizes, arrays, counts)
+    b2 = gpu_batch_score_prepacked(flat, offs, idx, sizes, strided, counts)
+    for j in range(T):
+        assert b1[j].tobytes() == b2[j].tobytes(), j
+    changed = np.concatenate(a1) != np.concatenate(b1)
+    assert changed.mean() > 0.9
+
+
 def test_split_evals_equal_one_lane_evals(gpu, oracle_lib):
     """weft_score_evals_split (pixels over lanes) == weft_score_evals (one lane per eval).
 

@@ -466,3 +466,116 @@ def test_prepacked_size_order_and_validation_per_tile():
         _validate_tiles(ev2, s2, n2, len2, ts.tolist(), 3 * 4096 * T)
     with pytest.raises(ValueError, match="pixel"):
         _validate_tiles(ev, s0, n_per, arr_len, ts.tolist(), 3 * 4096 * (T - 1))
+
+
+def _pack_case(rng, T, with_empty=True):
+    from weft_amd.records import PRIM_DTYPE
+
+    sizes = rng.choice([8, 16, 32, 64], T).tolist()
+    idx = rng.permutation(3 * T)[:T].tolist()
+    offs = {ti: 3 * 4096 * k for k, ti in enumerate(idx)}
+    lo = 0 if with_empty else 1
+    counts = [rng.integers(0, 6, int(rng.integers(lo, 7))).astype(rng.choice([np.int32, np.int64, np.uint16]))
+              for _ in range(T)]
+    arrays = []
+    for c in counts:
+        a = np.zeros(int(c.sum()) + int(rng.integers(0, 3)), PRIM_DTYPE)
+        a.view(np.uint8)[:] = rng.integers(0, 256, a.nbytes, dtype=np.uint8)
+        arrays.append(a)
+    return sizes, idx, offs, counts, arrays
+
+
+def test_pack_extension_equals_numpy_packing():
+    """The threaded packing extension writes the NumPy path's eval descriptors, size-grouped
+    launch order and records, byte for byte, and splits results into the same per-tile views."""
+    from weft_amd import _lib
+    from weft_amd.gpu_render import _pack_evals, _tile_split_order
+
+    pk = _lib.pack_module()
+    rng = np.random.default_rng(11)
+    for T, threads in ((1, 1), (37, 3), (700, 8), (700, 1)):
+        sizes, idx, offs, counts, arrays = _pack_case(rng, T)
+        n_floats = 3 * 4096 * T
+        r = pk.pack(arrays, counts, idx, sizes, offs, n_floats, np.zeros(16, np.uint8), threads)
+        assert r[0] == "need"
+        buf = np.zeros(int(r[1]), np.uint8)
+        E, P, off_ord, off_rec, groups, e_start, n_per = pk.pack(arrays, counts, idx, sizes, offs, n_floats, buf,
+                                                                  threads)
+        ev, (s0, n0), arr_len = _pack_evals(offs, idx, sizes, arrays, counts)
+        order, g0 = _tile_split_order(sizes, s0, n0)
+        assert E == len(ev) and P == int(arr_len.sum()) and groups == g0
+        assert np.array_equal(e_start, s0) and np.array_equal(n_per, n0)
+        assert off_ord % 256 == 0 and off_rec % 256 == 0 and off_ord >= 16 * E and off_rec >= off_ord + 4 * E
+        assert buf[:16 * E].tobytes() == ev.tobytes()
+        assert np.array_equal(buf[off_ord:off_ord + 4 * E].view(np.int32), order)
+        recs = np.concatenate([a.view(np.uint8) for a in arrays])
+        assert buf[off_rec:off_rec + 60 * P].tobytes() == recs.tobytes()
+        vals = rng.random(E).astype(np.float32)
+        views = pk.split(vals, e_start, n_per)
+        assert len(views) == T
+        for j in range(T):
+            assert views[j].dtype == np.float32 and views[j].base is vals
+            assert np.array_equal(views[j], vals[s0[j]:s0[j] + n0[j]])
+
+
+def test_pack_extension_validates_like_numpy_path():
+    """Stacks past their tile's records, tiles past the pixel buffer, unknown tile sizes,
+    missing offsets and negative counts raise the NumPy path's errors; layouts the extension
+    does not take raise NotImplementedError (the drop-in then takes the NumPy path)."""
+    from weft_amd import _lib
+    from weft_amd.records import PRIM_DTYPE
+
+    pk = _lib.pack_module()
+    rng = np.random.default_rng(12)
+    T = 50
+    sizes, idx, offs, counts, arrays = _pack_case(rng, T, with_empty=False)
+    n_floats = max(offs[ti] + 3 * s * s for ti, s in zip(idx, sizes))
+    buf = np.zeros(1 << 22, np.uint8)
+    pk.pack(arrays, counts, idx, sizes, offs, n_floats, buf, 4)
+    j = int(np.flatnonzero([c.sum() > 0 for c in counts])[0])
+    short = list(arrays)
+    short[j] = arrays[j][:int(counts[j].sum()) - 1]
+    with pytest.raises(ValueError, match="primitive"):
+        pk.pack(short, counts, idx, sizes, offs, n_floats, buf, 4)
+    with pytest.raises(ValueError, match="pixel"):
+        pk.pack(arrays, counts, idx, sizes, offs, n_floats - 1, buf, 4)
+    bad = list(sizes)
+    bad[3] = 12
+    with pytest.raises(ValueError, match="tile sizes"):
+        pk.pack(arrays, counts, idx, bad, offs, n_floats, buf, 4)
+    with pytest.raises(KeyError):
+        pk.pack(arrays, counts, idx, sizes, {k: v for k, v in offs.items() if k != idx[5]}, n_floats, buf, 4)
+    neg = list(counts)
+    neg[j] = counts[j].astype(np.int64)
+    neg[j][0] = -1
+    with pytest.raises(ValueError, match="primitive"):
+        pk.pack(arrays, neg, idx, sizes, offs, n_floats, buf, 4)
+    other = list(arrays)
+    other[2] = np.zeros(len(arrays[2]), np.dtype([("x", np.float32, 15)]))
+    with pytest.raises(NotImplementedError):
+        pk.pack(other, counts, idx, sizes, offs, n_floats, buf, 4)
+    strided = list(arrays)
+    strided[1] = np.zeros(2 * len(arrays[1]), PRIM_DTYPE)[::2]
+    with pytest.raises(NotImplementedError):
+        pk.pack(strided, counts, idx, sizes, offs, n_floats, buf, 4)
+    floats = list(counts)
+    floats[0] = counts[0].astype(np.float32)
+    with pytest.raises(NotImplementedError):
+        pk.pack(arrays, floats, idx, sizes, offs, n_floats, buf, 4)
+
+
+def test_pack_extension_equal_and_copy():
+    from weft_amd import _lib
+
+    pk = _lib.pack_module()
+    rng = np.random.default_rng(13)
+    a = rng.random(3 << 20).astype(np.float32)
+    b = np.empty_like(a)
+    pk.copy(b, a, 8)
+    assert np.array_equal(a, b) and pk.equal(a, b, 8) and pk.equal(a, b, 1)
+    b[-1] += 1.0
+    assert not pk.equal(a, b, 8)
+    b[-1] = a[-1]
+    b[0] += 1.0
+    assert not pk.equal(a, b, 8)
+    assert not pk.equal(a, b[:-1], 8)

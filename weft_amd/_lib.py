@@ -254,6 +254,42 @@ def stream_ptr(stream=None) -> int:
     return int(s.cuda_stream)
 
 
+_pack_mod = None
+
+
+def pack_module():
+    """The host-side packing extension (lib/_weft_pack, built with the library)."""
+    global _pack_mod
+    with _lock:
+        if _pack_mod is not None:
+            return _pack_mod
+        import importlib.machinery
+        import importlib.util
+        import sysconfig
+
+        path = os.path.join(PKG, "lib", "_weft_pack" + sysconfig.get_config_var("EXT_SUFFIX"))
+        if not os.path.exists(path):
+            raise WeftHipUnavailable(f"{path} is not built; run `python -m weft_amd.build`")
+        loader = importlib.machinery.ExtensionFileLoader("_weft_pack", path)
+        spec = importlib.util.spec_from_file_location("_weft_pack", path, loader=loader)
+        mod = importlib.util.module_from_spec(spec)
+        loader.exec_module(mod)
+        from .records import PRIM_DTYPE
+
+        mod.set_record_dtype(PRIM_DTYPE)
+        _pack_mod = mod
+        return mod
+
+
+def host_threads() -> int:
+    """Threads for host-side packing: the CPUs this process may run on, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def tables() -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     """The candidate tables the kernels use (log-radius codes, cos, sin)."""
     logr = np.zeros(4096, np.uint16)

@@ -7,6 +7,8 @@
 2. The code object is embedded into the library through an ``.incbin`` assembly stub.
 3. ``g++`` compiles csrc/weft_host.cpp against the HIP runtime and links
    ``lib/libweft_hip.so``.
+4. ``g++`` compiles csrc/weft_pack.cpp, the host-side packing of the prepacked scoring
+   drop-in, into the CPython extension ``lib/_weft_pack``.
 
 Run ``python -m weft_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -16,6 +18,7 @@ import os
 import shutil
 import subprocess
 import sys
+import sysconfig
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -70,7 +73,24 @@ def build(verbose: bool = True, libdir: str | None = None, defines: tuple[str, .
           "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-o", libso + ".tmp"])
     os.replace(libso + ".tmp", libso)
     os.remove(embed)
+    build_pack(libdir)
     return libso
+
+
+def pack_path(libdir: str | None = None) -> str:
+    return os.path.join(libdir or LIBDIR, "_weft_pack" + sysconfig.get_config_var("EXT_SUFFIX"))
+
+
+def build_pack(libdir: str | None = None) -> str:
+    """The CPython extension that packs the prepacked drop-in's inputs (no HIP code)."""
+    import numpy as np
+
+    out = pack_path(libdir)
+    _run(["g++", "-O3", "-fPIC", "-shared", "-std=c++17", "-Wall", "-Wextra", "-Wno-unused-parameter",
+          "-Wno-missing-field-initializers", "-I", sysconfig.get_paths()["include"], "-I", np.get_include(),
+          os.path.join(CSRC, "weft_pack.cpp"), "-pthread", "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
 
 
 if __name__ == "__main__":

@@ -133,17 +133,17 @@ def score_evals_split_device(d_evals, d_order, groups, d_prims, d_tiles, d_out_m
 
 
 class _Staging:
-    """Reused pinned host buffers and the device copy of the last pixel buffer.
+    """Reused pinned host buffers, device buffers and the device copy of the last pixel buffer.
 
     The pixel buffer the reference encoder passes (all tiles' pixels, built once per fit,
-    encoder.py:2657-2666) is uploaded again only when another array, another buffer or other
-    contents arrive: the contents are compared through a 64-bit xxh3 digest (without xxhash
-    the buffer is uploaded on every call)."""
+    encoder.py:2657-2666) is uploaded again only when other contents arrive: a host copy of
+    the last uploaded bytes is compared with the new buffer (a threaded memcmp, exact)."""
 
     def __init__(self):
-        self.pix_key = None
+        self.pix_host = None
         self.d_pix = None
         self.host = {}
+        self.dev = {}
 
     def pinned(self, name: str, nbytes: int):
         import torch
@@ -154,20 +154,29 @@ class _Staging:
             self.host[name] = buf
         return buf
 
+    def device(self, name: str, nbytes: int, dev):
+        import torch
+
+        buf = self.dev.get(name)
+        if buf is None or buf.numel() < nbytes:
+            self.dev.pop(name, None)
+            buf = torch.empty(max(nbytes, 1 << 16) * 5 // 4, dtype=torch.uint8, device=dev)
+            self.dev[name] = buf
+        return buf
+
     def pixels(self, tiles_flat: np.ndarray, dev):
         import torch
 
         flat = np.ascontiguousarray(tiles_flat, dtype=np.float32)
-        try:
-            import xxhash
-
-            digest = xxhash.xxh3_64_intdigest(flat)
-        except ImportError:
-            digest = None
-        key = (flat.ctypes.data, flat.nbytes, digest)
-        if digest is None or key != self.pix_key or self.d_pix is None:
-            self.d_pix = torch.from_numpy(flat).to(dev)
-            self.pix_key = key
+        pk = _lib.pack_module()
+        if self.d_pix is not None and self.pix_host is not None and self.pix_host.nbytes == flat.nbytes \
+                and pk.equal(flat, self.pix_host, _lib.host_threads()):
+            return self.d_pix
+        self.d_pix = None
+        self.d_pix = torch.from_numpy(flat).to(dev)
+        if self.pix_host is None or self.pix_host.nbytes != flat.nbytes:
+            self.pix_host = np.empty_like(flat)
+        pk.copy(self.pix_host, flat, _lib.host_threads())
         return self.d_pix
 
 
@@ -251,6 +260,46 @@ def _pack_evals(tile_pixel_offsets, tile_indices, tile_sizes, eval_prim_arrays, 
     return evals, (e_start, n_per), arr_len
 
 
+def _score_prepacked_native(tile_pixels_flat, tile_pixel_offsets, tile_indices, tile_sizes, eval_prim_arrays,
+                            eval_prim_counts):
+    """gpu_batch_score_prepacked through the packing extension: descriptors, launch order and
+    records written by several threads into one pinned buffer, one host-to-device copy, one
+    launch per tile size, one device-to-host copy, per-tile views of one result array.
+    Raises NotImplementedError for inputs the extension does not take (the NumPy path then
+    runs)."""
+    import torch
+
+    pk = _lib.pack_module()
+    threads = _lib.host_threads()
+    n_floats = len(tile_pixels_flat)
+    buf = _STAGING.pinned("packed", 1 << 20)
+    r = pk.pack(eval_prim_arrays, eval_prim_counts, tile_indices, tile_sizes, tile_pixel_offsets, n_floats,
+                buf.numpy(), threads)
+    if r[0] == "need":
+        buf = _STAGING.pinned("packed", int(r[1]))
+        r = pk.pack(eval_prim_arrays, eval_prim_counts, tile_indices, tile_sizes, tile_pixel_offsets, n_floats,
+                    buf.numpy(), threads)
+    n_ev, n_rec, off_ord, off_rec, groups, e_start, n_per = r
+    nbytes = int(off_rec) + max(int(n_rec), 1) * PRIM_DTYPE.itemsize
+    dev = torch.device("cuda", 0)
+    t_tiles = _STAGING.pixels(tile_pixels_flat, dev)
+    d_buf = _STAGING.device("packed", nbytes, dev)
+    d_buf[:nbytes].copy_(buf[:nbytes], non_blocking=True)
+    d_out = _STAGING.device("mse", 4 * n_ev, dev)
+    L = _lib.library()
+    base = d_buf.data_ptr()
+    pos = 0
+    for ts, n in groups:
+        _lib.check(L.weft_score_evals_split(base, base + int(off_ord) + 4 * pos, n, ts, base + int(off_rec),
+                                            t_tiles.data_ptr(), d_out.data_ptr(), None, _lib.stream_ptr()),
+                   "weft_score_evals_split")
+        pos += n
+    host = _STAGING.pinned("mse", 4 * n_ev)
+    host[:4 * n_ev].copy_(d_out[:4 * n_ev])
+    mse = host[:4 * n_ev].numpy().view(np.float32).copy()
+    return pk.split(mse, e_start, n_per)
+
+
 def gpu_batch_score_prepacked(
     tile_pixels_flat: np.ndarray,
     tile_pixel_offsets: dict[int, int],
@@ -265,12 +314,18 @@ def gpu_batch_score_prepacked(
     holds tile j's stacks back to back, ``eval_prim_counts[j]`` the length of each stack;
     returns one float32 MSE array per tile, or None when the GPU path is unavailable.
     """
-    total = int(sum(len(c) for c in eval_prim_counts))
-    if total == 0:
+    if not any(map(len, eval_prim_counts)):
         return [np.empty(0, dtype=np.float32) for _ in tile_indices]
     try:
         if not _ready():
             return None
+        if isinstance(tile_pixels_flat, np.ndarray) and tile_pixels_flat.dtype == np.float32 \
+                and tile_pixels_flat.ndim == 1:
+            try:
+                return _score_prepacked_native(tile_pixels_flat, tile_pixel_offsets, tile_indices, tile_sizes,
+                                               eval_prim_arrays, eval_prim_counts)
+            except NotImplementedError:
+                pass  # a layout the packing extension does not take: the NumPy path below
         if any(int(s) not in KERNEL_TILE_SIZES for s in tile_sizes):
             return _fail(ValueError(f"tile sizes must be in {KERNEL_TILE_SIZES}"))
         evals, (e_start, n_per), arr_len = _pack_evals(tile_pixel_offsets, tile_indices, tile_sizes,

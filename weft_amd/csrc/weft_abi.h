@@ -198,6 +198,8 @@ struct WeftTemplateFitArgs {
     int32_t height;
     int32_t tile_size;
     int32_t tile_cols;
+    int32_t tile_rows;
+    int32_t _pad;
 };
 
 struct WeftSelectArgs {

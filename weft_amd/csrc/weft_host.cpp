@@ -733,10 +733,10 @@ int weft_template_fit(const float* d_image, int width, int height, int tile_size
     if (!d_image || !d_mask || !d_model || !d_c0 || !d_c1 || !d_mse || !d_c0_all || !d_c1_all)
         return set_err(WEFT_E_INVALID, "weft_template_fit: null pointer");
     WeftTemplateFitArgs a{d_image, d_mask, d_model, d_c0, d_c1, d_mse, d_c0_all, d_c1_all,
-                          width, height, tile_size, (int32_t)cols};
-    const int n = tile_size * tile_size;
-    const unsigned lds = (unsigned)((3 * (n + 16) + n) * 4);
-    return launch(g_mod.f_tmpl_fit, (unsigned)(cols * rows), 256, lds, stream, &a);
+                          width, height, tile_size, (int32_t)cols, (int32_t)rows, 0};
+    // 14 tiles of 18 templates per 256-thread workgroup, static LDS only
+    const int64_t groups = (cols * rows + 13) / 14;
+    return launch(g_mod.f_tmpl_fit, (unsigned)groups, 256, 0, stream, &a);
 }
 
 // ---- selection primitives: workspace layout shared by the size query and the launchers

@@ -6,11 +6,9 @@
 // Reference: src/weft/gpu_encoder.py:42-251 (weft_model_bucket, weft_tile_fit_candidates).
 // The bucket of a pixel depends only on its position in the tile, so the 18 buckets of
 // every position are computed once per tile size into a bit table (bit m = bucket of
-// template m).  The fit reads a tile into LDS and forms the bucket sums and squared errors
-// with float64 accumulators in a fixed order: the result does not depend on scheduling
-// (the reference's float32 shared-memory atomics do).  The colour sums are exact when each
-// channel value of a tile is 0 or in [2^-17, 1] (a sum of at most 4096 such values needs
-// no more than 53 significant bits).
+// template m).  The fit gives each (tile, template) one thread that forms the bucket sums
+// and the squared errors with float64 accumulators in position order: the result does not
+// depend on scheduling (the reference's float32 shared-memory atomics do).
 
 constexpr int TMPL_MODELS = 18;
 constexpr int TMPL_MAX_TS = 64;
@@ -113,198 +111,220 @@ __device__ void tmpl_masks_body(const WeftTemplateMaskArgs& A) {
     }
 }
 
-// Dynamic LDS: three channel planes of n = ts^2 floats (each n + 16 apart) and n mask words.
-__host__ __device__ constexpr int tmpl_plane_stride(int n) { return n + 16; }
-
-// Both passes: thread = (pixel segment, group of 6 templates), 84 segments x 3 groups.  A
-// pixel read (three values and a mask word) feeds 36 float64 sums in the sum pass and 6 in
-// the error pass.
-constexpr int TMPL_GROUP = 6;
-constexpr int TMPL_GROUPS = TMPL_MODELS / TMPL_GROUP;
-constexpr int TMPL_SEGS = 84;
-constexpr int TMPL_THREADS = TMPL_SEGS * TMPL_GROUPS;   // 252
-constexpr int TMPL_CH_SUMS = TMPL_MODELS * 2;           // (template, bucket) sums of one channel
-constexpr int TMPL_L1 = TMPL_THREADS / TMPL_CH_SUMS;    // 7 first-level partial sums
-constexpr int TMPL_L1_SPAN = TMPL_SEGS / TMPL_L1;        // of 12 segments each
+// The fit: one thread per (tile, template), 14 tiles of 18 templates per 256-thread
+// workgroup.  Every thread walks its tile's positions in order 0 .. n-1 twice: the bucket
+// sums (float64), then the squared errors of the two-colour prediction (float32 per pixel,
+// accumulated in float64) -- the sequential order of gpu_encoder.py's per-template loops as
+// the oracle restates them, so sums, means and MSE do not depend on scheduling.  Positions
+// are staged through LDS in bands of 64 for all 14 tiles (float64 for the sums, float32 for
+// the errors) with the band's mask words beside them.
+constexpr int TMPL_WG = 256;
+constexpr int TMPL_SLOTS = 14;                       // tiles per workgroup
+constexpr int TMPL_BAND = 64;                        // positions per staged band
+constexpr int TMPL_ST64 = TMPL_BAND * 3 + 1;         // slot stride of the float64 band (odd: no bank conflicts)
+constexpr int TMPL_ST32 = TMPL_BAND * 3 + 1;         // slot stride of the float32 band
+static_assert(TMPL_SLOTS * TMPL_MODELS <= TMPL_WG, "one thread per (tile, template)");
 
 struct TmplShared {
-    double part[TMPL_SEGS * TMPL_CH_SUMS];  // per-segment partials (one channel at a time)
-    double part2[TMPL_L1][TMPL_CH_SUMS];
-    double sums[3][TMPL_CH_SUMS];
-    float col[TMPL_MODELS][2][3];
-    float mse[TMPL_MODELS];
+    union {
+        double v64[TMPL_SLOTS * TMPL_ST64];
+        float v32[TMPL_SLOTS * TMPL_ST32];
+    };
+    uint32_t mask[TMPL_BAND];
+    float mse[TMPL_SLOTS][TMPL_MODELS];
+    int x0[TMPL_SLOTS], y0[TMPL_SLOTS];
 };
-static_assert(TMPL_L1 * TMPL_CH_SUMS == TMPL_THREADS && TMPL_L1 * TMPL_L1_SPAN == TMPL_SEGS, "combine shape");
-static_assert(TMPL_SEGS * TMPL_MODELS <= TMPL_SEGS * TMPL_CH_SUMS, "error partials reuse the buffer");
 
-// Sum pass of one thread.  FMA: every value is finite, so b * v + s (b = 0 or 1) is the
-// exact b ? s + v : s with one instruction; otherwise the selects keep an infinite or NaN
-// value out of the other bucket.
-template <bool FMA>
-__device__ __forceinline__ void tmpl_sum_pass(const float* px0, int PS, const uint32_t* mask, int p0, int p1, int g,
-                                              double (&s)[TMPL_GROUP][2][3]) {
+// A band of positions [p0, p0 + cnt) of the workgroup's tiles, fetched into registers (so
+// that the next band's loads are in flight while the current one is scored) and then
+// written to LDS.  Edge pixels repeat the last row / column, as the reference kernel reads
+// them.
+constexpr int TMPL_PER = (TMPL_SLOTS * TMPL_BAND + TMPL_WG - 1) / TMPL_WG;  // (slot, position) pairs per thread
+
+struct TmplBand {
+    float v[TMPL_PER][3];
+};
+
+__device__ __forceinline__ void tmpl_fetch(const WeftTemplateFitArgs& A, const TmplShared& S, int tile0, int n_tiles,
+                                           int p0, int cnt, float inv_ts, TmplBand& B) {
+    const int ts = A.tile_size;
 #pragma unroll
-    for (int j = 0; j < TMPL_GROUP; ++j)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) s[j][0][c] = s[j][1][c] = 0.0;
-    for (int p = p0; p < p1; ++p) {
-        const double v[3] = {(double)px0[p], (double)px0[PS + p], (double)px0[2 * PS + p]};
-        const uint32_t mk = mask[p] >> (g * TMPL_GROUP);
-#pragma unroll
-        for (int j = 0; j < TMPL_GROUP; ++j) {
-            const uint32_t bit = (mk >> j) & 1u;
-            if constexpr (FMA) {
-                const double b1 = (double)bit, b0 = 1.0 - b1;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    s[j][0][c] = __builtin_fma(b0, v[c], s[j][0][c]);
-                    s[j][1][c] = __builtin_fma(b1, v[c], s[j][1][c]);
-                }
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    s[j][0][c] += bit ? 0.0 : v[c];
-                    s[j][1][c] += bit ? v[c] : 0.0;
-                }
-            }
+    for (int j = 0; j < TMPL_PER; ++j) {
+        const int e = threadIdx.x + j * TMPL_WG;
+        const int sl = e / TMPL_BAND, k = e - sl * TMPL_BAND;
+        if (sl < TMPL_SLOTS && k < cnt && tile0 + sl < n_tiles) {
+            const int i = p0 + k;
+            // i / ts exactly: (i + 1/2) / ts lies at least 1/128 from an integer for ts <= 64,
+            // and the float32 quotient is within 5e-4 of it for i < 4096
+            const int ly = (int)(((float)i + 0.5f) * inv_ts);
+            const int lx = i - ly * ts;
+            const int x = min(S.x0[sl] + lx, A.width - 1);
+            const int y = min(S.y0[sl] + ly, A.height - 1);
+            const float* src = A.image + ((int64_t)y * A.width + x) * 3;
+            B.v[j][0] = src[0];
+            B.v[j][1] = src[1];
+            B.v[j][2] = src[2];
         }
     }
 }
 
-// One 256-thread workgroup per tile.  Every float64 sum is taken in a fixed order.
+// Write a fetched band (float64 for the sums, float32 for the errors) and the band's mask
+// words; returns whether this thread wrote a non-finite value.
+template <bool F64>
+__device__ __forceinline__ int tmpl_store(const WeftTemplateFitArgs& A, TmplShared& S, int tile0, int n_tiles, int p0,
+                                          int cnt, const TmplBand& B) {
+    int nonfinite = 0;
+#pragma unroll
+    for (int j = 0; j < TMPL_PER; ++j) {
+        const int e = threadIdx.x + j * TMPL_WG;
+        const int sl = e / TMPL_BAND, k = e - sl * TMPL_BAND;
+        if (sl < TMPL_SLOTS && k < cnt && tile0 + sl < n_tiles) {
+            const float r = B.v[j][0], g = B.v[j][1], b = B.v[j][2];
+            nonfinite |= !(isfinite(r) && isfinite(g) && isfinite(b));
+            if constexpr (F64) {
+                double* d = S.v64 + sl * TMPL_ST64 + k * 3;
+                d[0] = (double)r;
+                d[1] = (double)g;
+                d[2] = (double)b;
+            } else {
+                float* d = S.v32 + sl * TMPL_ST32 + k * 3;
+                d[0] = r;
+                d[1] = g;
+                d[2] = b;
+            }
+        }
+    }
+    for (int k = threadIdx.x; k < cnt; k += TMPL_WG) S.mask[k] = A.mask[p0 + k];
+    return nonfinite;
+}
+
+__device__ __forceinline__ double tmpl_hi(uint32_t hi) { return __longlong_as_double((long long)((uint64_t)hi << 32)); }
+
 __device__ void tmpl_fit_body(const WeftTemplateFitArgs& A) {
     __shared__ TmplShared S;
-    extern __shared__ float4 weft_smem[];
-    const int tile = blockIdx.x;
     const int tid = threadIdx.x;
     const int ts = A.tile_size;
     const int n = ts * ts;
-    float* const px0 = reinterpret_cast<float*>(weft_smem);
-    const int PS = tmpl_plane_stride(n);
-    uint32_t* const mask = reinterpret_cast<uint32_t*>(px0 + 3 * PS);
-    const int x0 = (tile % A.tile_cols) * ts;
-    const int y0 = (tile / A.tile_cols) * ts;
-    int nonfinite = 0;
-    for (int i = tid; i < n; i += blockDim.x) {
-        const int ly = i / ts, lx = i - ly * ts;
-        const int x = min(x0 + lx, A.width - 1);
-        const int y = min(y0 + ly, A.height - 1);
-        const float* src = A.image + ((int64_t)y * A.width + x) * 3;
-        const float r = src[0], g = src[1], b = src[2];
-        nonfinite |= !(isfinite(r) && isfinite(g) && isfinite(b));
-        px0[i] = r;
-        px0[PS + i] = g;
-        px0[2 * PS + i] = b;
-        mask[i] = A.mask[i];
-    }
-    const bool all_finite = !__syncthreads_or(nonfinite);
-
-    const bool worker = tid < TMPL_THREADS;
-    const int seg = tid / TMPL_GROUPS;
-    const int g = tid - seg * TMPL_GROUPS;
-    const int p0 = (seg * n) / TMPL_SEGS, p1 = ((seg + 1) * n) / TMPL_SEGS;
-
-    // bucket sums, then per channel: segment partials -> 7 partials -> total
-    double s[TMPL_GROUP][2][3];
-    if (worker) {
-        if (all_finite) tmpl_sum_pass<true>(px0, PS, mask, p0, p1, g, s);
-        else tmpl_sum_pass<false>(px0, PS, mask, p0, p1, g, s);
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        if (worker) {
-#pragma unroll
-            for (int j = 0; j < TMPL_GROUP; ++j)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) S.part[seg * TMPL_CH_SUMS + (g * TMPL_GROUP + j) * 2 + b] = s[j][b][c];
-        }
-        __syncthreads();
-        if (worker) {
-            const int q = tid % TMPL_CH_SUMS, k = tid / TMPL_CH_SUMS;
-            double v = 0.0;
-            for (int i = 0; i < TMPL_L1_SPAN; ++i) v += S.part[(k * TMPL_L1_SPAN + i) * TMPL_CH_SUMS + q];
-            S.part2[k][q] = v;
-        }
-        __syncthreads();
-        if (tid < TMPL_CH_SUMS) {
-            double v = 0.0;
-#pragma unroll
-            for (int k = 0; k < TMPL_L1; ++k) v += S.part2[k][tid];
-            S.sums[c][tid] = v;
-        }
-    }
-    __syncthreads();
-    if (tid < TMPL_MODELS * 3) {
-        const int m = tid / 3, c = tid - m * 3;
-        const int c0 = (int)A.mask[n + m], c1 = n - c0;
-        const float v0 = (float)S.sums[c][m * 2] * (1.0f / (float)max(c0, 1));
-        S.col[m][0][c] = v0;
-        S.col[m][1][c] = m == 0 ? v0 : (float)S.sums[c][m * 2 + 1] * (1.0f / (float)max(c1, 1));
+    const int n_tiles = A.tile_cols * A.tile_rows;
+    const int tile0 = blockIdx.x * TMPL_SLOTS;
+    const int slot = tid / TMPL_MODELS;
+    const int m = tid - slot * TMPL_MODELS;
+    const int tile = tile0 + slot;
+    const bool valid = tid < TMPL_SLOTS * TMPL_MODELS && tile < n_tiles;
+    const float inv_ts = 1.0f / (float)ts;
+    if (tid < TMPL_SLOTS) {
+        const int t = tile0 + tid;
+        S.x0[tid] = (t % A.tile_cols) * ts;
+        S.y0[tid] = (t / A.tile_cols) * ts;
     }
     __syncthreads();
 
-    // squared error of each template's two-colour prediction
-    if (worker) {
-        float a[TMPL_GROUP][3], b[TMPL_GROUP][3];
-        double acc[TMPL_GROUP];
+    // bucket sums in position order; b * v + s with b = 0 or 1 is the exact conditional add
+    // for finite v, and bands holding inf / NaN take selects so that a non-finite value
+    // stays in its own bucket
+    double s0[3] = {0.0, 0.0, 0.0}, s1[3] = {0.0, 0.0, 0.0};
+    TmplBand B;
+    tmpl_fetch(A, S, tile0, n_tiles, 0, min(TMPL_BAND, n), inv_ts, B);
+    for (int p0 = 0; p0 < n; p0 += TMPL_BAND) {
+        const int cnt = min(TMPL_BAND, n - p0);
+        const int nf = __syncthreads_or(tmpl_store<true>(A, S, tile0, n_tiles, p0, cnt, B));
+        // the next band (or the error pass's first) is fetched while this one is summed
+        {
+            const int q0 = p0 + TMPL_BAND < n ? p0 + TMPL_BAND : 0;
+            tmpl_fetch(A, S, tile0, n_tiles, q0, min(TMPL_BAND, n - q0), inv_ts, B);
+        }
+        if (valid) {
+            const double* v = S.v64 + slot * TMPL_ST64;
+            if (!nf) {
+#pragma unroll 8
+                for (int k = 0; k < cnt; ++k) {
+                    // bit m of the mask word sign-extended: 0 or all ones
+                    const uint32_t hi1 = (uint32_t)__builtin_amdgcn_sbfe((int)S.mask[k], (unsigned)m, 1u) & 0x3FF00000u;
+                    const double b1 = tmpl_hi(hi1), b0 = tmpl_hi(hi1 ^ 0x3FF00000u);
 #pragma unroll
-        for (int j = 0; j < TMPL_GROUP; ++j) {
+                    for (int c = 0; c < 3; ++c) {
+                        s0[c] = __builtin_fma(b0, v[3 * k + c], s0[c]);
+                        s1[c] = __builtin_fma(b1, v[3 * k + c], s1[c]);
+                    }
+                }
+            } else {
+                for (int k = 0; k < cnt; ++k) {
+                    const bool o = (S.mask[k] >> m) & 1u;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        s0[c] += o ? 0.0 : v[3 * k + c];
+                        s1[c] += o ? v[3 * k + c] : 0.0;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // bucket means: (float)sum * (1 / max(count, 1)), the constant template's buckets alike
+    float a[3], b[3];
+    {
+        const int c0 = valid ? (int)A.mask[n + m] : 1, c1 = n - c0;
+        const float i0 = 1.0f / (float)max(c0, 1), i1 = 1.0f / (float)max(c1, 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            a[c] = (float)s0[c] * i0;
+            b[c] = m == 0 ? a[c] : (float)s1[c] * i1;
+        }
+    }
+
+    // squared errors in position order: e = dr^2, e += dg^2, e += db^2 in float32, summed in float64
+    const uint32_t ua[3] = {__float_as_uint(a[0]), __float_as_uint(a[1]), __float_as_uint(a[2])};
+    const uint32_t ub[3] = {__float_as_uint(b[0]), __float_as_uint(b[1]), __float_as_uint(b[2])};
+    double err = 0.0;
+    for (int p0 = 0; p0 < n; p0 += TMPL_BAND) {
+        const int cnt = min(TMPL_BAND, n - p0);
+        tmpl_store<false>(A, S, tile0, n_tiles, p0, cnt, B);
+        __syncthreads();
+        if (p0 + TMPL_BAND < n)
+            tmpl_fetch(A, S, tile0, n_tiles, p0 + TMPL_BAND, min(TMPL_BAND, n - p0 - TMPL_BAND), inv_ts, B);
+        if (valid) {
+            const float* v = S.v32 + slot * TMPL_ST32;
+#pragma unroll 4
+            for (int k = 0; k < cnt; ++k) {
+                // the bucket's colour by a bitfield select on the sign-extended bucket bit
+                const uint32_t o = (uint32_t)__builtin_amdgcn_sbfe((int)S.mask[k], (unsigned)m, 1u);
+                const float dr = v[3 * k] - __uint_as_float((o & ub[0]) | (~o & ua[0]));
+                const float dg = v[3 * k + 1] - __uint_as_float((o & ub[1]) | (~o & ua[1]));
+                const float db = v[3 * k + 2] - __uint_as_float((o & ub[2]) | (~o & ua[2]));
+                float e = dr * dr;
+                e = e + dg * dg;
+                e = e + db * db;
+                err += (double)e;
+            }
+        }
+        __syncthreads();
+    }
+
+    const float mse = (float)err * (1.0f / (float)n);
+    if (valid) {
+        S.mse[slot][m] = mse;
+        const int64_t o = (int64_t)tile * TMPL_MODELS + m;
+        A.out_mse[o] = mse;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            A.out_c0_all[o * 3 + c] = a[c];
+            A.out_c1_all[o * 3 + c] = b[c];
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        // first least MSE (a NaN never replaces the current best)
+        int best = 0;
+        for (int q = 1; q < TMPL_MODELS; ++q)
+            if (S.mse[slot][q] < S.mse[slot][best]) best = q;
+        if (best == m) {
+            A.out_model[tile] = m;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                a[j][c] = S.col[g * TMPL_GROUP + j][0][c];
-                b[j][c] = S.col[g * TMPL_GROUP + j][1][c];
+                A.out_c0[(int64_t)tile * 3 + c] = a[c];
+                A.out_c1[(int64_t)tile * 3 + c] = b[c];
             }
-            acc[j] = 0.0;
-        }
-        for (int p = p0; p < p1; ++p) {
-            const float r = px0[p], gg = px0[PS + p], bb = px0[2 * PS + p];
-            const uint32_t mk = mask[p] >> (g * TMPL_GROUP);
-#pragma unroll
-            for (int j = 0; j < TMPL_GROUP; ++j) {
-                const bool o = (mk >> j) & 1u;
-                const float dr = r - (o ? b[j][0] : a[j][0]);
-                const float dg = gg - (o ? b[j][1] : a[j][1]);
-                const float db = bb - (o ? b[j][2] : a[j][2]);
-                acc[j] += (double)(dr * dr + dg * dg + db * db);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < TMPL_GROUP; ++j) S.part[seg * TMPL_MODELS + g * TMPL_GROUP + j] = acc[j];
-    }
-    __syncthreads();
-    if (tid < TMPL_L1 * TMPL_MODELS) {  // 7 x 18: 12 segments each
-        const int k = tid / TMPL_MODELS, m = tid - k * TMPL_MODELS;
-        double e = 0.0;
-        for (int i = 0; i < TMPL_L1_SPAN; ++i) e += S.part[(k * TMPL_L1_SPAN + i) * TMPL_MODELS + m];
-        S.part2[k][m] = e;
-    }
-    __syncthreads();
-    if (tid < TMPL_MODELS) {
-        double e = 0.0;
-#pragma unroll
-        for (int k = 0; k < TMPL_L1; ++k) e += S.part2[k][tid];
-        const float mse = (float)e * (1.0f / (float)n);
-        S.mse[tid] = mse;
-        const int64_t o = (int64_t)tile * TMPL_MODELS + tid;
-        A.out_mse[o] = mse;
-        for (int c = 0; c < 3; ++c) {
-            A.out_c0_all[o * 3 + c] = S.col[tid][0][c];
-            A.out_c1_all[o * 3 + c] = S.col[tid][1][c];
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int best = 0;
-        float be = S.mse[0];
-        for (int m = 0; m < TMPL_MODELS; ++m)
-            if (S.mse[m] < be) {
-                be = S.mse[m];
-                best = m;
-            }
-        A.out_model[tile] = best;
-        for (int c = 0; c < 3; ++c) {
-            A.out_c0[(int64_t)tile * 3 + c] = S.col[best][0][c];
-            A.out_c1[(int64_t)tile * 3 + c] = S.col[best][1][c];
         }
     }
 }

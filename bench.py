@@ -243,15 +243,15 @@ def time_to_psnr(tiles, mask, n_cand_plain, depth, seed, lam, refined_cfg):
     del rf
     # refined + batched: the refined fit in the persistent kernel, r rounds per launch, the
     # whole fit a graph replay.  It takes its own work split and workgroup size, the fastest
-    # of a few: it handles one tile at a time, so small tiles want more lanes per candidate
-    # than the stepwise kernel's shared waves.  (With the stepwise split it equals the
-    # stepwise fit bit for bit; another split orders the soft sums differently.)
+    # of a few (the split decides how many small tiles a workgroup refines together).  (With
+    # the stepwise split it equals the stepwise fit bit for bit; another split orders the soft
+    # sums differently.)
     batched = None
     if reach:
         r = reach["rounds"]
         tr, block, bcpw = None, None, None
-        splits = [cpw] + [m for m in ({8: 1, 16: 1, 32: 1, 64: 1}, {8: 2, 16: 1, 32: 1, 64: 1},
-                                      {8: 4, 16: 2, 32: 1, 64: 1}) if m != cpw]
+        splits = [cpw] + [m for m in ({8: 8, 16: 4, 32: 2, 64: 1}, {8: 4, 16: 2, 32: 1, 64: 1},
+                                      {8: 16, 16: 4, 32: 2, 64: 1}) if m != cpw]
         for sp in splits:
             for b in (128, 256):
                 bfr = BatchFitter(tiles, n_cand=n_cand, depth=depth, n_steps=r, seed=seed, lam=lam, k=k,

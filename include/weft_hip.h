@@ -328,6 +328,13 @@ typedef struct WeftBatch {
     int32_t n_steps;
     int32_t grid;   /* persistent workgroups */
     int32_t block;  /* threads per workgroup: 64, 128, 192 or 256 */
+    /* weft_batch_refine_fit only (weft_batch_fit ignores them): the queue hands out groups
+     * of tiles, group g = d_tile_order[d_group_first[g] .. d_group_first[g + 1]) (n_groups + 1
+     * entries), whose runs of at most group_max tiles of one size are refined and re-scored
+     * together; NULL: one tile per queue entry. */
+    const int32_t* d_group_first;
+    int32_t n_groups;
+    int32_t group_max;  /* 1 .. 64 / k */
 } WeftBatch;
 
 int weft_batch_fit(const WeftFitState* st, const WeftCands* cands, const WeftBatch* b, void* stream);

@@ -214,10 +214,14 @@ def test_batch_rejects_bad_launch(gpu):
 
 @pytest.mark.parametrize("k,cpw,order,block", [(4, None, "sjf", 256), (8, {8: 8, 16: 2, 32: 1, 64: 1}, "lpt", 128),
                                                (2, {8: 1, 16: 2, 32: 2, 64: 1}, "fifo", 64),
-                                               (16, None, "lpt", 256), (64, {8: 64, 16: 16, 32: 4, 64: 1}, "sjf", 128)])
+                                               (16, None, "lpt", 256), (64, {8: 64, 16: 16, 32: 4, 64: 1}, "sjf", 128),
+                                               (1, {8: 16, 16: 4, 32: 2, 64: 1}, "fifo", 128),
+                                               (2, {8: 32, 16: 8, 32: 4, 64: 2}, "sjf", 64)])
 def test_batch_refine_equals_stepwise_refined(gpu, k, cpw, order, block):
     """The persistent refined fit = RefinedFitter round by round, bit for bit.  k <= 8 takes
-    the top k by k workgroup-wide minima, k = 16 and 64 by the bitonic sort of the keys."""
+    the top k by k workgroup-wide minima, k = 16 and 64 by the bitonic sort of the keys.
+    Where cpw > k, groups of same-size tiles are refined and re-scored together (fifo over
+    four images breaks the runs of one size at every image)."""
     from weft_amd.batch import BatchFitter
     from weft_amd.refine import AdamConfig, RefinedFitter
 
@@ -238,3 +242,32 @@ def test_batch_refine_equals_stepwise_refined(gpu, k, cpw, order, block):
     assert (bf.records()["win_idx"] > 0).any(), "refined candidates win somewhere"
     bf.run(use_graph=True)  # graph replay gives the same fit
     _same_records(bf.records(), rf.records(), "graph replay")
+
+
+def test_batch_refine_groups_equal_single_tiles(gpu):
+    """Refining groups of same-size tiles together gives the fit that one tile per queue
+    entry gives, and a group table cut by hand into runs of mixed sizes is split by size."""
+    import torch
+    from weft_amd.batch import BatchFitter
+    from weft_amd.refine import AdamConfig
+
+    tiles = _batch_tiles()
+    adam = AdamConfig(n_iter=4, w0=0.7, w1=0.4)
+    kw = dict(n_cand=96, depth=3, n_steps=3, seed=23, lam=1e-5, k=2, refine=adam,
+              cpw={8: 16, 16: 8, 32: 4, 64: 2}, order="fifo", block=128)
+    one = BatchFitter(tiles, groups=False, **kw)
+    one.run(use_graph=False)
+    grp = BatchFitter(tiles, **kw)
+    grp.run(use_graph=False)
+    assert grp._n_groups < tiles.n_tiles
+    _same_records(grp.records(), one.records(), "grouped vs one tile")
+    assert bits_equal(grp.canvas(), one.canvas())
+    # every queue entry one group of up to 32 tiles of any sizes: the kernel takes runs of one size
+    n = tiles.n_tiles
+    firsts = list(range(0, n, 32)) + [n]
+    grp._group_table()
+    grp.d_group_first = torch.tensor(firsts, dtype=torch.int32, device=grp.dev)
+    grp._n_groups, grp._group_max = len(firsts) - 1, 32
+    grp.run(use_graph=False)
+    _same_records(grp.records(), one.records(), "mixed-size groups")
+    assert bits_equal(grp.canvas(), one.canvas())

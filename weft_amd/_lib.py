@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 18
+ABI_VERSION = 19
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -78,7 +78,8 @@ class Batch(ctypes.Structure):
         ("d_image_left", c_void_p), ("d_image_done", c_void_p), ("d_t_start", c_void_p),
         ("d_queue", c_void_p), ("d_win_idx", c_void_p), ("d_win_obj", c_void_p), ("d_win_acc", c_void_p),
         ("d_win_code", c_void_p), ("n_queue", c_int32), ("n_images", c_int32), ("step0", c_uint32),
-        ("n_steps", c_int32), ("grid", c_int32), ("block", c_int32),
+        ("n_steps", c_int32), ("grid", c_int32), ("block", c_int32), ("d_group_first", c_void_p),
+        ("n_groups", c_int32), ("group_max", c_int32),
     ]
 
 
@@ -110,7 +111,7 @@ class SplitWork(ctypes.Structure):
 
 assert ctypes.sizeof(SplitWork) == 56
 assert ctypes.sizeof(SoftBatch) == 56 and ctypes.sizeof(RefineOpts) == 112
-assert ctypes.sizeof(Batch) == 112
+assert ctypes.sizeof(Batch) == 128
 assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 32 and ctypes.sizeof(Cands) == 56
 assert ctypes.sizeof(Adaptive) == 64
 

@@ -76,7 +76,7 @@ class BatchFitter:
                  seed: int = 0, lam: float | None = None, obj_mode: int = 0, step0: int = 0,
                  order: str = "sjf", block: int = 256, grid: int | None = None,
                  keep_codes: bool = True, device: int = 0, k: int = 0, refine=None,
-                 cpw: dict[int, int] | None = None):
+                 cpw: dict[int, int] | None = None, groups: bool = True):
         import torch
 
         _lib.ensure_init(device)
@@ -138,6 +138,11 @@ class BatchFitter:
         # round refined by Adam and re-scored exactly; k = 0 is the plain fit
         self.k = int(k)
         self._opts = None
+        # refined rounds of consecutive same-size tiles are refined and re-scored together
+        # (one workgroup holds up to group_size(ts) of them); False: one tile at a time
+        self.groups = bool(groups)
+        self._group_key = None
+        self.d_group_first = None
         if self.k:
             from .refine import DEFAULT_CPW, AdamConfig
 
@@ -181,14 +186,49 @@ class BatchFitter:
             self.grid = int(grid) if grid not in (None, _KEEP) else self.n_cu * (16 * 64 // self.block)
         self.graph = None
 
+    def group_size(self, ts: int) -> int:
+        """Tiles of `ts` px refined together.  Tiles whose refinement shares a wave (cpw > k:
+        cpw / k tiles per wave) are grouped so that every wave of a workgroup holds cpw
+        candidates, at most 64 / k tiles; larger tiles fill a wave alone and stay one per
+        queue entry, which keeps the long entries short at the end of a fit."""
+        cpw = int(self.cpw[ts])
+        if cpw <= self.k:
+            return 1
+        return max(1, min(64 // self.k, (self.block // 64) * cpw // self.k))
+
+    def _group_table(self):
+        """Queue entries of the refined fit: runs of consecutive same-size tiles of the queue
+        order, cut into groups of at most group_size(ts); (first index per group + end, largest)."""
+        key = (self.order, self.block, tuple(sorted(self.cpw.items())), self.groups)
+        if key != self._group_key:
+            order = queue_order(self.tiles, self.order)
+            ts = self.tiles.ts[order] if len(order) else np.zeros(0, np.int32)
+            firsts, gmax = [], 1
+            i, n = 0, len(order)
+            while i < n:
+                j = i
+                while j < n and ts[j] == ts[i]:
+                    j += 1
+                gs = self.group_size(int(ts[i])) if self.groups else 1
+                gmax = max(gmax, gs)
+                firsts.extend(range(i, j, gs))
+                i = j
+            firsts.append(n)
+            self.d_group_first = self.torch.tensor(firsts, dtype=self.torch.int32, device=self.dev)
+            self._n_groups = len(firsts) - 1
+            self._group_max = gmax
+            self._group_key = key
+        return self.d_group_first.data_ptr(), self._n_groups, self._group_max
+
     def _batch(self) -> _lib.Batch:
         code = self.d_win_code.data_ptr() if self.d_win_code is not None else None
+        gp, ng, gm = self._group_table() if self.k else (None, 0, 1)
         return _lib.Batch(
             self.d_order.data_ptr(), self.d_tile_image.data_ptr(), self.d_image_tiles.data_ptr(),
             self.d_image_left.data_ptr(), self.d_image_done.data_ptr(), self.d_t_start.data_ptr(),
             self.d_queue.data_ptr(), self.d_win_idx.data_ptr(), self.d_win_obj.data_ptr(),
             self.d_win_acc.data_ptr(), code, self.n_tiles, self.n_images, self.step0, self.n_steps,
-            self.grid, self.block)
+            self.grid, self.block, gp, ng, gm)
 
     def _launch_fit(self) -> None:
         """Reset the tile state with the initial objectives (one launch), then the persistent

@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 18
+#define WEFT_ABI_VERSION 19
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -163,6 +163,9 @@ struct WeftRansDecodeArgs {
 struct WeftBatchRefineArgs {
     WeftBatchArgs b;
     WeftRefineOpts o;
+    const int32_t* group_first;  // queue entries: groups of tile_order (nullptr: one tile each)
+    int32_t n_groups;
+    int32_t group_max;           // tiles refined together, at most 64 / k
     int32_t split_lanes[4];
     int32_t n_pow2;
     int32_t _pad;

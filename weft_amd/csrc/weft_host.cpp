@@ -182,7 +182,7 @@ static_assert(sizeof(WeftFitState) == 80, "WeftFitState layout");
 static_assert(sizeof(WeftWork) == 32, "WeftWork layout");
 static_assert(sizeof(WeftCands) == 56, "WeftCands layout");
 static_assert(sizeof(WeftAdaptive) == 64, "WeftAdaptive layout");
-static_assert(sizeof(WeftBatch) == 112, "WeftBatch layout");
+static_assert(sizeof(WeftBatch) == 128, "WeftBatch layout");
 static_assert(sizeof(WeftSoftBatch) == 56, "WeftSoftBatch layout");
 static_assert(sizeof(WeftRefineOpts) == 112, "WeftRefineOpts layout");
 static_assert(sizeof(WeftCode) == 20, "WeftCode layout");
@@ -645,7 +645,12 @@ int weft_batch_refine_fit(const WeftFitState* st, const WeftCands* cands, const 
         if (!refine_cpw_ok(o->cpw_by_size[c], o->k))
             return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: cpw must be powers of two that divide k or are "
                                            "multiples of it by at most WEFT_WORK_MAX_TILES");
+    if (b->d_group_first && (b->n_groups < 0 || b->group_max < 1 || b->group_max > 64 / o->k))
+        return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: groups need n_groups >= 0 and group_max in 1 .. 64 / k");
     WeftBatchRefineArgs a{};
+    a.group_first = b->d_group_first;
+    a.n_groups = b->d_group_first ? b->n_groups : 0;
+    a.group_max = b->d_group_first ? b->group_max : 1;
     a.b = WeftBatchArgs{*st, *cands, b->d_tile_order, b->d_tile_image, b->d_image_tiles, b->d_image_left,
                         reinterpret_cast<unsigned long long*>(b->d_image_done),
                         reinterpret_cast<unsigned long long*>(b->d_t_start), b->d_queue, b->d_win_idx,

@@ -384,9 +384,13 @@ def decode_report(tiles, records, height, width):
         times.append(a.elapsed_time(b) * 1e-3)
     dev_s = float(np.median(times[1:]))
     same = bool(np.array_equal(d_o.cpu().numpy().reshape(height, width, 3), img))
+    # algorithmic HBM bytes: every output pixel written once (12 B), descriptors and records read once
+    alg = height * width * 12 + descs.nbytes + prims.nbytes
     return dict(api="gpu_render_tiles_to_image", width=width, height=height, tiles=int(tiles.n_tiles),
                 prims=int(counts.sum()), kernel="weft_render_tiles_kernel", device_s=dev_s,
-                mpx_per_s=height * width / dev_s / 1e6, host_inclusive_s=host, device_equals_api=same)
+                mpx_per_s=height * width / dev_s / 1e6, algorithmic_bytes=int(alg),
+                hbm_gb_per_s=alg / dev_s / 1e9, hbm_frac=alg / dev_s / 8e12, host_inclusive_s=host,
+                device_equals_api=same)
 
 
 def template_report(image, ts=16):
@@ -734,7 +738,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES of device memory once (HBM counter calibration) and exit")
     ap.add_argument("--only", choices=["template_fit", "select", "candidates", "refined_round", "class_probe",
-                                       "drop_in"],
+                                       "drop_in", "decode"],
                     default=None,
                     help="run only that report (for profiling) and print it as JSON")
     args = ap.parse_args()
@@ -748,6 +752,17 @@ def main():
         elif args.only == "drop_in":
             c, _, tiles, _ = build_workload(args)
             rep = drop_in_report(tiles, c["seed"])
+        elif args.only == "decode":
+            from weft_amd.batch import BatchFitter
+            from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+            from weft_amd.rd import tile_lambda
+
+            img4 = synth_image(2160, 3840, 2500, seed=0, texture=True)
+            t4 = TileSet.from_images([img4], [decompose_quadtree_variance(img4, threshold=1e-3, min_tile=8,
+                                                                           max_tile=64)])
+            bf = BatchFitter(t4, n_cand=2048, depth=24, n_steps=24, seed=0, lam=tile_lambda(75))
+            bf.run()
+            rep = decode_report(t4, bf.records(), 2160, 3840)
         elif args.only == "refined_round":
             rep = refined_round_report(*build_workload(args)[:3])
         elif args.only == "class_probe":

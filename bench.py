@@ -347,6 +347,39 @@ def drop_in_report(tiles, seed):
                 split_speedup=one_s / dev_s)
 
 
+def rounds_report(c, img, tiles, rounds: int = 40):
+    """The plain fit's scoring launch, round by round from an empty canvas (every tile stays
+    active): its duration per round, timed with HIP events on the launch stream."""
+    import torch
+    from weft_amd import _lib
+    from weft_amd.fit import PlainFitter
+    from weft_amd.rd import tile_lambda
+
+    f = PlainFitter(tiles, n_cand=c["n_cand"], depth=c["depth"], seed=0, lam=tile_lambda(75), max_steps=rounds)
+    stream = torch.cuda.current_stream()
+    sp = _lib.stream_ptr()
+    evs = []
+    for s in range(rounds):
+        cands = f._cands(s)
+        f.order(s, sp)
+        for w, start in f._works:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            f.score_class(w, start, cands, None, None, sp)
+            b.record(stream)
+            evs.append((a, b))
+        f.select()
+        f.step_idx += 1
+    torch.cuda.synchronize()
+    rec = f.records(0, rounds)
+    acc = rec["win_acc"].astype(np.int64)
+    active = ((np.cumsum(acc, axis=0) - acc) < c["depth"]).mean(axis=1)
+    return dict(kernel="weft_fit_score_mixed", rounds=rounds,
+                launch_ms=[round(a.elapsed_time(b), 4) for a, b in evs],
+                active_tile_frac=[round(float(x), 4) for x in active])
+
+
 def decode_report(tiles, records, height, width):
     """Decode walk (gpu_render_tiles_to_image) of a fitted image's accepted primitives."""
     import torch
@@ -730,15 +763,15 @@ def class_probe_report(tiles, c, lam, reps: int = 3):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the time-to-PSNR, batched and refine reports")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES of device memory once (HBM counter calibration) and exit")
     ap.add_argument("--only", choices=["template_fit", "select", "candidates", "refined_round", "class_probe",
-                                       "drop_in", "decode"],
+                                       "drop_in", "decode", "rounds"],
                     default=None,
                     help="run only that report (for profiling) and print it as JSON")
     args = ap.parse_args()
@@ -752,6 +785,8 @@ def main():
         elif args.only == "drop_in":
             c, _, tiles, _ = build_workload(args)
             rep = drop_in_report(tiles, c["seed"])
+        elif args.only == "rounds":
+            rep = rounds_report(*build_workload(args)[:3])
         elif args.only == "decode":
             from weft_amd.batch import BatchFitter
             from weft_amd.quadtree import TileSet, decompose_quadtree_variance

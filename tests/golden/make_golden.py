@@ -28,10 +28,15 @@ pinned against.  Functions captured (reference file:line):
 - ``render.render_tile`` with ``source_size`` (the scaled grid, render.py:52-78) and
   ``render.render_scene_adaptive`` (render.py:270-358, with ``upsample_residual_map``
   474-515) -> ``ref_scene.npz``
+- ``edge_analysis.generate_edge_driven_candidates`` (src/weft/edge_analysis.py:139-315), the
+  Sobel magnitudes and angles of ``_sobel_edges`` (17-41) and the dominant orientations of
+  ``_dominant_orientations`` (44-68), and the merged pools of ``encoder._worker_gen_cands``
+  (encoder.py:2490-2517) -> ``ref_edge.npz``
 
 ``python tests/golden/make_golden.py entropy`` writes only ``ref_entropy.npz``;
 ``python tests/golden/make_golden.py candidates`` writes only ``ref_candidates.npz``;
-``python tests/golden/make_golden.py scene`` writes only ``ref_scene.npz``.
+``python tests/golden/make_golden.py scene`` writes only ``ref_scene.npz``;
+``python tests/golden/make_golden.py edge`` writes only ``ref_edge.npz``.
 
 Every primitive parameter is a float32-representable value, so the float32 records the
 HIP path receives carry exactly what the reference's Python floats carry.
@@ -695,6 +700,75 @@ def adaptive_cases():
     return d
 
 
+def _records(cands):
+    kinds, geoms, ngeom, c0, c1, has1, alpha = [], [], [], [], [], [], []
+    for c in cands:
+        kinds.append(c.kind)
+        geoms.append(list(c.geom) + [0.0] * (7 - len(c.geom)))
+        ngeom.append(len(c.geom))
+        c0.append(list(c.color0))
+        c1.append(list(c.color1) if c.color1 is not None else [0.0, 0.0, 0.0])
+        has1.append(c.color1 is not None)
+        alpha.append(c.alpha)
+    return kinds, geoms, ngeom, c0, c1, has1, alpha
+
+
+def edge_cases():
+    """generate_edge_driven_candidates (and its Sobel / orientation steps) and the merged
+    per-tile pools of _worker_gen_cands, on tiles of every kind of content."""
+    from weft.edge_analysis import _dominant_orientations, _sobel_edges, generate_edge_driven_candidates
+    from weft.encoder import _worker_gen_cands
+
+    rng = np.random.default_rng(139)
+    d = {k: [] for k in ("ts", "quality", "mode", "tile_off", "cand_off", "merged_off", "orient_off")}
+    tiles, mags, angles, orients = [], [], [], []
+    rec = {k: [] for k in ("kind", "geom", "ngeom", "color0", "color1", "has_color1", "alpha")}
+    mrec = {k: [] for k in ("kind", "geom", "ngeom", "color0", "color1", "has_color1", "alpha")}
+    toff = coff = moff = ooff = 0
+    for ts in (8, 16, 32, 64):
+        for mode in range(5):
+            for quality in ((50, 75, 85) if mode in (0, 3) else (75,)):
+                for rep in range(2 if ts <= 16 else 1):
+                    tile = _cand_tile(rng, ts, mode)
+                    mag, angle, _ = _sobel_edges(tile)
+                    orient = _dominant_orientations(angle, mag)
+                    cands = generate_edge_driven_candidates(tile, quality)
+                    merged, _ = _worker_gen_cands((tile, quality))
+                    for k, v in (("ts", ts), ("quality", quality), ("mode", mode), ("tile_off", toff),
+                                 ("cand_off", coff), ("merged_off", moff), ("orient_off", ooff)):
+                        d[k].append(v)
+                    tiles.append(tile.ravel())
+                    mags.append(np.asarray(mag, np.float32).ravel())
+                    angles.append(np.asarray(angle, np.float32).ravel())
+                    orients.extend(float(o) for o in orient)
+                    toff += tile.size
+                    coff += len(cands)
+                    moff += len(merged)
+                    ooff += len(orient)
+                    for dst, src in ((rec, cands), (mrec, merged)):
+                        for key, vals in zip(("kind", "geom", "ngeom", "color0", "color1", "has_color1", "alpha"),
+                                             _records(src)):
+                            dst[key].extend(vals)
+    out = {k: np.asarray(v, np.int64) for k, v in d.items()}
+    out["tile_off"] = np.append(out["tile_off"], toff)
+    out["cand_off"] = np.append(out["cand_off"], coff)
+    out["merged_off"] = np.append(out["merged_off"], moff)
+    out["orient_off"] = np.append(out["orient_off"], ooff)
+    out["tiles"] = np.concatenate(tiles).astype(np.float32)
+    out["mag"] = np.concatenate(mags)
+    out["angle"] = np.concatenate(angles)
+    out["orient"] = np.asarray(orients, np.float64)
+    for prefix, src in (("", rec), ("merged_", mrec)):
+        out[prefix + "kind"] = np.asarray(src["kind"], np.int32)
+        out[prefix + "geom"] = np.asarray(src["geom"], np.float64)
+        out[prefix + "ngeom"] = np.asarray(src["ngeom"], np.int32)
+        out[prefix + "color0"] = np.asarray(src["color0"], np.float64)
+        out[prefix + "color1"] = np.asarray(src["color1"], np.float64)
+        out[prefix + "has_color1"] = np.asarray(src["has_color1"], bool)
+        out[prefix + "alpha"] = np.asarray(src["alpha"], np.float64)
+    return out
+
+
 def main() -> None:
     sys.path.insert(0, REF_SRC)
     if len(sys.argv) > 1 and sys.argv[1] == "adaptive":
@@ -704,6 +778,10 @@ def main() -> None:
     if len(sys.argv) > 1 and sys.argv[1] == "scene":
         np.savez_compressed(os.path.join(OUT_DIR, "ref_scene.npz"), **scene_cases())
         print("wrote ref_scene.npz to", OUT_DIR)
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "edge":
+        np.savez_compressed(os.path.join(OUT_DIR, "ref_edge.npz"), **edge_cases())
+        print("wrote ref_edge.npz to", OUT_DIR)
         return
     if len(sys.argv) > 1 and sys.argv[1] == "candidates":
         np.savez_compressed(os.path.join(OUT_DIR, "ref_candidates.npz"), **candidate_cases())
@@ -721,6 +799,7 @@ def main() -> None:
     np.savez_compressed(os.path.join(OUT_DIR, "ref_entropy.npz"), **entropy_cases())
     np.savez_compressed(os.path.join(OUT_DIR, "ref_candidates.npz"), **candidate_cases())
     np.savez_compressed(os.path.join(OUT_DIR, "ref_scene.npz"), **scene_cases())
+    np.savez_compressed(os.path.join(OUT_DIR, "ref_edge.npz"), **edge_cases())
     print("wrote fixtures to", OUT_DIR)
 
 

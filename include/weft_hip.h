@@ -36,6 +36,10 @@
  *                                 for every tile of one size
  *   weft_gen_residual_candidates  encoder._generate_residual_candidates
  *                                 (src/weft/encoder.py:1255-1366) for every tile of one size
+ *   weft_gen_edge_candidates      edge_analysis.generate_edge_driven_candidates
+ *                                 (src/weft/edge_analysis.py:139-315) for every tile of one size
+ *   weft_merge_pools              _dedup_candidates(edge + template), as encoder._fit_tile
+ *                                 (encoder.py:1383-1385) and _worker_gen_cands (2513-2515) form it
  *   weft_argmin_f64, weft_exclusive_scan_i64, weft_compact_flags, weft_segmented_topk
  *                                 cccl_argmin, cccl_exclusive_scan, cccl_compact,
  *                                 cccl_segmented_topk (src/weft/cuda_backend.py:80-181)
@@ -580,6 +584,34 @@ int weft_gen_candidates(const float* d_pixels, const int64_t* d_tile_pix, const 
 int weft_gen_residual_candidates(const float* d_pixels, const float* d_pred, int clip_pred, const int64_t* d_tile_pix,
                                  const int32_t* d_tiles, int n_tiles, int tile_size, WeftPrim* d_out,
                                  double* d_out_f64, int32_t* d_out_count, int max_cands, void* stream);
+
+/* Edge-driven candidate pools (src/weft/edge_analysis.py:139-315,
+ * generate_edge_driven_candidates, with _sobel_edges 17-41, _dominant_orientations 44-68,
+ * _extract_edge_positions 71-80 and _ct 133-136; encoder._dedup_candidates 447-530 over the
+ * result).  Layout and output as for weft_gen_candidates, max_cands >= WEFT_GEN_EDGE_MAX_CANDS.
+ * The Sobel gradients are formed as the reference's slices form them (every tap reads the
+ * luminance one row up and one column left, so interior taps cancel); the float32 angle is
+ * the arctangent of min / max rounded to float32 with float32 quadrant steps and
+ * atan(1) = 0.7853981.  The line and curve sections, reached only when the 60th percentile of
+ * the magnitudes is at least 1e-6, are not built: a tile with fewer than
+ * floor(0.6 (n - 1)) + 2 magnitudes below 9.99e-7 (values far outside [0, 1]) gets the count
+ * -1 and no pool. */
+#define WEFT_GEN_EDGE_MAX_CANDS 114
+int weft_gen_edge_candidates(const float* d_pixels, const int64_t* d_tile_pix, const int32_t* d_tiles, int n_tiles,
+                             int tile_size, int quality, WeftPrim* d_out, double* d_out_f64, int32_t* d_out_count,
+                             int max_cands, void* stream);
+
+/* Merged pools: pool i of d_out is _dedup_candidates(a_i + b_i) (src/weft/encoder.py:447-530
+ * over the concatenation that _fit_tile 1383-1385 and _worker_gen_cands 2513-2515 form from the
+ * edge-driven and the template pool).  Pools as weft_gen_*_candidates write them: a_cap / b_cap
+ * records and 14 doubles each per pool, counts in d_a_count / d_b_count; the keys are formed
+ * from the doubles.  a_cap + b_cap <= WEFT_MERGE_MAX_CANDS <= out_cap is required; a count
+ * outside [0, cap] (an edge pool's -1 among them) gives the merged count -1.  d_out_f64 may be
+ * NULL. */
+#define WEFT_MERGE_MAX_CANDS 512
+int weft_merge_pools(const WeftPrim* d_a, const double* d_a_f64, const int32_t* d_a_count, int a_cap,
+                     const WeftPrim* d_b, const double* d_b_f64, const int32_t* d_b_count, int b_cap, int n_pools,
+                     WeftPrim* d_out, double* d_out_f64, int32_t* d_out_count, int out_cap, void* stream);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 19
+ABI_VERSION = 20
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -134,6 +134,10 @@ def _bind(L: ctypes.CDLL) -> None:
                                       c_int, c_void_p]
     L.weft_gen_residual_candidates.argtypes = [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                                c_void_p, c_void_p, c_int, c_void_p]
+    L.weft_gen_edge_candidates.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                                           c_void_p, c_int, c_void_p]
+    L.weft_merge_pools.argtypes = [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                   c_void_p, c_void_p, c_void_p, c_int, c_void_p]
     L.weft_select_work_bytes.argtypes = [c_int, ctypes.c_int64]
     L.weft_select_work_bytes.restype = ctypes.c_int64
     L.weft_argmin_f64.argtypes = [c_void_p, ctypes.c_int64, c_void_p, c_void_p, ctypes.c_int64, c_void_p]
@@ -189,7 +193,8 @@ def _bind(L: ctypes.CDLL) -> None:
                  "weft_fit_score_split", "weft_rans_encode_chunks", "weft_rans_decode_chunks",
                  "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks", "weft_template_fit",
                  "weft_argmin_f64", "weft_exclusive_scan_i64", "weft_compact_flags", "weft_segmented_topk",
-                 "weft_gen_candidates", "weft_gen_residual_candidates"):
+                 "weft_gen_candidates", "weft_gen_residual_candidates", "weft_gen_edge_candidates",
+                 "weft_merge_pools"):
         getattr(L, name).restype = c_int
 
 
@@ -311,5 +316,6 @@ EXPORTED_SYMBOLS = (
     "weft_refine", "weft_fit_score_split", "weft_rans_stage_bytes", "weft_rans_encode_chunks",
     "weft_rans_decode_chunks", "weft_batch_refine_fit", "weft_score_evals_split", "weft_template_masks",
     "weft_template_fit", "weft_select_work_bytes", "weft_argmin_f64", "weft_exclusive_scan_i64", "weft_compact_flags",
-    "weft_segmented_topk", "weft_gen_candidates", "weft_gen_residual_candidates",
+    "weft_segmented_topk", "weft_gen_candidates", "weft_gen_residual_candidates", "weft_gen_edge_candidates",
+    "weft_merge_pools",
 )

@@ -4,7 +4,7 @@
 
 #include "weft_hip.h"
 
-#define WEFT_ABI_VERSION 19
+#define WEFT_ABI_VERSION 20
 #define WEFT_MAX_BLOCK 1024
 #define WEFT_N_CLASSES 6
 #define WEFT_TABLE_N 4096
@@ -245,6 +245,40 @@ struct WeftCandGenArgs {
     double sin_ang[16];
     double cos6[6];
     double sin6[6];
+};
+
+struct WeftEdgeGenArgs {
+    const float* pixels;
+    const int64_t* tile_pix;
+    const int32_t* tiles;
+    WeftPrim* out;
+    double* out_f64;
+    int32_t* out_count;
+    int32_t n_tiles;
+    int32_t tile_size;
+    int32_t quality;
+    int32_t max_cands;
+    int32_t min_small;  // floor(0.6 (n - 1)) + 2 magnitudes below 9.99e-7: no edge points
+    int32_t _pad;
+    double edges[13];   // np.linspace(0, pi, 13)
+    double cos_o[28];   // 12 bin midpoints, 0, pi/4, pi/2, 3pi/4, 12 bin starts + pi/24
+    double sin_o[28];
+};
+
+struct WeftMergePoolsArgs {
+    const WeftPrim* a;
+    const double* a_f64;
+    const int32_t* a_count;
+    const WeftPrim* b;
+    const double* b_f64;
+    const int32_t* b_count;
+    WeftPrim* out;
+    double* out_f64;
+    int32_t* out_count;
+    int32_t a_cap;
+    int32_t b_cap;
+    int32_t out_cap;
+    int32_t n_pools;
 };
 
 struct WeftResGenArgs {

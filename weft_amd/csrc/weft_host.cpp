@@ -58,6 +58,8 @@ struct Module {
     hipFunction_t f_sel_rs_emit = nullptr;
     hipFunction_t f_gen_cands = nullptr;
     hipFunction_t f_gen_res = nullptr;
+    hipFunction_t f_gen_edge = nullptr;
+    hipFunction_t f_merge_pools = nullptr;
     hipFunction_t f_rans_enc = nullptr;
     hipFunction_t f_rans_scan = nullptr;
     hipFunction_t f_rans_pack = nullptr;
@@ -243,6 +245,8 @@ int weft_init(int device) {
         {&g_mod.f_sel_rs_emit, "weft_sel_rs_emit_kernel"},
         {&g_mod.f_gen_cands, "weft_gen_candidates_kernel"},
         {&g_mod.f_gen_res, "weft_gen_residual_candidates_kernel"},
+        {&g_mod.f_gen_edge, "weft_gen_edge_candidates_kernel"},
+        {&g_mod.f_merge_pools, "weft_merge_pools_kernel"},
         {&g_mod.f_rans_enc, "weft_rans_encode_kernel"},
         {&g_mod.f_rans_scan, "weft_rans_scan_kernel"},
         {&g_mod.f_rans_pack, "weft_rans_pack_kernel"},
@@ -1034,6 +1038,70 @@ int weft_gen_residual_candidates(const float* d_pixels, const float* d_pred, int
                      n_tiles, tile_size, clip_pred ? 1 : 0, max_cands};
     const unsigned lds = (unsigned)(28 * tile_size * tile_size);
     return launch(g_mod.f_gen_res, (unsigned)n_tiles, 256, lds, stream, &a);
+}
+
+int weft_gen_edge_candidates(const float* d_pixels, const int64_t* d_tile_pix, const int32_t* d_tiles, int n_tiles,
+                             int tile_size, int quality, WeftPrim* d_out, double* d_out_f64, int32_t* d_out_count,
+                             int max_cands, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (tile_size != 8 && tile_size != 16 && tile_size != 32 && tile_size != 64)
+        return set_err(WEFT_E_UNSUPPORTED_TILE, "weft_gen_edge_candidates: tile size must be 8, 16, 32 or 64");
+    if (n_tiles < 0) return set_err(WEFT_E_INVALID, "weft_gen_edge_candidates: n_tiles < 0");
+    if (max_cands < WEFT_GEN_EDGE_MAX_CANDS)
+        return set_err(WEFT_E_INVALID, "weft_gen_edge_candidates: max_cands below WEFT_GEN_EDGE_MAX_CANDS");
+    if (n_tiles == 0) return WEFT_OK;
+    if (!d_pixels || !d_tile_pix || !d_tiles || !d_out || !d_out_count)
+        return set_err(WEFT_E_INVALID, "weft_gen_edge_candidates: null pointer");
+    WeftEdgeGenArgs a{};
+    a.pixels = d_pixels;
+    a.tile_pix = d_tile_pix;
+    a.tiles = d_tiles;
+    a.out = d_out;
+    a.out_f64 = d_out_f64;
+    a.out_count = d_out_count;
+    a.n_tiles = n_tiles;
+    a.tile_size = tile_size;
+    a.quality = quality;
+    a.max_cands = max_cands;
+    const int n = tile_size * tile_size;
+    a.min_small = (int)std::floor((double)(n - 1) * 0.6) + 2;
+    // np.linspace(0, pi, 13): i * (pi / 12), the last edge pi itself; the orientations as the
+    // reference's Python floats, their cos / sin from the host's libm (math.cos / math.sin)
+    const double pi = 3.141592653589793;
+    const double step = pi / 12.0;
+    for (int i = 0; i < 12; ++i) a.edges[i] = (double)i * step;
+    a.edges[12] = pi;
+    double theta[28];
+    for (int i = 0; i < 12; ++i) theta[i] = (a.edges[i] + a.edges[i + 1]) / 2.0;
+    theta[12] = 0.0;
+    theta[13] = pi / 4.0;
+    theta[14] = pi / 2.0;
+    theta[15] = 3.0 * pi / 4.0;
+    for (int i = 0; i < 12; ++i) theta[16 + i] = a.edges[i] + pi / 24.0;
+    for (int i = 0; i < 28; ++i) {
+        a.cos_o[i] = std::cos(theta[i]);
+        a.sin_o[i] = std::sin(theta[i]);
+    }
+    const unsigned lds = (unsigned)(33 * n);
+    return launch(g_mod.f_gen_edge, (unsigned)n_tiles, 256, lds, stream, &a);
+}
+
+int weft_merge_pools(const WeftPrim* d_a, const double* d_a_f64, const int32_t* d_a_count, int a_cap,
+                     const WeftPrim* d_b, const double* d_b_f64, const int32_t* d_b_count, int b_cap, int n_pools,
+                     WeftPrim* d_out, double* d_out_f64, int32_t* d_out_count, int out_cap, void* stream) {
+    int rc = require_init();
+    if (rc) return rc;
+    if (n_pools < 0 || a_cap < 0 || b_cap < 0) return set_err(WEFT_E_INVALID, "weft_merge_pools: negative size");
+    if ((int64_t)a_cap + b_cap > WEFT_MERGE_MAX_CANDS)
+        return set_err(WEFT_E_INVALID, "weft_merge_pools: a_cap + b_cap above WEFT_MERGE_MAX_CANDS");
+    if (out_cap < a_cap + b_cap) return set_err(WEFT_E_INVALID, "weft_merge_pools: out_cap below a_cap + b_cap");
+    if (n_pools == 0) return WEFT_OK;
+    if (!d_a || !d_a_f64 || !d_a_count || !d_b || !d_b_f64 || !d_b_count || !d_out || !d_out_count)
+        return set_err(WEFT_E_INVALID, "weft_merge_pools: null pointer");
+    WeftMergePoolsArgs m{d_a, d_a_f64, d_a_count, d_b, d_b_f64, d_b_count, d_out, d_out_f64, d_out_count,
+                         a_cap, b_cap, out_cap, n_pools};
+    return launch(g_mod.f_merge_pools, (unsigned)n_pools, 256, 0, stream, &m);
 }
 
 }  // extern "C"

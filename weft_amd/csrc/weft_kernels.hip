@@ -1354,6 +1354,7 @@ __device__ void batch_fit_body(const WeftBatchArgs& A) {
 #include "weft_template.h"
 #include "weft_select.h"
 #include "weft_candgen.h"
+#include "weft_edgegen.h"
 
 }  // namespace
 
@@ -1837,3 +1838,6 @@ extern "C" __global__ __launch_bounds__(256) void weft_gen_candidates_kernel(Wef
 extern "C" __global__ __launch_bounds__(256) void weft_gen_residual_candidates_kernel(WeftResGenArgs A) {
     cr_body(A);
 }
+// Edge-driven candidate pools, one workgroup per tile of one size.
+extern "C" __global__ __launch_bounds__(256) void weft_gen_edge_candidates_kernel(WeftEdgeGenArgs A) { ed_body(A); }
+extern "C" __global__ __launch_bounds__(256) void weft_merge_pools_kernel(WeftMergePoolsArgs A) { mg_body(A); }

@@ -505,7 +505,52 @@ def candidates_report(tiles, quality=75, sample=48):
                cpu_numpy_tiles=len(pick), speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / dev_s,
                sample_equal_restatement=f"{same}/{len(pick)}")
     rep["residual"] = residual_candidates_report(tiles, d_px, d_off, pick, quality)
+    rep["edge"] = edge_candidates_report(tiles, d_px, d_off, pick, quality)
     return rep
+
+
+def edge_candidates_report(tiles, d_px, d_off, pick, quality=75):
+    """Edge-driven pools (edge_analysis.generate_edge_driven_candidates) and the merged pools
+    of encoder._worker_gen_cands (edge-driven + template, de-duplicated) for every tile."""
+    import torch
+    from oracle import candgen_np, edge_np
+    from weft_amd.candidates import EDGE_MAX_CANDS, MERGED_MAX_CANDS, generate_edge_pools, generate_merged_pools
+    from weft_amd.records import PRIM_DTYPE
+
+    out = {}
+    for name, fn, cap, ref_fn in (("edge", generate_edge_pools, EDGE_MAX_CANDS, edge_np.generate),
+                                  ("merged", generate_merged_pools, MERGED_MAX_CANDS, edge_np.merged)):
+        fn(tiles, quality, d_pixels=d_px, d_tile_pix=d_off)  # warm, and no tile with edge points
+        stream = torch.cuda.current_stream()
+        times = []
+        for _ in range(3):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            recs, counts, _ = fn(tiles, quality, d_pixels=d_px, d_tile_pix=d_off, check=False)
+            b.record(stream)
+            torch.cuda.synchronize()
+            times.append(a.elapsed_time(b) * 1e-3)
+        dev_s = float(np.median(times))
+        cnt = counts.cpu().numpy()
+        host = recs.cpu().numpy().view(PRIM_DTYPE).reshape(-1, cap)
+        t0 = time.perf_counter()
+        same = 0
+        for t in pick:
+            s = int(tiles.ts[t])
+            o = 3 * int(tiles.pix_off[t])
+            ref = ref_fn(tiles.target[o:o + 3 * s * s].reshape(s, s, 3), quality)
+            same += int(cnt[t] == len(ref) and host[t, :cnt[t]].tobytes()
+                        == candgen_np.to_records(ref, PRIM_DTYPE).tobytes())
+        cpu_per_tile = (time.perf_counter() - t0) / len(pick)
+        out[name] = dict(api=fn.__name__, device_s=dev_s, tiles_per_s=tiles.n_tiles / dev_s,
+                         mean_pool=float(cnt.mean()), cpu_numpy_s_per_tile=cpu_per_tile, cpu_numpy_tiles=len(pick),
+                         speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / dev_s,
+                         sample_equal_restatement=f"{same}/{len(pick)}")
+    out["edge"]["kernel"] = "weft_gen_edge_candidates_kernel"
+    out["merged"]["kernels"] = ["weft_gen_edge_candidates_kernel", "weft_gen_candidates_kernel",
+                                "weft_merge_pools_kernel"]
+    return out
 
 
 def residual_candidates_report(tiles, d_px, d_off, pick, quality=75):

@@ -123,9 +123,11 @@ def _raise_on_edge_points(counts, n_tiles: int) -> None:
             "curves to them, which the edge-driven generator does not build")
 
 
-def generate_edge_pools(tiles, quality: int = 75, want_f64: bool = False, d_pixels=None, d_tile_pix=None):
+def generate_edge_pools(tiles, quality: int = 75, want_f64: bool = False, d_pixels=None, d_tile_pix=None,
+                        check: bool = True):
     """Edge-driven pools of every tile: (records uint8 [T * EDGE_MAX_CANDS * 60], counts int32
-    [T], f64 or None).  Synchronises to check that no tile reached the edge-point sections."""
+    [T], f64 or None).  With ``check`` it synchronises and raises NotImplementedError if a
+    tile reached the edge-point sections; without, such tiles have the count -1."""
     q = int(quality)
 
     def launch(L, px, off, idx, n, s, out, f64, cnt, st):
@@ -133,14 +135,17 @@ def generate_edge_pools(tiles, quality: int = 75, want_f64: bool = False, d_pixe
 
     out, counts, f64 = _class_pools(tiles, EDGE_MAX_CANDS, want_f64, d_pixels, d_tile_pix, launch,
                                     "weft_gen_edge_candidates")
-    _raise_on_edge_points(counts, len(np.asarray(tiles.ts)))
+    if check:
+        _raise_on_edge_points(counts, len(np.asarray(tiles.ts)))
     return out, counts, f64
 
 
-def generate_merged_pools(tiles, quality: int = 75, want_f64: bool = False, d_pixels=None, d_tile_pix=None):
+def generate_merged_pools(tiles, quality: int = 75, want_f64: bool = False, d_pixels=None, d_tile_pix=None,
+                          check: bool = True):
     """The pools ``_worker_gen_cands`` forms for every tile (edge-driven then template
     candidates, de-duplicated): (records uint8 [T * MERGED_MAX_CANDS * 60], counts int32 [T],
-    f64 or None)."""
+    f64 or None).  ``check`` as for ``generate_edge_pools`` (unchecked, such tiles have the
+    merged count -1)."""
     import torch
 
     dev = torch.device("cuda", 0)
@@ -148,7 +153,7 @@ def generate_merged_pools(tiles, quality: int = 75, want_f64: bool = False, d_pi
         d_pixels = torch.from_numpy(np.ascontiguousarray(tiles.target, dtype=np.float32)).to(dev)
     if d_tile_pix is None:
         d_tile_pix = torch.from_numpy(np.asarray(tiles.pix_off, np.int64)).to(dev)
-    ea, ec, ef = generate_edge_pools(tiles, quality, True, d_pixels, d_tile_pix)
+    ea, ec, ef = generate_edge_pools(tiles, quality, True, d_pixels, d_tile_pix, check)
     ta, tc, tf = generate_pools(tiles, quality, True, d_pixels, d_tile_pix)
     T = max(len(np.asarray(tiles.ts)), 1)
     out = torch.empty(T * MERGED_MAX_CANDS * PRIM_DTYPE.itemsize, dtype=torch.uint8, device=dev)

@@ -28,9 +28,9 @@ struct EdShared {
     double line_geom[ED_BINS][4];
     double line_col[ED_BINS][2][3];
     double pal[10][3];
-    int n_orient, n_pal, n_total, n_kept, small_cnt;
-    int km_cnt[3], km_imin[3], km_imax[3];
-    double km_mx[3], km_my[3];
+    int n_orient, n_pal, n_kept, small_cnt;
+    // 3-means regions: pixel count, x and y sums, min of x n + p, max of x n + (n - 1 - p)
+    int km_cnt[3], km_sx[3], km_sy[3], km_lo[3], km_hi[3];
 };
 
 // NumPy's float32 pairwise sum of a[0 .. n) as np.add.reduce forms it: 0 + pw(a, n), where
@@ -130,15 +130,69 @@ __device__ __forceinline__ float ed_angle(float gx, float gy) {
     return mod;
 }
 
+// s + a[0] + a[1] + ... + a[n - 1], left to right in float32 (loads issued ahead of the adds).
+__device__ __forceinline__ float ed_seq_add(float s, const float* a, int n) {
+    int i = 0;
+    for (; i + 8 <= n; i += 8) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = a[i + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s += v[j];
+    }
+    for (; i < n; ++i) s += a[i];
+    return s;
+}
+
+__device__ __forceinline__ int ed_wave_sum(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ int ed_wave_min(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int ed_wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// The dominant orientations from the histogram (table indices into cos_o / sin_o); returns
+// their number.  np.max and np.argmax: NaN propagates and is the first maximum.
+__device__ int ed_orientations(const float* hist, int* orient) {
+    float hmax = hist[0];
+    int amax = 0;
+    for (int i = 1; i < ED_BINS && hmax == hmax; ++i)
+        if (hist[i] > hmax || hist[i] != hist[i]) {
+            hmax = hist[i];
+            amax = i;
+        }
+    int m = 0;
+    if (hmax < 1e-6f) {  // no edges: 0, pi/4, pi/2, 3pi/4 (table entries 12 .. 15)
+        for (int i = 0; i < 4; ++i) orient[m++] = ED_BINS + i;
+    } else {
+        const float thr = hmax * 0.2f;
+        for (int i = 0; i < ED_BINS; ++i)
+            if (hist[i] >= thr) orient[m++] = i;
+        if (m == 0) orient[m++] = ED_BINS + 4 + amax;  // edges[argmax] + pi / 24 (entries 16 .. 27)
+    }
+    return m;
+}
+
+// Work is split so that each independent in-order sum runs on lanes of its own wavefront
+// (lanes of one wavefront running different loops would take turns).
 __device__ void ed_body(const WeftEdgeGenArgs& A) {
     __shared__ EdShared S;
     extern __shared__ float4 weft_smem[];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int ts = A.tile_size, n = ts * ts;
     const int slot = blockIdx.x;
     const int tile = A.tiles[slot];
     const double hf = (double)(ts - 1), wf = (double)(ts - 1);
-    float* R = reinterpret_cast<float*>(weft_smem);
+    float* R = reinterpret_cast<float*>(weft_smem);              // R, G, B planes
     float* LUM = R + 3 * n;
     float* MAG = LUM + n;
     float* MS = MAG + n;                                         // each bin's magnitudes, in order
@@ -156,9 +210,14 @@ __device__ void ed_body(const WeftEdgeGenArgs& A) {
         LUM[p] = (0.2126f * r + 0.7152f * g) + 0.0722f * b;
     }
     if (tid == 0) S.small_cnt = 0;
+    if (tid < 3) {
+        S.km_cnt[tid] = S.km_sx[tid] = S.km_sy[tid] = 0;
+        S.km_lo[tid] = 2147483647;
+        S.km_hi[tid] = -1;
+    }
     __syncthreads();
-    // gradients, magnitude, histogram bin; magnitudes below 9.99e-7 counted for the edge-point
-    // test (a float32 interpolation of two of them stays below the reference's 1e-6)
+    // gradients, magnitude, histogram bin, sort key; magnitudes below 9.99e-7 counted for the
+    // edge-point test (a float32 interpolation of two of them stays below the reference's 1e-6)
     int small = 0;
     for (int p = tid; p < n; p += CG_THREADS) {
         const int y = p / ts, x = p - y * ts;
@@ -174,34 +233,60 @@ __device__ void ed_body(const WeftEdgeGenArgs& A) {
         BIN[p] = (uint8_t)bin;
         SORT[p] = ((uint64_t)cg_fkey(LUM[p]) << 32) | (uint32_t)p;
     }
-    atomicAdd(&S.small_cnt, small);
+    small = ed_wave_sum(small);
+    if (lane == 0) atomicAdd(&S.small_cnt, small);
     __syncthreads();
-    // bin sizes; tile and quadrant means (row by row)
-    if (tid < ED_BINS) {
-        int c = 0;
-        for (int p = 0; p < n; ++p) c += BIN[p] == tid;
-        S.bin_cnt[tid] = c;
-    } else if (tid >= 16 && tid < 19) {
-        const int c = tid - 16;
-        S.mean_all[c] = cg_block_sum(R + c * n, ts, 0, ts, 0, ts) / (float)n;
-    } else if (tid >= 32 && tid < 44) {
-        const int q = (tid - 32) / 3, c = (tid - 32) % 3, qh = ts / 2;
-        const int x0 = (q & 1) * qh, y0 = (q >> 1) * qh;
-        S.quad[q][c] = cg_block_sum(R + c * n, ts, x0, x0 + qh, y0, y0 + qh) / (float)(qh * qh);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int o = 0;
-        for (int i = 0; i < ED_BINS; ++i) {
-            S.bin_off[i] = o;
-            o += S.bin_cnt[i];
+    if (wave == 0) {
+        // each bin's magnitudes gathered in pixel order: counts by ballots, then each lane's
+        // slot = its bin's running offset + the earlier lanes of that bin (n is a multiple of 64)
+        int cnt[ED_BINS];
+#pragma unroll
+        for (int b = 0; b < ED_BINS; ++b) cnt[b] = 0;
+        for (int p0 = 0; p0 < n; p0 += 64) {
+            const int bin = BIN[p0 + lane];
+#pragma unroll
+            for (int b = 0; b < ED_BINS; ++b) cnt[b] += __popcll(__ballot(bin == b));
         }
-    }
-    __syncthreads();
-    if (tid < ED_BINS) {  // each bin's magnitudes in pixel order
-        int o = S.bin_off[tid];
-        for (int p = 0; p < n; ++p)
-            if (BIN[p] == tid) MS[o++] = MAG[p];
+        int off[ED_BINS];
+        int run = 0;
+#pragma unroll
+        for (int b = 0; b < ED_BINS; ++b) {
+            off[b] = run;
+            run += cnt[b];
+        }
+        if (lane < ED_BINS) {
+#pragma unroll
+            for (int b = 0; b < ED_BINS; ++b)
+                if (lane == b) {
+                    S.bin_cnt[b] = cnt[b];
+                    S.bin_off[b] = off[b];
+                }
+        }
+        for (int p0 = 0; p0 < n; p0 += 64) {
+            const int p = p0 + lane;
+            const int bin = BIN[p];
+            const float mag = MAG[p];
+#pragma unroll
+            for (int b = 0; b < ED_BINS; ++b) {
+                const uint64_t m = __ballot(bin == b);
+                if (bin == b) {
+                    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                    MS[off[b] + below] = mag;
+                }
+                off[b] += __popcll(m);
+            }
+        }
+    } else if (wave == 1) {
+        if (lane < 3) S.mean_all[lane] = ed_seq_add(0.0f, R + lane * n, n) / (float)n;  // row by row = in order
+    } else if (wave == 2) {
+        if (lane < 12) {
+            const int q = lane / 3, c = lane % 3, qh = ts / 2;
+            const int x0 = (q & 1) * qh, y0 = (q >> 1) * qh;
+            float s = 0.0f;
+            for (int y = y0; y < y0 + qh; ++y) s = ed_seq_add(s, R + c * n + y * ts + x0, qh);
+            S.quad[q][c] = s / (float)(qh * qh);
+        }
     }
     __syncthreads();
     if (tid < ED_BINS) S.hist[tid] = ed_pairwise(MS + S.bin_off[tid], S.bin_cnt[tid]);
@@ -220,44 +305,127 @@ __device__ void ed_body(const WeftEdgeGenArgs& A) {
             }
         }
     __syncthreads();
-    // orientations (thread 0); k-means groups (one thread per group and channel)
-    if (tid == 0) {
-        // np.max and np.argmax: NaN propagates and is the first maximum
-        float hmax = S.hist[0];
-        int amax = 0;
-        for (int i = 1; i < ED_BINS && hmax == hmax; ++i)
-            if (S.hist[i] > hmax || S.hist[i] != S.hist[i]) {
-                hmax = S.hist[i];
-                amax = i;
+    if (wave == 0) {
+        if (lane == 0) S.n_orient = ed_orientations(S.hist, S.orient);
+    } else if (wave == 1) {
+        // k-means groups: equal-count runs of the luminance-sorted pixels, summed in order
+        if (lane < 27) {
+            const int g = lane / 3, c = lane % 3;
+            const int k = g < 2 ? 2 : (g < 5 ? 3 : 4);
+            const int jj = g < 2 ? g : (g < 5 ? g - 2 : g - 5);
+            const double step = (double)n / (double)k;
+            const int a = (int)((double)jj * step), b = (int)((double)(jj + 1) * step);
+            const float* plane = R + c * n;
+            float v;
+            if (b <= a) {
+                v = plane[(int)(SORT[a < n - 1 ? a : n - 1] & 0xffffffffu)];
+            } else {
+                float s = 0.0f;
+                int i = a;
+                for (; i + 8 <= b; i += 8) {
+                    float w[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) w[q] = plane[(int)(SORT[i + q] & 0xffffffffu)];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) s += w[q];
+                }
+                for (; i < b; ++i) s += plane[(int)(SORT[i] & 0xffffffffu)];
+                v = s / (float)(b - a);
             }
-        int m = 0;
-        if (hmax < 1e-6f) {  // no edges: 0, pi/4, pi/2, 3pi/4 (table entries 12 .. 15)
-            for (int i = 0; i < 4; ++i) S.orient[m++] = ED_BINS + i;
-        } else {
-            const float thr = hmax * 0.2f;
-            for (int i = 0; i < ED_BINS; ++i)
-                if (S.hist[i] >= thr) S.orient[m++] = i;
-            if (m == 0) S.orient[m++] = ED_BINS + 4 + amax;  // edges[argmax] + pi / 24 (entries 16 .. 27)
+            S.km[g][c] = v;
         }
-        S.n_orient = m;
-    } else if (tid >= 16 && tid < 16 + 27) {
-        const int g = (tid - 16) / 3, c = (tid - 16) % 3;
-        const int k = g < 2 ? 2 : (g < 5 ? 3 : 4);
-        const int jj = g < 2 ? g : (g < 5 ? g - 2 : g - 5);
-        const double step = (double)n / (double)k;
-        const int a = (int)((double)jj * step), b = (int)((double)(jj + 1) * step);
-        float v;
-        if (b <= a) {
-            v = R[c * n + (int)(SORT[a < n - 1 ? a : n - 1] & 0xffffffffu)];
-        } else {
-            float s = 0.0f;
-            for (int i = a; i < b; ++i) s += R[c * n + (int)(SORT[i] & 0xffffffffu)];
-            v = s / (float)(b - a);
+    } else if (wave == 2) {
+        // two-colour split of every orientation's line: float32 side test, sums in pixel order
+        int orient[ED_BINS];
+        const int n_or = ed_orientations(S.hist, orient);
+        const int l = lane / 3, c = lane % 3;
+        if (l < n_or) {
+            const int o = orient[l];
+            const double vx = A.cos_o[o], vy = A.sin_o[o];
+            const double nx = -vy, ny = vx;
+            const double cx = wf * 0.5, cy = hf * 0.5;
+            const double lx0 = cx - nx * wf * 0.6, ly0 = cy - ny * hf * 0.6;
+            const double lx1 = cx + nx * wf * 0.6, ly1 = cy + ny * hf * 0.6;
+            if (c == 0) {
+                S.line_geom[l][0] = lx0;
+                S.line_geom[l][1] = ly0;
+                S.line_geom[l][2] = lx1;
+                S.line_geom[l][3] = ly1;
+            }
+            const float x0 = (float)lx0, y0 = (float)ly0;
+            const float dyl = (float)(ly1 - ly0), dxl = (float)(lx1 - lx0);
+            const float* plane = R + c * n;
+            float sa = 0.0f, sb = 0.0f;
+            int na = 0;
+            for (int y = 0; y < ts; ++y) {
+                const float ty = ((float)y - y0) * dxl;
+                const float* row = plane + y * ts;
+                for (int x = 0; x < ts; x += 8) {
+                    float v[8];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) v[j] = row[x + j];
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float side = ((float)(x + j) - x0) * dyl - ty;
+                        if (side >= 0.0f) {
+                            sa += v[j];
+                            ++na;
+                        } else {
+                            sb += v[j];
+                        }
+                    }
+                }
+            }
+            const int nb = n - na;
+            const double avg = cg_ct((double)S.mean_all[c]);
+            S.line_col[l][0][c] = na ? cg_ct((double)(sa / (float)na)) : avg;
+            S.line_col[l][1][c] = nb ? cg_ct((double)(sb / (float)nb)) : avg;
         }
-        S.km[g][c] = v;
     }
     __syncthreads();
-    // palette (thread 0); linear-patch geometry (one thread per orientation)
+    // 3-means labels (first of the smallest float32 distances) and their regions' integer
+    // statistics, reduced per wavefront; the palette (one thread)
+    if (A.quality >= 60) {
+        int cnt[3] = {0, 0, 0}, sx[3] = {0, 0, 0}, sy[3] = {0, 0, 0};
+        int lo[3] = {2147483647, 2147483647, 2147483647}, hi[3] = {-1, -1, -1};
+        for (int p = tid; p < n; p += CG_THREADS) {
+            int best = 0;
+            float bd = 0.0f;
+            for (int k = 0; k < 3; ++k) {
+                float d = 0.0f;
+                for (int c = 0; c < 3; ++c) {
+                    const float e = R[c * n + p] - S.km[2 + k][c];
+                    d = d + e * e;
+                }
+                if (k == 0 || d < bd) {
+                    bd = d;
+                    best = k;
+                }
+            }
+            const int y = p / ts, x = p - y * ts;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (best == k) {
+                    ++cnt[k];
+                    sx[k] += x;
+                    sy[k] += y;
+                    lo[k] = min(lo[k], x * n + p);
+                    hi[k] = max(hi[k], x * n + (n - 1 - p));
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int c = ed_wave_sum(cnt[k]), a = ed_wave_sum(sx[k]), b = ed_wave_sum(sy[k]);
+            const int l = ed_wave_min(lo[k]), h = ed_wave_max(hi[k]);
+            if (lane == 0 && c) {
+                atomicAdd(&S.km_cnt[k], c);
+                atomicAdd(&S.km_sx[k], a);
+                atomicAdd(&S.km_sy[k], b);
+                atomicMin(&S.km_lo[k], l);
+                atomicMax(&S.km_hi[k], h);
+            }
+        }
+    }
     if (tid == 0) {
         auto near = [&](const double* c) {
             for (int i = 0; i < S.n_pal; ++i) {
@@ -281,113 +449,32 @@ __device__ void ed_body(const WeftEdgeGenArgs& A) {
                 ++S.n_pal;
             }
         }
-    } else if (tid >= 32 && tid < 32 + S.n_orient) {
-        const int l = tid - 32, o = S.orient[l];
-        const double vx = A.cos_o[o], vy = A.sin_o[o];
-        const double nx = -vy, ny = vx;
-        const double cx = wf * 0.5, cy = hf * 0.5;
-        S.line_geom[l][0] = cx - nx * wf * 0.6;
-        S.line_geom[l][1] = cy - ny * hf * 0.6;
-        S.line_geom[l][2] = cx + nx * wf * 0.6;
-        S.line_geom[l][3] = cy + ny * hf * 0.6;
-    }
-    __syncthreads();
-    // side means of every orientation's line (float32 side test, sums row by row); the
-    // 3-means labels and their region statistics
-    if (tid < 3 * S.n_orient) {
-        const int l = tid / 3, c = tid % 3;
-        const double* lg = S.line_geom[l];
-        const float x0 = (float)lg[0], y0 = (float)lg[1];
-        const float dyl = (float)(lg[3] - lg[1]), dxl = (float)(lg[2] - lg[0]);
-        float sa = 0.0f, sb = 0.0f;
-        int na = 0;
-        for (int p = 0; p < n; ++p) {
-            const int y = p / ts, x = p - y * ts;
-            const float side = ((float)x - x0) * dyl - ((float)y - y0) * dxl;
-            if (side >= 0.0f) {
-                sa += R[c * n + p];
-                ++na;
-            } else {
-                sb += R[c * n + p];
-            }
-        }
-        const int nb = n - na;
-        S.line_col[l][0][c] = na ? cg_ct((double)(sa / (float)na)) : S.pal[0][c];
-        S.line_col[l][1][c] = nb ? cg_ct((double)(sb / (float)nb)) : S.pal[0][c];
-    }
-    int* LB = reinterpret_cast<int*>(MS);  // the labels overwrite the bins' magnitudes
-    if (A.quality >= 60) {
-        for (int p = tid; p < n; p += CG_THREADS) {
-            int best = 0;
-            float bd = 0.0f;
-            for (int k = 0; k < 3; ++k) {
-                float d = 0.0f;
-                for (int c = 0; c < 3; ++c) {
-                    const float e = R[c * n + p] - S.km[2 + k][c];
-                    d = d + e * e;
-                }
-                if (k == 0 || d < bd) {
-                    bd = d;
-                    best = k;
-                }
-            }
-            LB[p] = best;
-        }
-    }
-    __syncthreads();
-    if (A.quality >= 60 && tid < 3) {
-        int cnt = 0, imin = -1, imax = -1, xmin = 0, xmax = 0;
-        int64_t sx = 0, sy = 0;
-        for (int p = 0; p < n; ++p)
-            if (LB[p] == tid) {
-                const int y = p / ts, x = p - y * ts;
-                if (cnt == 0 || x < xmin) {
-                    xmin = x;
-                    imin = p;
-                }
-                if (cnt == 0 || x > xmax) {
-                    xmax = x;
-                    imax = p;
-                }
-                sx += x;
-                sy += y;
-                ++cnt;
-            }
-        S.km_cnt[tid] = cnt;
-        S.km_imin[tid] = imin;
-        S.km_imax[tid] = imax;
-        S.km_mx[tid] = cnt ? (double)sx / (double)cnt : 0.0;
-        S.km_my[tid] = cnt ? (double)sy / (double)cnt : 0.0;
-    }
-    if (tid == 0 && A.quality < 60)
-        for (int k = 0; k < 3; ++k) S.km_cnt[k] = 0;
-    __syncthreads();
-    if (tid == 0) {
-        int m = 0;
-        for (int k = 0; k < 3; ++k) m += S.km_cnt[k] >= 3 ? 2 : 0;
-        S.n_total = 4 * S.n_pal + 8 + 5 * S.n_orient + m;
     }
     __syncthreads();
 
     // the pool, in the reference's order
-    const int n_total = S.n_total;
+    int n_tri = 0;
+    if (A.quality >= 60)
+        for (int k = 0; k < 3; ++k) n_tri += S.km_cnt[k] >= 3 ? 2 : 0;
+    const int n_pal = S.n_pal, n_orient = S.n_orient;
+    const int n_total = 4 * n_pal + 8 + 5 * n_orient + n_tri;
     for (int j = tid; j < n_total; j += CG_THREADS) {
         double g[7] = {0, 0, 0, 0, 0, 0, 0};
         double c0[3];
         int r = j;
-        if (r < 4 * S.n_pal) {  // constant patches of the palette
+        if (r < 4 * n_pal) {  // constant patches of the palette
             const double al[4] = {1.0, 0.7, 0.4, 0.2};
             cg_emit(S.keys, out, f64, j, 0, g, 0, S.pal[r / 4], nullptr, al[r % 4]);
             continue;
         }
-        r -= 4 * S.n_pal;
+        r -= 4 * n_pal;
         if (r < 8) {  // quadrant means
             for (int c = 0; c < 3; ++c) c0[c] = cg_ct((double)S.quad[r / 2][c]);
             cg_emit(S.keys, out, f64, j, 0, g, 0, c0, nullptr, (r & 1) ? 0.3 : 0.5);
             continue;
         }
         r -= 8;
-        if (r < 5 * S.n_orient) {  // linear patches across each orientation
+        if (r < 5 * n_orient) {  // linear patches across each orientation
             const int l = r / 5, q = r % 5;
             const double* lg = S.line_geom[l];
             double a = 0.5;
@@ -409,22 +496,23 @@ __device__ void ed_body(const WeftEdgeGenArgs& A) {
             cg_emit(S.keys, out, f64, j, 1, g, 4, S.line_col[l][0], S.line_col[l][1], a);
             continue;
         }
-        r -= 5 * S.n_orient;
-        {  // triangles over the 3-means regions
+        r -= 5 * n_orient;
+        {  // triangles over the 3-means regions of 3 pixels or more: leftmost, rightmost, centroid
             int lab = 0, q = r;
             while (lab < 3 && (S.km_cnt[lab] < 3 || q >= 2)) {
                 if (S.km_cnt[lab] >= 3) q -= 2;
                 ++lab;
             }
-            const int a = S.km_imin[lab], b = S.km_imax[lab];
+            const int a = S.km_lo[lab] % n, b = (n - 1) - S.km_hi[lab] % n;
+            const double cnt = (double)S.km_cnt[lab];
             auto clampx = [&](double v) { return v > wf ? wf : (v < 0.0 ? 0.0 : v); };
             auto clampy = [&](double v) { return v > hf ? hf : (v < 0.0 ? 0.0 : v); };
             g[0] = clampx((double)(a % ts));
             g[1] = clampy((double)(a / ts));
             g[2] = clampx((double)(b % ts));
             g[3] = clampy((double)(b / ts));
-            g[4] = clampx(S.km_mx[lab]);
-            g[5] = clampy(S.km_my[lab]);
+            g[4] = clampx((double)S.km_sx[lab] / cnt);
+            g[5] = clampy((double)S.km_sy[lab] / cnt);
             for (int c = 0; c < 3; ++c) c0[c] = cg_ct((double)S.km[2 + lab][c]);
             cg_emit(S.keys, out, f64, j, 4, g, 6, c0, nullptr, q ? 0.3 : 0.6);
         }

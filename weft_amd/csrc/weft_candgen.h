@@ -73,17 +73,18 @@ __device__ float cg_percentile(const uint32_t* keys, int n, float q) {
 struct CgShared {
     int keys[CG_MAXC][CG_KEY];
     int keep[CG_MAXC];
+    uint32_t hash[CG_MAXC];
     float line_col[CG_MAX_LINES][2][3];
     double line_geom[CG_MAX_LINES][4];
     float mean_all[3], quad[4][3], sub[16][3], km[9][3], ext[3][3];
     int ext_n[3];
-    float avg_mag, gmx, gmy, p25, p75, gthr;
+    float avg_mag, gmx, gmy;
     double pal[13][3];
-    int n_pal, grad_lines, n_tri_x, n_km_tri, n_total, n_kept;
+    int n_pal, grad_lines, n_tri_x, n_kept;
     double base_alpha, gline[4];
     int pk[6], vl[6];
-    int km_cnt[3], km_imin[3], km_imax[3];
-    double km_mx[3], km_my[3];
+    // 3-means regions: pixel count, x and y sums, min of x n + p, max of x n + (n - 1 - p)
+    int km_cnt[3], km_sx[3], km_sy[3], km_lo[3], km_hi[3];
 };
 
 // Sequential float32 sum of plane values over the pixels (x, y) in [x0, x1) x [y0, y1), row-major.
@@ -92,6 +93,43 @@ __device__ __forceinline__ float cg_block_sum(const float* plane, int ts, int x0
     for (int y = y0; y < y1; ++y)
         for (int x = x0; x < x1; ++x) s += plane[y * ts + x];
     return s;
+}
+
+// s + a[0] + a[1] + ... + a[n - 1], left to right in float32 (loads issued ahead of the adds).
+__device__ __forceinline__ float cg_seq_add(float s, const float* a, int n) {
+    int i = 0;
+    for (; i + 8 <= n; i += 8) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = a[i + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s += v[j];
+    }
+    for (; i < n; ++i) s += a[i];
+    return s;
+}
+
+// The same sum over the pixels of [x0, x0 + w) x [y0, y0 + h), row by row.
+__device__ __forceinline__ float cg_rows_add(const float* plane, int ts, int x0, int w, int y0, int h) {
+    float s = 0.0f;
+    for (int y = y0; y < y0 + h; ++y) s = cg_seq_add(s, plane + y * ts + x0, w);
+    return s;
+}
+
+__device__ __forceinline__ int cg_wave_sum(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ int cg_wave_min(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int cg_wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
 }
 
 __device__ __forceinline__ int cg_q(double v, double s) {  // int(v * s + 0.5), truncated
@@ -136,18 +174,38 @@ __device__ void cg_emit(int (*keys)[CG_KEY], WeftPrim* out, double* f64, int j, 
 // De-duplication (keep the first candidate of every key, _dedup_candidates), compaction in
 // place (every kept record is read before any is written), and records of no kind in the
 // slots past the pool.
-__device__ void cg_finish(int (*keys)[CG_KEY], int* keep, int* n_kept_lds, WeftPrim* out, double* f64, int n_total,
-                          int max_cands, int32_t* out_count) {
+__device__ __forceinline__ uint32_t cg_key_hash(const int* k) {
+    uint32_t h = 0x811c9dc5u;
+#pragma unroll
+    for (int t = 0; t < CG_KEY; ++t) h = (h ^ (uint32_t)k[t]) * 0x01000193u;
+    return h ^ (h >> 15);
+}
+
+// keep[j] = 1 for the first candidate of every key (hashes compared first, keys on a match).
+__device__ void cg_first_of_key(int (*keys)[CG_KEY], uint32_t* hash, int* keep, int n_total) {
     const int tid = threadIdx.x;
+    for (int j = tid; j < n_total; j += CG_THREADS) hash[j] = cg_key_hash(keys[j]);
+    __syncthreads();
     for (int j = tid; j < n_total; j += CG_THREADS) {
+        const uint32_t hj = hash[j];
         int k = 1;
-        for (int i = 0; i < j && k; ++i) {
-            bool same = true;
-            for (int t = 0; t < CG_KEY && same; ++t) same = keys[i][t] == keys[j][t];
-            if (same) k = 0;
-        }
+        for (int i = 0; i < j; ++i)
+            if (hash[i] == hj) {
+                bool same = true;
+                for (int t = 0; t < CG_KEY && same; ++t) same = keys[i][t] == keys[j][t];
+                if (same) {
+                    k = 0;
+                    break;
+                }
+            }
         keep[j] = k;
     }
+}
+
+__device__ void cg_finish(int (*keys)[CG_KEY], uint32_t* hash, int* keep, int* n_kept_lds, WeftPrim* out,
+                          double* f64, int n_total, int max_cands, int32_t* out_count) {
+    const int tid = threadIdx.x;
+    cg_first_of_key(keys, hash, keep, n_total);
     __syncthreads();
     if (tid == 0) {
         int run = 0;
@@ -193,10 +251,12 @@ __device__ void cg_finish(int (*keys)[CG_KEY], int* keep, int* n_kept_lds, WeftP
     }
 }
 
+// Work is split so that each independent in-order sum runs on lanes of its own wavefront
+// (lanes of one wavefront running different loops would take turns).
 __device__ void cg_body(const WeftCandGenArgs& A) {
     __shared__ CgShared S;
     extern __shared__ float4 weft_smem[];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int ts = A.tile_size, n = ts * ts;
     const int slot = blockIdx.x;
     const int tile = A.tiles[slot];
@@ -233,19 +293,31 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
         MAG[p] = sqrtf(gx * gx + gy * gy);
     }
     __syncthreads();
-    // flat means (pairwise) and block means (row by row)
-    if (tid == 0) S.avg_mag = cg_pairwise(MAG, n) / (float)n;
-    else if (tid == 1) S.gmx = cg_pairwise(GX, n) / (float)n;
-    else if (tid == 2) S.gmy = cg_pairwise(GY, n) / (float)n;
-    else if (tid < 6) S.mean_all[tid - 3] = cg_block_sum((R + (tid - 3) * n), ts, 0, ts, 0, ts) / (float)n;
-    else if (tid < 18) {
-        const int q = (tid - 6) / 3, c = (tid - 6) % 3, qh = ts / 2;
-        const int x0 = (q & 1) * qh, y0 = (q >> 1) * qh;
-        S.quad[q][c] = cg_block_sum((R + c * n), ts, x0, x0 + qh, y0, y0 + qh) / (float)(qh * qh);
-    } else if (tid < 66) {
-        const int q = (tid - 18) / 3, c = (tid - 18) % 3, sh = ts / 4;
-        const int x0 = (q & 3) * sh, y0 = (q >> 2) * sh;
-        S.sub[q][c] = cg_block_sum((R + c * n), ts, x0, x0 + sh, y0, y0 + sh) / (float)(sh * sh);
+    // flat means (pairwise) and block means (row by row), one kind per wavefront
+    if (wave == 0) {
+        if (lane < 3) {
+            const float v = cg_pairwise(lane == 0 ? MAG : (lane == 1 ? GX : GY), n) / (float)n;
+            if (lane == 0) S.avg_mag = v;
+            else if (lane == 1) S.gmx = v;
+            else S.gmy = v;
+        }
+    } else if (wave == 1) {
+        if (lane < 3) S.mean_all[lane] = cg_seq_add(0.0f, R + lane * n, n) / (float)n;
+    } else if (wave == 2) {
+        if (lane < 12) {
+            const int q = lane / 3, c = lane % 3, qh = ts / 2;
+            S.quad[q][c] = cg_rows_add(R + c * n, ts, (q & 1) * qh, qh, (q >> 1) * qh, qh) / (float)(qh * qh);
+        }
+    } else {
+        if (lane < 48) {
+            const int q = lane / 3, c = lane % 3, sh = ts / 4;
+            S.sub[q][c] = cg_rows_add(R + c * n, ts, (q & 3) * sh, sh, (q >> 2) * sh, sh) / (float)(sh * sh);
+        }
+    }
+    if (tid < 3) {
+        S.km_cnt[tid] = S.km_sx[tid] = S.km_sy[tid] = 0;
+        S.km_lo[tid] = 2147483647;
+        S.km_hi[tid] = -1;
     }
     __syncthreads();
     // sort keys: (luminance, index) for the palette and the extreme pixels; magnitude values
@@ -302,8 +374,12 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
         S.line_geom[l][3] = cy + vy * hf * f;
     }
     __syncthreads();
-    if (tid == 0) {
-        // luminance percentiles from the high words of the sorted (luminance, index) keys
+    // percentiles (every thread reads the few sorted keys it needs); then the masked means and
+    // the linear patches' side means in one loop over the pixels in order (waves 0-2: lanes
+    // 0-8 the bright / dark / edge masks, lanes 9 .. 9 + 3 n_lines - 1 the lines), the k-means
+    // groups on wave 3
+    float p25, p75, gthr;
+    {
         auto lum_at = [&](int i) { return cg_funkey((uint32_t)(SORT[i] >> 32)); };
         const float virt25 = (float)(n - 1) * 0.25f, virt75 = (float)(n - 1) * 0.75f;
         auto pct = [&](float virt) {
@@ -315,62 +391,126 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
             if (gamma >= 0.5f) r = b - diff * (1.0f - gamma);
             return r;
         };
-        S.p25 = pct(virt25);
-        S.p75 = pct(virt75);
-        S.gthr = cg_percentile(MSORT, n, 0.75f);
+        p25 = pct(virt25);
+        p75 = pct(virt75);
+        gthr = cg_percentile(MSORT, n, 0.75f);
     }
-    __syncthreads();
-    // masked means, k-means groups, linear-patch side means
-    if (tid < 9) {
-        const int e = tid / 3, c = tid % 3;
-        float s = 0.0f;
-        int cnt = 0;
-        for (int p = 0; p < n; ++p) {
-            const bool m = e == 0 ? LUM[p] > S.p75 : (e == 1 ? LUM[p] < S.p25 : MAG[p] >= S.gthr);
-            if (m) {
-                s += (R + c * n)[p];
-                ++cnt;
+    if (tid < 9 + 3 * n_lines) {
+        const bool side_lane = tid >= 9;
+        const int e = tid / 3, l = (tid - 9) / 3, c = side_lane ? (tid - 9) % 3 : tid % 3;
+        float x0 = 0.0f, y0 = 0.0f, dyl = 0.0f, dxl = 0.0f;
+        if (side_lane) {
+            x0 = (float)S.line_geom[l][0];
+            y0 = (float)S.line_geom[l][1];
+            const float x1 = (float)S.line_geom[l][2], y1 = (float)S.line_geom[l][3];
+            dyl = y1 - y0;
+            dxl = x1 - x0;
+        }
+        const float* plane = R + c * n;
+        float sa = 0.0f, sb = 0.0f;
+        int na = 0;
+        for (int y = 0; y < ts; ++y) {
+            const float ty = ((float)y - y0) * dxl;
+            for (int x = 0; x < ts; x += 8) {
+                const int p = y * ts + x;
+                float v[8], lm[8], mg[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    v[j] = plane[p + j];
+                    lm[j] = LUM[p + j];
+                    mg[j] = MAG[p + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const float side = ((float)(x + j) - x0) * dyl - ty;
+                    const bool m = side_lane ? side >= 0.0f
+                                             : (e == 0 ? lm[j] > p75 : (e == 1 ? lm[j] < p25 : mg[j] >= gthr));
+                    if (m) {
+                        sa += v[j];
+                        ++na;
+                    } else {
+                        sb += v[j];
+                    }
+                }
             }
         }
-        S.ext[e][c] = cnt ? s / (float)cnt : 0.0f;
-        if (c == 0) S.ext_n[e] = cnt;
-    } else if (tid >= 16 && tid < 16 + 27) {
-        const int g = (tid - 16) / 3, c = (tid - 16) % 3;
+        if (!side_lane) {
+            S.ext[e][c] = na ? sa / (float)na : 0.0f;
+            if (c == 0) S.ext_n[e] = na;
+        } else {
+            const int nb = n - na;
+            S.line_col[l][0][c] = na ? sa / (float)na : S.mean_all[c];
+            S.line_col[l][1][c] = nb ? sb / (float)nb : S.mean_all[c];
+        }
+    } else if (wave == 3 && lane < 27) {
+        const int g = lane / 3, c = lane % 3;
         const int k = g < 2 ? 2 : (g < 5 ? 3 : 4);
         const int jj = g < 2 ? g : (g < 5 ? g - 2 : g - 5);
         const double step = (double)n / (double)k;
         const int a = (int)((double)jj * step), b = (int)((double)(jj + 1) * step);
+        const float* plane = R + c * n;
         float v;
         if (b <= a) {
-            v = (R + c * n)[(int)(SORT[a < n - 1 ? a : n - 1] & 0xffffffffu)];
+            v = plane[(int)(SORT[a < n - 1 ? a : n - 1] & 0xffffffffu)];
         } else {
             float s = 0.0f;
-            for (int i = a; i < b; ++i) s += (R + c * n)[(int)(SORT[i] & 0xffffffffu)];
+            int i = a;
+            for (; i + 8 <= b; i += 8) {
+                float w[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) w[q] = plane[(int)(SORT[i + q] & 0xffffffffu)];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) s += w[q];
+            }
+            for (; i < b; ++i) s += plane[(int)(SORT[i] & 0xffffffffu)];
             v = s / (float)(b - a);
         }
         S.km[g][c] = v;
-    } else if (tid >= 64 && tid < 64 + 3 * n_lines) {
-        const int l = (tid - 64) / 3, c = (tid - 64) % 3;
-        const float x0 = (float)S.line_geom[l][0], y0 = (float)S.line_geom[l][1];
-        const float x1 = (float)S.line_geom[l][2], y1 = (float)S.line_geom[l][3];
-        const float dyl = y1 - y0, dxl = x1 - x0;
-        float sa = 0.0f, sb = 0.0f;
-        int na = 0;
-        for (int p = 0; p < n; ++p) {
-            const int y = p / ts, x = p - y * ts;
-            const float side = ((float)x - x0) * dyl - ((float)y - y0) * dxl;
-            if (side >= 0.0f) {
-                sa += (R + c * n)[p];
-                ++na;
-            } else {
-                sb += (R + c * n)[p];
-            }
-        }
-        const int nb = n - na;
-        S.line_col[l][0][c] = na ? sa / (float)na : S.mean_all[c];
-        S.line_col[l][1][c] = nb ? sb / (float)nb : S.mean_all[c];
     }
     __syncthreads();
+    // k-means labels (quality >= 80; first of the smallest float32 distances) and their regions'
+    // integer statistics, reduced per wavefront
+    if (A.quality >= 80) {
+        int cnt[3] = {0, 0, 0}, sx[3] = {0, 0, 0}, sy[3] = {0, 0, 0};
+        int lo[3] = {2147483647, 2147483647, 2147483647}, hi[3] = {-1, -1, -1};
+        for (int p = tid; p < n; p += CG_THREADS) {
+            int best = 0;
+            float bd = 0.0f;
+            for (int k = 0; k < 3; ++k) {
+                float d = 0.0f;
+                for (int c = 0; c < 3; ++c) {
+                    const float e = (R + c * n)[p] - S.km[2 + k][c];
+                    d = d + e * e;
+                }
+                if (k == 0 || d < bd) {
+                    bd = d;
+                    best = k;
+                }
+            }
+            const int y = p / ts, x = p - y * ts;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (best == k) {
+                    ++cnt[k];
+                    sx[k] += x;
+                    sy[k] += y;
+                    lo[k] = min(lo[k], x * n + p);
+                    hi[k] = max(hi[k], x * n + (n - 1 - p));
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int c = cg_wave_sum(cnt[k]), a = cg_wave_sum(sx[k]), b = cg_wave_sum(sy[k]);
+            const int l = cg_wave_min(lo[k]), h = cg_wave_max(hi[k]);
+            if (lane == 0 && c) {
+                atomicAdd(&S.km_cnt[k], c);
+                atomicAdd(&S.km_sx[k], a);
+                atomicAdd(&S.km_sy[k], b);
+                atomicMin(&S.km_lo[k], l);
+                atomicMax(&S.km_hi[k], h);
+            }
+        }
+    }
     // palette, alphas, gradient line, section sizes
     if (tid == 0) {
         auto near = [&](const double* c) {
@@ -421,64 +561,13 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
         S.n_tri_x = 8;  // 4 brightest-pixel and 4 darkest-pixel triangles (6 extreme pixels each)
     }
     __syncthreads();
-    // k-means labels and their region triangles (quality >= 80); the labels overwrite MAG
-    if (A.quality >= 80) {
-        int* L = reinterpret_cast<int*>(MAG);
-        for (int p = tid; p < n; p += CG_THREADS) {
-            int best = 0;
-            float bd = 0.0f;
-            for (int k = 0; k < 3; ++k) {
-                float d = 0.0f;
-                for (int c = 0; c < 3; ++c) {
-                    const float e = (R + c * n)[p] - S.km[2 + k][c];
-                    d = d + e * e;
-                }
-                if (k == 0 || d < bd) {
-                    bd = d;
-                    best = k;
-                }
-            }
-            L[p] = best;
-        }
-        __syncthreads();
-        if (tid < 3) {
-            int cnt = 0, imin = -1, imax = -1, xmin = 0, xmax = 0;
-            int64_t sx = 0, sy = 0;
-            for (int p = 0; p < n; ++p)
-                if (L[p] == tid) {
-                    const int y = p / ts, x = p - y * ts;
-                    if (cnt == 0 || x < xmin) {
-                        xmin = x;
-                        imin = p;
-                    }
-                    if (cnt == 0 || x > xmax) {
-                        xmax = x;
-                        imax = p;
-                    }
-                    sx += x;
-                    sy += y;
-                    ++cnt;
-                }
-            S.km_cnt[tid] = cnt;
-            S.km_imin[tid] = imin;
-            S.km_imax[tid] = imax;
-            S.km_mx[tid] = cnt ? (double)sx / (double)cnt : 0.0;
-            S.km_my[tid] = cnt ? (double)sy / (double)cnt : 0.0;
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        int m = 0;
-        if (A.quality >= 80)
-            for (int k = 0; k < 3; ++k) m += S.km_cnt[k] >= 3 ? 2 : 0;
-        S.n_km_tri = m;
-        const int pc = S.n_pal < 3 ? S.n_pal : 3;
-        S.n_total = 4 * S.n_pal + 8 + 16 + 3 * n_ang + 36 + 80 + S.grad_lines + 45 + 8 * pc + 8 + S.n_tri_x + m;
-    }
-    __syncthreads();
-
     // the pool, in the reference's order
-    const int n_total = S.n_total;
+    int n_km_tri = 0;
+    if (A.quality >= 80)
+        for (int k = 0; k < 3; ++k) n_km_tri += S.km_cnt[k] >= 3 ? 2 : 0;
+    const int pc_total = S.n_pal < 3 ? S.n_pal : 3;
+    const int n_total =
+        4 * S.n_pal + 8 + 16 + 3 * n_ang + 36 + 80 + S.grad_lines + 45 + 8 * pc_total + 8 + S.n_tri_x + n_km_tri;
     for (int j = tid; j < n_total; j += CG_THREADS) {
         double g[7] = {0, 0, 0, 0, 0, 0, 0};
         double c0[3], c1[3];
@@ -611,16 +700,17 @@ __device__ void cg_body(const WeftCandGenArgs& A) {
                 if (S.km_cnt[lab] >= 3) q -= 2;
                 ++lab;
             }
-            const int a = S.km_imin[lab], b = S.km_imax[lab];
+            const int a = S.km_lo[lab] % n, b = (n - 1) - S.km_hi[lab] % n;
+            const double cnt = (double)S.km_cnt[lab];
             g[0] = (double)(a % ts); g[1] = (double)(a / ts);
             g[2] = (double)(b % ts); g[3] = (double)(b / ts);
-            g[4] = S.km_mx[lab]; g[5] = S.km_my[lab];
+            g[4] = (double)S.km_sx[lab] / cnt; g[5] = (double)S.km_sy[lab] / cnt;
             colct(c0, S.km[2 + lab]);
             cg_emit(S.keys, out, f64, j, 4, g, 6, c0, nullptr, q ? 0.3 : 0.5);
         }
     }
     __syncthreads();
-    cg_finish(S.keys, S.keep, &S.n_kept, out, f64, n_total, A.max_cands, A.out_count + slot);
+    cg_finish(S.keys, S.hash, S.keep, &S.n_kept, out, f64, n_total, A.max_cands, A.out_count + slot);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -633,13 +723,18 @@ constexpr int CR_MAXC = 128;  // 117 before de-duplication at most
 struct CrShared {
     int keys[CR_MAXC][CG_KEY];
     int keep[CR_MAXC];
+    uint32_t hash[CR_MAXC];
     float mt[3], mp[3], me[3], mall[3], quad[4][3];
     int quad_n[4];
     float line_col[4][2][3];
     double line_geom[4][4];
-    float thresh, emx, emy;
-    int n_hi, n_total, n_kept;
-    int64_t sx, sy;
+    int line_na[4];
+    float emx, emy;
+    int n_total, n_kept;
+    // error pixels: count, coordinate sums, min of x n + p, max of x n + (n - 1 - p), first and
+    // last pixel, pixels per column
+    int n_hi, sx, sy, key_lo, key_hi, p_first, p_last;
+    int colc[64];
     int xmin, y_at_xmin, xmax, y_at_xmax, ymin, ymax;  // extent of the error pixels
     int o_first, o_mid, o_last;                       // pixel indices in the stable x-order
     double hi_col[3], corr[3];
@@ -647,20 +742,23 @@ struct CrShared {
 
 __device__ __forceinline__ float cr_clip(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
 
+// Work is split so that each independent in-order sum runs on lanes of its own wavefront;
+// the prediction is staged in LDS once the gradients and sort keys are consumed.
 __device__ void cr_body(const WeftResGenArgs& A) {
     __shared__ CrShared S;
     extern __shared__ float4 weft_smem[];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int ts = A.tile_size, n = ts * ts;
     const int slot = blockIdx.x;
     const int tile = A.tiles[slot];
     const double hf = (double)(ts - 1), wf = (double)(ts - 1);
-    float* T = reinterpret_cast<float*>(weft_smem);  // 3 planes; the prediction stays in HBM
-    float* EL = T + 3 * n;
+    float* T = reinterpret_cast<float*>(weft_smem);  // target planes
+    float* EL = T + 3 * n;                            // mean absolute error, then the error mask
     float* GX = EL + n;
     float* GY = GX + n;
     uint32_t* SK = reinterpret_cast<uint32_t*>(GY + n);
-    int* HI = reinterpret_cast<int*>(GX);  // after the gradient means
+    float* P = GX;                                    // prediction planes, over GX, GY and SK
+    int* HI = reinterpret_cast<int*>(EL);
     WeftPrim* out = A.out + (int64_t)slot * A.max_cands;
     double* f64 = A.out_f64 ? A.out_f64 + (int64_t)slot * A.max_cands * 14 : nullptr;
 
@@ -678,6 +776,12 @@ __device__ void cr_body(const WeftResGenArgs& A) {
         }
         EL[p] = ((0.0f + fabsf(e[0]) + fabsf(e[1])) + fabsf(e[2])) / 3.0f;
     }
+    if (tid < 64) S.colc[tid] = 0;
+    if (tid == 0) {
+        S.n_hi = S.sx = S.sy = 0;
+        S.key_lo = S.p_first = 2147483647;
+        S.key_hi = S.p_last = -1;
+    }
     __syncthreads();
     for (int p = tid; p < n; p += CG_THREADS) {
         const int y = p / ts, x = p - y * ts;
@@ -686,6 +790,11 @@ __device__ void cr_body(const WeftResGenArgs& A) {
         SK[p] = cg_fkey(EL[p]);
     }
     __syncthreads();
+    if (tid < 2) {  // mean error gradient (pairwise), while the others start sorting
+        const float v = cg_pairwise(tid == 0 ? GX : GY, n) / (float)n;
+        if (tid == 0) S.emx = v;
+        else S.emy = v;
+    }
     for (int k = 2; k <= n; k <<= 1)
         for (int j = k >> 1; j >= 1; j >>= 1) {
             for (int t = tid; t < n / 2; t += CG_THREADS) {
@@ -699,147 +808,193 @@ __device__ void cr_body(const WeftResGenArgs& A) {
             }
             __syncthreads();
         }
-    if (tid == 0) {
-        S.thresh = cg_percentile(SK, n, 60.0f / 100.0f);
-        S.emx = cg_pairwise(GX, n) / (float)n;
-    } else if (tid == 1) {
-        S.emy = cg_pairwise(GY, n) / (float)n;
-    }
+    const float thresh = cg_percentile(SK, n, 60.0f / 100.0f);
     __syncthreads();
-    for (int p = tid; p < n; p += CG_THREADS) HI[p] = EL[p] >= S.thresh ? 1 : 0;
-    __syncthreads();
-    // means over the error pixels (row-major), quadrant means, the tile mean, positions
-    if (tid < 9) {
-        const int w = tid / 3, c = tid % 3;
-        float s = 0.0f;
-        int cnt = 0;
-        for (int p = 0; p < n; ++p)
-            if (HI[p]) {
-                const float v = w == 0 ? T[c * n + p] : (w == 1 ? pred_at(p, c) : T[c * n + p] - pred_at(p, c));
-                s += v;
-                ++cnt;
-            }
-        const float m = cnt ? s / (float)cnt : 0.0f;
-        if (w == 0) S.mt[c] = m;
-        else if (w == 1) S.mp[c] = m;
-        else S.me[c] = m;
-        if (tid == 0) S.n_hi = cnt;
-    } else if (tid < 21) {
-        const int q = (tid - 9) / 3, c = (tid - 9) % 3, qh = ts / 2;
-        const int x0 = (q & 1) * qh, y0 = (q >> 1) * qh;
-        float s = 0.0f;
-        int cnt = 0;
-        for (int y = y0; y < y0 + qh; ++y)
-            for (int x = x0; x < x0 + qh; ++x)
-                if (HI[y * ts + x]) {
-                    s += T[c * n + y * ts + x];
-                    ++cnt;
-                }
-        S.quad[q][c] = cnt ? s / (float)cnt : 0.0f;
-        if (c == 0) S.quad_n[q] = cnt;
-    } else if (tid < 24) {
-        const int c = tid - 21;
-        S.mall[c] = cg_block_sum(T + c * n, ts, 0, ts, 0, ts) / (float)n;
-    } else if (tid == 24) {
-        // error pixels in row-major order: coordinate sums, first min / max x, first max y, and
-        // the stable x-order's first, middle and last element
-        int cnt = 0, xmin = 0, xmax = 0, ymin = 0, ymax = 0;
-        int64_t sx = 0, sy = 0;
-        int colc[64];
-        for (int x = 0; x < 64; ++x) colc[x] = 0;
-        for (int p = 0; p < n; ++p)
-            if (HI[p]) {
+    // error mask, prediction planes, and the error pixels' positions reduced per wavefront
+    {
+        int cnt = 0, sx = 0, sy = 0, lo = 2147483647, hi = -1, pf = 2147483647, pl = -1;
+        for (int p = tid; p < n; p += CG_THREADS) {
+            const int h = EL[p] >= thresh ? 1 : 0;
+            HI[p] = h;
+            for (int c = 0; c < 3; ++c) P[c * n + p] = pred_at(p, c);
+            if (h) {
                 const int y = p / ts, x = p - y * ts;
-                if (cnt == 0 || x < xmin) {  // np.argmin / argmax: the first occurrence
-                    xmin = x;
-                    S.y_at_xmin = y;
-                }
-                if (cnt == 0 || x > xmax) {
-                    xmax = x;
-                    S.y_at_xmax = y;
-                }
-                if (cnt == 0) ymin = y;
-                if (cnt == 0 || y > ymax) ymax = y;
+                ++cnt;
                 sx += x;
                 sy += y;
-                ++colc[x];
-                ++cnt;
+                lo = min(lo, x * n + p);
+                hi = max(hi, x * n + (n - 1 - p));
+                pf = min(pf, p);
+                pl = max(pl, p);
+                atomicAdd(&S.colc[x], 1);
             }
-        S.xmin = xmin;
-        S.xmax = xmax;
-        S.ymin = ymin;
-        S.ymax = ymax;
-        S.sx = sx;
-        S.sy = sy;
+        }
+        cnt = cg_wave_sum(cnt);
+        sx = cg_wave_sum(sx);
+        sy = cg_wave_sum(sy);
+        lo = cg_wave_min(lo);
+        hi = cg_wave_max(hi);
+        pf = cg_wave_min(pf);
+        pl = cg_wave_max(pl);
+        if (lane == 0 && cnt) {
+            atomicAdd(&S.n_hi, cnt);
+            atomicAdd(&S.sx, sx);
+            atomicAdd(&S.sy, sy);
+            atomicMin(&S.key_lo, lo);
+            atomicMax(&S.key_hi, hi);
+            atomicMin(&S.p_first, pf);
+            atomicMax(&S.p_last, pl);
+        }
+    }
+    __syncthreads();
+    // means over the error pixels (row-major), quadrant means, the tile mean, line side means
+    if (wave == 0) {
+        if (lane < 9) {
+            const int w = lane / 3, c = lane % 3;
+            const float* tp = T + c * n;
+            const float* pp = P + c * n;
+            float s = 0.0f;
+            int cnt = 0;
+            for (int p0 = 0; p0 < n; p0 += 8) {
+                int h[8];
+                float t[8], q[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    h[j] = HI[p0 + j];
+                    t[j] = tp[p0 + j];
+                    q[j] = pp[p0 + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (h[j]) {
+                        s += w == 0 ? t[j] : (w == 1 ? q[j] : t[j] - q[j]);
+                        ++cnt;
+                    }
+            }
+            const float m = cnt ? s / (float)cnt : 0.0f;
+            if (w == 0) S.mt[c] = m;
+            else if (w == 1) S.mp[c] = m;
+            else S.me[c] = m;
+        }
+    } else if (wave == 1) {
+        if (lane < 12) {
+            const int q = lane / 3, c = lane % 3, qh = ts / 2;
+            const int x0 = (q & 1) * qh, y0 = (q >> 1) * qh;
+            float s = 0.0f;
+            int cnt = 0;
+            for (int y = y0; y < y0 + qh; ++y)
+                for (int x = x0; x < x0 + qh; x += 4) {
+                    int h[4];
+                    float v[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        h[j] = HI[y * ts + x + j];
+                        v[j] = T[c * n + y * ts + x + j];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (h[j]) {
+                            s += v[j];
+                            ++cnt;
+                        }
+                }
+            S.quad[q][c] = cnt ? s / (float)cnt : 0.0f;
+            if (c == 0) S.quad_n[q] = cnt;
+        }
+    } else if (wave == 2) {
+        if (lane < 3) S.mall[lane] = cg_seq_add(0.0f, T + lane * n, n) / (float)n;
+    } else if (lane < 12) {
+        // the four lines: along the mean error gradient at 0.5, 0.3, 0.7 of the tile and across
+        // it at the centre (_split_by_line: the Python-float line enters as float32)
+        const int l = lane / 3, c = lane % 3;
+        const double emx = S.emx, emy = S.emy;
+        const double en = sqrt(emx * emx + emy * emy) + 1e-8;
+        const double vx = emx / en, vy = emy / en;
+        double g[4];
+        if (l < 3) {
+            const double offs[3] = {0.5, 0.3, 0.7};
+            const double cx = wf * offs[l], cy = hf * offs[l];
+            g[0] = cx - vx * wf * 0.6;
+            g[1] = cy - vy * hf * 0.6;
+            g[2] = cx + vx * wf * 0.6;
+            g[3] = cy + vy * hf * 0.6;
+        } else {
+            const double pvx = -vy, pvy = vx, cx = wf * 0.5, cy = hf * 0.5;
+            g[0] = cx - pvx * wf * 0.6;
+            g[1] = cy - pvy * hf * 0.6;
+            g[2] = cx + pvx * wf * 0.6;
+            g[3] = cy + pvy * hf * 0.6;
+        }
+        if (c == 0)
+            for (int i = 0; i < 4; ++i) S.line_geom[l][i] = g[i];
+        const float x0 = (float)g[0], y0 = (float)g[1];
+        const float dy = (float)(g[3] - g[1]), dx = (float)(g[2] - g[0]);
+        const float* plane = T + c * n;
+        float sa = 0.0f, sb = 0.0f;
+        int na = 0;
+        for (int y = 0; y < ts; ++y) {
+            const float ty = ((float)y - y0) * dx;
+            for (int x = 0; x < ts; x += 8) {
+                float v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = plane[y * ts + x + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const float side = ((float)(x + j) - x0) * dy - ty;
+                    if (side >= 0.0f) {
+                        sa += v[j];
+                        ++na;
+                    } else {
+                        sb += v[j];
+                    }
+                }
+            }
+        }
+        const int nb = n - na;
+        S.line_col[l][0][c] = na ? sa / (float)na : 0.0f;  // an empty side takes the tile mean below
+        S.line_col[l][1][c] = nb ? sb / (float)nb : 0.0f;
+        if (c == 0) S.line_na[l] = na;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int cnt = S.n_hi;
+        S.xmin = cnt ? S.key_lo / n : 0;
+        S.y_at_xmin = cnt ? (S.key_lo % n) / ts : 0;
+        S.xmax = cnt ? S.key_hi / n : 0;
+        S.y_at_xmax = cnt ? ((n - 1) - S.key_hi % n) / ts : 0;
+        S.ymin = cnt ? S.p_first / ts : 0;
+        S.ymax = cnt ? S.p_last / ts : 0;
         // stable sort by x: rank r lies in the first column whose running count passes r, at
         // the (r - count before) -th error pixel of that column from the top
         int ranks[3] = {0, cnt / 2, cnt - 1};
         int res[3] = {0, 0, 0};
         for (int k = 0; k < 3; ++k) {
             int r = ranks[k], x = 0;
-            while (x < ts && r >= colc[x]) {
-                r -= colc[x];
+            while (x < ts && r >= S.colc[x]) {
+                r -= S.colc[x];
                 ++x;
             }
             int y = 0;
-            for (; y < ts; ++y)
-                if (HI[y * ts + x]) {
-                    if (r == 0) break;
-                    --r;
-                }
+            if (x < ts)
+                for (; y < ts; ++y)
+                    if (HI[y * ts + x]) {
+                        if (r == 0) break;
+                        --r;
+                    }
             res[k] = y * ts + x;
         }
         S.o_first = res[0];
         S.o_mid = res[1];
         S.o_last = res[2];
-    }
-    __syncthreads();
-    if (tid == 0) {
         for (int c = 0; c < 3; ++c) {
             S.hi_col[c] = cg_ct((double)cr_clip(S.mt[c]));
             S.corr[c] = cg_ct((double)cr_clip(S.mp[c] + S.me[c]));
         }
-        const double emx = S.emx, emy = S.emy;
-        const double en = sqrt(emx * emx + emy * emy) + 1e-8;
-        const double vx = emx / en, vy = emy / en;
-        const double offs[3] = {0.5, 0.3, 0.7};
-        for (int l = 0; l < 3; ++l) {
-            const double cx = wf * offs[l], cy = hf * offs[l];
-            S.line_geom[l][0] = cx - vx * wf * 0.6;
-            S.line_geom[l][1] = cy - vy * hf * 0.6;
-            S.line_geom[l][2] = cx + vx * wf * 0.6;
-            S.line_geom[l][3] = cy + vy * hf * 0.6;
-        }
-        const double pvx = -vy, pvy = vx, cx = wf * 0.5, cy = hf * 0.5;
-        S.line_geom[3][0] = cx - pvx * wf * 0.6;
-        S.line_geom[3][1] = cy - pvy * hf * 0.6;
-        S.line_geom[3][2] = cx + pvx * wf * 0.6;
-        S.line_geom[3][3] = cy + pvy * hf * 0.6;
-    }
-    __syncthreads();
-    // side means of the four lines (_split_by_line: the Python-float line enters as float32)
-    if (tid < 12) {
-        const int l = tid / 3, c = tid % 3;
-        const double* g = S.line_geom[l];
-        const float x0 = (float)g[0], y0 = (float)g[1];
-        const float dy = (float)(g[3] - g[1]), dx = (float)(g[2] - g[0]);
-        float sa = 0.0f, sb = 0.0f;
-        int na = 0;
-        for (int p = 0; p < n; ++p) {
-            const int y = p / ts, x = p - y * ts;
-            const float side = ((float)x - x0) * dy - ((float)y - y0) * dx;
-            if (side >= 0.0f) {
-                sa += T[c * n + p];
-                ++na;
-            } else {
-                sb += T[c * n + p];
+        for (int l = 0; l < 4; ++l)
+            for (int c = 0; c < 3; ++c) {
+                if (S.line_na[l] == 0) S.line_col[l][0][c] = S.mall[c];
+                if (S.line_na[l] == n) S.line_col[l][1][c] = S.mall[c];
             }
-        }
-        const int nb = n - na;
-        S.line_col[l][0][c] = na ? sa / (float)na : S.mall[c];
-        S.line_col[l][1][c] = nb ? sb / (float)nb : S.mall[c];
-    }
-    if (tid == 0) {
         int nq = 0;
         for (int q = 0; q < 4; ++q) nq += S.quad_n[q] ? 3 : 0;
         S.n_total = S.n_hi == 0 ? 0 : 15 + nq + 11 + 64 + (S.n_hi >= 6 ? 3 : 0) + (S.n_hi >= 3 ? 12 : 0);
@@ -940,5 +1095,5 @@ __device__ void cr_body(const WeftResGenArgs& A) {
         }
     }
     __syncthreads();
-    cg_finish(S.keys, S.keep, &S.n_kept, out, f64, n_total, A.max_cands, A.out_count + slot);
+    cg_finish(S.keys, S.hash, S.keep, &S.n_kept, out, f64, n_total, A.max_cands, A.out_count + slot);
 }

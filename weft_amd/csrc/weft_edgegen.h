@@ -21,6 +21,7 @@ constexpr int ED_BINS = 12;
 struct EdShared {
     int keys[ED_MAXC][CG_KEY];
     int keep[ED_MAXC];
+    uint32_t hash[ED_MAXC];
     float mean_all[3], quad[4][3], km[9][3];
     float hist[ED_BINS];
     int bin_cnt[ED_BINS], bin_off[ED_BINS];
@@ -130,36 +131,6 @@ __device__ __forceinline__ float ed_angle(float gx, float gy) {
     return mod;
 }
 
-// s + a[0] + a[1] + ... + a[n - 1], left to right in float32 (loads issued ahead of the adds).
-__device__ __forceinline__ float ed_seq_add(float s, const float* a, int n) {
-    int i = 0;
-    for (; i + 8 <= n; i += 8) {
-        float v[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = a[i + j];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) s += v[j];
-    }
-    for (; i < n; ++i) s += a[i];
-    return s;
-}
-
-__device__ __forceinline__ int ed_wave_sum(int v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-__device__ __forceinline__ int ed_wave_min(int v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
-    return v;
-}
-__device__ __forceinline__ int ed_wave_max(int v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
 // The dominant orientations from the histogram (table indices into cos_o / sin_o); returns
 // their number.  np.max and np.argmax: NaN propagates and is the first maximum.
 __device__ int ed_orientations(const float* hist, int* orient) {
@@ -233,7 +204,7 @@ __device__ void ed_body(const WeftEdgeGenArgs& A) {
         BIN[p] = (uint8_t)bin;
         SORT[p] = ((uint64_t)cg_fkey(LUM[p]) << 32) | (uint32_t)p;
     }
-    small = ed_wave_sum(small);
+    small = cg_wave_sum(small);
     if (lane == 0) atomicAdd(&S.small_cnt, small);
     __syncthreads();
     if (wave == 0) {
@@ -278,14 +249,12 @@ __device__ void ed_body(const WeftEdgeGenArgs& A) {
             }
         }
     } else if (wave == 1) {
-        if (lane < 3) S.mean_all[lane] = ed_seq_add(0.0f, R + lane * n, n) / (float)n;  // row by row = in order
+        if (lane < 3) S.mean_all[lane] = cg_seq_add(0.0f, R + lane * n, n) / (float)n;  // row by row = in order
     } else if (wave == 2) {
         if (lane < 12) {
             const int q = lane / 3, c = lane % 3, qh = ts / 2;
             const int x0 = (q & 1) * qh, y0 = (q >> 1) * qh;
-            float s = 0.0f;
-            for (int y = y0; y < y0 + qh; ++y) s = ed_seq_add(s, R + c * n + y * ts + x0, qh);
-            S.quad[q][c] = s / (float)(qh * qh);
+            S.quad[q][c] = cg_rows_add(R + c * n, ts, x0, qh, y0, qh) / (float)(qh * qh);
         }
     }
     __syncthreads();
@@ -415,8 +384,8 @@ __device__ void ed_body(const WeftEdgeGenArgs& A) {
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const int c = ed_wave_sum(cnt[k]), a = ed_wave_sum(sx[k]), b = ed_wave_sum(sy[k]);
-            const int l = ed_wave_min(lo[k]), h = ed_wave_max(hi[k]);
+            const int c = cg_wave_sum(cnt[k]), a = cg_wave_sum(sx[k]), b = cg_wave_sum(sy[k]);
+            const int l = cg_wave_min(lo[k]), h = cg_wave_max(hi[k]);
             if (lane == 0 && c) {
                 atomicAdd(&S.km_cnt[k], c);
                 atomicAdd(&S.km_sx[k], a);
@@ -518,7 +487,7 @@ __device__ void ed_body(const WeftEdgeGenArgs& A) {
         }
     }
     __syncthreads();
-    cg_finish(S.keys, S.keep, &S.n_kept, out, f64, n_total, A.max_cands, A.out_count + slot);
+    cg_finish(S.keys, S.hash, S.keep, &S.n_kept, out, f64, n_total, A.max_cands, A.out_count + slot);
     // Edge points (the 60th percentile of the magnitudes at or above 1e-6) would need the line
     // and curve sections, which are not built: unless sorted magnitude floor(0.6 (n - 1)) + 1
     // is below 9.99e-7, the tile gets no pool
@@ -545,6 +514,7 @@ __device__ __forceinline__ void mg_key(int* k, int kind, const double* d) {
 __device__ void mg_body(const WeftMergePoolsArgs& A) {
     __shared__ int keys[MG_MAXC][CG_KEY];
     __shared__ int keep[MG_MAXC];
+    __shared__ uint32_t hash[MG_MAXC];
     __shared__ int n_kept_s;
     const int tid = threadIdx.x, slot = blockIdx.x;
     const int na = A.a_count[slot], nb = A.b_count[slot];
@@ -561,15 +531,7 @@ __device__ void mg_body(const WeftMergePoolsArgs& A) {
         mg_key(keys[j], (in_a ? a : b)[i].kind, (in_a ? af : bf) + (int64_t)i * 14);
     }
     __syncthreads();
-    for (int j = tid; j < n; j += CG_THREADS) {
-        int k = 1;
-        for (int i = 0; i < j && k; ++i) {
-            bool same = true;
-            for (int t = 0; t < CG_KEY && same; ++t) same = keys[i][t] == keys[j][t];
-            if (same) k = 0;
-        }
-        keep[j] = k;
-    }
+    cg_first_of_key(keys, hash, keep, n);
     __syncthreads();
     if (tid == 0) {
         int run = 0;

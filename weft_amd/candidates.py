@@ -72,10 +72,25 @@ def generate_residual_pools(tiles, d_pred, clip_pred: bool = True, want_f64: boo
                         "weft_gen_residual_candidates")
 
 
+_SIDE_STREAMS: dict = {}
+
+
+def _side_streams(dev, n: int = 4):
+    """Default-priority streams of the device, made once, for concurrent size classes."""
+    import torch
+
+    key = dev.index
+    if key not in _SIDE_STREAMS:
+        _SIDE_STREAMS[key] = [torch.cuda.Stream(dev) for _ in range(n)]
+    return _SIDE_STREAMS[key]
+
+
 def _class_pools(tiles, cap: int, want_f64: bool, d_pixels, d_tile_pix, launch, name: str):
     """Pools of every tile from one launch per tile size: ``launch(L, d_pixels, d_tile_pix,
     d_idx, n, ts, out, f64, counts, stream)`` writes the pools of the tiles d_idx, all of
-    size ts."""
+    size ts.  The size classes run concurrently on side streams, joined back into the
+    current stream before returning (a class of few large tiles leaves most of the device
+    to the others)."""
     import torch
 
     _lib.ensure_init(0)
@@ -94,23 +109,35 @@ def _class_pools(tiles, cap: int, want_f64: bool, d_pixels, d_tile_pix, launch, 
     f64 = torch.empty(max(T, 1) * cap * 14, dtype=torch.float64, device=dev) if want_f64 else None
     counts = torch.zeros(max(T, 1), dtype=torch.int32, device=dev)
     L = _lib.library()
-    for s in GEN_TILE_SIZES:
-        idx = np.nonzero(ts == s)[0].astype(np.int32)
-        if len(idx) == 0:
-            continue
-        # each size class writes its tiles' slots: launch over the class, slot = tile index
-        d_idx = torch.from_numpy(idx).to(dev)
-        cls_out = torch.empty(len(idx) * cap * rec, dtype=torch.uint8, device=dev)
-        cls_f64 = torch.empty(len(idx) * cap * 14, dtype=torch.float64, device=dev) if want_f64 else None
-        cls_cnt = torch.empty(len(idx), dtype=torch.int32, device=dev)
-        _lib.check(launch(L, d_pixels.data_ptr(), d_tile_pix.data_ptr(), d_idx.data_ptr(), len(idx), s,
-                          cls_out.data_ptr(), None if cls_f64 is None else cls_f64.data_ptr(), cls_cnt.data_ptr(),
-                          _lib.stream_ptr()), name)
-        di = d_idx.long()
-        out.view(-1, cap * rec)[di] = cls_out.view(-1, cap * rec)
-        counts[di] = cls_cnt
-        if want_f64:
-            f64.view(-1, cap * 14)[di] = cls_f64.view(-1, cap * 14)
+    main = torch.cuda.current_stream(dev)
+    fork = torch.cuda.Event()
+    fork.record(main)
+    joins = []
+    classes = [(s, np.nonzero(ts == s)[0].astype(np.int32)) for s in sorted(GEN_TILE_SIZES, reverse=True)]
+    classes = [(s, idx) for s, idx in classes if len(idx)]
+    side = _side_streams(dev)
+    for i, (s, idx) in enumerate(classes):
+        st = side[i % len(side)]
+        st.wait_event(fork)
+        with torch.cuda.stream(st):
+            # each size class writes its tiles' slots: launch over the class, slot = tile index
+            d_idx = torch.from_numpy(idx).to(dev)
+            cls_out = torch.empty(len(idx) * cap * rec, dtype=torch.uint8, device=dev)
+            cls_f64 = torch.empty(len(idx) * cap * 14, dtype=torch.float64, device=dev) if want_f64 else None
+            cls_cnt = torch.empty(len(idx), dtype=torch.int32, device=dev)
+            _lib.check(launch(L, d_pixels.data_ptr(), d_tile_pix.data_ptr(), d_idx.data_ptr(), len(idx), s,
+                              cls_out.data_ptr(), None if cls_f64 is None else cls_f64.data_ptr(),
+                              cls_cnt.data_ptr(), _lib.stream_ptr(st)), name)
+            di = d_idx.long()
+            out.view(-1, cap * rec)[di] = cls_out.view(-1, cap * rec)
+            counts[di] = cls_cnt
+            if want_f64:
+                f64.view(-1, cap * 14)[di] = cls_f64.view(-1, cap * 14)
+            ev = torch.cuda.Event()
+            ev.record(st)
+            joins.append(ev)
+    for ev in joins:
+        main.wait_event(ev)
     return out, counts, f64
 
 
@@ -153,7 +180,7 @@ def generate_merged_pools(tiles, quality: int = 75, want_f64: bool = False, d_pi
         d_pixels = torch.from_numpy(np.ascontiguousarray(tiles.target, dtype=np.float32)).to(dev)
     if d_tile_pix is None:
         d_tile_pix = torch.from_numpy(np.asarray(tiles.pix_off, np.int64)).to(dev)
-    ea, ec, ef = generate_edge_pools(tiles, quality, True, d_pixels, d_tile_pix, check)
+    ea, ec, ef = generate_edge_pools(tiles, quality, True, d_pixels, d_tile_pix, check=False)
     ta, tc, tf = generate_pools(tiles, quality, True, d_pixels, d_tile_pix)
     T = max(len(np.asarray(tiles.ts)), 1)
     out = torch.empty(T * MERGED_MAX_CANDS * PRIM_DTYPE.itemsize, dtype=torch.uint8, device=dev)
@@ -165,6 +192,8 @@ def generate_merged_pools(tiles, quality: int = 75, want_f64: bool = False, d_pi
                                   tf.data_ptr(), tc.data_ptr(), MAX_CANDS, n, out.data_ptr(),
                                   None if f64 is None else f64.data_ptr(), counts.data_ptr(), MERGED_MAX_CANDS,
                                   _lib.stream_ptr()), "weft_merge_pools")
+    if check:
+        _raise_on_edge_points(counts, n)
     return out, counts, f64
 
 

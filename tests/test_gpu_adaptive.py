@@ -32,6 +32,35 @@ def _tileset(run):
     return TileSet(t[:, 0], t[:, 1], t[:, 2], np.zeros(len(t), np.int32), flat, off)
 
 
+def _check_against_reference(f, run, tiles, k):
+    """Every round of fitter f (run with a trace) against the reference's recorded run."""
+    ref = [run["calls"][0]] + run["calls"][2:]
+    assert len(f.trace) == len(ref), f"run {k}: {len(f.trace)} rounds against {len(ref)}"
+    gauss_stack = np.zeros(tiles.n_tiles, bool)
+    n_exact = n_values = 0
+    for r, (g, e) in enumerate(zip(f.trace, ref)):
+        assert np.array_equal(g["tiles"], e["tiles"]), f"run {k} round {r}: tiles differ"
+        for j, t in enumerate(e["tiles"]):
+            w = f"run {k} round {r} tile {t}"
+            gm, em, kinds = g["mses"][j], e["mses"][j], g["kinds"][j]
+            assert gm.shape == em.shape, f"{w}: pool sizes differ"
+            best = int(np.argmin(e["objs"][j]))
+            assert int(g["win_idx"][j]) == best, f"{w}: winner {g['win_idx'][j]} against {best}"
+            gauss = np.isin(kinds, (2, 3)) | gauss_stack[t]
+            same = gm.view(np.uint32) == em.view(np.uint32)
+            assert gauss[~same].all(), f"{w}: a hard-edge stack differs"
+            np.testing.assert_allclose(gm, em, rtol=1e-6, atol=0, err_msg=w)
+            if not gauss[best]:
+                assert np.float32(g["win_obj"][j]) == e["objs"][j][best], w
+            n_exact += int(same.sum())
+            n_values += len(gm)
+            if g["win_acc"][j]:
+                gauss_stack[t] |= bool(np.isin(kinds[best], (2, 3)))
+    _say(f"  {len(ref)} rounds, {n_values} MSEs ({n_exact} bit-exact), stalls {f.stalls}")
+    for t, (a, b) in enumerate(zip(f.selected(), run["selected"])):
+        assert bits_equal(a.view(np.uint8), b.view(np.uint8)), f"run {k} tile {t}: selected stacks differ"
+
+
 def test_adaptive_rounds_match_reference(gpu):
     from weft_amd.adaptive import AdaptiveFitter
 
@@ -39,31 +68,41 @@ def test_adaptive_rounds_match_reference(gpu):
         tiles = _tileset(run)
         _say(f"adaptive run {k}: quality {run['quality']}, {tiles.n_tiles} tiles")
         f = AdaptiveFitter(tiles, run["pools"], run["quality"], lam=run["lam"], keep_trace=True).run()
-        ref = [run["calls"][0]] + run["calls"][2:]
-        assert len(f.trace) == len(ref), f"run {k}: {len(f.trace)} rounds against {len(ref)}"
-        gauss_stack = np.zeros(tiles.n_tiles, bool)
-        n_exact = n_values = 0
-        for r, (g, e) in enumerate(zip(f.trace, ref)):
-            assert np.array_equal(g["tiles"], e["tiles"]), f"run {k} round {r}: tiles differ"
-            for j, t in enumerate(e["tiles"]):
-                w = f"run {k} round {r} tile {t}"
-                gm, em, kinds = g["mses"][j], e["mses"][j], g["kinds"][j]
-                assert gm.shape == em.shape, f"{w}: pool sizes differ"
-                best = int(np.argmin(e["objs"][j]))
-                assert int(g["win_idx"][j]) == best, f"{w}: winner {g['win_idx'][j]} against {best}"
-                gauss = np.isin(kinds, (2, 3)) | gauss_stack[t]
-                same = gm.view(np.uint32) == em.view(np.uint32)
-                assert gauss[~same].all(), f"{w}: a hard-edge stack differs"
-                np.testing.assert_allclose(gm, em, rtol=1e-6, atol=0, err_msg=w)
-                if not gauss[best]:
-                    assert np.float32(g["win_obj"][j]) == e["objs"][j][best], w
-                n_exact += int(same.sum())
-                n_values += len(gm)
-                if g["win_acc"][j]:
-                    gauss_stack[t] |= bool(np.isin(kinds[best], (2, 3)))
-        _say(f"  {len(ref)} rounds, {n_values} MSEs ({n_exact} bit-exact), stalls {f.stalls}")
-        for t, (a, b) in enumerate(zip(f.selected(), run["selected"])):
-            assert bits_equal(a.view(np.uint8), b.view(np.uint8)), f"run {k} tile {t}: selected stacks differ"
+        _check_against_reference(f, run, tiles, k)
+
+
+def test_adaptive_fit_from_device_merged_pools(gpu):
+    """The fit phase on the device end to end: the merged pools of _worker_gen_cands generated on
+    the device from the runs' tiles, then the rounds, against the reference's own runs.  The
+    pools equal the reference's but for the template pool's extreme-pixel triangles (their
+    pairing follows np.argpartition's unspecified order there); where every tile's pool is
+    identical the whole run must match (rounds couple the tiles through the stall rule)."""
+    from weft_amd.adaptive import AdaptiveFitter
+    from weft_amd.candidates import MERGED_MAX_CANDS, generate_merged_pools
+    from weft_amd.records import PRIM_DTYPE
+
+    n_full = 0
+    for k, run in enumerate(adaptive_runs()):
+        tiles = _tileset(run)
+        _say(f"device pools, adaptive run {k}: quality {run['quality']}, {tiles.n_tiles} tiles")
+        recs, counts, _ = generate_merged_pools(tiles, int(run["quality"]))
+        host = recs.cpu().numpy().view(PRIM_DTYPE).reshape(-1, MERGED_MAX_CANDS)
+        cnt = counts.cpu().numpy()
+        identical = True
+        for t in range(tiles.n_tiles):
+            ref = np.ascontiguousarray(run["pools"][t], dtype=PRIM_DTYPE)
+            assert cnt[t] == len(ref), f"run {k} tile {t}: pool sizes {cnt[t]} and {len(ref)}"
+            got = host[t, :cnt[t]]
+            diff = (got.view(np.uint8).reshape(len(ref), -1) != ref.view(np.uint8).reshape(len(ref), -1)).any(axis=1)
+            assert (got["kind"][diff] == 4).all() and (ref["kind"][diff] == 4).all() and diff.sum() <= 12, (k, t)
+            identical &= not diff.any()
+        if not identical:
+            continue
+        n_full += 1
+        f = AdaptiveFitter(tiles, (recs, counts, MERGED_MAX_CANDS), run["quality"], lam=run["lam"],
+                           keep_trace=True).run()
+        _check_against_reference(f, run, tiles, k)
+    assert n_full >= 2
 
 
 def test_adaptive_equals_oracle_on_device_pools(gpu, oracle_lib):

@@ -60,8 +60,11 @@ def _mean_colour_patch(tile: np.ndarray) -> np.ndarray:
 
 class AdaptiveFitter:
     """tiles: the packed tile set (quadtree tiles of 8-64 px); pools: per tile, its packed
-    candidate records in the reference's order; quality: the encode quality (lambda,
-    threshold, rounds).  ``run()`` fits every tile; ``selected()`` returns the stacks."""
+    candidate records in the reference's order -- a list of host arrays, or the device pools
+    of ``weft_amd.candidates`` as ``(records, counts, slots)`` (for example
+    ``generate_merged_pools``, the pools of ``_worker_gen_cands``), which stay on the device;
+    quality: the encode quality (lambda, threshold, rounds).  ``run()`` fits every tile;
+    ``selected()`` returns the stacks."""
 
     MAX_STALLS = 3  # encoder.py:2747
 
@@ -74,7 +77,8 @@ class AdaptiveFitter:
         self.dev = torch.device("cuda", device)
         self.tiles = tiles
         T = self.n_tiles = tiles.n_tiles
-        if len(pools) != T:
+        device_pools = isinstance(pools, tuple)
+        if not device_pools and len(pools) != T:
             raise ValueError("one pool per tile")
         bad = set(np.unique(tiles.ts).tolist()) - set(GEN_TILE_SIZES)
         if bad:
@@ -84,18 +88,37 @@ class AdaptiveFitter:
         self.threshold, self.max_rounds = round_params(self.quality)
         self.keep_trace = keep_trace
         self.block = int(block)
-        pools = [np.ascontiguousarray(p, dtype=PRIM_DTYPE) for p in pools]
-        pools = [p if len(p) else _mean_colour_patch(tiles.tile_pixels(t)) for t, p in enumerate(pools)]
-        n0 = np.array([len(p) for p in pools], np.int32)
+        dev = self.dev
+        rec = PRIM_DTYPE.itemsize
+        if device_pools:
+            d_recs, d_counts, slots = pools
+            if d_counts.numel() < T or d_recs.numel() < T * slots * rec:
+                raise ValueError("device pools must hold one pool of `slots` records per tile")
+            n0 = d_counts[:T].cpu().numpy().astype(np.int32)
+            if (n0 < 0).any() or (n0 > slots).any():
+                raise ValueError("device pool counts must lie in [0, slots]")
+            empty = np.flatnonzero(n0 == 0)
+            n0[empty] = 1
+        else:
+            pools = [np.ascontiguousarray(p, dtype=PRIM_DTYPE) for p in pools]
+            pools = [p if len(p) else _mean_colour_patch(tiles.tile_pixels(t)) for t, p in enumerate(pools)]
+            n0 = np.array([len(p) for p in pools], np.int32)
         # live + accepted records of a tile never exceed its first pool plus what the re-seeds
         # add (at most one re-seed per round)
         self.cap = int(n0.max()) + self.max_rounds * RES_MAX_CANDS
         self.n0 = n0
-        host = np.zeros(T * self.cap, PRIM_DTYPE)
-        for t, p in enumerate(pools):
-            host[t * self.cap:t * self.cap + len(p)] = p
-        dev = self.dev
-        self.d_pools = torch.from_numpy(host.view(np.uint8)).to(dev)
+        if device_pools:
+            self.d_pools = torch.zeros(T * self.cap * rec, dtype=torch.uint8, device=dev)
+            m = min(int(slots), self.cap)
+            self.d_pools.view(T, self.cap, rec)[:, :m] = d_recs.view(-1, int(slots), rec)[:T, :m]
+            for t in empty:  # a tile without candidates takes its mean colour (encoder.py:2690-2693)
+                patch = _mean_colour_patch(tiles.tile_pixels(int(t))).view(np.uint8)
+                self.d_pools.view(T, self.cap, rec)[int(t), 0] = torch.from_numpy(patch.copy()).to(dev)
+        else:
+            host = np.zeros(T * self.cap, PRIM_DTYPE)
+            for t, p in enumerate(pools):
+                host[t * self.cap:t * self.cap + len(p)] = p
+            self.d_pools = torch.from_numpy(host.view(np.uint8)).to(dev)
         self.d_ncand = torch.from_numpy(n0.copy()).to(dev)
         maxp = np.array([MAX_PRIMITIVES_BY_TILE_SIZE.get(int(s), MAX_PRIMITIVES_PER_TILE) for s in tiles.ts], np.int32)
         self.d_maxp = torch.from_numpy(maxp).to(dev)

@@ -553,6 +553,60 @@ def edge_candidates_report(tiles, d_px, d_off, pick, quality=75):
     return out
 
 
+def adaptive_report(img, quality=75, sample=24, reps=3):
+    """The reference's fit phase (encoder._fit_adaptive_state, encoder.py:2603-2831) on the
+    1920x1080 image: its quadtree (complexity criterion, split threshold 0.08 at quality 75,
+    8-32 px), the merged candidate pools of _worker_gen_cands for every tile, and the batched
+    greedy rounds with winner removal, stalled-tile drop-out and residual re-seeding -- all on
+    the device (generate_merged_pools, AdaptiveFitter), timed on the host clock with the
+    device synchronised.  The CPU figure is the NumPy restatement of the same loop
+    (oracle/adaptive_np.py, pools from oracle/edge_np.py) on a sample of the tiles, one core."""
+    import torch
+    from oracle import adaptive_np, candgen_np, edge_np
+    from oracle.coracle import PRIM_DTYPE as ORC_PRIM
+    from weft_amd.adaptive import AdaptiveFitter
+    from weft_amd.candidates import MERGED_MAX_CANDS, generate_merged_pools
+    from weft_amd.quadtree import TileSet, decompose_quadtree
+
+    qt = decompose_quadtree(img, split_threshold=0.08, min_tile=8, max_tile=32)
+    tiles = TileSet.from_images([img], [qt], pad_multiple=32)
+    d_px = torch.from_numpy(tiles.target).cuda()
+    d_off = torch.from_numpy(np.asarray(tiles.pix_off, np.int64)).cuda()
+
+    def once():
+        recs, counts, _ = generate_merged_pools(tiles, quality, d_pixels=d_px, d_tile_pix=d_off)
+        f = AdaptiveFitter(tiles, (recs, counts, MERGED_MAX_CANDS), quality).run()
+        torch.cuda.synchronize()
+        return f, counts
+
+    once()  # warm
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f, counts = once()
+        times.append(time.perf_counter() - t0)
+    wall = float(np.median(times))
+    canvas = np.clip(f.canvas(), 0.0, 1.0)
+    mse = float(np.mean((canvas.astype(np.float64) - tiles.target) ** 2))
+    nsel = f.d_nsel.cpu().numpy()
+    # CPU: pools and the loop restated in NumPy, on a sample of the tiles
+    rng = np.random.default_rng(1)
+    pick = np.sort(rng.choice(tiles.n_tiles, min(sample, tiles.n_tiles), replace=False))
+    patches = [tiles.tile_pixels(int(t)) for t in pick]
+    t0 = time.perf_counter()
+    packed = [candgen_np.to_records(edge_np.merged(p, quality), ORC_PRIM) for p in patches]
+    adaptive_np.fit(patches, packed, quality, f.lam)
+    cpu_per_tile = (time.perf_counter() - t0) / len(pick)
+    return dict(api="generate_merged_pools + AdaptiveFitter", width=int(img.shape[1]), height=int(img.shape[0]),
+                quality=quality, tiles=int(tiles.n_tiles), tile_sizes={int(s): int((tiles.ts == s).sum())
+                                                                    for s in np.unique(tiles.ts)},
+                mean_pool=float(counts[:tiles.n_tiles].float().mean().item()), rounds=f.rounds_run,
+                primitives=int(nsel.sum()), psnr_db=10.0 * np.log10(1.0 / mse), wall_s=wall,
+                tiles_per_s=tiles.n_tiles / wall, cpu_numpy_s_per_tile=cpu_per_tile, cpu_numpy_tiles=len(pick),
+                speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / wall)
+
+
 def residual_candidates_report(tiles, d_px, d_off, pick, quality=75):
     """Residual pools (encoder._generate_residual_candidates) of every tile against a prediction.
 
@@ -816,7 +870,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES of device memory once (HBM counter calibration) and exit")
     ap.add_argument("--only", choices=["template_fit", "select", "candidates", "refined_round", "class_probe",
-                                       "drop_in", "decode", "rounds"],
+                                       "drop_in", "decode", "rounds", "adaptive"],
                     default=None,
                     help="run only that report (for profiling) and print it as JSON")
     args = ap.parse_args()
@@ -827,6 +881,8 @@ def main():
             rep = template_report(synth_image(2160, 3840, 2500, seed=0, texture=True))
         elif args.only == "candidates":
             rep = candidates_report(build_workload(args)[2])
+        elif args.only == "adaptive":
+            rep = adaptive_report(build_workload(args)[1])
         elif args.only == "drop_in":
             c, _, tiles, _ = build_workload(args)
             rep = drop_in_report(tiles, c["seed"])

@@ -243,3 +243,23 @@ def test_fit_on_merged_pools(gpu, oracle_lib):
         np.testing.assert_allclose(sse[t][gauss], osse[gauss], rtol=1e-5, atol=1e-9)
         if rec["win_acc"][0, t]:
             assert 0 <= rec["win_idx"][0, t] < cnt[t]
+    # every slot, Gaussian kinds included, bit for bit against the per-pixel stack walk of
+    # weft_score_evals (one-primitive stacks on the black canvas the fit starts from)
+    import torch
+    from weft_amd import _lib
+    from weft_amd.records import EVAL_DTYPE
+
+    tt, jj = np.nonzero(np.arange(M)[None, :] < cnt[:, None])
+    ev = np.zeros(len(tt), EVAL_DTYPE)
+    ev["prim_offset"] = tt * M + jj
+    ev["prim_count"] = 1
+    ev["tile_offset"] = 3 * tiles.pix_off[tt]
+    ev["tile_size"] = tiles.ts[tt]
+    d_ev = torch.from_numpy(ev.view(np.uint8)).cuda()
+    d_t = torch.from_numpy(tiles.target).cuda()
+    m1, s1 = (torch.empty(len(ev), dtype=torch.float32, device="cuda") for _ in range(2))
+    _lib.check(_lib.library().weft_score_evals(d_ev.data_ptr(), len(ev), recs.data_ptr(), d_t.data_ptr(),
+                                               m1.data_ptr(), s1.data_ptr(), _lib.stream_ptr()), "weft_score_evals")
+    kinds = host["kind"][tt, jj]
+    assert set(np.unique(kinds).tolist()) >= {1, 2, 3}
+    assert np.array_equal(sse[tt, jj].view(np.uint32), s1.cpu().numpy().view(np.uint32))

@@ -246,9 +246,9 @@ __device__ __forceinline__ void composite_row8(const WeftPrim& P, float x0, floa
         break;
     }
     case 3: {
-        float best[8];
+        float best[8];  // squared distances; 1e18 > RN(1e9)^2
 #pragma unroll
-        for (int i = 0; i < 8; ++i) best[i] = 1e9f;
+        for (int i = 0; i < 8; ++i) best[i] = 1e18f;
         float prevx = 0.0f, prevy = 0.0f;
         for (int k = 0; k < 32; ++k) {
             const float t = (k == 31) ? 1.0f : (float)((double)k * (1.0 / 31.0));
@@ -266,7 +266,7 @@ __device__ __forceinline__ void composite_row8(const WeftPrim& P, float x0, floa
                     const float tt2 = clip01(((px - prevx) * vx + ty) / den);
                     const float qx = prevx + tt2 * vx, qy = prevy + tt2 * vy;
                     const float dx = px - qx, dy = py - qy;
-                    best[i] = fminf(best[i], sqrtf(dx * dx + dy * dy));
+                    best[i] = fminf(best[i], dx * dx + dy * dy);
                 }
             }
             prevx = x;
@@ -275,7 +275,10 @@ __device__ __forceinline__ void composite_row8(const WeftPrim& P, float x0, floa
         const double sigma = (double)P.geom[6] > 0.5 ? (double)P.geom[6] : 0.5;
         const float k2 = (float)(1.0 / (2.0 * sigma * sigma));
 #pragma unroll
-        for (int i = 0; i < 8; ++i) al[i] = expf((-(best[i] * best[i])) * k2) * a;
+        for (int i = 0; i < 8; ++i) {
+            const float d = sqrtf(best[i]);  // = the minimum of the segments' sqrtf (monotone)
+            al[i] = expf((-(d * d)) * k2) * a;
+        }
         break;
     }
     case 4:
@@ -665,22 +668,37 @@ struct PxTri : PxLdsBase<TS> {
     }
 };
 
-// Any kind over the LDS canvas through the generic composite.
+// Any kind over the LDS canvas through the generic composite, 8 pixels of a row at a time
+// (composite_row8: the per-primitive and per-row terms, a curve's polyline among them,
+// formed once per group).
 template <int TS>
 struct PxGeneral {
     static constexpr bool kUnrollRow = false;
-    static constexpr bool kGroup = false;
+    static constexpr bool kGroup = true;
     LdsBand L;
-    int y0;
+    int y0, yr;
     WeftPrim P;
     float py;
-    __device__ __forceinline__ void row(int y) { py = (float)y + 0.5f; }
-    __device__ __forceinline__ void pixel(int y, int x, int, float pxf, float& e0, float& e1, float& e2) {
+    float cr[8], cg[8], cb[8];
+    __device__ __forceinline__ void row(int y) {
+        yr = y;
+        py = (float)y + 0.5f;
+    }
+    __device__ __forceinline__ void group(int gx, float) {
+        const int p0 = (yr - y0) * TS + gx;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float4 a = L.A[2 * (p0 + i)];
+            cr[i] = a.x;
+            cg[i] = a.y;
+            cb[i] = a.z;
+        }
+        composite_row8(P, (float)gx, py, cr, cg, cb);
+    }
+    __device__ __forceinline__ void pixel(int y, int x, int i, float, float& e0, float& e1, float& e2) {
         const int p = (y - y0) * TS + x;
         const float4 a = L.A[2 * p], b = L.A[2 * p + 1];
-        float r = a.x, g = a.y, bb = a.z;
-        composite_px(P, pxf, py, r, g, bb);
-        const float d0 = a.w - clip01(r), d1 = b.w - clip01(g), d2 = L.T[p] - clip01(bb);
+        const float d0 = a.w - clip01(cr[i]), d1 = b.w - clip01(cg[i]), d2 = L.T[p] - clip01(cb[i]);
         e0 = d0 * d0;
         e1 = d1 * d1;
         e2 = d2 * d2;

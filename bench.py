@@ -1050,6 +1050,11 @@ def main():
         log(f"[bench] candidate pools 1080p: {extras['candidates']['device_s'] * 1e3:.2f} ms for "
             f"{extras['candidates']['tiles']} tiles ({extras['candidates']['speedup_vs_cpu_numpy_one_core']:.0f}x "
             f"one CPU core); sample equal: {extras['candidates']['sample_equal_restatement']}")
+        extras["adaptive"] = adaptive_report(img)
+        log(f"[bench] adaptive fit phase 1080p: {extras['adaptive']['wall_s'] * 1e3:.1f} ms for "
+            f"{extras['adaptive']['tiles']} tiles (pools + {extras['adaptive']['rounds']} rounds, "
+            f"{extras['adaptive']['psnr_db']:.2f} dB; {extras['adaptive']['speedup_vs_cpu_numpy_one_core']:.0f}x "
+            f"one CPU core)")
         extras["select"] = select_report()
         log(f"[bench] select 4M: argmin {extras['select']['argmin']['gb_per_s']:.0f} GB/s, scan "
             f"{extras['select']['exclusive_scan']['gb_per_s']:.0f} GB/s, segmented top-k "

@@ -499,8 +499,11 @@ def candidates_report(tiles, quality=75, sample=48):
         same += int(cnt[t] == len(ref) and host[t, :cnt[t]].tobytes()
                     == candgen_np.to_records(ref, PRIM_DTYPE).tobytes())
     cpu_per_tile = (time.perf_counter() - t0) / len(pick)
+    # algorithmic HBM bytes: every tile's pixels read once (12 B per pixel), every slot written (60 B)
+    nbytes = 12 * int(np.sum(np.asarray(tiles.ts, np.int64) ** 2)) + 60 * MAX_CANDS * int(tiles.n_tiles)
     rep = dict(api="generate_pools", tiles=int(tiles.n_tiles), quality=quality, kernel="weft_gen_candidates_kernel",
                device_s=dev_s, tiles_per_s=tiles.n_tiles / dev_s, mean_pool=float(cnt.mean()),
+               algorithmic_bytes=nbytes, hbm_gb_per_s=nbytes / dev_s / 1e9, hbm_frac=nbytes / dev_s / 8e12,
                candidates_per_s=float(cnt.sum()) / dev_s, cpu_numpy_s_per_tile=cpu_per_tile,
                cpu_numpy_tiles=len(pick), speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / dev_s,
                sample_equal_restatement=f"{same}/{len(pick)}")
@@ -514,7 +517,8 @@ def edge_candidates_report(tiles, d_px, d_off, pick, quality=75):
     of encoder._worker_gen_cands (edge-driven + template, de-duplicated) for every tile."""
     import torch
     from oracle import candgen_np, edge_np
-    from weft_amd.candidates import EDGE_MAX_CANDS, MERGED_MAX_CANDS, generate_edge_pools, generate_merged_pools
+    from weft_amd.candidates import (EDGE_MAX_CANDS, MAX_CANDS, MERGED_MAX_CANDS, generate_edge_pools,
+                                     generate_merged_pools)
     from weft_amd.records import PRIM_DTYPE
 
     out = {}
@@ -543,8 +547,14 @@ def edge_candidates_report(tiles, d_px, d_off, pick, quality=75):
             same += int(cnt[t] == len(ref) and host[t, :cnt[t]].tobytes()
                         == candgen_np.to_records(ref, PRIM_DTYPE).tobytes())
         cpu_per_tile = (time.perf_counter() - t0) / len(pick)
+        px_bytes = 12 * int(np.sum(np.asarray(tiles.ts, np.int64) ** 2))
+        nbytes = px_bytes + 60 * cap * int(tiles.n_tiles)
+        if name == "merged":  # both generators' pixels, records and float64 values, read back by the merge
+            nbytes = 2 * px_bytes + 2 * (60 + 112) * (EDGE_MAX_CANDS + MAX_CANDS) * int(tiles.n_tiles) \
+                + 60 * cap * int(tiles.n_tiles)
         out[name] = dict(api=fn.__name__, device_s=dev_s, tiles_per_s=tiles.n_tiles / dev_s,
-                         mean_pool=float(cnt.mean()), cpu_numpy_s_per_tile=cpu_per_tile, cpu_numpy_tiles=len(pick),
+                         mean_pool=float(cnt.mean()), algorithmic_bytes=nbytes, hbm_gb_per_s=nbytes / dev_s / 1e9,
+                         hbm_frac=nbytes / dev_s / 8e12, cpu_numpy_s_per_tile=cpu_per_tile, cpu_numpy_tiles=len(pick),
                          speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / dev_s,
                          sample_equal_restatement=f"{same}/{len(pick)}")
     out["edge"]["kernel"] = "weft_gen_edge_candidates_kernel"
@@ -645,8 +655,10 @@ def residual_candidates_report(tiles, d_px, d_off, pick, quality=75):
         same += int(cnt[t] == len(ref) and host[t, :cnt[t]].tobytes()
                     == candgen_np.to_records(ref, PRIM_DTYPE).tobytes())
     cpu_per_tile = (time.perf_counter() - t0) / len(pick)
+    nbytes = 24 * int(np.sum(np.asarray(tiles.ts, np.int64) ** 2)) + 60 * RES_MAX_CANDS * int(tiles.n_tiles)
     return dict(api="generate_residual_pools", kernel="weft_gen_residual_candidates_kernel", device_s=dev_s,
-                tiles_per_s=tiles.n_tiles / dev_s, mean_pool=float(cnt.mean()),
+                tiles_per_s=tiles.n_tiles / dev_s, mean_pool=float(cnt.mean()), algorithmic_bytes=nbytes,
+                hbm_gb_per_s=nbytes / dev_s / 1e9, hbm_frac=nbytes / dev_s / 8e12,
                 cpu_numpy_s_per_tile=cpu_per_tile, cpu_numpy_tiles=len(pick),
                 speedup_vs_cpu_numpy_one_core=cpu_per_tile * tiles.n_tiles / dev_s,
                 sample_equal_restatement=f"{same}/{len(pick)}")

@@ -263,3 +263,25 @@ def test_fit_on_merged_pools(gpu, oracle_lib):
     kinds = host["kind"][tt, jj]
     assert set(np.unique(kinds).tolist()) >= {1, 2, 3}
     assert np.array_equal(sse[tt, jj].view(np.uint32), s1.cpu().numpy().view(np.uint32))
+
+
+def test_edge_and_merge_argument_errors(gpu):
+    """Sizes the kernels cannot take are refused with WEFT_E_INVALID / WEFT_E_UNSUPPORTED_TILE
+    and a message, before any launch."""
+    import torch
+    from weft_amd import _lib
+
+    L = _lib.library()
+    d = torch.zeros(1 << 16, dtype=torch.uint8, device="cuda")
+    p = d.data_ptr()
+    assert L.weft_gen_edge_candidates(p, p, p, 1, 12, 75, p, None, p, 114, None) == -1002
+    assert L.weft_gen_edge_candidates(p, p, p, 1, 8, 75, p, None, p, 113, None) == -1001
+    assert b"WEFT_GEN_EDGE_MAX_CANDS" in L.weft_last_error()
+    assert L.weft_gen_edge_candidates(p, p, p, -1, 8, 75, p, None, p, 114, None) == -1001
+    assert L.weft_gen_edge_candidates(None, p, p, 1, 8, 75, p, None, p, 114, None) == -1001
+    assert L.weft_gen_edge_candidates(None, None, None, 0, 8, 75, None, None, None, 114, None) == 0
+    assert L.weft_merge_pools(p, p, p, 200, p, p, p, 352, 1, p, None, p, 552, None) == -1001
+    assert b"WEFT_MERGE_MAX_CANDS" in L.weft_last_error()
+    assert L.weft_merge_pools(p, p, p, 114, p, p, p, 352, 1, p, None, p, 465, None) == -1001
+    assert L.weft_merge_pools(p, None, p, 114, p, p, p, 352, 1, p, None, p, 466, None) == -1001
+    assert L.weft_merge_pools(p, p, p, -1, p, p, p, 352, 1, p, None, p, 466, None) == -1001

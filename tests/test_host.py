@@ -47,6 +47,8 @@ def test_entry_points_fail_cleanly_without_init():
         pytest.skip("a GPU is visible; the no-device path is exercised on CPU hosts")
     assert L.weft_score_evals(None, 1, None, None, None, None, None) == -1000
     assert b"weft_init" in L.weft_last_error()
+    assert L.weft_gen_edge_candidates(None, None, None, 1, 8, 75, None, None, None, 114, None) == -1000
+    assert L.weft_merge_pools(None, None, None, 114, None, None, None, 352, 1, None, None, None, 466, None) == -1000
 
 
 def test_lds_request_checked_against_the_limit():

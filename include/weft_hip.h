@@ -595,7 +595,8 @@ int weft_gen_residual_candidates(const float* d_pixels, const float* d_pred, int
  * atan(1) = 0.7853981.  The line and curve sections, reached only when the 60th percentile of
  * the magnitudes is at least 1e-6, are not built: a tile with fewer than
  * floor(0.6 (n - 1)) + 2 magnitudes below 9.99e-7 (values far outside [0, 1]) gets the count
- * -1 and no pool. */
+ * -1 and no pool.  Pixels must be finite: the reference's merged pool raises on NaN, and NaN
+ * distances are not ranked as np.argmin ranks them. */
 #define WEFT_GEN_EDGE_MAX_CANDS 114
 int weft_gen_edge_candidates(const float* d_pixels, const int64_t* d_tile_pix, const int32_t* d_tiles, int n_tiles,
                              int tile_size, int quality, WeftPrim* d_out, double* d_out_f64, int32_t* d_out_count,

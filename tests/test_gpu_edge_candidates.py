@@ -285,3 +285,17 @@ def test_edge_and_merge_argument_errors(gpu):
     assert L.weft_merge_pools(p, p, p, 114, p, p, p, 352, 1, p, None, p, 465, None) == -1001
     assert L.weft_merge_pools(p, None, p, 114, p, p, p, 352, 1, p, None, p, 466, None) == -1001
     assert L.weft_merge_pools(p, p, p, -1, p, p, p, 352, 1, p, None, p, 466, None) == -1001
+
+
+def test_empty_tile_set(gpu):
+    """No tiles: no launch, no error, empty pools."""
+    from weft_amd.candidates import generate_edge_pools, generate_merged_pools
+
+    class _Empty:
+        ts = np.zeros(0, np.int32)
+        target = np.zeros(0, np.float32)
+        pix_off = np.zeros(0, np.int64)
+
+    for fn in (generate_edge_pools, generate_merged_pools):
+        _, counts, _ = fn(_Empty(), 75)
+        assert int(counts.sum().item()) == 0

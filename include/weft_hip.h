@@ -112,7 +112,16 @@ typedef struct WeftFitState {
 } WeftFitState;
 
 /* Work list for one tile-size class: work item i scores candidates
- * [d_work_cand0[i], d_work_cand0[i] + block) of tile d_work_tile[i]. */
+ * [d_work_cand0[i], d_work_cand0[i] + block) of tile d_work_tile[i].
+ *
+ * Split items (64-px tiles only; n_split > 0): a 64-px tile's candidates may be scored by two
+ * work items, rows 0-31 and rows 32-63.  Their 128-element pairwise leaves split exactly there
+ * (48 + 48 leaves), so each half's partial subtree sums (two floats per candidate, into
+ * d_split_sse) combine into the single-pass SSE bit for bit; a combine launch after scoring
+ * forms the SSE, the tables and the work item's first minimum.  d_work_split[i] is -1 for an
+ * unsplit item, else 2 * pair + half; d_split_pairs[2 * pair + half] is the pair's work items;
+ * d_split_sse holds n_split * 2 * 2 * block floats.  Few candidates per 64-px tile leave few
+ * wavefronts for them, each walking 4096 pixels alone: the split halves that walk. */
 typedef struct WeftWork {
     const int32_t* d_work_tile;
     const int32_t* d_work_cand0;
@@ -120,7 +129,10 @@ typedef struct WeftWork {
     int32_t tile_size;       /* 8, 16, 32 or 64: every tile in the list has this size;
                                 0: tiles of any supported size (one launch for all) */
     int32_t block;           /* threads per workgroup (= candidates per work item), 64..1024 */
-    int32_t _pad;
+    int32_t n_split;         /* split pairs (0: none; the fields below may then be NULL) */
+    const int32_t* d_work_split;
+    const int32_t* d_split_pairs;
+    float* d_split_sse;
 } WeftWork;
 
 /* Candidate source: explicit records or drawn on the device from (seed, step).
@@ -286,7 +298,8 @@ int weft_adaptive_append(const WeftFitState* st, const WeftAdaptive* a, WeftPrim
  * Every call on a device shares that device's one array: calls whose launches overlap
  * on one device (other streams, other threads) race and must be serialised by the
  * caller.  Nothing in the array has to start at zero (every entry a call reads, it wrote
- * first in the same call). */
+ * first in the same call).  Work lists with split 64-px items (n_split > 0) are refused with
+ * WEFT_E_INVALID: their partial sums would be scratch too. */
 #define WEFT_STATIC_SCRATCH_BYTES (16u << 20)
 int weft_fit_step_static(const WeftFitState* st, const WeftWork* work, const WeftCands* cands,
                          const int32_t* d_tile_work0, const int32_t* d_tile_nwork,

@@ -470,3 +470,44 @@ def test_fit_reset_equals_fill_and_init(gpu):
     assert (f.canvas() == 0).all()
     assert np.array_equal(got["nsel"], ref["nsel"]) and np.array_equal(got["bits"], ref["bits"])
     assert bits_equal(got["obj"], ref["obj"])
+
+
+@pytest.mark.parametrize("n_cand,explicit", [(256, False), (128, False), (96, True)])
+def test_split64_equals_unsplit(gpu, n_cand, explicit):
+    """64-px tiles scored as two row halves (split64) give the single pass's SSE tables,
+    winners, objectives, codes and canvases bit for bit, over several rounds, with drawn and
+    with explicit candidates (of every kind, the generic class included)."""
+    import torch
+    from weft_amd.fit import PlainFitter
+    from weft_amd.quadtree import TileSet, decompose_quadtree_variance
+    from weft_amd.records import PRIM_DTYPE
+    from weft_amd.synth import synth_image
+
+    img = synth_image(256, 384, 15, seed=4)
+    tiles = TileSet.from_images([img], [decompose_quadtree_variance(img, threshold=1e-3, min_tile=8, max_tile=64)])
+    assert (tiles.ts == 64).sum() >= 10 and len(set(tiles.ts.tolist())) == 4
+    prims = None
+    if explicit:
+        rng = np.random.default_rng(5)
+        d = load_golden("ref_render_tile.npz")
+        pool = np.concatenate([golden_prims(d, i) for i in range(len(d["tile_size"])) if not d["has_residual"][i]])
+        prims = pool[rng.integers(0, len(pool), tiles.n_tiles * n_cand)].astype(PRIM_DTYPE)
+    runs = []
+    for split in (False, True):
+        _say(f"split64 {split}: {tiles.n_tiles} tiles, {n_cand} candidates, explicit {explicit}")
+        f = PlainFitter(tiles, n_cand=n_cand, depth=6, seed=3, cand_prims=prims, keep_sse=True, split64=split)
+        assert (f.n_split > 0) == split
+        tabs = []
+        for _ in range(3):
+            f.step(1)
+            tabs.append(f.sse_table().copy())
+        torch.cuda.synchronize()
+        runs.append((tabs, f.records(), f.canvas(), f.state()))
+    (t0, r0, c0, s0), (t1, r1, c1, s1) = runs
+    for a, b in zip(t0, t1):
+        assert bits_equal(a, b)
+    for k in r0:
+        assert bits_equal(r0[k], r1[k]), k
+    assert bits_equal(c0, c1)
+    for k in s0:
+        assert bits_equal(s0[k], s1[k]), k

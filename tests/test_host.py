@@ -581,3 +581,35 @@ def test_pack_extension_equal_and_copy():
     b[0] += 1.0
     assert not pk.equal(a, b, 8)
     assert not pk.equal(a, b[:-1], 8)
+
+
+def test_split64_work_lists():
+    """Split work lists: every 64-px item becomes a pair of halves on the same XCD (8 items
+    apart in a group of 8 tiles), the pair table and the per-item codes agree, the 64-px class
+    starts the list, and other sizes are untouched."""
+    from weft_amd.fit import DEFAULT_BLOCK, build_work_lists, split64_helps
+
+    rng = np.random.default_rng(0)
+    ts = rng.choice([8, 16, 32, 64], 300).astype(np.int32)
+    plain = build_work_lists(ts, 256, dict(DEFAULT_BLOCK))
+    wl = build_work_lists(ts, 256, dict(DEFAULT_BLOCK), split64=True)
+    assert (plain["work_split"] == -1).all() and plain["split_pairs"].shape == (0, 2)
+    assert wl["classes"][0]["ts"] == 64 and wl["classes"][0]["start"] == 0
+    n64 = int((ts == 64).sum())
+    per = (256 + 255) // 256
+    assert len(wl["split_pairs"]) == n64 * per
+    assert (wl["tile_nwork"][ts == 64] == 2 * plain["tile_nwork"][ts == 64]).all()
+    assert (wl["tile_nwork"][ts != 64] == plain["tile_nwork"][ts != 64]).all()
+    for p, (a, b) in enumerate(wl["split_pairs"]):
+        assert wl["work_split"][a] == 2 * p and wl["work_split"][b] == 2 * p + 1
+        assert wl["work_tile"][a] == wl["work_tile"][b] and wl["work_c0"][a] == wl["work_c0"][b]
+        assert (b - a) % 8 == 0 or b - a == wl["tile_wstride"][wl["work_tile"][a]]
+    covered = set()
+    for t in np.nonzero(ts == 64)[0]:
+        items = [wl["tile_work0"][t] + j * wl["tile_wstride"][t] for j in range(wl["tile_nwork"][t])]
+        assert all(wl["work_tile"][i] == t for i in items)
+        assert sorted(wl["work_split"][items] % 2) == [0, 1] * per or sorted(wl["work_split"][items] % 2) == [0] * per + [1] * per
+        covered.update(items)
+    assert (wl["work_split"][wl["classes"][0]["n"]:] == -1).all()
+    assert split64_helps(ts, 256, DEFAULT_BLOCK) and not split64_helps(ts[ts != 64], 256, DEFAULT_BLOCK)
+    assert not split64_helps(np.full(128, 64), 1024, DEFAULT_BLOCK)

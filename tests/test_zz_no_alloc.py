@@ -35,7 +35,8 @@ def _twin(explicit: bool, h: int, w: int, n_cand: int, seed: int):
     if explicit:
         ids = np.arange(tiles.n_tiles, dtype=np.int32)
         _, cand = draw_codes(ids, tiles.ts, n_cand, seed=seed + 100, step=0)
-    kw = dict(n_cand=n_cand, depth=6, seed=seed, lam=tile_lambda(75), max_steps=4, cand_prims=cand)
+    # the static step takes unsplit work lists only (split partial sums would be scratch too)
+    kw = dict(n_cand=n_cand, depth=6, seed=seed, lam=tile_lambda(75), max_steps=4, cand_prims=cand, split64=False)
     return PlainFitter(tiles, **kw), PlainFitter(tiles, **kw)
 
 

@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 20
+ABI_VERSION = 21
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -45,7 +45,8 @@ class FitState(ctypes.Structure):
 class Work(ctypes.Structure):
     _fields_ = [
         ("d_work_tile", c_void_p), ("d_work_cand0", c_void_p), ("n_work", c_int32),
-        ("tile_size", c_int32), ("block", c_int32), ("_pad", c_int32),
+        ("tile_size", c_int32), ("block", c_int32), ("n_split", c_int32), ("d_work_split", c_void_p),
+        ("d_split_pairs", c_void_p), ("d_split_sse", c_void_p),
     ]
 
 
@@ -112,7 +113,7 @@ class SplitWork(ctypes.Structure):
 assert ctypes.sizeof(SplitWork) == 56
 assert ctypes.sizeof(SoftBatch) == 56 and ctypes.sizeof(RefineOpts) == 112
 assert ctypes.sizeof(Batch) == 128
-assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 32 and ctypes.sizeof(Cands) == 56
+assert ctypes.sizeof(FitState) == 80 and ctypes.sizeof(Work) == 56 and ctypes.sizeof(Cands) == 56
 assert ctypes.sizeof(Adaptive) == 64
 
 _lock = threading.Lock()

@@ -33,6 +33,7 @@ struct Module {
     hipFunction_t f_score_gen[4] = {nullptr, nullptr, nullptr, nullptr};
     hipFunction_t f_score_mixed = nullptr;
     hipFunction_t f_score_gen_mixed = nullptr;
+    hipFunction_t f_score_combine = nullptr;
     hipFunction_t f_order = nullptr;
     hipFunction_t f_select = nullptr;
     hipFunction_t f_draw = nullptr;
@@ -181,7 +182,7 @@ int weft_check_lds(int64_t bytes) { return lds_check(bytes, "weft_check_lds"); }
 
 // The Python binding (weft_amd/_lib.py) mirrors these layouts.
 static_assert(sizeof(WeftFitState) == 80, "WeftFitState layout");
-static_assert(sizeof(WeftWork) == 32, "WeftWork layout");
+static_assert(sizeof(WeftWork) == 56, "WeftWork layout");
 static_assert(sizeof(WeftCands) == 56, "WeftCands layout");
 static_assert(sizeof(WeftAdaptive) == 64, "WeftAdaptive layout");
 static_assert(sizeof(WeftBatch) == 128, "WeftBatch layout");
@@ -220,6 +221,7 @@ int weft_init(int device) {
         {&g_mod.f_score_gen[3], "weft_fit_score_gen_ts64"},
         {&g_mod.f_score_mixed, "weft_fit_score_mixed"},
         {&g_mod.f_score_gen_mixed, "weft_fit_score_gen_mixed"},
+        {&g_mod.f_score_combine, "weft_fit_score_combine_kernel"},
         {&g_mod.f_order, "weft_fit_order_kernel"},
         {&g_mod.f_select, "weft_fit_select_kernel"},
         {&g_mod.f_draw, "weft_draw_kernel"},
@@ -386,12 +388,25 @@ int weft_fit_score(const WeftFitState* st, const WeftWork* work, const WeftCands
     if (work->block < 64 || work->block > WEFT_MAX_BLOCK || (work->block % 64) != 0)
         return set_err(WEFT_E_INVALID, "weft_fit_score: block must be a multiple of 64 in [64, 1024]");
     if (cands->n_cand < 1) return set_err(WEFT_E_INVALID, "weft_fit_score: n_cand < 1");
+    if (work->n_split < 0 || work->n_split > work->n_work)
+        return set_err(WEFT_E_INVALID, "weft_fit_score: n_split must be in [0, n_work]");
+    if (work->n_split > 0) {
+        if (!work->d_work_split || !work->d_split_pairs || !work->d_split_sse)
+            return set_err(WEFT_E_INVALID, "weft_fit_score: split items need d_work_split, d_split_pairs and d_split_sse");
+        if (work->tile_size != 0 && work->tile_size != 64)
+            return set_err(WEFT_E_INVALID, "weft_fit_score: only 64-px tiles are split");
+    }
     if (work->n_work <= 0) return WEFT_OK;
     WeftFitScoreArgs a{*st, *work, *cands, d_order, d_out_sse, d_out_mse, d_part_obj, d_part_idx};
+    if (work->n_split == 0) a.wk.d_work_split = nullptr;
     hipFunction_t f = slot == 4 ? (cands->has_general ? g_mod.f_score_gen_mixed : g_mod.f_score_mixed)
                                 : (cands->has_general ? g_mod.f_score_gen[slot] : g_mod.f_score[slot]);
-    return launch(f, (unsigned)work->n_work, (unsigned)work->block,
-                  score_lds_bytes(work->tile_size), stream, &a);
+    if ((rc = launch(f, (unsigned)work->n_work, (unsigned)work->block, score_lds_bytes(work->tile_size), stream,
+                     &a)))
+        return rc;
+    if (work->n_split == 0) return WEFT_OK;
+    // the split pairs' partial sums -> SSE, tables and each pair's first minimum
+    return launch(g_mod.f_score_combine, (unsigned)work->n_split, (unsigned)work->block, 0, stream, &a);
 }
 
 int weft_fit_select(const WeftFitState* st, const WeftCands* cands, const int32_t* d_tile_work0,
@@ -445,6 +460,8 @@ int weft_fit_step_static(const WeftFitState* st, const WeftWork* work, const Wef
     if (!st || !work || !cands) return set_err(WEFT_E_INVALID, "weft_fit_step_static: null argument");
     if (st->n_tiles < 0 || work->n_work < 0 || cands->n_cand < 1)
         return set_err(WEFT_E_INVALID, "weft_fit_step_static: bad sizes");
+    if (work->n_split != 0)
+        return set_err(WEFT_E_INVALID, "weft_fit_step_static: work lists with split items are not supported");
     // Scratch layout: [class order, explicit candidates only][partial objectives][partial indices].
     const bool explicit_cands = cands->d_prims != nullptr;
     const uint64_t order_bytes =

@@ -1061,11 +1061,23 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
     const float* Cv = st.d_canvas + base;
     float* LAf = reinterpret_cast<float*>(LA);
     const LdsBand L{LA, LT};
+    // A split item of a 64-px tile scores rows [32 half, 32 half + 32) only: leaves 0-47 or
+    // 48-95 of the tile's 96 pairwise leaves (see the combine kernel).
+    int y_lo = 0, y_hi = TS, split = -1;
+    if constexpr (TS == 64) {
+        if (A.wk.d_work_split) {
+            split = A.wk.d_work_split[w];
+            if (split >= 0) {
+                y_lo = 32 * (split & 1);
+                y_hi = y_lo + 32;
+            }
+        }
+    }
     PwState pw;
     pw_init(pw);
 #pragma unroll 1
-    for (int y0 = 0; y0 < TS; y0 += ROWS) {
-        if (y0 > 0) __syncthreads();  // every wave is done with the previous band
+    for (int y0 = y_lo; y0 < y_hi; y0 += ROWS) {
+        if (y0 > y_lo) __syncthreads();  // every wave is done with the previous band
         const int e0 = 3 * y0 * TS;
         for (int i = tid; i < 3 * BPX; i += nthr) {
             const int p = i / 3, ch = i - 3 * (i / 3);
@@ -1078,6 +1090,20 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
         }
         __syncthreads();
         score_band<TS, GEN>(mycls, cc, P, L, pw, y0, y0 + ROWS);
+    }
+    if constexpr (TS == 64) {
+        if (split >= 0) {
+            // rows 0-31: subtrees [0..31] (stack level 5) and [32..47] (level 4); rows 32-63,
+            // from a zero stack: [48..63] (level 6) and the 32-leaf rest chunk (level 5)
+            const int half = split & 1, pair = split >> 1;
+            const float2 v = half == 0 ? make_float2(pw.s[5], pw.s[4]) : make_float2(pw.s[6], pw.s[5]);
+            reinterpret_cast<float2*>(A.wk.d_split_sse)[((int64_t)pair * 2 + half) * nthr + tid] = v;
+            if (tid == 0) {
+                A.part_obj[w] = __builtin_huge_val();
+                A.part_idx[w] = -1;
+            }
+            return;
+        }
     }
     const float sse = pw.template result<TS>();
 
@@ -1109,6 +1135,69 @@ __device__ void fit_score_body(const WeftFitScoreArgs& A) {
         for (int ww = 1; ww < nwaves; ++ww) argmin_pair(bo, bi, red_o[ww], red_i[ww]);
         A.part_obj[w] = bo;
         A.part_idx[w] = bi == 0x7fffffff ? -1 : bi;
+    }
+}
+
+// Split pairs of 64-px tiles (one workgroup of the scoring block size per pair): each
+// candidate's SSE from the two halves' partial sums, exactly as the single pass forms it --
+// chunk A = [0..31] + ([32..47] + [48..63]), then chunk A + the rest chunk (NumPy's 8192-element
+// buffer, then the remaining 4096) -- then the tables, the objective and the pair's first
+// minimum, written to the first half's partial slot (the second half's holds +inf).
+__device__ void fit_score_combine_body(const WeftFitScoreArgs& A) {
+    __shared__ double red_o[16];
+    __shared__ int red_i[16];
+    constexpr int TS = 64;
+    const int sp = blockIdx.x;
+    const int tid = threadIdx.x, nthr = blockDim.x, wave = tid >> 6, nwaves = nthr >> 6;
+    const int w0 = A.wk.d_split_pairs[2 * sp];
+    const int tile = A.wk.d_work_tile[w0];
+    const int c0 = A.wk.d_work_cand0[w0];
+    const WeftFitState& st = A.st;
+    const int ncand = tile_ncand(A.cd, tile);
+    if (tile_skipped(st, A.cd, tile) || c0 >= ncand) return;  // both halves wrote +inf
+    const bool drawn = A.cd.d_prims == nullptr;
+    const uint64_t dbase = draw_base(A.cd.seed, A.cd.step, (uint32_t)tile);
+    const int64_t cand_base = drawn ? 0
+                              : (A.cd.d_tile_cand ? A.cd.d_tile_cand[tile]
+                                                  : (int64_t)tile * A.cd.n_cand);
+    const int slot = c0 + tid;
+    const int ks = slot >= ncand ? -1
+                   : (drawn ? drawn_sorted_index(dbase, A.cd.n_cand, slot)
+                            : A.order[(int64_t)tile * A.cd.n_cand + slot]);
+    double obj = __builtin_huge_val();
+    int idx = 0x7fffffff;
+    if (ks >= 0) {
+        const int kind = drawn ? code_to_prim(draw_code(dbase, (uint32_t)ks, TS)).kind
+                               : A.cd.d_prims[cand_base + ks].kind;
+        const float2* part = reinterpret_cast<const float2*>(A.wk.d_split_sse);
+        const float2 h0 = part[((int64_t)sp * 2 + 0) * nthr + tid];
+        const float2 h1 = part[((int64_t)sp * 2 + 1) * nthr + tid];
+        const float chunk_a = h0.x + (h0.y + h1.x);
+        const float sse = chunk_a + h1.y;
+        const float mse = sse / (float)(3 * TS * TS);
+        const int64_t ti = (int64_t)tile * A.cd.n_cand + ks;
+        if (A.out_sse) A.out_sse[ti] = sse;
+        if (A.out_mse) A.out_mse[ti] = mse;
+        obj = objective(mse, st.d_tile_bits[tile] + prim_bytes(kind) * 8, st.lam, st.obj_mode);
+        idx = ks;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o2 = __shfl_xor(obj, off, 64);
+        const int i2 = __shfl_xor(idx, off, 64);
+        argmin_pair(obj, idx, o2, i2);
+    }
+    if ((tid & 63) == 0) {
+        red_o[wave] = obj;
+        red_i[wave] = idx;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double bo = red_o[0];
+        int bi = red_i[0];
+        for (int ww = 1; ww < nwaves; ++ww) argmin_pair(bo, bi, red_o[ww], red_i[ww]);
+        A.part_obj[w0] = bo;
+        A.part_idx[w0] = bi == 0x7fffffff ? -1 : bi;
     }
 }
 
@@ -1472,6 +1561,9 @@ __device__ __forceinline__ void fit_score_mixed(const WeftFitScoreArgs& A) {
 }
 extern "C" __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void weft_fit_score_mixed(WeftFitScoreArgs A) { fit_score_mixed<false>(A); }
 extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_mixed(WeftFitScoreArgs A) { fit_score_mixed<true>(A); }
+extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_combine_kernel(WeftFitScoreArgs A) {
+    fit_score_combine_body(A);
+}
 
 // Variants that also accept kinds 1-3 (linear gradient, Gaussian line, Gaussian curve).
 extern "C" __global__ __launch_bounds__(1024) void weft_fit_score_gen_ts8(WeftFitScoreArgs A) { fit_score_body<8, true>(A); }

@@ -35,7 +35,7 @@ def _round64(n: int) -> int:
     return max(64, (n + 63) // 64 * 64)
 
 
-def build_work_lists(tile_ts: np.ndarray, n_cand: int, blocks: dict[int, int]) -> dict:
+def build_work_lists(tile_ts: np.ndarray, n_cand: int, blocks: dict[int, int], split64: bool = False) -> dict:
     """Work items (tile, first candidate slot) for the scoring launches.
 
     Tile-size classes in descending size (the long work items start first); within a
@@ -43,6 +43,11 @@ def build_work_lists(tile_ts: np.ndarray, n_cand: int, blocks: dict[int, int]) -
     the items of one tile are XCDS workgroups apart and run on the same XCD (workgroup i
     is dispatched to XCD i mod 8; each XCD has its own L2, so the tile is read from HBM
     once).  Item j of tile t is work_tile/work_c0[tile_work0[t] + j * tile_wstride[t]].
+
+    With split64, every work item of a 64-px tile becomes two, rows 0-31 and rows 32-63
+    (``work_split``: -1, or 2 * pair + half; ``split_pairs[pair]``: the two items); the
+    scoring launch combines each pair exactly.  The 64-px class comes first, so its item
+    indices are the same relative to the whole list and to its own class launch.
     """
     tile_ts = np.asarray(tile_ts)
     T = len(tile_ts)
@@ -51,32 +56,57 @@ def build_work_lists(tile_ts: np.ndarray, n_cand: int, blocks: dict[int, int]) -
     tile_work0 = np.zeros(T, np.int32)
     tile_nwork = np.zeros(T, np.int32)
     tile_wstride = np.ones(T, np.int32)
+    work_split, split_pairs = [], []
     for ts in sorted(set(tile_ts.tolist()), reverse=True):
         blk = min(blocks[ts], _round64(n_cand))
         if blk % 64 or not 64 <= blk <= 1024:
             raise ValueError(f"block for ts={ts} must be a multiple of 64 in [64, 1024]")
         ids = np.nonzero(tile_ts == ts)[0].astype(np.int32)
         per = (n_cand + blk - 1) // blk
+        split = bool(split64) and ts == 64
         start = len(work_tile)
         for g0 in range(0, len(ids), XCDS):
             grp = ids[g0:g0 + XCDS]
             tile_wstride[grp] = len(grp)
             tile_work0[grp] = len(work_tile) + np.arange(len(grp))
             for j in range(per):
-                work_tile.extend(grp.tolist())
-                work_c0.extend([j * blk] * len(grp))
-        tile_nwork[ids] = per
+                for half in ((0, 1) if split else (-1,)):
+                    first = len(work_tile)
+                    work_tile.extend(grp.tolist())
+                    work_c0.extend([j * blk] * len(grp))
+                    if half < 0:
+                        work_split.extend([-1] * len(grp))
+                    elif half == 0:
+                        pair0 = len(split_pairs)
+                        split_pairs.extend([first + q, -1] for q in range(len(grp)))
+                        work_split.extend(2 * (pair0 + q) for q in range(len(grp)))
+                    else:
+                        for q in range(len(grp)):
+                            split_pairs[pair0 + q][1] = first + q
+                        work_split.extend(2 * (pair0 + q) + 1 for q in range(len(grp)))
+        tile_nwork[ids] = per * (2 if split else 1)
         classes.append(dict(ts=int(ts), block=blk, start=start, n=len(work_tile) - start))
     return dict(classes=classes, work_tile=np.array(work_tile, np.int32),
                 work_c0=np.array(work_c0, np.int32), tile_work0=tile_work0, tile_nwork=tile_nwork,
-                tile_wstride=tile_wstride)
+                tile_wstride=tile_wstride, work_split=np.array(work_split, np.int32),
+                split_pairs=np.array(split_pairs, np.int32).reshape(-1, 2))
+
+
+def split64_helps(tile_ts: np.ndarray, n_cand: int, blocks: dict[int, int], simds: int = 1024) -> bool:
+    """Whether to split 64-px tiles' work items: when their wavefronts (64 candidates each)
+    number at most 1.5 per SIMD, each walks its tile's 4096 pixels with little else to hide
+    its latency, and the class sets the launch's length (measured on MI355X: 1080p quadtree
+    at 256 candidates 437 -> 319 us, at 128 432 -> 236 us; 4K at 128 750 -> 726 us; no gain
+    from 1024 candidates at 1080p or 256 at 4K on)."""
+    n64 = int(np.sum(np.asarray(tile_ts) == 64))
+    return n64 > 0 and 2 * n64 * ((int(n_cand) + 63) // 64) <= 3 * simds
 
 
 class PlainFitter:
     def __init__(self, tiles: TileSet, n_cand: int, depth: int, seed: int = 0,
                  lam: float | None = None, obj_mode: int = 0, max_steps: int = 64,
                  blocks: dict[int, int] | None = None, cand_prims=None,
-                 keep_sse: bool = False, device: int = 0, merged: bool = True):
+                 keep_sse: bool = False, device: int = 0, merged: bool = True, split64="auto"):
         import torch
 
         _lib.ensure_init(device)
@@ -126,7 +156,9 @@ class PlainFitter:
         self.blocks = dict(DEFAULT_BLOCK)
         if blocks:
             self.blocks.update(blocks)
-        wl = build_work_lists(tiles.ts, self.n_cand, self.blocks)
+        if split64 == "auto":
+            split64 = split64_helps(tiles.ts, self.n_cand, self.blocks)
+        wl = build_work_lists(tiles.ts, self.n_cand, self.blocks, split64=bool(split64))
         self.classes = wl["classes"]
         work_tile, work_c0 = wl["work_tile"], wl["work_c0"]
         tile_work0, tile_nwork, tile_wstride = wl["tile_work0"], wl["tile_nwork"], wl["tile_wstride"]
@@ -141,6 +173,14 @@ class PlainFitter:
                                    device=dev)
         self.d_part_obj = torch.empty(max(1, self.n_work), dtype=torch.float64, device=dev)
         self.d_part_idx = torch.empty(max(1, self.n_work), dtype=torch.int32, device=dev)
+        # split 64-px items: which pair and half each item is, the pairs, their partial sums
+        self.n_split = len(wl["split_pairs"])
+        self.d_work_split = torch.from_numpy(wl["work_split"]).to(dev) if self.n_split else None
+        self.d_split_pairs = torch.from_numpy(np.ascontiguousarray(wl["split_pairs"]).reshape(-1)).to(dev) \
+            if self.n_split else None
+        blk64 = next((c["block"] for c in self.classes if c["ts"] == 64), 64)
+        self.d_split_sse = torch.empty(max(1, self.n_split * 4 * blk64), dtype=torch.float32, device=dev) \
+            if self.n_split else None
 
         S = self.max_steps
         self.d_win_idx = torch.full((S, T), -1, dtype=torch.int32, device=dev)
@@ -159,14 +199,18 @@ class PlainFitter:
         # which needs one block size for all classes), or one launch per size class.
         self.merged = merged and len({c["block"] for c in self.classes}) == 1
         self._works = []
+        split = (self.n_split, self.d_work_split.data_ptr(), self.d_split_pairs.data_ptr(),
+                 self.d_split_sse.data_ptr()) if self.n_split else (0, None, None, None)
         if self.merged:
             blk = self.classes[0]["block"] if self.classes else 64
             self._works.append((_lib.Work(self.d_work_tile.data_ptr(), self.d_work_c0.data_ptr(),
-                                          self.n_work, 0, blk, 0), 0))
+                                          self.n_work, 0, blk, *split), 0))
         else:
             for c in self.classes:
+                # the 64-px class starts the list, so its split indices need no offset
+                sp = split if c["ts"] == 64 else (0, None, None, None)
                 w = _lib.Work(self.d_work_tile.data_ptr() + 4 * c["start"],
-                              self.d_work_c0.data_ptr() + 4 * c["start"], c["n"], c["ts"], c["block"], 0)
+                              self.d_work_c0.data_ptr() + 4 * c["start"], c["n"], c["ts"], c["block"], *sp)
                 self._works.append((w, c["start"]))
         self.L = _lib.library()
         _lib.check(self.L.weft_fit_init(ctypes.byref(self._state), _lib.stream_ptr()), "weft_fit_init")

@@ -8,7 +8,7 @@ objective (lambda of quality 75).  A step is one greedy round over every tile: s
 candidates against the cached canvases (one launch per tile-size class), take the first
 minimum, accept and composite (one launch).  Inputs are resident in HBM before timing.
 
-Prints one JSON line (see the driver contract): value = candidate*pixel evaluations per
+Prints one JSON line (metric, value, unit, n_gpus, steps, warmup, ms_per_step, ...): value = candidate*pixel evaluations per
 second over the K timed steps; roofline = the dominant scoring kernel's algorithmic FP32
 rate (FLOP per evaluation x evaluations per launch / mean launch time measured with HIP
 events on the launch stream) against MI355X's 157.3 TFLOP/s FP32 vector peak;
@@ -37,7 +37,13 @@ Further objects on the same line (skipped with --no-extras):
 - candidates: the template candidate pools (encoder._generate_candidates) of every tile
   of the 1920x1080 quadtree on the device, and the NumPy restatement on one CPU core for a
   sample of the same tiles; under "residual" the same for the residual pools
-  (encoder._generate_residual_candidates) against a seeded noisy prediction.
+  (encoder._generate_residual_candidates) against a seeded noisy prediction, and under
+  "edge" for the edge-driven pools (edge_analysis.generate_edge_driven_candidates) and the
+  merged pools of encoder._worker_gen_cands.
+- adaptive: the reference encoder's fit phase (encoder._fit_adaptive_state) on the
+  1920x1080 image: its quadtree, the merged pools of every tile and the batched greedy
+  rounds with re-seeding, all on the device, with the NumPy restatement of pools and loop
+  on one CPU core for a sample of the tiles.
 - select: the selection helpers (argmin, exclusive scan, compaction, per-segment top-k)
   on 2^22 device elements.
 - template_fit: the stage-3 template fit (fit_tiles_gpu_constant) of the 3840x2160 image at

@@ -249,17 +249,21 @@ def time_to_psnr(tiles, mask, n_cand_plain, depth, seed, lam, refined_cfg):
     del rf
     # refined + batched: the refined fit in the persistent kernel, r rounds per launch, the
     # whole fit a graph replay.  It takes its own work split and workgroup size, the fastest
-    # of a few (the split decides how many small tiles a workgroup refines together).  (With
-    # the stepwise split it equals the stepwise fit bit for bit; another split orders the soft
-    # sums differently.)
+    # of a few (the split decides how many small tiles a workgroup refines together, and
+    # cpw = -2 refines each 32/64-px candidate on two waves, which shortens the long queue
+    # entries).  (With the stepwise split it equals the stepwise fit bit for bit; another split
+    # orders the soft sums differently.)
     batched = None
     if reach:
         r = reach["rounds"]
         tr, block, bcpw = None, None, None
         splits = [cpw] + [m for m in ({8: 8, 16: 4, 32: 2, 64: 1}, {8: 4, 16: 2, 32: 1, 64: 1},
-                                      {8: 16, 16: 4, 32: 2, 64: 1}) if m != cpw]
+                                      {8: 16, 16: 4, 32: 2, 64: 1}, {8: 16, 16: 4, 32: -2, 64: -2},
+                                      {8: 8, 16: 4, 32: -2, 64: -2}) if m != cpw]
         for sp in splits:
             for b in (128, 256):
+                if 64 * max(-min(sp.values()), 1) > b:  # cpw = -W takes W waves of the workgroup
+                    continue
                 bfr = BatchFitter(tiles, n_cand=n_cand, depth=depth, n_steps=r, seed=seed, lam=lam, k=k,
                                   refine=adam, cpw=sp, block=b)
                 t = min(bfr.time_runs(3, warmup=1))

@@ -376,7 +376,9 @@ typedef struct WeftSoftBatch {
     int32_t n_groups;
     int32_t k;                    /* candidates per group: 1, 2, 4, ..., 64 */
     float w;                      /* edge width in pixels (> 0) */
-    int32_t _pad;
+    int32_t waves;                /* 0 or 1: 64 / k lanes per candidate; 2, 4, 8 (k = 1): the
+                                     candidate's pixels over the 64 * waves lanes of that many
+                                     waves, as weft_refine splits them for cpw = -waves */
 } WeftSoftBatch;
 
 int weft_soft_eval(const WeftFitState* st, const WeftSoftBatch* b, void* stream);
@@ -402,16 +404,19 @@ typedef struct WeftRefineOpts {
     int32_t k;       /* 1, 2, 4, ..., 64 */
     int32_t n_iter;  /* >= 0 */
     const float* d_sched;  /* [n_iter][4] */
-    /* Work items, one 64-lane workgroup each, holding cpw = cpw_by_size[log2(ts) - 3]
-     * candidates (a power of two that divides k or is a multiple of it); the 64 / cpw
-     * lanes of a candidate split its pixels, which fixes the order of the soft sums, so
-     * cpw is part of the refinement's arithmetic.  cpw <= k: candidates d_work_j0[i] ..
-     * + cpw - 1 of tile d_work_tile[i * WEFT_WORK_MAX_TILES]; cpw > k: all k candidates of
-     * each of the cpw / k (<= WEFT_WORK_MAX_TILES) tiles d_work_tile[i *
-     * WEFT_WORK_MAX_TILES + g] (-1 for an unused slot; slot 0 is always a tile).  Every
-     * (tile, candidate) must be covered once; tiles whose stack is full are skipped.
-     * weft_batch_refine_fit reads only cpw_by_size (one tile at a time, 64 / cpw lanes per
-     * candidate, idle lanes when cpw > k). */
+    /* Work items, one workgroup each, holding cpw = cpw_by_size[log2(ts) - 3] candidates
+     * (a power of two that divides k or is a multiple of it); the 64 / cpw lanes of a
+     * candidate split its pixels, which fixes the order of the soft sums, so cpw is part
+     * of the refinement's arithmetic.  cpw <= k: candidates d_work_j0[i] .. + cpw - 1 of
+     * tile d_work_tile[i * WEFT_WORK_MAX_TILES]; cpw > k: all k candidates of each of the
+     * cpw / k (<= WEFT_WORK_MAX_TILES) tiles d_work_tile[i * WEFT_WORK_MAX_TILES + g] (-1
+     * for an unused slot; slot 0 is always a tile).  cpw = -W (W = 2, 4 or 8; one W for
+     * every size that uses it): one candidate per item, its pixels over the 64 W lanes of
+     * W waves (the large tiles of a fit with few of them, whose one-wave refinement would
+     * be the launch's longest).  Every (tile, candidate) must be covered once; tiles whose
+     * stack is full are skipped.  weft_batch_refine_fit reads only cpw_by_size (one tile
+     * at a time, 64 / cpw lanes per candidate, idle lanes when cpw > k; cpw = -W takes W of
+     * its workgroup's waves per candidate). */
     const int32_t* d_work_tile;
     const int32_t* d_work_j0;
     int32_t n_work;

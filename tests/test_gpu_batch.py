@@ -216,12 +216,16 @@ def test_batch_rejects_bad_launch(gpu):
                                                (2, {8: 1, 16: 2, 32: 2, 64: 1}, "fifo", 64),
                                                (16, None, "lpt", 256), (64, {8: 64, 16: 16, 32: 4, 64: 1}, "sjf", 128),
                                                (1, {8: 16, 16: 4, 32: 2, 64: 1}, "fifo", 128),
-                                               (2, {8: 32, 16: 8, 32: 4, 64: 2}, "sjf", 64)])
+                                               (2, {8: 32, 16: 8, 32: 4, 64: 2}, "sjf", 64),
+                                               (2, {8: 8, 16: 2, 32: 1, 64: -4}, "sjf", 256),
+                                               (1, {8: 16, 16: -2, 32: -2, 64: -2}, "fifo", 192),
+                                               (4, {8: 8, 16: 4, 32: -4, 64: -4}, "lpt", 256)])
 def test_batch_refine_equals_stepwise_refined(gpu, k, cpw, order, block):
     """The persistent refined fit = RefinedFitter round by round, bit for bit.  k <= 8 takes
     the top k by k workgroup-wide minima, k = 16 and 64 by the bitonic sort of the keys.
     Where cpw > k, groups of same-size tiles are refined and re-scored together (fifo over
-    four images breaks the runs of one size at every image)."""
+    four images breaks the runs of one size at every image).  cpw = -W spreads a candidate
+    over W waves (block 192 with W = 2 leaves a spare wave that keeps the barriers)."""
     from weft_amd.batch import BatchFitter
     from weft_amd.refine import AdamConfig, RefinedFitter
 

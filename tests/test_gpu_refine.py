@@ -64,8 +64,8 @@ def _random_params(rng, tiles, k):
     return kinds, prm
 
 
-@pytest.mark.parametrize("k", [1, 8, 64])
-def test_soft_sse_and_gradient_vs_finite_differences(gpu, k):
+@pytest.mark.parametrize("k,waves", [(1, 1), (8, 1), (64, 1), (1, 2), (1, 4), (1, 8)])
+def test_soft_sse_and_gradient_vs_finite_differences(gpu, k, waves):
     import torch
     from oracle import soft_np
     from weft_amd.refine import soft_eval
@@ -74,9 +74,9 @@ def test_soft_sse_and_gradient_vs_finite_differences(gpu, k):
     f = _fitter(tiles, canvas)
     rng = np.random.default_rng(100 + k)
     kinds, prm = _random_params(rng, tiles, k)
-    _say(f"soft_eval: {tiles.n_tiles} tiles x {k} candidates, w 0.5")
+    _say(f"soft_eval: {tiles.n_tiles} tiles x {k} candidates, {waves} waves each, w 0.5")
     gt = torch.arange(tiles.n_tiles, dtype=torch.int32, device="cuda")
-    sse, g = soft_eval(f, gt, torch.from_numpy(kinds).cuda(), torch.from_numpy(prm).cuda(), k, 0.5)
+    sse, g = soft_eval(f, gt, torch.from_numpy(kinds).cuda(), torch.from_numpy(prm).cuda(), k, 0.5, waves=waves)
     sse, g = sse.cpu().numpy(), g.cpu().numpy()
     worst = 0.0
     for i in range(len(kinds)):
@@ -178,7 +178,7 @@ def test_topk_matches_stable_sort(gpu):
 def _mirror_grads(f, tiles, kinds, prm, k, cpw, w):
     """Kernel gradients with the refinement's lane split: per tile size, 64 / cpw[ts] lanes
     per candidate (groups of cpw[ts] candidates of one tile; when cpw[ts] > k, a tile's k
-    candidates padded with copies of themselves)."""
+    candidates padded with copies of themselves), or 64 W lanes over W waves for cpw = -W."""
     import torch
     from weft_amd.refine import soft_eval
 
@@ -187,7 +187,13 @@ def _mirror_grads(f, tiles, kinds, prm, k, cpw, w):
         c = cpw[ts]
         ids = np.nonzero(tiles.ts == ts)[0]
         rows = ids[:, None] * k + np.arange(k)[None, :]
-        if c <= k:
+        if c < 0:
+            gt = torch.from_numpy(np.repeat(ids, k).astype(np.int32)).cuda()
+            r = rows.ravel()
+            _, gg = soft_eval(f, gt, torch.from_numpy(kinds[r]).cuda(), torch.from_numpy(prm[r]).cuda(), 1, w,
+                              waves=-c)
+            g[r] = gg.cpu().numpy()
+        elif c <= k:
             gt = torch.from_numpy(np.repeat(ids, k // c).astype(np.int32)).cuda()
             r = rows.ravel()
             _, gg = soft_eval(f, gt, torch.from_numpy(kinds[r]).cuda(), torch.from_numpy(prm[r]).cuda(), c, w)
@@ -202,7 +208,9 @@ def _mirror_grads(f, tiles, kinds, prm, k, cpw, w):
 
 @pytest.mark.parametrize("k,n_iter,cpw", [(8, 0, None), (8, 3, None), (64, 2, None), (2, 5, None),
                                           (16, 3, {8: 16, 16: 2, 32: 1, 64: 4}), (64, 2, {8: 64, 16: 64, 32: 64, 64: 64}),
-                                          (4, 3, {8: 64, 16: 16, 32: 8, 64: 2}), (1, 2, {8: 32, 16: 4, 32: 1, 64: 2})])
+                                          (4, 3, {8: 64, 16: 16, 32: 8, 64: 2}), (1, 2, {8: 32, 16: 4, 32: 1, 64: 2}),
+                                          (2, 3, {8: 8, 16: 2, 32: 1, 64: -4}), (1, 3, {8: 8, 16: -2, 32: -2, 64: -2}),
+                                          (2, 2, {8: -8, 16: 2, 32: -8, 64: -8})])
 def test_refine_matches_kernel_gradient_and_cpu_adam(gpu, oracle_lib, k, n_iter, cpw):
     import torch
     from oracle import soft_np

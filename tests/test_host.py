@@ -325,8 +325,10 @@ def test_refine_work_lists_cover_each_candidate_once():
     ts = rng.choice([8, 16, 32, 64], 203)
     M = 32
     for k, cpw in ((8, None), (64, None), (1, None), (2, None), (16, {8: 16, 16: 2, 32: 1, 64: 4}),
-                   (64, {32: 64}), (4, {8: 64, 16: 16, 32: 8, 64: 2})):
+                   (64, {32: 64}), (4, {8: 64, 16: 16, 32: 8, 64: 2}), (2, {32: -4, 64: -4}),
+                   (1, {8: -2, 64: -2})):
         wt, wj, cl = refine_work(ts, k, cpw)
+        cl = [max(c, 1) for c in cl]  # cpw = -W: one candidate per item
         per = dict(zip((8, 16, 32, 64), cl))
         assert len(wt) == M * len(wj)
         assert all(k % c == 0 or (c % k == 0 and c // k <= M) for c in cl)
@@ -355,6 +357,15 @@ def test_refine_work_lists_cover_each_candidate_once():
     assert set(x for x in wt.tolist() if x >= 0) == set(sub.tolist()) and len(wj) == 4 * len(sub)
     with pytest.raises(ValueError):
         refine_work(ts, 4, {8: 3})
+    with pytest.raises(ValueError):
+        refine_work(ts, 4, {64: -3})
+    with pytest.raises(ValueError):  # one block size: a single waves-per-candidate value
+        refine_work(ts, 4, {32: -2, 64: -4})
+    wt, wj, cl = refine_work(ts, 2, {64: -4})
+    n64 = int((ts == 64).sum())
+    big = {(int(wt[i * M]), int(wj[i])) for i in range(2 * n64)}
+    assert cl[3] == -4 and all(ts[t] == 64 for t, _ in big)  # the 64-px items lead, one candidate each
+    assert big == {(int(t), j) for t in np.nonzero(ts == 64)[0] for j in range(2)}
 
 
 def test_split_work_tables():

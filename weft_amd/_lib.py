@@ -15,7 +15,7 @@ import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libweft_hip.so")
-ABI_VERSION = 21
+ABI_VERSION = 22
 
 c_void_p = ctypes.c_void_p
 c_int = ctypes.c_int
@@ -87,7 +87,7 @@ class Batch(ctypes.Structure):
 class SoftBatch(ctypes.Structure):
     _fields_ = [
         ("d_group_tile", c_void_p), ("d_kinds", c_void_p), ("d_params", c_void_p), ("d_out_sse", c_void_p),
-        ("d_out_grad", c_void_p), ("n_groups", c_int32), ("k", c_int32), ("w", ctypes.c_float), ("_pad", c_int32),
+        ("d_out_grad", c_void_p), ("n_groups", c_int32), ("k", c_int32), ("w", ctypes.c_float), ("waves", c_int32),
     ]
 
 

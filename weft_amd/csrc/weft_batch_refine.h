@@ -35,7 +35,8 @@ struct RefineShared {
 
 // Adam on the top k of the group's G tiles, with the work split of weft_refine_kernel: work
 // item i holds cpw candidates (64 / cpw lanes each) -- cpw of one tile when cpw <= k, or all
-// k of each of cpw / k tiles -- and the workgroup's waves take the items in turn.  Writes
+// k of each of cpw / k tiles -- and the workgroup's waves take the items in turn; with
+// cpw = -W each aligned group of W waves takes one candidate at a time.  Writes
 // rs.codes[g * (k + 1) + 1 + j].
 template <int TS>
 __device__ __forceinline__ void refine_group(const WeftBatchRefineArgs& R, uint32_t step, int G, RefineShared& rs) {
@@ -44,25 +45,37 @@ __device__ __forceinline__ void refine_group(const WeftBatchRefineArgs& R, uint3
     const int tid = threadIdx.x, wave = tid >> 6, nwaves = blockDim.x >> 6, lane = tid & 63;
     const int k = R.o.k;
     const int K1 = k + 1;
-    const int cpw = R.o.cpw_by_size[CLS];
+    const int cpw_raw = R.o.cpw_by_size[CLS];
+    // cpw = -W: every wave runs every round's n_iter steps, each with a workgroup barrier, so
+    // the barriers line up; a wave group without a live candidate in a round (or the spare
+    // waves when W does not divide the waves) refines a stand-in and keeps nothing.
+    const bool mw = cpw_raw < 0;
+    const int W = mw ? -cpw_raw : 1;
+    const int cpw = mw ? 1 : cpw_raw;
     const int cpt = cpw < k ? cpw : k;           // candidates of one tile in an item
     const int tpw = cpw > k ? cpw / k : 1;       // tiles per item
     const int ipt = cpw < k ? k / cpw : 1;       // items per tile
     const int items = cpw > k ? (G + tpw - 1) / tpw : G * ipt;
-    const int Ln = 64 / cpw;
-    const int q = lane / Ln, slice = lane - q * Ln;
+    const int L = 64 * W / cpw;
+    const int per = nwaves / W;                  // items in flight (aligned groups of W waves)
+    const int gi = wave / W;
+    const int rounds = (items + per - 1) / per;
+    const int q = mw ? 0 : lane / L;
+    const int slice = mw ? (wave - gi * W) * 64 + lane : lane - q * L;
 #pragma unroll 1
-    for (int it = wave; it < items; it += nwaves) {
+    for (int r = 0; mw ? r < rounds : r * per + gi < items; ++r) {
+        const int it = r * per + gi;
         const int g = cpw > k ? it * tpw + q / cpt : it / ipt;
         const int j = cpw > k ? q - (q / cpt) * cpt : (it - (it / ipt) * ipt) * cpw + q;
-        if (g >= G || !rs.live[g]) continue;  // a candidate's lanes leave together
-        const int t = rs.gtile[g];
+        const bool keep = gi < per && it < items && g < G && rs.live[g];
+        if (!mw && !keep) continue;  // a candidate's lanes leave together
+        const int t = rs.gtile[keep ? g : 0];
         const int64_t base = st.d_tile_pix[t] * 3;
         const uint64_t dbase = draw_base(R.b.cd.seed, step, (uint32_t)t);
-        SoftCand C = soft_from_code(draw_code(dbase, (uint32_t)rs.topk[g * k + j], TS));
-        refine_candidate<WEFT_BATCH_REFINE_UNROLL, false>(R.o, st.d_target + base, st.d_canvas + base, TS, slice, Ln,
+        SoftCand C = soft_from_code(draw_code(dbase, keep ? (uint32_t)rs.topk[g * k + j] : 0u, TS));
+        refine_candidate<WEFT_BATCH_REFINE_UNROLL, false>(R.o, st.d_target + base, st.d_canvas + base, TS, slice, L,
                                                           C);
-        if (slice == 0) rs.codes[g * K1 + 1 + j] = soft_to_code(C, TS);
+        if (keep && slice == 0) rs.codes[g * K1 + 1 + j] = soft_to_code(C, TS);
     }
 }
 

@@ -542,9 +542,12 @@ int weft_soft_eval(const WeftFitState* st, const WeftSoftBatch* b, void* stream)
     if (!pow2_le64(b->k)) return set_err(WEFT_E_INVALID, "weft_soft_eval: k must be a power of two <= 64");
     if (b->n_groups < 0) return set_err(WEFT_E_INVALID, "weft_soft_eval: n_groups < 0");
     if (!(b->w > 0.0f)) return set_err(WEFT_E_INVALID, "weft_soft_eval: edge width must be > 0");
+    const int waves = b->waves > 1 ? b->waves : 1;
+    if (b->waves < 0 || (waves > 1 && (!pow2_le64(waves) || waves > 8 || b->k != 1)))
+        return set_err(WEFT_E_INVALID, "weft_soft_eval: waves must be 0, 1, 2, 4 or 8, and k 1 when above 1");
     if (b->n_groups == 0) return WEFT_OK;
     WeftSoftArgs a{*st, *b};
-    return launch(g_mod.f_soft, (unsigned)b->n_groups, 64, 0, stream, &a);
+    return launch(g_mod.f_soft, (unsigned)b->n_groups, (unsigned)(64 * waves), 0, stream, &a);
 }
 
 int weft_topk(const WeftFitState* st, const float* d_table, int n, int k, int32_t* d_out_idx, void* stream) {
@@ -569,15 +572,24 @@ int weft_refine(const WeftFitState* st, const WeftCands* cands, const int32_t* d
     if (!pow2_le64(o->k) || o->k > cands->n_cand)
         return set_err(WEFT_E_INVALID, "weft_refine: k must be a power of two <= min(64, n_cand)");
     if (o->n_iter < 0 || (o->n_iter > 0 && !o->d_sched)) return set_err(WEFT_E_INVALID, "weft_refine: bad schedule");
-    for (int c = 0; c < 4; ++c)
-        if (!refine_cpw_ok(o->cpw_by_size[c], o->k))
+    int waves = 1;
+    for (int c = 0; c < 4; ++c) {
+        const int cpw = o->cpw_by_size[c];
+        if (cpw < 0) {
+            if (!(cpw == -2 || cpw == -4 || cpw == -8) || (waves > 1 && waves != -cpw))
+                return set_err(WEFT_E_INVALID, "weft_refine: a negative cpw (waves per candidate) must be -2, -4 or "
+                                               "-8, the same for every size");
+            waves = -cpw;
+        } else if (!refine_cpw_ok(cpw, o->k)) {
             return set_err(WEFT_E_INVALID, "weft_refine: cpw must be powers of two that divide k or are "
                                            "multiples of it by at most WEFT_WORK_MAX_TILES");
+        }
+    }
     if (o->n_work < 0 || (o->n_work > 0 && (!o->d_work_tile || !o->d_work_j0)))
         return set_err(WEFT_E_INVALID, "weft_refine: bad work list");
     if (o->n_work == 0) return WEFT_OK;
     WeftRefineArgs a{*st, *cands, *o, d_topk, d_out_codes, d_out_prims, d_out_params};
-    return launch(g_mod.f_refine, (unsigned)o->n_work, 64, 0, stream, &a);
+    return launch(g_mod.f_refine, (unsigned)o->n_work, (unsigned)(64 * waves), 0, stream, &a);
 }
 
 int weft_fit_score_split(const WeftFitState* st, const WeftCands* cands, const WeftSplitWork* w,
@@ -662,10 +674,13 @@ int weft_batch_refine_fit(const WeftFitState* st, const WeftCands* cands, const 
         return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: k must be a power of two <= min(64, n_cand)");
     if (o->n_iter < 0 || (o->n_iter > 0 && !o->d_sched))
         return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: bad schedule");
-    for (int c = 0; c < 4; ++c)
-        if (!refine_cpw_ok(o->cpw_by_size[c], o->k))
+    for (int c = 0; c < 4; ++c) {
+        const int cpw = o->cpw_by_size[c];
+        if (cpw < 0 ? !((cpw == -2 || cpw == -4) && 64 * -cpw <= b->block) : !refine_cpw_ok(cpw, o->k))
             return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: cpw must be powers of two that divide k or are "
-                                           "multiples of it by at most WEFT_WORK_MAX_TILES");
+                                           "multiples of it by at most WEFT_WORK_MAX_TILES, or -2 / -4 waves per "
+                                           "candidate within the workgroup");
+    }
     if (b->d_group_first && (b->n_groups < 0 || b->group_max < 1 || b->group_max > 64 / o->k))
         return set_err(WEFT_E_INVALID, "weft_batch_refine_fit: groups need n_groups >= 0 and group_max in 1 .. 64 / k");
     WeftBatchRefineArgs a{};

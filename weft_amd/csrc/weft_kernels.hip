@@ -1825,19 +1825,21 @@ extern "C" __global__ __launch_bounds__(256) void weft_render_tiles_kernel(WeftR
     }
 }
 
-// Soft-edge SSE and its gradient for groups of k candidates (one 64-lane workgroup each).
+// Soft-edge SSE and its gradient for groups of k candidates (one workgroup each: 64 lanes, or
+// 64 W lanes for one candidate over W waves).
 #ifndef WEFT_SOFT_MIN_WAVES
 #define WEFT_SOFT_MIN_WAVES 1
 #endif
-extern "C" __global__ __launch_bounds__(64, WEFT_SOFT_MIN_WAVES) void weft_soft_eval_kernel(WeftSoftArgs A) {
+extern "C" __global__ __launch_bounds__(512, WEFT_SOFT_MIN_WAVES) void weft_soft_eval_kernel(WeftSoftArgs A) {
     soft_eval_body(A);
 }
 
 // Per-tile top-k indices of a value table (one workgroup per tile).
 extern "C" __global__ __launch_bounds__(256) void weft_topk_kernel(WeftTopkArgs A) { topk_body(A); }
 
-// Adam refinement of each tile's top-k candidates (one 64-lane workgroup per tile).
-extern "C" __global__ __launch_bounds__(64, WEFT_SOFT_MIN_WAVES) void weft_refine_kernel(WeftRefineArgs A) {
+// Adam refinement of each tile's top-k candidates (one workgroup per work item: 64 lanes, or
+// 64 W lanes when a tile size spreads a candidate over W waves).
+extern "C" __global__ __launch_bounds__(512, WEFT_SOFT_MIN_WAVES) void weft_refine_kernel(WeftRefineArgs A) {
     refine_body(A);
 }
 

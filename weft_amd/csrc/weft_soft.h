@@ -21,9 +21,9 @@
 // to the parameters once per candidate.
 
 #ifndef WEFT_SOFT_UNROLL
-// Pixels in flight per lane (memory-level parallelism vs registers).  One keeps the
-// refinement kernel at 117 VGPRs (4 waves per SIMD); measured on the 4096-tile refinement it
-// is the fastest setting (20.0 ms against 21.5 ms with 2 and 24.4 ms with 4).
+// Pixels in flight per lane (memory-level parallelism vs registers).  Measured on the
+// 4096-tile refinement, one is the fastest setting (20.0 ms against 21.5 ms with 2 and
+// 24.4 ms with 4).
 #define WEFT_SOFT_UNROLL 1
 #endif
 
@@ -292,8 +292,59 @@ __device__ __forceinline__ void soft_acc_reduce(SoftAcc& A, int L) {
     if (L > 32) soft_acc_level<32>(A);
 }
 
+// Waves per candidate at most (a candidate's lanes span W = 2, 4 or 8 waves of one workgroup).
+constexpr int SOFT_MAX_WAVES = 8;
+
+// The butterfly's levels above one wave, for a candidate whose L = 64 W lanes are the W waves
+// of the workgroup: after soft_acc_reduce every lane of wave w holds its wave's totals s_w;
+// level 64 forms s_w + s_(w^1), level 128 adds the partner pair's sum, level 256 the partner
+// quad's, so every wave ends with the same bits (float addition commutes).  All W waves must
+// call it; `parity` alternates between calls (two barriers per call already order the buffers'
+// reuse; the alternation keeps consecutive calls' writes apart).
+__device__ __forceinline__ void soft_acc_waves(SoftAcc& A, int W, int parity) {
+    constexpr int NS = 17;
+    __shared__ float s_part[2][SOFT_MAX_WAVES][NS];
+    __shared__ float s_tot[2][SOFT_MAX_WAVES][NS];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (lane == 0) {
+        float* mine = s_part[parity][wave];
+        mine[0] = A.sse;
+        mine[1] = A.gc0;
+        mine[2] = A.gc1;
+        mine[3] = A.gc2;
+        mine[4] = A.ga;
+#pragma unroll
+        for (int i = 0; i < 12; ++i) mine[5 + i] = A.q[i];
+    }
+    __syncthreads();
+    // lane i < 17 of each wave forms sum i over the aligned group of W waves holding the
+    // candidate, level by level
+    if (lane < NS) {
+        const int g0 = wave & ~(W - 1);
+        float lv[SOFT_MAX_WAVES];
+#pragma unroll
+        for (int w = 0; w < SOFT_MAX_WAVES; ++w) lv[w] = w < W ? s_part[parity][g0 + w][lane] : 0.0f;
+#pragma unroll
+        for (int off = 1; off < SOFT_MAX_WAVES; off <<= 1)
+#pragma unroll
+            for (int w = 0; w + off < SOFT_MAX_WAVES; w += 2 * off)
+                if (off < W) lv[w] = lv[w] + lv[w + off];
+        s_tot[parity][wave][lane] = lv[0];
+    }
+    __syncthreads();
+    const float* t = s_tot[parity][wave];
+    A.sse = t[0];
+    A.gc0 = t[1];
+    A.gc1 = t[2];
+    A.gc2 = t[3];
+    A.ga = t[4];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) A.q[i] = t[5 + i];
+}
+
 // SSE (and sums for the gradient) of candidate C over tile pixels slice, slice + L, ...
-// (ts a power of two, L a power of two <= ts * ts, so every lane takes ts * ts / L pixels).
+// (ts and L powers of two: every lane takes ts * ts / L pixels, or none when L > ts * ts and
+// slice >= ts * ts).
 // With PREFETCH the next pixel's canvas and target are loaded before the current pixel is
 // computed, so a lane keeps one pixel's loads in flight behind its arithmetic (six more
 // registers; the persistent kernel, short of registers, goes without).  UNROLL pixels per
@@ -306,8 +357,9 @@ __device__ __forceinline__ void soft_tile(const SoftCand& C, const SoftPre& P, f
     const int npx = ts * ts;
     const int lg = __builtin_ctz((unsigned)ts);
     if constexpr (PREFETCH) {
-        float n0 = cv[3 * slice], n1 = cv[3 * slice + 1], n2 = cv[3 * slice + 2];
-        float m0 = tg[3 * slice], m1 = tg[3 * slice + 1], m2 = tg[3 * slice + 2];
+        const int p0 = slice < npx ? slice : 0;  // a lane past the tile (L > ts * ts) takes no pixel
+        float n0 = cv[3 * p0], n1 = cv[3 * p0 + 1], n2 = cv[3 * p0 + 2];
+        float m0 = tg[3 * p0], m1 = tg[3 * p0 + 1], m2 = tg[3 * p0 + 2];
 #pragma unroll UNROLL
         for (int px = slice; px < npx; px += L) {
             const float c0 = n0, c1 = n1, c2 = n2, t0 = m0, t1 = m1, t2 = m2;
@@ -400,12 +452,13 @@ __device__ __forceinline__ WeftCode soft_to_code(const SoftCand& C, int ts) {
 }
 
 // ---- kernels' bodies -------------------------------------------------------------
-// One 64-lane workgroup per group of k candidates of one tile; L = 64 / k lanes per
-// candidate split the tile's pixels.
+// One workgroup per group of k candidates of one tile; L = 64 / k lanes per candidate split
+// the tile's pixels, or with A.b.waves = W > 1 (k = 1) the 64 W lanes of W waves.
 __device__ void soft_eval_body(const WeftSoftArgs& A) {
     const int g = blockIdx.x;
     const int k = A.b.k;
-    const int L = 64 / k;
+    const int W = A.b.waves > 1 ? A.b.waves : 1;
+    const int L = W > 1 ? 64 * W : 64 / k;
     const int lane = threadIdx.x;
     const int j = lane / L, slice = lane - j * L;
     const int t = A.b.d_group_tile[g];
@@ -422,6 +475,7 @@ __device__ void soft_eval_body(const WeftSoftArgs& A) {
     const float invw = 1.0f / A.b.w;
     if (A.b.d_out_grad) {
         soft_tile<true>(C, P, invw, A.st.d_target + base, A.st.d_canvas + base, ts, slice, L, S);
+        if (W > 1) soft_acc_waves(S, W, 0);
         if (slice == 0) {
             float gr[SOFT_NP];
             soft_finish(C, P, S, gr);
@@ -430,6 +484,7 @@ __device__ void soft_eval_body(const WeftSoftArgs& A) {
         }
     } else {
         soft_tile<false>(C, P, invw, A.st.d_target + base, A.st.d_canvas + base, ts, slice, L, S);
+        if (W > 1) soft_acc_waves(S, W, 0);
     }
     if (slice == 0) A.b.d_out_sse[ci] = S.sse;
 }
@@ -517,6 +572,7 @@ __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const 
         soft_pre(C, P);
         SoftAcc S;
         soft_tile<true, UNROLL, PREFETCH>(C, P, 1.0f / w, tg, cv, ts, slice, L, S);
+        if (L > 64) soft_acc_waves(S, L >> 6, it & 1);
         float gr[SOFT_NP];
         soft_finish(C, P, S, gr);
 #pragma unroll
@@ -531,19 +587,24 @@ __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const 
     }
 }
 
-// Adam on the soft SSE of each tile's top-k drawn candidates; one 64-lane workgroup
-// per work item: cpw candidates, 64 / cpw lanes each, of one tile (cpw <= k) or of
-// cpw / k tiles (cpw > k, all k candidates of each).  Every lane of a candidate holds the
-// same parameters and moments (the butterfly sums give all of them the same totals).
+// Adam on the soft SSE of each tile's top-k drawn candidates; one workgroup per work item:
+// cpw candidates, 64 / cpw lanes of the first wave each, of one tile (cpw <= k) or of
+// cpw / k tiles (cpw > k, all k candidates of each); a size with cpw = -W spreads one
+// candidate over W waves (64 W lanes).  The workgroup has 64 times the largest such W lanes;
+// an item of one wave leaves the others idle.  Every lane of a candidate holds the same
+// parameters and moments (the butterfly sums give all of them the same totals).
 __device__ void refine_body(const WeftRefineArgs& A) {
     const int item = blockIdx.x;
     const WeftFitState& st = A.st;
     const int k = A.o.k;
     const int ts = st.d_tile_ts[A.o.d_work_tile[(int64_t)item * WEFT_WORK_MAX_TILES]];
     const int cls = ts == 8 ? 0 : (ts == 16 ? 1 : (ts == 32 ? 2 : 3));
-    const int cpw = A.o.cpw_by_size[cls];
+    const int cpw_raw = A.o.cpw_by_size[cls];
+    const int W = cpw_raw < 0 ? -cpw_raw : 1;
+    if ((int)(threadIdx.x >> 6) >= W) return;  // the whole wave: no barrier waits for it
+    const int cpw = cpw_raw < 0 ? 1 : cpw_raw;
     const int cpt = cpw < k ? cpw : k;  // candidates per tile in this item
-    const int L = 64 / cpw;
+    const int L = 64 * W / cpw;
     const int lane = threadIdx.x;
     const int q = lane / L, slice = lane - q * L;
     const int g = q / cpt;  // tile slot: a candidate's lanes share it, so they leave together

@@ -36,11 +36,14 @@ def _state_of(fitter) -> _lib.FitState:
     return fitter._state
 
 
-def soft_eval(fitter, group_tile, kinds, params, k: int, w: float, grad: bool = True, stream=None):
+def soft_eval(fitter, group_tile, kinds, params, k: int, w: float, grad: bool = True, stream=None,
+              waves: int = 1):
     """Soft SSE (and d SSE / d params) of candidates in groups of k per tile.
 
     fitter: a PlainFitter (its target, canvas and tile table); group_tile: int32 tensor
     [G]; kinds: int32 tensor [G * k] (6, 7, 8); params: float32 tensor [G * k, 9].
+    waves > 1 (with k = 1) spreads each candidate's pixels over that many waves, the split
+    of a refinement with cpw = -waves.
     Returns (sse [G * k], grad [G * k, 9] or None) as new tensors.
     """
     torch = fitter.torch
@@ -53,7 +56,7 @@ def soft_eval(fitter, group_tile, kinds, params, k: int, w: float, grad: bool = 
     sse = torch.empty(G * k, dtype=torch.float32, device=params.device)
     g = torch.empty((G * k, 9), dtype=torch.float32, device=params.device) if grad else None
     b = _lib.SoftBatch(group_tile.data_ptr(), kinds.data_ptr(), params.data_ptr(), sse.data_ptr(),
-                       g.data_ptr() if g is not None else None, G, int(k), float(w), 0)
+                       g.data_ptr() if g is not None else None, G, int(k), float(w), int(waves))
     sp = _lib.stream_ptr() if stream is None else stream
     _lib.check(fitter.L.weft_soft_eval(ctypes.byref(_state_of(fitter)), ctypes.byref(b), sp), "weft_soft_eval")
     return sse, g
@@ -153,6 +156,20 @@ class AdamConfig:
 # fix the order of its soft sums, so this is part of the refinement's arithmetic.
 DEFAULT_CPW = {8: 8, 16: 4, 32: 2, 64: 1}
 XCDS = 8
+WAVE_OPTIONS = (-2, -4, -8)   # cpw = -W: one candidate over W waves
+
+
+def _check_cpw(cpw: dict[int, int], k: int) -> None:
+    M = _lib.WORK_MAX_TILES
+    waves = {c for c in cpw.values() if c < 0}
+    for ts, c in cpw.items():
+        if c < 0:
+            if c not in WAVE_OPTIONS:
+                raise ValueError(f"cpw[{ts}] = {c}: waves per candidate must be -2, -4 or -8")
+        elif not (c & (c - 1) == 0 and 1 <= c <= 64 and (k % c == 0 or (c % k == 0 and c // k <= M))):
+            raise ValueError(f"cpw[{ts}] = {c}: a power of two <= 64 dividing k = {k} or a multiple of it")
+    if len(waves) > 1:
+        raise ValueError(f"cpw: one negative value (waves per candidate) for every size, got {sorted(waves)}")
 
 
 def refine_work(tile_ts, k: int, cpw: dict[int, int] | None = None, tiles=None):
@@ -161,12 +178,10 @@ def refine_work(tile_ts, k: int, cpw: dict[int, int] | None = None, tiles=None):
     item holds candidates j0 .. j0 + cpw - 1 of one tile, and the items of a group of XCDS
     tiles are interleaved so that a tile's items are XCDS apart, run on one XCD (workgroup
     i goes to XCD i mod 8) and share its L2; with cpw > k an item holds all k candidates
-    of cpw / k tiles."""
+    of cpw / k tiles; with cpw = -W an item is one candidate, refined by W waves."""
     M = _lib.WORK_MAX_TILES
     cpw = {**DEFAULT_CPW, **(cpw or {})}
-    for ts, c in cpw.items():
-        if not (c & (c - 1) == 0 and 1 <= c <= 64 and (k % c == 0 or (c % k == 0 and c // k <= M))):
-            raise ValueError(f"cpw[{ts}] = {c}: a power of two <= 64 dividing k = {k} or a multiple of it")
+    _check_cpw(cpw, k)
     tile_ts = np.asarray(tile_ts)
     ids = np.arange(len(tile_ts)) if tiles is None else np.asarray(tiles)
     ts_ids = tile_ts[ids]
@@ -182,7 +197,7 @@ def refine_work(tile_ts, k: int, cpw: dict[int, int] | None = None, tiles=None):
     sizes = tile_ts[order]
     for ts in (64, 32, 16, 8):
         sel = order[sizes == ts]
-        c = cpw[ts]
+        c = max(cpw[ts], 1)
         if c > k:
             g = c // k
             for i0 in range(0, len(sel), g):
@@ -274,23 +289,29 @@ def split_work(tile_ts, n: int, cpw: dict[int, int] | None = None):
 
 
 def autotune_cpw(fitter, cands: _lib.Cands, topk_idx, k: int, cfg: AdamConfig,
-                 options=(1, 2, 4, 8, 16, 32, 64), reps: int = 2) -> dict[int, int]:
+                 options=(1, 2, 4, 8, 16, 32, 64), reps: int = 2,
+                 wave_options: dict[int, tuple[int, ...]] | None = None) -> dict[int, int]:
     """Candidates per work item for each tile size, the fastest on this workload.
 
     Each size class is timed alone (HIP events around weft_refine over that class's
-    tiles) for every valid option.  The choice sets the lanes per candidate and with them
-    the order of the soft sums, so a fit's refined candidates follow it; the choice itself
-    depends on timing, so pass a fixed cpw where runs must repeat bit for bit."""
+    tiles) for every valid option, and with `wave_options` (by tile size, e.g. {64: (-2,
+    -4)}) also with one candidate over that many waves.  None by default: timed alone, the
+    64-px class gains from more waves, but the launch then gives every item a workgroup of
+    that many waves, which costs the whole launch more than it saves.  The choice sets the
+    lanes per candidate and with them the order of the soft sums, so a fit's refined
+    candidates follow it; the choice itself depends on timing, so pass a fixed cpw where runs
+    must repeat bit for bit."""
     torch = fitter.torch
     stream = torch.cuda.current_stream()
+    wave_options = wave_options or {}
     best = {}
     for ts in sorted(set(np.asarray(fitter.tiles.ts).tolist())):
         ids = np.nonzero(np.asarray(fitter.tiles.ts) == ts)[0]
         times = {}
-        for c in options:
-            if not (k % c == 0 or (c % k == 0 and c // k <= _lib.WORK_MAX_TILES)):
+        for c in (*options, *wave_options.get(int(ts), ())):
+            if c > 0 and not (k % c == 0 or (c % k == 0 and c // k <= _lib.WORK_MAX_TILES)):
                 continue
-            wt, wj, cl = refine_work(fitter.tiles.ts, k, {ts: c}, tiles=ids)
+            wt, wj, cl = refine_work(fitter.tiles.ts, k, {8: 1, 16: 1, 32: 1, 64: 1, ts: c}, tiles=ids)
             work = (torch.from_numpy(wt).to(fitter.dev), torch.from_numpy(wj).to(fitter.dev), cl)
             refine(fitter, cands, topk_idx, k, cfg, work=work)  # warm
             a = torch.cuda.Event(enable_timing=True)

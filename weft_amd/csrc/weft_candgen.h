@@ -189,17 +189,58 @@ __device__ void cg_first_of_key(int (*keys)[CG_KEY], uint32_t* hash, int* keep, 
     for (int j = tid; j < n_total; j += CG_THREADS) {
         const uint32_t hj = hash[j];
         int k = 1;
-        for (int i = 0; i < j; ++i)
+        int i = 0;
+        // four hashes per trip (their loads in flight together); a trip with a match looks at
+        // its four in order, keys compared in full
+        for (; i + 4 <= j && k; i += 4) {
+            const uint32_t h0 = hash[i], h1 = hash[i + 1], h2 = hash[i + 2], h3 = hash[i + 3];
+            if (h0 == hj || h1 == hj || h2 == hj || h3 == hj) {
+                for (int u = i; u < i + 4 && k; ++u)
+                    if (hash[u] == hj) {
+                        bool same = true;
+                        for (int t = 0; t < CG_KEY && same; ++t) same = keys[u][t] == keys[j][t];
+                        if (same) k = 0;
+                    }
+            }
+        }
+        for (; i < j && k; ++i)
             if (hash[i] == hj) {
                 bool same = true;
                 for (int t = 0; t < CG_KEY && same; ++t) same = keys[i][t] == keys[j][t];
-                if (same) {
-                    k = 0;
-                    break;
-                }
+                if (same) k = 0;
             }
         keep[j] = k;
     }
+}
+
+// keep[j] (1 to keep, 0 to drop, j < n <= 2 * CG_THREADS) becomes the candidate's rank among
+// the kept ones, or -1: ballots over chunks of 64 candidates, then the chunks' counts in
+// order.  Returns the number kept (to every thread); ends with a barrier.
+__device__ int cg_keep_ranks(int* keep, int n) {
+    __shared__ int s_chunk[2 * CG_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63;
+    int kq[2], rank[2];
+    for (int q = 0; q < 2; ++q) {
+        const int j = tid + q * CG_THREADS;
+        kq[q] = j < n ? keep[j] : 0;
+        const unsigned long long m = __ballot(kq[q] != 0);
+        rank[q] = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) s_chunk[(tid >> 6) + q * (CG_THREADS / 64)] = __popcll(m);
+    }
+    __syncthreads();
+    int off[2] = {0, 0}, total = 0;
+    for (int c = 0; c < 2 * CG_THREADS / 64; ++c) {
+        const int v = s_chunk[c];
+        if (c < (tid >> 6)) off[0] += v;
+        if (c < (tid >> 6) + CG_THREADS / 64) off[1] += v;
+        total += v;
+    }
+    for (int q = 0; q < 2; ++q) {
+        const int j = tid + q * CG_THREADS;
+        if (j < n) keep[j] = kq[q] ? off[q] + rank[q] : -1;
+    }
+    __syncthreads();
+    return total;
 }
 
 __device__ void cg_finish(int (*keys)[CG_KEY], uint32_t* hash, int* keep, int* n_kept_lds, WeftPrim* out,
@@ -207,17 +248,11 @@ __device__ void cg_finish(int (*keys)[CG_KEY], uint32_t* hash, int* keep, int* n
     const int tid = threadIdx.x;
     cg_first_of_key(keys, hash, keep, n_total);
     __syncthreads();
+    const int total = cg_keep_ranks(keep, n_total);
     if (tid == 0) {
-        int run = 0;
-        for (int j = 0; j < n_total; ++j) {
-            const int k = keep[j];
-            keep[j] = k ? run : -1;
-            run += k;
-        }
-        *out_count = run;
-        *n_kept_lds = run;
+        *out_count = total;
+        *n_kept_lds = total;
     }
-    __syncthreads();
     WeftPrim mine[2];
     int dst[2] = {-1, -1};
     for (int q = 0; q < 2; ++q) {
@@ -230,7 +265,7 @@ __device__ void cg_finish(int (*keys)[CG_KEY], uint32_t* hash, int* keep, int* n
     __syncthreads();
     for (int q = 0; q < 2; ++q)
         if (dst[q] >= 0) out[dst[q]] = mine[q];
-    const int n_kept = *n_kept_lds;
+    const int n_kept = total;
     for (int j = n_kept + tid; j < max_cands; j += CG_THREADS) {
         WeftPrim P;
         P.kind = -1;

@@ -515,7 +515,6 @@ __device__ void mg_body(const WeftMergePoolsArgs& A) {
     __shared__ int keys[MG_MAXC][CG_KEY];
     __shared__ int keep[MG_MAXC];
     __shared__ uint32_t hash[MG_MAXC];
-    __shared__ int n_kept_s;
     const int tid = threadIdx.x, slot = blockIdx.x;
     const int na = A.a_count[slot], nb = A.b_count[slot];
     const bool ok = na >= 0 && na <= A.a_cap && nb >= 0 && nb <= A.b_cap;
@@ -533,17 +532,8 @@ __device__ void mg_body(const WeftMergePoolsArgs& A) {
     __syncthreads();
     cg_first_of_key(keys, hash, keep, n);
     __syncthreads();
-    if (tid == 0) {
-        int run = 0;
-        for (int j = 0; j < n; ++j) {
-            const int k = keep[j];
-            keep[j] = k ? run : -1;
-            run += k;
-        }
-        n_kept_s = run;
-        A.out_count[slot] = ok ? run : -1;
-    }
-    __syncthreads();
+    const int run = cg_keep_ranks(keep, n);
+    if (tid == 0) A.out_count[slot] = ok ? run : -1;
     for (int j = tid; j < n; j += CG_THREADS) {
         const int d = keep[j];
         if (d < 0) continue;
@@ -556,7 +546,7 @@ __device__ void mg_body(const WeftMergePoolsArgs& A) {
             for (int q = 0; q < 14; ++q) o[q] = s[q];
         }
     }
-    for (int j = n_kept_s + tid; j < A.out_cap; j += CG_THREADS) {
+    for (int j = run + tid; j < A.out_cap; j += CG_THREADS) {
         WeftPrim P;
         P.kind = -1;
         for (int i = 0; i < 7; ++i) P.geom[i] = 0.0f;

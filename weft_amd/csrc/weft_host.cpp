@@ -41,6 +41,7 @@ struct Module {
     hipFunction_t f_soft = nullptr;
     hipFunction_t f_topk = nullptr;
     hipFunction_t f_refine = nullptr;
+    hipFunction_t f_refine_mw = nullptr;
     hipFunction_t f_split = nullptr;
     hipFunction_t f_batch_refine = nullptr;
     hipFunction_t f_evals_split = nullptr;
@@ -229,6 +230,7 @@ int weft_init(int device) {
         {&g_mod.f_soft, "weft_soft_eval_kernel"},
         {&g_mod.f_topk, "weft_topk_kernel"},
         {&g_mod.f_refine, "weft_refine_kernel"},
+        {&g_mod.f_refine_mw, "weft_refine_mw_kernel"},
         {&g_mod.f_split, "weft_fit_score_split_kernel"},
         {&g_mod.f_batch_refine, "weft_batch_refine_kernel"},
         {&g_mod.f_evals_split, "weft_score_evals_split_kernel"},
@@ -589,7 +591,8 @@ int weft_refine(const WeftFitState* st, const WeftCands* cands, const int32_t* d
         return set_err(WEFT_E_INVALID, "weft_refine: bad work list");
     if (o->n_work == 0) return WEFT_OK;
     WeftRefineArgs a{*st, *cands, *o, d_topk, d_out_codes, d_out_prims, d_out_params};
-    return launch(g_mod.f_refine, (unsigned)o->n_work, (unsigned)(64 * waves), 0, stream, &a);
+    return launch(waves > 1 ? g_mod.f_refine_mw : g_mod.f_refine, (unsigned)o->n_work, (unsigned)(64 * waves), 0,
+                  stream, &a);
 }
 
 int weft_fit_score_split(const WeftFitState* st, const WeftCands* cands, const WeftSplitWork* w,

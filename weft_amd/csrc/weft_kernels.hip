@@ -1837,10 +1837,14 @@ extern "C" __global__ __launch_bounds__(512, WEFT_SOFT_MIN_WAVES) void weft_soft
 // Per-tile top-k indices of a value table (one workgroup per tile).
 extern "C" __global__ __launch_bounds__(256) void weft_topk_kernel(WeftTopkArgs A) { topk_body(A); }
 
-// Adam refinement of each tile's top-k candidates (one workgroup per work item: 64 lanes, or
-// 64 W lanes when a tile size spreads a candidate over W waves).
-extern "C" __global__ __launch_bounds__(512, WEFT_SOFT_MIN_WAVES) void weft_refine_kernel(WeftRefineArgs A) {
-    refine_body(A);
+// Adam refinement of each tile's top-k candidates (one 64-lane workgroup per work item), and
+// the form for a work split that spreads a size's candidates over W waves (one workgroup of
+// 64 W lanes per item).
+extern "C" __global__ __launch_bounds__(64, WEFT_SOFT_MIN_WAVES) void weft_refine_kernel(WeftRefineArgs A) {
+    refine_body<false>(A);
+}
+extern "C" __global__ __launch_bounds__(512, WEFT_SOFT_MIN_WAVES) void weft_refine_mw_kernel(WeftRefineArgs A) {
+    refine_body<true>(A);
 }
 
 // Exact scores of a few candidates per tile, each candidate's pixels split over lanes.

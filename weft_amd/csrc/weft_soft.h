@@ -557,8 +557,10 @@ __device__ void topk_body(const WeftTopkArgs& A) {
 
 // n_iter Adam steps on candidate C's soft SSE over its tile, L lanes per candidate
 // (this lane takes pixels slice, slice + L, ...), projected onto the code ranges after
-// every step.  Every lane of the candidate ends with the same parameters.
-template <int UNROLL = SOFT_UNROLL, bool PREFETCH = true>
+// every step.  Every lane of the candidate ends with the same parameters.  MW: L may exceed
+// 64 (the candidate's lanes span L / 64 waves, every one of which must call this); without
+// it L <= 64 and the code keeps no cross-wave step (fewer registers).
+template <int UNROLL = SOFT_UNROLL, bool PREFETCH = true, bool MW = true>
 __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const float* tg, const float* cv,
                                                  int ts, int slice, int L, SoftCand& C) {
     float m1[SOFT_NP], m2[SOFT_NP];
@@ -572,7 +574,9 @@ __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const 
         soft_pre(C, P);
         SoftAcc S;
         soft_tile<true, UNROLL, PREFETCH>(C, P, 1.0f / w, tg, cv, ts, slice, L, S);
-        if (L > 64) soft_acc_waves(S, L >> 6, it & 1);
+        if constexpr (MW) {
+            if (L > 64) soft_acc_waves(S, L >> 6, it & 1);
+        }
         float gr[SOFT_NP];
         soft_finish(C, P, S, gr);
 #pragma unroll
@@ -593,6 +597,7 @@ __device__ __forceinline__ void refine_candidate(const WeftRefineOpts& o, const 
 // candidate over W waves (64 W lanes).  The workgroup has 64 times the largest such W lanes;
 // an item of one wave leaves the others idle.  Every lane of a candidate holds the same
 // parameters and moments (the butterfly sums give all of them the same totals).
+template <bool MW>
 __device__ void refine_body(const WeftRefineArgs& A) {
     const int item = blockIdx.x;
     const WeftFitState& st = A.st;
@@ -600,7 +605,7 @@ __device__ void refine_body(const WeftRefineArgs& A) {
     const int ts = st.d_tile_ts[A.o.d_work_tile[(int64_t)item * WEFT_WORK_MAX_TILES]];
     const int cls = ts == 8 ? 0 : (ts == 16 ? 1 : (ts == 32 ? 2 : 3));
     const int cpw_raw = A.o.cpw_by_size[cls];
-    const int W = cpw_raw < 0 ? -cpw_raw : 1;
+    const int W = MW && cpw_raw < 0 ? -cpw_raw : 1;
     if ((int)(threadIdx.x >> 6) >= W) return;  // the whole wave: no barrier waits for it
     const int cpw = cpw_raw < 0 ? 1 : cpw_raw;
     const int cpt = cpw < k ? cpw : k;  // candidates per tile in this item
@@ -626,7 +631,7 @@ __device__ void refine_body(const WeftRefineArgs& A) {
         A.out_prims[slot0] = code_to_prim(best);
     }
     SoftCand C = soft_from_code(code0);
-    refine_candidate(A.o, tg, cv, ts, slice, L, C);
+    refine_candidate<SOFT_UNROLL, true, MW>(A.o, tg, cv, ts, slice, L, C);
     if (slice == 0) {
         const WeftCode qc = soft_to_code(C, ts);
         if (A.out_codes) A.out_codes[slot0 + 1 + j] = qc;

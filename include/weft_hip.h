@@ -412,11 +412,12 @@ typedef struct WeftRefineOpts {
      * cpw / k (<= WEFT_WORK_MAX_TILES) tiles d_work_tile[i * WEFT_WORK_MAX_TILES + g] (-1
      * for an unused slot; slot 0 is always a tile).  cpw = -W (W = 2, 4 or 8; one W for
      * every size that uses it): one candidate per item, its pixels over the 64 W lanes of
-     * W waves (the large tiles of a fit with few of them, whose one-wave refinement would
-     * be the launch's longest).  Every (tile, candidate) must be covered once; tiles whose
-     * stack is full are skipped.  weft_batch_refine_fit reads only cpw_by_size (one tile
-     * at a time, 64 / cpw lanes per candidate, idle lanes when cpw > k; cpw = -W takes W of
-     * its workgroup's waves per candidate). */
+     * W waves; every item's workgroup then has 64 W lanes (an item of another size uses
+     * its first wave).  It shortens the longest queue entries of weft_batch_refine_fit.
+     * Every (tile, candidate) must be covered once; tiles whose stack is full are skipped.
+     * weft_batch_refine_fit reads only cpw_by_size (one tile at a time, 64 / cpw lanes per
+     * candidate, idle lanes when cpw > k; cpw = -W takes W of its workgroup's waves per
+     * candidate, 64 W <= its block). */
     const int32_t* d_work_tile;
     const int32_t* d_work_j0;
     int32_t n_work;
